@@ -1,0 +1,333 @@
+#!/usr/bin/env python3
+"""Gate-sweep benchmark: algorithmic HBM GB/s of a 30-qubit complex128 sweep on one MI355X.
+
+One step = one pass of the 200-gate sweep (1q/2q Haar dense, RX/RY/RZ, CZ/CNOT,
+fused 5q dense; seeded) over a 30-qubit state (16 GiB) that stays resident in
+HBM.  value = sum over gates of 2 x 16 B x (amplitudes the gate can change),
+divided by the time of the timed steps.  Gates are issued through the C ABI
+(libqulacs_mi355) on one HIP stream in stream-ordered mode.
+
+Besides the headline the JSON line carries:
+  roofline      -- the dominant kernel's achieved GB/s (algorithmic bytes per
+                   launch / mean launch time from HIP events on the launch
+                   stream) against the 8 TB/s HBM peak;
+  per_kind      -- the same per gate kind and target position;
+  dm_sweep      -- the 14-qubit density-matrix variant (depolarizing p = 0.01
+                   after every gate on each touched qubit);
+  cpu_baseline  -- the reference csim compiled from its sources
+                   (oracle/_ref/libcsim_ref.so; the in-repo restatement when
+                   that is absent) on the host cores, on a time-boxed prefix of
+                   the same gate list at the same size.
+
+Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBPS = 8000.0
+
+
+class HipEvents:
+    """hipEvent timing on an explicit stream (libamdhip64 through ctypes)."""
+
+    def __init__(self):
+        self.h = C.CDLL("libamdhip64.so")
+        self.h.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.h.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.h.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.h.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.h.hipEventDestroy.argtypes = [C.c_void_p]
+        self.h.hipStreamSynchronize.argtypes = [C.c_void_p]
+        self.h.hipDeviceSynchronize.argtypes = []
+        self.h.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+
+    def create(self, n: int) -> list:
+        evs = []
+        for _ in range(n):
+            e = C.c_void_p()
+            assert self.h.hipEventCreate(C.byref(e)) == 0
+            evs.append(e)
+        return evs
+
+    def record(self, e, stream) -> None:
+        assert self.h.hipEventRecord(e, stream) == 0
+
+    def elapsed_ms(self, a, b) -> float:
+        self.h.hipEventSynchronize(b)
+        f = C.c_float()
+        assert self.h.hipEventElapsedTime(C.byref(f), a, b) == 0
+        return float(f.value)
+
+    def destroy(self, evs) -> None:
+        for e in evs:
+            self.h.hipEventDestroy(e)
+
+
+KERNEL_OF_KIND = {
+    # gate kind -> kernel that executes it (qulacs_amd/csrc/qm_gates.hip)
+    "dense1": "k_gather<1,OpDenseArg<1>>", "RX": "k_gather<1,OpDenseArg<1>>", "RY": "k_gather<1,OpDenseArg<1>>",
+    "RZ": "k_gather<1,OpDiag<1>>", "dense2": "k_gather<2,OpDenseArg<2>>", "dense5": "k_dense_stream<5>",
+    "CNOT": "k_gather<1,OpSwap01>", "CZ": "k_gather<1,OpNeg1>",
+}
+
+
+def sv_sweep(args, lib_mod, hip):
+    from qulacs_amd.workloads import GateLauncher, alg_bytes, gate_sweep
+
+    L = lib_mod.lib()
+    n = args.qubits
+    dim = 1 << n
+    gates = gate_sweep(n, args.gates, seed=args.seed)
+    launcher = GateLauncher(lib_mod, gates)
+    stream = lib_mod.call("allocate_cuda_stream_host", 1, 0)
+    hstream = C.c_void_p.from_address(stream).value
+    state = lib_mod.call("allocate_quantum_state_host", dim, 0)
+    lib_mod.call("initialize_Haar_random_state_with_seed_host", state, dim, 1, stream, 0)
+    lib_mod.call("qm_set_synchronous", 0)
+
+    def sweep():
+        for i in range(len(gates)):
+            launcher.launch(i, state, dim, stream)
+
+    for _ in range(args.warmup):
+        sweep()
+    lib_mod.call("qm_synchronize", stream, 0)
+    lib_mod.check()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sweep()
+    lib_mod.call("qm_synchronize", stream, 0)
+    t1 = time.perf_counter()
+    lib_mod.check()
+    total_bytes = sum(alg_bytes(g, n) for g in gates)
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = total_bytes * args.steps / (t1 - t0) / 1e9
+
+    # per-gate device time: events between consecutive launches on the launch stream
+    evs = hip.create(len(gates) + 1)
+    hip.record(evs[0], hstream)
+    for i in range(len(gates)):
+        launcher.launch(i, state, dim, stream)
+        hip.record(evs[i + 1], hstream)
+    lib_mod.call("qm_synchronize", stream, 0)
+    durs = [hip.elapsed_ms(evs[i], evs[i + 1]) for i in range(len(gates))]
+    hip.destroy(evs)
+    norm = lib_mod.call("state_norm_squared_host", state, dim, stream, 0)
+
+    per_kind = {}
+    per_kernel = {}
+    for g, d in zip(gates, durs):
+        key = f"{g.kind}/{g.position}"
+        e = per_kind.setdefault(key, {"count": 0, "ms": 0.0, "bytes": 0})
+        e["count"] += 1
+        e["ms"] += d
+        e["bytes"] += alg_bytes(g, n)
+        k = per_kernel.setdefault(KERNEL_OF_KIND[g.kind], {"count": 0, "ms": 0.0, "bytes": 0})
+        k["count"] += 1
+        k["ms"] += d
+        k["bytes"] += alg_bytes(g, n)
+    for e in list(per_kind.values()) + list(per_kernel.values()):
+        e["GBps"] = round(e["bytes"] / (e["ms"] * 1e-3) / 1e9, 1)
+        e["frac_of_peak"] = round(e["GBps"] / HBM_PEAK_GBPS, 4)
+        e["mean_ms"] = round(e["ms"] / e["count"], 4)
+        e["ms"] = round(e["ms"], 3)
+    dom_name, dom = max(per_kernel.items(), key=lambda kv: kv[1]["ms"])
+    per_launch = dom["bytes"] / dom["count"]
+    achieved = per_launch / (dom["mean_ms"] * 1e-3) / 1e9
+
+    # measured device-to-device copy rate for reference (read + write bytes)
+    half = dim * 16 // 2
+    ev2 = hip.create(2)
+    hip.record(ev2[0], hstream)
+    hip.h.hipMemcpyAsync(C.c_void_p(state + half), C.c_void_p(state), half, 3, hstream)
+    hip.record(ev2[1], hstream)
+    copy_ms = hip.elapsed_ms(ev2[0], ev2[1])
+    hip.destroy(ev2)
+
+    lib_mod.call("qm_set_synchronous", 1)
+    lib_mod.call("release_quantum_state_host", state, 0)
+    lib_mod.call("release_cuda_stream_host", stream, 1, 0)
+    return {
+        "value": value, "ms_per_step": ms_per_step, "gates": gates, "per_kind": per_kind, "per_kernel": per_kernel,
+        "norm_after": norm,
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "bytes_per_launch": per_launch, "mean_launch_ms": dom["mean_ms"]},
+        "hbm_copy_GBps": round(2 * half / (copy_ms * 1e-3) / 1e9, 1),
+    }
+
+
+def dm_sweep(args, lib_mod, hip):
+    """14-qubit density matrix: every gate followed by depolarizing(0.01) on its qubits."""
+    from qulacs_amd.workloads import gate_sweep, rotation_matrix
+
+    n = args.dm_qubits
+    dim = 1 << n
+    gates = gate_sweep(n, args.gates, seed=args.seed + 1)
+    stream = lib_mod.call("allocate_cuda_stream_host", 1, 0)
+    hstream = C.c_void_p.from_address(stream).value
+    rho = lib_mod.call("allocate_quantum_state_host", dim * dim, 0)
+    lib_mod.call("dm_initialize_quantum_state_host", rho, dim, stream, 0)
+    lib_mod.call("qm_set_synchronous", 0)
+    L = lib_mod.lib()
+    from qulacs_amd.workloads import _cmat
+    X = np.array([[0, 1], [1, 0]])
+    Z = np.diag([1, -1])
+    prepared = []
+    for g in gates:
+        if g.kind in ("CNOT", "CZ"):
+            m = _cmat(lib_mod, X if g.kind == "CNOT" else Z)
+            prepared.append(("c", (C.c_uint * 1)(g.targets[0]), (C.c_uint * 1)(1), (C.c_uint * 1)(g.targets[1]), m))
+        else:
+            mat = g.matrix if g.matrix is not None else rotation_matrix(g.kind, g.angle)
+            prepared.append(("u", (C.c_uint * len(g.targets))(*g.targets), len(g.targets), _cmat(lib_mod, mat)))
+
+    def run():
+        for g, p in zip(gates, prepared):
+            if p[0] == "c":
+                L.dm_multi_qubit_control_multi_qubit_dense_matrix_gate_host(p[1], p[2], 1, p[3], 1, p[4], rho, dim,
+                                                                             stream, 0)
+            elif p[2] == 1:
+                L.dm_single_qubit_dense_matrix_gate_host(g.targets[0], p[3], rho, dim, stream, 0)
+            else:
+                L.dm_multi_qubit_dense_matrix_gate_host(p[1], p[2], p[3], rho, dim, stream, 0)
+            for q in g.targets:
+                L.dm_depolarizing_host(q, 0.01, rho, dim, stream, 0)
+
+    run()
+    lib_mod.call("qm_synchronize", stream, 0)
+    lib_mod.check()
+    ev = hip.create(2)
+    hip.record(ev[0], hstream)
+    run()
+    hip.record(ev[1], hstream)
+    ms = hip.elapsed_ms(ev[0], ev[1])
+    hip.destroy(ev)
+    lib_mod.check()
+    tr = lib_mod.call("dm_state_norm_squared_host", rho, dim, stream, 0)
+    ops = len(gates) + sum(len(g.targets) for g in gates)
+    alg = ops * 32 * dim * dim  # each gate / channel may change every entry once
+    lib_mod.call("qm_set_synchronous", 1)
+    lib_mod.call("release_quantum_state_host", rho, 0)
+    lib_mod.call("release_cuda_stream_host", stream, 1, 0)
+    return {"qubits": n, "gates": len(gates), "channels": ops - len(gates), "ms": round(ms, 2),
+            "alg_GBps": round(alg / (ms * 1e-3) / 1e9, 1), "trace_after": tr}
+
+
+def cpu_baseline(args, gates, n):
+    """Reference csim on the host cores, on a time-boxed prefix of the same sweep."""
+    from oracle import oracle as O
+    from qulacs_amd.workloads import alg_bytes, rotation_matrix
+
+    if O.reference_available():
+        k, kind = O.Reference(), "reference"
+    else:
+        k, kind = O.Oracle(), "port"
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    threads = min(threads, os.cpu_count() or threads)
+    k.set_num_threads(threads)
+    dim = 1 << n
+    psi = np.empty(dim, dtype=np.complex128)
+    psi.fill(1.0 / np.sqrt(dim))
+    done_bytes = 0
+    spent = 0.0
+    count = 0
+    for g in gates:
+        t0 = time.perf_counter()
+        if g.kind == "dense1":
+            k.dense1(g.targets[0], g.matrix, psi)
+        elif g.kind == "dense2":
+            k.dense2(g.targets[0], g.targets[1], g.matrix, psi)
+        elif g.kind == "dense5":
+            k.dense(g.targets, g.matrix, psi)
+        elif g.kind in ("RX", "RY"):
+            k.dense1(g.targets[0], rotation_matrix(g.kind, g.angle), psi)
+        elif g.kind == "RZ":
+            m = rotation_matrix("RZ", g.angle)
+            k.diag1(g.targets[0], [m[0, 0], m[1, 1]], psi)
+        else:
+            k.two(g.kind, g.targets[0], g.targets[1], psi)
+        spent += time.perf_counter() - t0
+        done_bytes += alg_bytes(g, n)
+        count += 1
+        if spent >= args.cpu_seconds:
+            break
+    return {"value": round(done_bytes / spent / 1e9, 2), "unit": "GB/s", "cores": threads, "kind": kind,
+            "sample": f"first {count} of {len(gates)} gates of the same {n}-qubit sweep, {spent:.1f} s, "
+                      f"{threads} OpenMP threads"}
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--qubits", type=int, default=30)
+    ap.add_argument("--gates", type=int, default=200)
+    ap.add_argument("--dm-qubits", type=int, default=14)
+    ap.add_argument("--seed", type=int, default=2002)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-dm", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the sweep is a single 30-qubit "
+              "state on one device; see the sharding notes in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from qulacs_amd import _lib
+
+    hip = HipEvents()
+    print(f"[bench] {args.qubits}-qubit sweep, {args.gates} gates, warmup {args.warmup}, steps {args.steps}",
+          file=sys.stderr, flush=True)
+    sv = sv_sweep(args, _lib, hip)
+    print(f"[bench] sweep {sv['value']:.1f} GB/s, {sv['ms_per_step']:.1f} ms/step", file=sys.stderr, flush=True)
+    dm = None
+    if not args.no_dm:
+        dm = dm_sweep(args, _lib, hip)
+        print(f"[bench] dm sweep {dm['ms']} ms", file=sys.stderr, flush=True)
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args, sv["gates"], args.qubits)
+        print(f"[bench] cpu baseline {cpu}", file=sys.stderr, flush=True)
+    line = {
+        "metric": "30q gate-sweep HBM GB/s",
+        "value": round(sv["value"], 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sv["ms_per_step"], 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Haar gates, Haar-random initial state)",
+        "config": {"workload": f"single large state: {args.qubits}-qubit complex128 state vector, {args.gates}-gate "
+                               "sweep over 1q/2q Haar dense, RX/RY/RZ, CZ/CNOT and fused 5q dense",
+                   "qubits": args.qubits, "gates": args.gates, "seed": args.seed},
+        "roofline": sv["roofline"],
+        "cpu_baseline": cpu,
+        "hbm_copy_GBps": sv["hbm_copy_GBps"],
+        "per_kind": sv["per_kind"],
+        "per_kernel": sv["per_kernel"],
+        "dm_sweep": dm,
+        "norm_after": sv["norm_after"],
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

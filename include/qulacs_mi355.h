@@ -1,0 +1,244 @@
+/*
+ * libqulacs_mi355 -- C ABI of the MI355X (gfx950) state-vector and
+ * density-matrix kernels.
+ *
+ * The state-vector entry points keep the names, argument order and meaning of
+ * qulacs's GPU seam, the `*_host` family that cppsim calls whenever
+ * `state->get_device_name() == "gpu"` (reference: src/gpusim/update_ops_cuda.h,
+ * src/gpusim/stat_ops.h, src/gpusim/memory_ops.h, src/gpusim/util_func.h).  Each
+ * declaration below cites the reference declaration it replaces.
+ *
+ * Types (reference src/gpusim/util_type.h:9-17):
+ *   ITYPE      = unsigned long long (dimension / basis index)
+ *   UINT       = unsigned int
+ *   qm_complex = two doubles (re, im); layout-compatible with std::complex<double>
+ *                and returned in the same registers on x86-64 SysV.
+ * `state`  : device pointer to dim complex128 values (qubit q = index bit q).
+ * `stream` : pointer to a hipStream_t (as returned by allocate_cuda_stream_host),
+ *            or NULL for the null stream.
+ * `device_number` : HIP device the state lives on; selected if not current.
+ * Matrices are host pointers, row-major; for a k-qubit gate bit i of the
+ * row/column index belongs to target_qubit_index_list[i] (reference
+ * src/csim/utility.cpp:77-93).  They may be reused as soon as the call returns.
+ *
+ * Behaviour that differs from the reference, on purpose:
+ *   * errors do not exit the process (reference: src/gpusim/util.cuh:240-254);
+ *     an invalid argument or a HIP failure is recorded and the call returns
+ *     without touching the state.  Read it with qm_get_last_error().
+ *   * update calls synchronise their stream before returning, as the reference
+ *     does (src/gpusim/update_ops_single.cu:130), unless qm_set_synchronous(0)
+ *     makes them stream-ordered only.  Reductions always return a host value
+ *     and therefore always wait for their stream.
+ *
+ * The density-matrix entry points (dm_*_host) have no GPU counterpart in the
+ * reference, whose density matrix only runs on the CPU (src/csim/update_ops_dm.cpp);
+ * they mirror those csim functions' arguments with the device/stream tail added.
+ */
+#ifndef QULACS_MI355_H_
+#define QULACS_MI355_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef unsigned long long ITYPE;
+typedef unsigned int UINT;
+typedef struct qm_complex {
+    double re;
+    double im;
+} qm_complex;
+
+#define QM_EXPORT __attribute__((visibility("default")))
+
+/* ------------------------------------------------------------------ errors */
+#define QM_OK 0
+#define QM_ERR_INVALID_ARGUMENT 1
+#define QM_ERR_HIP 2
+#define QM_ERR_OUT_OF_MEMORY 3
+#define QM_ERR_UNSUPPORTED 4
+
+/* Last error recorded on the calling thread (0 if none); copies its message
+ * into buf (may be NULL).  No reference counterpart: the reference exits. */
+QM_EXPORT int qm_get_last_error(char* buf, size_t buf_len);
+QM_EXPORT void qm_clear_last_error(void);
+/* 1 (default): update calls wait for their stream, as the reference does.
+ * 0: update calls only enqueue work.  No reference counterpart. */
+QM_EXPORT void qm_set_synchronous(int on);
+QM_EXPORT int qm_get_synchronous(void);
+/* Waits for the given stream (NULL = null stream) on device_number. */
+QM_EXPORT void qm_synchronize(void* stream, UINT device_number);
+/* Library version string. */
+QM_EXPORT const char* qm_version(void);
+
+/* ------------------------------------------------------------------ devices / memory
+ * reference src/gpusim/util_func.h:9-25, src/gpusim/memory_ops.h:7-20 */
+QM_EXPORT int get_num_device(void);                                      /* util_func.h:9  */
+QM_EXPORT void set_device(unsigned int device_num);                      /* util_func.h:10 */
+QM_EXPORT int get_current_device(void);                                  /* util_func.h:11 */
+QM_EXPORT void* allocate_cuda_stream_host(unsigned int max_cuda_stream,
+    unsigned int device_number);                                         /* memory_ops.h:7  */
+QM_EXPORT void release_cuda_stream_host(void* cuda_stream,
+    unsigned int max_cuda_stream, unsigned int device_number);           /* memory_ops.h:9  */
+QM_EXPORT void* allocate_quantum_state_host(ITYPE dim,
+    unsigned int device_number);                                         /* memory_ops.h:11 */
+QM_EXPORT void initialize_quantum_state_host(void* state, ITYPE dim,
+    void* stream, unsigned int device_number);                           /* memory_ops.h:13 */
+QM_EXPORT void release_quantum_state_host(void* state,
+    unsigned int device_number);                                         /* memory_ops.h:15 */
+/* Haar-random state from a counter-based generator of this library; the
+ * bitstream differs from the reference's hipRAND XORWOW (memory_ops.cu:86-161). */
+QM_EXPORT void initialize_Haar_random_state_host(void* state, ITYPE dim,
+    void* stream, unsigned int device_number);                           /* memory_ops.h:17 */
+QM_EXPORT void initialize_Haar_random_state_with_seed_host(void* state,
+    ITYPE dim, UINT seed, void* stream, unsigned int device_number);     /* memory_ops.h:19 */
+QM_EXPORT void set_computational_basis_host(ITYPE comp_basis, void* state,
+    ITYPE dim, void* stream, unsigned int device_number);                /* util_func.h:12 */
+QM_EXPORT void copy_quantum_state_from_device_to_device(void* state_gpu_copy,
+    const void* state_gpu, ITYPE dim, void* stream,
+    unsigned int device_number);                                         /* util_func.h:14 */
+QM_EXPORT void copy_quantum_state_from_host_to_device(void* state_gpu_copy,
+    const void* state, ITYPE dim, void* stream,
+    unsigned int device_number);                                         /* util_func.h:16 */
+QM_EXPORT void copy_quantum_state_from_cppstate_host(void* state_gpu_copy,
+    const qm_complex* cppstate, ITYPE dim, void* stream,
+    UINT device_number);                                                 /* util_func.h:18 */
+QM_EXPORT void copy_quantum_state_from_device_to_host(void* state_cpu_copy,
+    const void* state_gpu_original, ITYPE dim, void* stream,
+    unsigned int device_number);                                         /* util_func.h:20 */
+QM_EXPORT void get_quantum_state_host(void* state_gpu, void* psi_cpu_copy,
+    ITYPE dim, void* stream, unsigned int device_number);                /* util_func.h:23 */
+QM_EXPORT void print_quantum_state_host(void* state, ITYPE dim,
+    unsigned int device_number);                                         /* util_func.h:25 */
+
+/* ------------------------------------------------------------------ named gates
+ * reference src/gpusim/update_ops_cuda.h:10-55 */
+QM_EXPORT void H_gate_host(unsigned int target_qubit_index, void* state, ITYPE dim, void* stream, unsigned int device_number);   /* :10 */
+QM_EXPORT void X_gate_host(unsigned int target_qubit_index, void* state, ITYPE dim, void* stream, unsigned int device_number);   /* :12 */
+QM_EXPORT void Y_gate_host(unsigned int target_qubit_index, void* state, ITYPE dim, void* stream, unsigned int device_number);   /* :14 */
+QM_EXPORT void Z_gate_host(unsigned int target_qubit_index, void* state, ITYPE dim, void* stream, unsigned int device_number);   /* :16 */
+QM_EXPORT void CZ_gate_host(unsigned int control_qubit_index, unsigned int target_qubit_index, void* state, ITYPE dim, void* stream, unsigned int device_number);     /* :18 */
+QM_EXPORT void CNOT_gate_host(unsigned int control_qubit_index, unsigned int target_qubit_index, void* state, ITYPE dim, void* stream, unsigned int device_number);   /* :21 */
+QM_EXPORT void SWAP_gate_host(unsigned int target_qubit_index0, unsigned int target_qubit_index1, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :24 */
+QM_EXPORT void RX_gate_host(UINT target_qubit_index, double angle, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :27 */
+QM_EXPORT void RY_gate_host(UINT target_qubit_index, double angle, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :29 */
+QM_EXPORT void RZ_gate_host(UINT target_qubit_index, double angle, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :31 */
+QM_EXPORT void S_gate_host(UINT target_qubit_index, void* state, ITYPE dim, void* stream, unsigned int device_number);          /* :33 */
+QM_EXPORT void Sdag_gate_host(UINT target_qubit_index, void* state, ITYPE dim, void* stream, unsigned int device_number);       /* :35 */
+QM_EXPORT void T_gate_host(UINT target_qubit_index, void* state, ITYPE dim, void* stream, unsigned int device_number);          /* :37 */
+QM_EXPORT void Tdag_gate_host(UINT target_qubit_index, void* state, ITYPE dim, void* stream, unsigned int device_number);       /* :39 */
+QM_EXPORT void sqrtX_gate_host(UINT target_qubit_index, void* state, ITYPE dim, void* stream, unsigned int device_number);      /* :41 */
+QM_EXPORT void sqrtXdag_gate_host(UINT target_qubit_index, void* state, ITYPE dim, void* stream, unsigned int device_number);   /* :43 */
+QM_EXPORT void sqrtY_gate_host(UINT target_qubit_index, void* state, ITYPE dim, void* stream, unsigned int device_number);      /* :45 */
+QM_EXPORT void sqrtYdag_gate_host(UINT target_qubit_index, void* state, ITYPE dim, void* stream, unsigned int device_number);   /* :47 */
+QM_EXPORT void P0_gate_host(UINT target_qubit_index, void* state, ITYPE dim, void* stream, unsigned int device_number);         /* :49 */
+QM_EXPORT void P1_gate_host(UINT target_qubit_index, void* state, ITYPE dim, void* stream, unsigned int device_number);         /* :51 */
+QM_EXPORT void normalize_host(double squared_norm, void* state, ITYPE dim, void* stream, unsigned int device_number);          /* :53 */
+
+/* ------------------------------------------------------------------ single-qubit matrices
+ * reference src/gpusim/update_ops_cuda.h:57-80 */
+QM_EXPORT void single_qubit_Pauli_gate_host(UINT target_qubit_index, UINT Pauli_operator_type, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :57 */
+QM_EXPORT void single_qubit_Pauli_rotation_gate_host(unsigned int target_qubit_index, unsigned int op_idx, double angle, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :60 */
+QM_EXPORT void single_qubit_dense_matrix_gate_host(unsigned int target_qubit_index, const qm_complex matrix[4], void* state, ITYPE dim, void* stream, unsigned int device_number); /* :63 */
+QM_EXPORT void single_qubit_diagonal_matrix_gate_host(unsigned int target_qubit_index, const qm_complex diagonal_matrix[2], void* state, ITYPE dim, void* stream, unsigned int device_number); /* :66 */
+QM_EXPORT void single_qubit_control_single_qubit_dense_matrix_gate_host(unsigned int control_qubit_index, unsigned int control_value, unsigned int target_qubit_index, const qm_complex matrix[4], void* state, ITYPE dim, void* stream, unsigned int device_number); /* :69 */
+QM_EXPORT void single_qubit_phase_gate_host(unsigned int target_qubit_index, qm_complex phase, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :73 */
+QM_EXPORT void multi_qubit_control_single_qubit_dense_matrix_gate_host(const UINT* control_qubit_index_list, const UINT* control_value_list, UINT control_qubit_index_count, UINT target_qubit_index, const qm_complex matrix[4], void* state, ITYPE dim, void* stream, unsigned int device_number); /* :76 */
+
+/* ------------------------------------------------------------------ multi-qubit matrices
+ * reference src/gpusim/update_ops_cuda.h:83-140 */
+QM_EXPORT void penta_qubit_dense_matrix_gate_host(const unsigned int target_qubit_index[5], const qm_complex matrix[1024], void* state, ITYPE dim, void* stream, UINT device_number); /* :83 */
+QM_EXPORT void quad_qubit_dense_matrix_gate_host(const unsigned int target_qubit_index[4], const qm_complex matrix[256], void* state, ITYPE dim, void* stream, unsigned int device_number); /* :86 */
+QM_EXPORT void triple_qubit_dense_matrix_gate_host(unsigned int target0_qubit_index, unsigned int target1_qubit_index, unsigned int target2_qubit_index, const qm_complex matrix[64], void* state, ITYPE dim, void* stream, unsigned int device_number); /* :89 */
+QM_EXPORT void double_qubit_dense_matrix_gate_host(unsigned int target0_qubit_index, unsigned int target1_qubit_index, const qm_complex matrix[16], void* state, ITYPE dim, void* stream, unsigned int device_number); /* :93 */
+QM_EXPORT void multi_qubit_Pauli_gate_Z_mask_host(ITYPE phase_flip_mask, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :97 */
+QM_EXPORT void multi_qubit_Pauli_gate_XZ_mask_host(ITYPE bit_flip_mask, ITYPE phase_flip_mask, UINT global_phase_90rot_count, UINT pivot_qubit_index, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :99 */
+QM_EXPORT void multi_qubit_Pauli_rotation_gate_XZ_mask_host(ITYPE bit_flip_mask, ITYPE phase_flip_mask, UINT global_phase_90rot_count, UINT pivot_qubit_index, double angle, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :103 */
+QM_EXPORT void multi_qubit_Pauli_rotation_gate_Z_mask_host(ITYPE phase_flip_mask, double angle, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :107 */
+QM_EXPORT void multi_qubit_Pauli_gate_partial_list_host(const UINT* target_qubit_index_list, const UINT* Pauli_operator_type_list, UINT target_qubit_index_count, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :110 */
+QM_EXPORT void multi_qubit_Pauli_gate_whole_list_host(const UINT* Pauli_operator_type_list, UINT qubit_count, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :114 */
+QM_EXPORT void multi_qubit_Pauli_rotation_gate_partial_list_host(const UINT* target_qubit_index_list, const UINT* Pauli_operator_type_list, UINT target_qubit_index_count, double angle, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :117 */
+QM_EXPORT void multi_qubit_Pauli_rotation_gate_whole_list_host(const UINT* Pauli_operator_type_list, UINT qubit_count, double angle, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :121 */
+QM_EXPORT void multi_qubit_dense_matrix_gate_host(const UINT* target_qubit_index_list, UINT target_qubit_index_count, const qm_complex* matrix, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :124 */
+QM_EXPORT void single_qubit_control_multi_qubit_dense_matrix_gate_host(UINT control_qubit_index, UINT control_value, const UINT* target_qubit_index_list, UINT target_qubit_index_count, const qm_complex* matrix, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :128 */
+QM_EXPORT void multi_qubit_control_multi_qubit_dense_matrix_gate_host(const UINT* control_qubit_index_list, const UINT* control_value_list, UINT control_qubit_index_count, const UINT* target_qubit_index_list, UINT target_qubit_index_count, const qm_complex* matrix, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :133 */
+/* Whole-state diagonal: state[i] *= diagonal_matrix[i] for all i < dim. */
+QM_EXPORT void multi_qubit_diagonal_matrix_gate_host(const qm_complex* diagonal_matrix, void* state, ITYPE dim, void* stream, UINT device_number); /* :138 */
+
+/* Diagonal gate on a target list, optionally controlled: the GPU form of
+ * csim multi_qubit_control_multi_qubit_diagonal_matrix_gate
+ * (src/csim/update_ops_matrix_diagonal_multi.cpp:60-112), which the reference
+ * GPU path lacks (src/cppsim/gate_matrix_diagonal.cpp:101-104). */
+QM_EXPORT void qm_multi_qubit_control_multi_qubit_diagonal_matrix_gate_host(const UINT* control_qubit_index_list, const UINT* control_value_list, UINT control_qubit_index_count, const UINT* target_qubit_index_list, UINT target_qubit_index_count, const qm_complex* diagonal_element, void* state, ITYPE dim, void* stream, unsigned int device_number);
+
+/* ------------------------------------------------------------------ reductions / state arithmetic
+ * reference src/gpusim/stat_ops.h:7-63 */
+QM_EXPORT double state_norm_squared_cublas_host(void* state, ITYPE dim); /* :7 (no BLAS here: same as state_norm_squared_host on the null stream) */
+QM_EXPORT double state_norm_squared_host(void* state, ITYPE dim, void* stream, unsigned int device_number); /* :8 */
+QM_EXPORT double measurement_distribution_entropy_host(void* state, ITYPE dim, void* stream, unsigned int device_number); /* :10 */
+QM_EXPORT void state_add_host(void* state_added, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :12 */
+QM_EXPORT void state_multiply_host(qm_complex coef, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :14 */
+QM_EXPORT qm_complex inner_product_host(const void* bra_state, const void* ket_state, ITYPE dim, void* stream, unsigned int device_number); /* :16 */
+QM_EXPORT double expectation_value_single_qubit_Pauli_operator_host(unsigned int operator_index, unsigned int target_qubit_index, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :18 */
+QM_EXPORT double M0_prob_host(UINT target_qubit_index, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :26 */
+QM_EXPORT double M1_prob_host(UINT target_qubit_index, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :28 */
+QM_EXPORT double marginal_prob_host(UINT* sorted_target_qubit_index_list, UINT* measured_value_list, UINT target_qubit_index_count, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :30 */
+QM_EXPORT double expectation_value_multi_qubit_Pauli_operator_XZ_mask_host(ITYPE bit_flip_mask, ITYPE phase_flip_mask, UINT global_phase_90rot_count, UINT pivot_qubit_index, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :33 */
+QM_EXPORT double expectation_value_multi_qubit_Pauli_operator_Z_mask_host(ITYPE phase_flip_mask, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :37 */
+QM_EXPORT double expectation_value_multi_qubit_Pauli_operator_partial_list_host(const UINT* target_qubit_index_list, const UINT* Pauli_operator_type_list, UINT target_qubit_index_count, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :40 */
+QM_EXPORT double expectation_value_multi_qubit_Pauli_operator_whole_list_host(const UINT* Pauli_operator_type_list, UINT qubit_count, void* state, ITYPE dim, void* stream, unsigned int device_number); /* :44 */
+QM_EXPORT qm_complex transition_amplitude_multi_qubit_Pauli_operator_XZ_mask_host(ITYPE bit_flip_mask, ITYPE phase_flip_mask, UINT global_phase_90rot_count, UINT pivot_qubit_index, void* state_bra, void* state_ket, ITYPE dim, void* stream, unsigned int device_number); /* :47 */
+QM_EXPORT qm_complex transition_amplitude_multi_qubit_Pauli_operator_Z_mask_host(ITYPE phase_flip_mask, void* state_bra, void* state_ket, ITYPE dim, void* stream, unsigned int device_number); /* :51 */
+QM_EXPORT qm_complex transition_amplitude_multi_qubit_Pauli_operator_partial_list_host(const UINT* target_qubit_index_list, const UINT* Pauli_operator_type_list, UINT target_qubit_index_count, void* state_bra, void* state_ket, ITYPE dim, void* stream, unsigned int device_number); /* :54 */
+QM_EXPORT qm_complex transition_amplitude_multi_qubit_Pauli_operator_whole_list_host(const UINT* Pauli_operator_type_list, UINT qubit_count, void* state_bra, void* state_ket, ITYPE dim, void* stream, unsigned int device_number); /* :59 */
+
+/* state += coef * state_added (csim state_add_with_coef,
+ * src/csim/update_ops_named_state.cpp:62-76; cppsim reaches it through
+ * QuantumStateBase::add_state_with_coef, src/cppsim/state.hpp). */
+QM_EXPORT void qm_state_add_with_coef_host(qm_complex coef, const void* state_added, void* state, ITYPE dim, void* stream, unsigned int device_number);
+
+/* Whole-observable expectation in one call: returns sum_k coef_k <P_k> with
+ * P_k given by its masks (bit_flip, phase_flip, Y count; the pivot is derived).
+ * Replaces the per-term loop of GeneralQuantumOperator::get_expectation_value
+ * (src/cppsim/general_quantum_operator.cpp:105-141), which on the GPU issues one
+ * allocation, launch and synchronisation per term. */
+QM_EXPORT qm_complex qm_observable_expectation_host(const ITYPE* bit_flip_masks, const ITYPE* phase_flip_masks, const UINT* global_phase_90rot_counts, const qm_complex* coefs, UINT term_count, void* state, ITYPE dim, void* stream, unsigned int device_number);
+
+/* Sampling with caller-supplied uniforms: out[c] = lower_bound(cdf, u[c]) - 1
+ * over the sequential un-normalised CDF cdf[0] = 0, cdf[i+1] = cdf[i] + |psi_i|^2,
+ * exactly as QuantumStateCpu::sampling (src/cppsim/state.hpp:909-930), with
+ * |psi|^2 = fma(re, re, im*im).  The CDF is built on the device with the same
+ * sequence of IEEE roundings, so the indices equal the sequential CPU loop's.
+ * uniforms and out are host arrays of `count` entries. */
+QM_EXPORT void qm_sampling_host(const double* uniforms, UINT count, ITYPE* out, void* state, ITYPE dim, void* stream, unsigned int device_number);
+
+/* ------------------------------------------------------------------ density matrix
+ * `state` holds dim*dim complex128 values, row-major rho[y*dim + x]; `dim` is
+ * 2^n as in csim's dm_* functions (src/csim/update_ops_dm.hpp, stat_ops_dm.hpp). */
+QM_EXPORT void dm_initialize_quantum_state_host(void* state, ITYPE dim, void* stream, unsigned int device_number); /* memory_ops_dm.cpp:22-37 */
+QM_EXPORT void dm_initialize_with_pure_state_host(void* state, const void* pure_state, ITYPE dim, void* stream, unsigned int device_number); /* memory_ops_dm.cpp:39-51 */
+QM_EXPORT void dm_single_qubit_dense_matrix_gate_host(UINT target_qubit_index, const qm_complex matrix[4], void* state, ITYPE dim, void* stream, unsigned int device_number); /* update_ops_dm.cpp:33-102 */
+QM_EXPORT void dm_multi_qubit_dense_matrix_gate_host(const UINT* target_qubit_index_list, UINT target_qubit_index_count, const qm_complex* matrix, void* state, ITYPE dim, void* stream, unsigned int device_number); /* update_ops_dm.cpp:342-498 */
+QM_EXPORT void dm_multi_qubit_control_multi_qubit_dense_matrix_gate_host(const UINT* control_qubit_index_list, const UINT* control_value_list, UINT control_qubit_index_count, const UINT* target_qubit_index_list, UINT target_qubit_index_count, const qm_complex* matrix, void* state, ITYPE dim, void* stream, unsigned int device_number); /* update_ops_dm.cpp:500-688 */
+/* General CPTP map on k qubits given as its superoperator S = sum_i K_i (x) conj(K_i)
+ * (4^k x 4^k, row index = (row bits of the k targets) * 2^k + (column bits)),
+ * applied in one pass.  Covers QuantumGate_Probabilistic / CPTP on a density
+ * matrix (src/cppsim/gate_general.hpp:265-285) without its ~10 passes. */
+QM_EXPORT void dm_superoperator_host(const UINT* target_qubit_index_list, UINT target_qubit_index_count, const qm_complex* superop, void* state, ITYPE dim, void* stream, unsigned int device_number);
+/* Single-qubit depolarizing channel with probability p (DepolarizingNoise,
+ * src/cppsim/gate_factory.cpp:454-467) in one pass. */
+QM_EXPORT void dm_depolarizing_host(UINT target_qubit_index, double p, void* state, ITYPE dim, void* stream, unsigned int device_number);
+QM_EXPORT double dm_state_norm_squared_host(void* state, ITYPE dim, void* stream, unsigned int device_number); /* stat_ops_dm.cpp:12-26 */
+QM_EXPORT double dm_M0_prob_host(UINT target_qubit_index, void* state, ITYPE dim, void* stream, unsigned int device_number); /* stat_ops_dm.cpp:49-63 */
+QM_EXPORT double dm_marginal_prob_host(const UINT* sorted_target_qubit_index_list, const UINT* measured_value_list, UINT target_qubit_index_count, void* state, ITYPE dim, void* stream, unsigned int device_number); /* stat_ops_dm.cpp:89-114 */
+QM_EXPORT double dm_expectation_value_multi_qubit_Pauli_operator_partial_list_host(const UINT* target_qubit_index_list, const UINT* Pauli_operator_type_list, UINT target_qubit_index_count, void* state, ITYPE dim, void* stream, unsigned int device_number); /* stat_ops_dm.cpp:164-191 */
+QM_EXPORT void dm_normalize_host(double squared_norm, void* state, ITYPE dim, void* stream, unsigned int device_number); /* update_ops_dm.cpp:14-31 */
+/* DM sampling over |rho_ii| (src/cppsim/state_dm.hpp:383-408), same contract as qm_sampling_host. */
+QM_EXPORT void dm_sampling_host(const double* uniforms, UINT count, ITYPE* out, void* state, ITYPE dim, void* stream, unsigned int device_number);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QULACS_MI355_H_ */

@@ -1,0 +1,276 @@
+// Density-matrix kernels.
+//
+// rho is row-major rho[y * 2^n + x] (src/csim/update_ops_dm.cpp:73-76), so the
+// flat index has 2n bits: bit q is column qubit q and bit q + n is row qubit q.
+// With that view:
+//   * U rho U^dagger is U applied to the row bits followed by conj(U) applied
+//     to the column bits -- two passes of the state-vector gather kernels over
+//     the 2n-bit flat array, each as fast as a state-vector gate of 2n qubits;
+//     a single-qubit unitary is fused into one pass with E = U (x) conj(U) on
+//     (column t, row t + n) (the reference's ext_matrix, update_ops_dm.cpp:41-50);
+//   * a k-qubit CPTP map sum_i K_i rho K_i^dagger is one pass of a 2k-bit dense
+//     gate whose matrix is the superoperator sum_i K_i (x) conj(K_i), replacing
+//     the copy / gate / scale / add loop of QuantumGate_Probabilistic on a
+//     density matrix (src/cppsim/gate_general.hpp:265-285, about ten passes).
+#include <cmath>
+#include <complex>
+
+#include "qm_internal.hpp"
+
+namespace qm {
+
+using cd = std::complex<double>;
+
+static bool dm_ok(void* state, u64 dim, int* n, const char* what) {
+    *n = log2_dim(dim);
+    if (!state || *n < 0 || *n > 31) {
+        set_error(QM_ERR_INVALID_ARGUMENT, "%s: null state or dim %llu is not a power of two in [2, 2^31]", what, dim);
+        return false;
+    }
+    return true;
+}
+
+// sum over the diagonal rho_ii (real parts) with (i & mask) == value;
+// ENT selects -p log p over entries with p > 1e-15 (stat_ops_dm.cpp:28-47).
+template <bool ENT>
+__global__ __launch_bounds__(256) void k_dm_diag(const double2* __restrict__ r, u64 dim, u64 mask, u64 value,
+                                                  double* __restrict__ partials) {
+    __shared__ double lds[32];
+    double acc = 0.0;
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
+        if ((i & mask) != value) continue;
+        const double p = r[i * dim + i].x;
+        if (ENT) {
+            if (p > 1e-15) acc += -p * log(p);
+        } else {
+            acc += p;
+        }
+    }
+    acc = block_sum(acc, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+}
+
+// Re sum_i i^rot (-1)^popcount(i & pf) rho[i, i ^ bf]  (stat_ops_dm.cpp:164-191)
+__global__ __launch_bounds__(256) void k_dm_pauli(const double2* __restrict__ r, u64 dim, u64 bf, u64 pf,
+                                                   unsigned rot, double* __restrict__ partials) {
+    __shared__ double lds[32];
+    double acc = 0.0;
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
+        const double2 v = r[i * dim + (i ^ bf)];
+        double re;
+        switch (rot & 3) {
+            case 0: re = v.x; break;
+            case 1: re = -v.y; break;
+            case 2: re = -v.x; break;
+            default: re = v.y; break;
+        }
+        acc += (__popcll(i & pf) & 1) ? -re : re;
+    }
+    acc = block_sum(acc, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__ p, unsigned n, double* __restrict__ out) {
+    __shared__ double lds[32];
+    double acc = 0.0;
+    for (unsigned k = threadIdx.x; k < n; k += blockDim.x) acc += p[k];
+    acc = block_sum(acc, lds);
+    if (threadIdx.x == 0) out[0] = acc;
+}
+
+static double finish_scalar(Scratch* sc, unsigned nb, hipStream_t st, const char* what) {
+    double* dout = sc->partials + nb;
+    k_sum_partials<<<1, 256, 0, st>>>(sc->partials, nb, dout);
+    if (!hip_ok(hipMemcpyAsync(sc->host_result, dout, sizeof(double), hipMemcpyDeviceToHost, st), what)) return 0.0;
+    if (!hip_ok(hipStreamSynchronize(st), what)) return 0.0;
+    return sc->host_result[0];
+}
+
+static double dm_diag_sum(bool ent, u64 mask, u64 value, void* state, u64 dim, void* stream, unsigned dev,
+                          const char* what) {
+    int n;
+    if (!dm_ok(state, dim, &n, what) || !use_device(dev)) return 0.0;
+    hipStream_t st = stream_of(stream);
+    Scratch* sc = scratch_for(st);
+    if (!sc) return 0.0;
+    const unsigned nb = grid_for(dim, 256 * 4, 1024);
+    if (ent)
+        k_dm_diag<true><<<nb, 256, 0, st>>>((const double2*)state, dim, mask, value, sc->partials);
+    else
+        k_dm_diag<false><<<nb, 256, 0, st>>>((const double2*)state, dim, mask, value, sc->partials);
+    return finish_scalar(sc, nb, st, what);
+}
+
+__global__ __launch_bounds__(256) void k_outer(double2* __restrict__ r, const double2* __restrict__ psi, u64 dim) {
+    const u64 total = dim * dim, stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const double2 a = psi[i / dim], b = psi[i % dim];
+        // a * conj(b)
+        r[i] = make_double2(fma(a.x, b.x, a.y * b.y), fma(a.y, b.x, -a.x * b.y));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dm_scale(double2* __restrict__ r, u64 total, double f) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) r[i] = cscale(r[i], f);
+}
+
+// Row/column split of a (controlled) k-qubit unitary on an n-qubit density matrix.
+static void dm_unitary(const u32* cl, const u32* cvl, u32 nc, const u32* tl, u32 nt, const qm_complex* m,
+                       void* state, u64 dim, void* stream, unsigned dev, const char* what) {
+    int n;
+    if (!dm_ok(state, dim, &n, what)) return;
+    QM_CHECK_ARG(tl && m && nt >= 1 && nt + nc <= (u32)n, "%s: bad target/control counts", what);
+    u32 all[64];
+    for (u32 i = 0; i < nt; ++i) all[i] = tl[i];
+    for (u32 i = 0; i < nc; ++i) all[nt + i] = cl[i];
+    if (!check_qubits(all, nt + nc, n, what)) return;
+    for (u32 i = 0; i < nc; ++i) QM_CHECK_ARG(cvl[i] <= 1, "%s: control value %u", what, cvl[i]);
+    const u64 flat = dim * dim;
+    if (nt == 1 && nc == 0) {
+        // one pass: E[(oy,ox),(iy,ix)] = U[oy,iy] conj(U[ox,ix]); member bit0 = column t, bit1 = row t+n
+        qm_complex e[16];
+        for (int y = 0; y < 4; ++y)
+            for (int x = 0; x < 4; ++x) {
+                const cd a(m[(y >> 1) * 2 + (x >> 1)].re, m[(y >> 1) * 2 + (x >> 1)].im);
+                const cd b(m[(y & 1) * 2 + (x & 1)].re, m[(y & 1) * 2 + (x & 1)].im);
+                const cd v = a * std::conj(b);
+                e[y * 4 + x] = qm_complex{v.real(), v.imag()};
+            }
+        const u32 ft[2] = {tl[0], tl[0] + (u32)n};
+        dense_any(ft, 2, nullptr, nullptr, 0, e, state, flat, stream, dev, what);
+        return;
+    }
+    u32 rt[32], rc[32], ct[32];
+    for (u32 i = 0; i < nt; ++i) { rt[i] = tl[i] + n; ct[i] = tl[i]; }
+    for (u32 i = 0; i < nc; ++i) rc[i] = cl[i] + n;
+    const size_t N = (size_t)1 << nt;
+    std::vector<qm_complex> mc(N * N);
+    for (size_t i = 0; i < N * N; ++i) mc[i] = qm_complex{m[i].re, -m[i].im};
+    dense_any(rt, nt, nc ? rc : nullptr, cvl, nc, m, state, flat, stream, dev, what);
+    dense_any(ct, nt, nc ? cl : nullptr, cvl, nc, mc.data(), state, flat, stream, dev, what);
+}
+
+}  // namespace qm
+
+using namespace qm;
+
+extern "C" {
+
+void dm_initialize_quantum_state_host(void* state, ITYPE dim, void* stream, unsigned dev) {
+    int n;
+    if (!dm_ok(state, dim, &n, "dm_initialize_quantum_state_host")) return;
+    set_computational_basis_host(0, state, dim * dim, stream, dev);
+}
+
+void dm_initialize_with_pure_state_host(void* state, const void* pure, ITYPE dim, void* stream, unsigned dev) {
+    int n;
+    if (!dm_ok(state, dim, &n, "dm_initialize_with_pure_state_host")) return;
+    QM_CHECK_ARG(pure, "dm_initialize_with_pure_state_host: null pure state");
+    if (!use_device(dev)) return;
+    hipStream_t st = stream_of(stream);
+    k_outer<<<grid_for(dim * dim, 256 * 4), 256, 0, st>>>((double2*)state, (const double2*)pure, dim);
+    finish_update(st, "dm_initialize_with_pure_state_host");
+}
+
+void dm_single_qubit_dense_matrix_gate_host(UINT t, const qm_complex m[4], void* state, ITYPE dim, void* stream,
+                                            unsigned dev) {
+    dm_unitary(nullptr, nullptr, 0, &t, 1, m, state, dim, stream, dev, "dm_single_qubit_dense_matrix_gate_host");
+}
+void dm_multi_qubit_dense_matrix_gate_host(const UINT* tl, UINT nt, const qm_complex* m, void* state, ITYPE dim,
+                                           void* stream, unsigned dev) {
+    dm_unitary(nullptr, nullptr, 0, tl, nt, m, state, dim, stream, dev, "dm_multi_qubit_dense_matrix_gate_host");
+}
+void dm_multi_qubit_control_multi_qubit_dense_matrix_gate_host(const UINT* cl, const UINT* cvl, UINT nc,
+                                                               const UINT* tl, UINT nt, const qm_complex* m,
+                                                               void* state, ITYPE dim, void* stream, unsigned dev) {
+    dm_unitary(cl, cvl, nc, tl, nt, m, state, dim, stream, dev,
+               "dm_multi_qubit_control_multi_qubit_dense_matrix_gate_host");
+}
+
+void dm_superoperator_host(const UINT* tl, UINT nt, const qm_complex* superop, void* state, ITYPE dim, void* stream,
+                           unsigned dev) {
+    const char* what = "dm_superoperator_host";
+    int n;
+    if (!dm_ok(state, dim, &n, what)) return;
+    QM_CHECK_ARG(tl && superop && nt >= 1 && nt <= 6 && nt <= (u32)n, "%s: bad target count %u", what, nt);
+    if (!check_qubits(tl, nt, n, what)) return;
+    // member bits 0..k-1: column bits t_i; bits k..2k-1: row bits t_i + n
+    u32 ft[12];
+    for (u32 i = 0; i < nt; ++i) {
+        ft[i] = tl[i];
+        ft[nt + i] = tl[i] + n;
+    }
+    dense_any(ft, 2 * nt, nullptr, nullptr, 0, superop, state, dim * dim, stream, dev, what);
+}
+
+void dm_depolarizing_host(UINT t, double p, void* state, ITYPE dim, void* stream, unsigned dev) {
+    // S = (1-p) I(x)I + p/3 (X(x)X + Y(x)conj(Y) + Z(x)Z) on (row, column) bits of t:
+    // rho00' = (1-2p/3) rho00 + (2p/3) rho11, rho01' = (1-4p/3) rho01, and symmetric.
+    const double a = 1.0 - 2.0 * p / 3.0, b = 2.0 * p / 3.0, c = 1.0 - 4.0 * p / 3.0;
+    // member index = row_bit * 2 + column_bit
+    const qm_complex S[16] = {
+        {a, 0}, {0, 0}, {0, 0}, {b, 0},
+        {0, 0}, {c, 0}, {0, 0}, {0, 0},
+        {0, 0}, {0, 0}, {c, 0}, {0, 0},
+        {b, 0}, {0, 0}, {0, 0}, {a, 0},
+    };
+    dm_superoperator_host(&t, 1, S, state, dim, stream, dev);
+}
+
+double dm_state_norm_squared_host(void* state, ITYPE dim, void* stream, unsigned dev) {
+    return dm_diag_sum(false, 0, 0, state, dim, stream, dev, "dm_state_norm_squared_host");
+}
+double dm_M0_prob_host(UINT t, void* state, ITYPE dim, void* stream, unsigned dev) {
+    QM_CHECK_ARG_RET(log2_dim(dim) > (int)t, 0.0, "dm_M0_prob_host: qubit %u out of range", t);
+    return dm_diag_sum(false, 1ULL << t, 0, state, dim, stream, dev, "dm_M0_prob_host");
+}
+double dm_marginal_prob_host(const UINT* tl, const UINT* vl, UINT nt, void* state, ITYPE dim, void* stream,
+                             unsigned dev) {
+    const int nq = log2_dim(dim);
+    u64 mask = 0, value = 0;
+    for (UINT i = 0; i < nt; ++i) {
+        QM_CHECK_ARG_RET((int)tl[i] < nq && vl[i] <= 1 && !((mask >> tl[i]) & 1), 0.0,
+                         "dm_marginal_prob_host: bad qubit %u / value %u", tl[i], vl[i]);
+        mask |= 1ULL << tl[i];
+        value |= (u64)vl[i] << tl[i];
+    }
+    return dm_diag_sum(false, mask, value, state, dim, stream, dev, "dm_marginal_prob_host");
+}
+double dm_expectation_value_multi_qubit_Pauli_operator_partial_list_host(const UINT* tl, const UINT* pl, UINT nt,
+                                                                         void* state, ITYPE dim, void* stream,
+                                                                         unsigned dev) {
+    const char* what = "dm_expectation_value_multi_qubit_Pauli_operator_partial_list_host";
+    int n;
+    if (!dm_ok(state, dim, &n, what)) return 0.0;
+    u64 bf = 0, pf = 0;
+    unsigned rot = 0;
+    for (UINT c = 0; c < nt; ++c) {
+        QM_CHECK_ARG_RET((int)tl[c] < n && pl[c] <= 3, 0.0, "%s: bad qubit %u or Pauli id %u", what, tl[c], pl[c]);
+        switch (pl[c]) {
+            case 1: bf ^= 1ULL << tl[c]; break;
+            case 2: bf ^= 1ULL << tl[c]; pf ^= 1ULL << tl[c]; rot++; break;
+            case 3: pf ^= 1ULL << tl[c]; break;
+            default: break;
+        }
+    }
+    if (!use_device(dev)) return 0.0;
+    hipStream_t st = stream_of(stream);
+    Scratch* sc = scratch_for(st);
+    if (!sc) return 0.0;
+    const unsigned nb = grid_for(dim, 256 * 4, 1024);
+    k_dm_pauli<<<nb, 256, 0, st>>>((const double2*)state, dim, bf, pf, rot, sc->partials);
+    return finish_scalar(sc, nb, st, what);
+}
+
+void dm_normalize_host(double squared_norm, void* state, ITYPE dim, void* stream, unsigned dev) {
+    int n;
+    if (!dm_ok(state, dim, &n, "dm_normalize_host") || !use_device(dev)) return;
+    hipStream_t st = stream_of(stream);
+    k_dm_scale<<<grid_for(dim * dim, 256 * 4), 256, 0, st>>>((double2*)state, dim * dim, 1.0 / squared_norm);
+    finish_update(st, "dm_normalize_host");
+}
+
+}  // extern "C"

@@ -1,0 +1,749 @@
+// State-vector gate kernels for gfx950.
+//
+// One kernel shape covers every gate that acts on a fixed set of k <= 5 target
+// qubits (dense, controlled, diagonal, named permutations): each thread owns a
+// "base" index b whose target and control bits are zero (control bits then set
+// to their required values), gathers the 2^k amplitudes b + spread(x) into
+// registers, applies the gate's operation and scatters back only what changed.
+// Consecutive lanes take consecutive bases, i.e. they vary the lowest bits that
+// are neither target nor control: when no target is below bit 6 every wave
+// load is a contiguous 1 KiB run; with a or two low targets a load touches 2^a
+// KiB with 1/2^a use and its partner loads (issued back to back by the same
+// wave) use the rest of the same cache lines, so HBM sees each byte once.
+//
+// Reference loops restated: src/csim/update_ops_matrix_dense_single.cpp:44-68,
+// update_ops_matrix_dense_double.cpp:64-113, update_ops_matrix_dense_multi.cpp:59-128,
+// update_ops_control_*.cpp, update_ops_named_{X,Y,Z,H,CNOT,CZ,SWAP}.cpp,
+// update_ops_named_projection.cpp:38-72, update_ops_matrix_diagonal_*.cpp,
+// update_ops_matrix_phase_single.cpp:14-60.
+#include <cmath>
+#include <complex>
+
+#include "qm_internal.hpp"
+
+namespace qm {
+
+// ------------------------------------------------------------------ launch plumbing
+template <int K>
+struct GatherArgs {
+    unsigned char ins[24];  // ascending insert positions (targets and controls)
+    int n_ins;
+    u64 cmask;              // control bits that must be 1
+    u64 offs[1 << K];       // offset of tuple member x: sum of 2^targets[i] over set bits i of x
+};
+
+template <int K>
+bool fill_args(GatherArgs<K>& a, const u32* targets, const u32* controls, const u32* cvals, u32 nc) {
+    const GatherPlan p = make_plan(targets, K, controls, cvals, nc);
+    if (p.n_ins > 24) {
+        set_error(QM_ERR_UNSUPPORTED, "more than 24 target+control qubits");
+        return false;
+    }
+    for (int i = 0; i < p.n_ins; ++i) a.ins[i] = p.ins[i];
+    a.n_ins = p.n_ins;
+    a.cmask = p.control_mask;
+    for (int x = 0; x < (1 << K); ++x) {
+        u64 o = 0;
+        for (int i = 0; i < K; ++i)
+            if ((x >> i) & 1) o |= 1ULL << targets[i];
+        a.offs[x] = o;
+    }
+    return true;
+}
+
+template <int K>
+__device__ __forceinline__ u64 base_of(u64 i, const GatherArgs<K>& a) {
+    for (int k = 0; k < a.n_ins; ++k) i = insert_zero(i, a.ins[k]);
+    return i | a.cmask;
+}
+
+// Op interface: READ / WRITE are bit masks over tuple members; apply() maps
+// the registers in place.
+template <int K, int U, class Op>
+__global__ __launch_bounds__(256) void k_gather(double2* __restrict__ s, u64 n_bases, GatherArgs<K> a, Op op) {
+    constexpr int N = 1 << K;
+    const u64 stride = (u64)gridDim.x * blockDim.x * U;
+    for (u64 i0 = ((u64)blockIdx.x * blockDim.x) * U + threadIdx.x; i0 < n_bases; i0 += stride) {
+        double2 v[U][N];
+        u64 b[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const u64 i = i0 + (u64)u * blockDim.x;
+            b[u] = i < n_bases ? base_of<K>(i, a) : ~0ULL;
+#pragma unroll
+            for (int x = 0; x < N; ++x)
+                if ((Op::READ >> x) & 1)
+                    if (b[u] != ~0ULL) v[u][x] = s[b[u] + a.offs[x]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (b[u] == ~0ULL) continue;
+            op.apply(v[u]);
+#pragma unroll
+            for (int x = 0; x < N; ++x)
+                if ((Op::WRITE >> x) & 1) s[b[u] + a.offs[x]] = v[u][x];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ operations
+template <int K>
+struct MatArg {
+    double2 m[1 << (2 * K)];
+};
+
+template <int K>
+struct OpDenseArg {  // matrix passed in the kernel arguments (K <= 3)
+    static constexpr unsigned READ = (1u << (1 << K)) - 1;
+    static constexpr unsigned WRITE = READ;
+    MatArg<K> mat;
+    __device__ __forceinline__ void apply(double2* v) const {
+        constexpr int N = 1 << K;
+        double2 out[N];
+#pragma unroll
+        for (int y = 0; y < N; ++y) {
+            double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int x = 0; x < N; ++x) acc = cmac(acc, mat.m[y * N + x], v[x]);
+            out[y] = acc;
+        }
+#pragma unroll
+        for (int y = 0; y < N; ++y) v[y] = out[y];
+    }
+};
+
+// Matrix sources for the streaming dense kernel.
+template <int K>
+struct MatInArg {  // K <= 3: the whole matrix rides in the kernel arguments
+    MatArg<K> mat;
+    __device__ __forceinline__ double2 at(int y, int x) const { return mat.m[y * (1 << K) + x]; }
+};
+template <int K>
+struct MatInPtr {  // K = 4, 5: device copy, uniform across the wave (scalar loads)
+    const double2* __restrict__ p;
+    __device__ __forceinline__ double2 at(int y, int x) const { return p[y * (1 << K) + x]; }
+};
+
+// Dense k-qubit gate (k = 3..5): the tuple sits in registers and each output
+// row is written as soon as it is formed, so live registers stay near 4 * 2^k.
+template <int K, class M>
+__global__ __launch_bounds__(256) void k_dense_stream(double2* __restrict__ s, u64 n_bases, GatherArgs<K> a, M mat) {
+    constexpr int N = 1 << K;
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n_bases; i += stride) {
+        const u64 b = base_of<K>(i, a);
+        double2 v[N];
+#pragma unroll
+        for (int x = 0; x < N; ++x) v[x] = s[b + a.offs[x]];
+#pragma unroll
+        for (int y = 0; y < N; ++y) {
+            double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int x = 0; x < N; ++x) acc = cmac(acc, mat.at(y, x), v[x]);
+            s[b + a.offs[y]] = acc;
+        }
+    }
+}
+
+struct OpSwap01 {  // X, CNOT
+    static constexpr unsigned READ = 3, WRITE = 3;
+    __device__ __forceinline__ void apply(double2* v) const {
+        const double2 t = v[0];
+        v[0] = v[1];
+        v[1] = t;
+    }
+};
+struct OpSwap12 {  // SWAP (members 01 <-> 10)
+    static constexpr unsigned READ = 6, WRITE = 6;
+    __device__ __forceinline__ void apply(double2* v) const {
+        const double2 t = v[1];
+        v[1] = v[2];
+        v[2] = t;
+    }
+};
+struct OpY {  // (a0, a1) -> (-i a1, i a0)
+    static constexpr unsigned READ = 3, WRITE = 3;
+    __device__ __forceinline__ void apply(double2* v) const {
+        const double2 a0 = v[0], a1 = v[1];
+        v[0] = make_double2(a1.y, -a1.x);
+        v[1] = make_double2(-a0.y, a0.x);
+    }
+};
+struct OpH {
+    static constexpr unsigned READ = 3, WRITE = 3;
+    double r;
+    __device__ __forceinline__ void apply(double2* v) const {
+        const double2 a0 = v[0], a1 = v[1];
+        v[0] = make_double2((a0.x + a1.x) * r, (a0.y + a1.y) * r);
+        v[1] = make_double2((a0.x - a1.x) * r, (a0.y - a1.y) * r);
+    }
+};
+struct OpNeg1 {  // Z, CZ: negate member 1
+    static constexpr unsigned READ = 2, WRITE = 2;
+    __device__ __forceinline__ void apply(double2* v) const { v[1] = make_double2(-v[1].x, -v[1].y); }
+};
+struct OpPhase1 {  // S, T, phase gate: member 1 *= phase
+    static constexpr unsigned READ = 2, WRITE = 2;
+    double2 ph;
+    __device__ __forceinline__ void apply(double2* v) const { v[1] = cmul(v[1], ph); }
+};
+struct OpZero0 {  // P1 projection zeroes member 0
+    static constexpr unsigned READ = 0, WRITE = 1;
+    __device__ __forceinline__ void apply(double2* v) const { v[0] = make_double2(0.0, 0.0); }
+};
+struct OpZero1 {  // P0 projection zeroes member 1
+    static constexpr unsigned READ = 0, WRITE = 2;
+    __device__ __forceinline__ void apply(double2* v) const { v[1] = make_double2(0.0, 0.0); }
+};
+template <int K>
+struct OpDiag {  // member x *= d[x]
+    static constexpr unsigned READ = (K >= 5) ? 0xFFFFFFFFu : ((1u << (1 << K)) - 1);
+    static constexpr unsigned WRITE = READ;
+    double2 d[1 << K];
+    __device__ __forceinline__ void apply(double2* v) const {
+#pragma unroll
+        for (int x = 0; x < (1 << K); ++x) v[x] = cmul(v[x], d[x]);
+    }
+};
+
+// ------------------------------------------------------------------ generic driver
+template <int K, int U, class Op>
+void launch_gather(double2* s, u64 dim, const u32* targets, const u32* controls, const u32* cvals, u32 nc,
+                   const Op& op, hipStream_t st) {
+    GatherArgs<K> a;
+    if (!fill_args<K>(a, targets, controls, cvals, nc)) return;
+    const u64 n_bases = dim >> (K + nc);
+    const unsigned block = 256;
+    k_gather<K, U, Op><<<grid_for(n_bases, (u64)block * U), block, 0, st>>>(s, n_bases, a, op);
+}
+
+template <int K, class M>
+void launch_dense_stream(double2* s, u64 dim, const u32* targets, const u32* controls, const u32* cvals, u32 nc,
+                         const M& m, hipStream_t st) {
+    GatherArgs<K> a;
+    if (!fill_args<K>(a, targets, controls, cvals, nc)) return;
+    const u64 n_bases = dim >> (K + nc);
+    k_dense_stream<K, M><<<grid_for(n_bases, 256), 256, 0, st>>>(s, n_bases, a, m);
+}
+
+// Validates state/dim/targets/controls; returns n (qubit count) or -1.
+int check_gate(void* state, u64 dim, const u32* targets, u32 nt, const u32* controls, u32 nc, const char* what) {
+    const int n = log2_dim(dim);
+    if (!state || n < 0) {
+        set_error(QM_ERR_INVALID_ARGUMENT, "%s: null state or dim %llu is not a power of two >= 2", what, dim);
+        return -1;
+    }
+    u32 all[64];
+    if (nt + nc > 64 || (int)(nt + nc) > n) {
+        set_error(QM_ERR_INVALID_ARGUMENT, "%s: %u targets + %u controls exceed %d qubits", what, nt, nc, n);
+        return -1;
+    }
+    for (u32 i = 0; i < nt; ++i) all[i] = targets[i];
+    for (u32 i = 0; i < nc; ++i) all[nt + i] = controls[i];
+    if (!check_qubits(all, nt + nc, n, what)) return -1;
+    return n;
+}
+
+template <int K>
+MatArg<K> mat_arg(const qm_complex* m) {
+    MatArg<K> a;
+    for (int i = 0; i < (1 << (2 * K)); ++i) a.m[i] = make_double2(m[i].re, m[i].im);
+    return a;
+}
+
+// Dense k-qubit matrix with c controls, k >= 6: one block per tuple, tuple in
+// LDS, matrix read transposed from device memory so a wave's loads of one
+// column are contiguous.
+__global__ __launch_bounds__(256) void k_dense_wide(double2* __restrict__ s, u64 n_bases, int K,
+                                                     const unsigned char* __restrict__ ins_dev, int n_ins,
+                                                     u64 cmask, const u64* __restrict__ offs,
+                                                     const double2* __restrict__ mt) {
+    extern __shared__ double2 tup[];
+    const int N = 1 << K;
+    for (u64 t = blockIdx.x; t < n_bases; t += gridDim.x) {
+        u64 b = t;
+        for (int k = 0; k < n_ins; ++k) b = insert_zero(b, ins_dev[k]);
+        b |= cmask;
+        for (int x = threadIdx.x; x < N; x += blockDim.x) tup[x] = s[b + offs[x]];
+        __syncthreads();
+        for (int y = threadIdx.x; y < N; y += blockDim.x) {
+            double2 acc = make_double2(0.0, 0.0);
+            for (int x = 0; x < N; ++x) acc = cmac(acc, mt[(u64)x * N + y], tup[x]);
+            s[b + offs[y]] = acc;
+        }
+        __syncthreads();
+    }
+}
+
+void dense_any(const u32* targets, u32 nt, const u32* controls, const u32* cvals, u32 nc, const qm_complex* matrix,
+               void* state, u64 dim, void* stream, unsigned device, const char* what) {
+    if (check_gate(state, dim, targets, nt, controls, nc, what) < 0) return;
+    QM_CHECK_ARG(nt >= 1 && matrix, "%s: no targets or null matrix", what);
+    if (!use_device(device)) return;
+    hipStream_t st = stream_of(stream);
+    double2* s = (double2*)state;
+    switch (nt) {
+        case 1: launch_gather<1, 2>(s, dim, targets, controls, cvals, nc, OpDenseArg<1>{mat_arg<1>(matrix)}, st); break;
+        case 2: launch_gather<2, 2>(s, dim, targets, controls, cvals, nc, OpDenseArg<2>{mat_arg<2>(matrix)}, st); break;
+        case 3: launch_dense_stream<3>(s, dim, targets, controls, cvals, nc, MatInArg<3>{mat_arg<3>(matrix)}, st); break;
+        case 4:
+        case 5: {
+            Scratch* sc = scratch_for(st);
+            if (!sc) return;
+            const size_t nel = (size_t)1 << (2 * nt);
+            const double2* dm = (const double2*)upload(sc, matrix, nel * sizeof(double2));
+            if (!dm) return;
+            if (nt == 4)
+                launch_dense_stream<4>(s, dim, targets, controls, cvals, nc, MatInPtr<4>{dm}, st);
+            else
+                launch_dense_stream<5>(s, dim, targets, controls, cvals, nc, MatInPtr<5>{dm}, st);
+            break;
+        }
+        default: {
+            if (nt > 12) {
+                set_error(QM_ERR_UNSUPPORTED, "%s: %u targets (at most 12 supported)", what, nt);
+                return;
+            }
+            Scratch* sc = scratch_for(st);
+            if (!sc) return;
+            const u64 N = 1ULL << nt;
+            std::vector<double2> mt(N * N);
+            for (u64 y = 0; y < N; ++y)
+                for (u64 x = 0; x < N; ++x) mt[x * N + y] = make_double2(matrix[y * N + x].re, matrix[y * N + x].im);
+            const GatherPlan p = make_plan(targets, nt, controls, cvals, nc);
+            std::vector<u64> offs(N);
+            for (u64 x = 0; x < N; ++x) {
+                u64 o = 0;
+                for (u32 i = 0; i < nt; ++i)
+                    if ((x >> i) & 1) o |= 1ULL << targets[i];
+                offs[x] = o;
+            }
+            // Matrix beyond the ring goes to the grow-on-demand buffer.
+            const size_t mbytes = N * N * sizeof(double2);
+            double2* dmt;
+            if (mbytes + 8192 + N * 8 <= kRingBytes / 2) {
+                dmt = (double2*)upload(sc, mt.data(), mbytes);
+            } else {
+                dmt = (double2*)big_buffer(sc, mbytes);
+                if (dmt && !hip_ok(hipMemcpyAsync(dmt, mt.data(), mbytes, hipMemcpyHostToDevice, st), "matrix upload"))
+                    dmt = nullptr;
+            }
+            const u64* doffs = (const u64*)upload(sc, offs.data(), N * sizeof(u64));
+            const unsigned char* dins = (const unsigned char*)upload(sc, p.ins, 64);
+            if (!dmt || !doffs || !dins) return;
+            const u64 n_bases = dim >> (nt + nc);
+            k_dense_wide<<<grid_for(n_bases, 1, 1u << 16), 256, N * sizeof(double2), st>>>(
+                s, n_bases, (int)nt, dins, p.n_ins, p.control_mask, doffs, dmt);
+            if (mbytes + 8192 + N * 8 > kRingBytes / 2) hip_ok(hipStreamSynchronize(st), "wide gate");
+            break;
+        }
+    }
+    finish_update(st, what);
+}
+
+// Named single-target gate through the gather kernel.
+template <class Op>
+void named1(u32 t, const Op& op, void* state, u64 dim, void* stream, unsigned device, const char* what) {
+    if (check_gate(state, dim, &t, 1, nullptr, 0, what) < 0) return;
+    if (!use_device(device)) return;
+    hipStream_t st = stream_of(stream);
+    launch_gather<1, 2>((double2*)state, dim, &t, nullptr, nullptr, 0, op, st);
+    finish_update(st, what);
+}
+
+// Single target with one control bit set (CNOT, CZ).
+template <class Op>
+void controlled1(u32 c, u32 t, const Op& op, void* state, u64 dim, void* stream, unsigned device, const char* what) {
+    const u32 one = 1;
+    if (check_gate(state, dim, &t, 1, &c, 1, what) < 0) return;
+    if (!use_device(device)) return;
+    hipStream_t st = stream_of(stream);
+    launch_gather<1, 2>((double2*)state, dim, &t, &c, &one, 1, op, st);
+    finish_update(st, what);
+}
+
+// Elementwise diagonal over the controlled subspace for any number of targets:
+// s[b] *= d[index of b's target bits in target-list order].
+__global__ __launch_bounds__(256) void k_diag_any(double2* __restrict__ s, u64 n, const unsigned char* __restrict__ cins,
+                                                   int n_cins, u64 cmask, const unsigned char* __restrict__ tg, int nt,
+                                                   const double2* __restrict__ d) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        u64 b = i;
+        for (int k = 0; k < n_cins; ++k) b = insert_zero(b, cins[k]);
+        b |= cmask;
+        unsigned y = 0;
+        for (int k = 0; k < nt; ++k) y |= (unsigned)((b >> tg[k]) & 1) << k;
+        s[b] = cmul(s[b], d[y]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_diag_whole(double2* __restrict__ s, u64 dim, const double2* __restrict__ d) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) s[i] = cmul(s[i], d[i]);
+}
+
+// ------------------------------------------------------------------ Pauli gates
+// src/csim/update_ops_pauli_multi.cpp:51-190.  PHASE_M90ROT[j] = (-i)^j.
+__device__ __forceinline__ double2 mul_mi_pow(double2 v, unsigned j) {
+    switch (j & 3) {
+        case 0: return v;
+        case 1: return make_double2(v.y, -v.x);    // * (-i)
+        case 2: return make_double2(-v.x, -v.y);   // * (-1)
+        default: return make_double2(-v.y, v.x);   // * (+i)
+    }
+}
+__device__ __forceinline__ unsigned par(u64 x) { return __popcll(x) & 1; }
+
+__global__ __launch_bounds__(256) void k_pauli_xz(double2* __restrict__ s, u64 half, u64 bf, u64 pf, unsigned rot,
+                                                   unsigned pivot) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < half; i += stride) {
+        const u64 b0 = insert_zero(i, pivot), b1 = b0 ^ bf;
+        const double2 v0 = s[b0], v1 = s[b1];
+        s[b0] = mul_mi_pow(v1, rot + 2 * par(b0 & pf));
+        s[b1] = mul_mi_pow(v0, rot + 2 * par(b1 & pf));
+    }
+}
+__global__ __launch_bounds__(256) void k_pauli_z(double2* __restrict__ s, u64 dim, u64 pf) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride)
+        if (par(i & pf)) {
+            const double2 v = s[i];
+            s[i] = make_double2(-v.x, -v.y);
+        }
+}
+// exp(+i angle P / 2): v0' = c v0 + i s (phase(b0) v1), v1' = c v1 + i s (phase(b1) v0)
+__global__ __launch_bounds__(256) void k_pauli_rot_xz(double2* __restrict__ s, u64 half, u64 bf, u64 pf,
+                                                       unsigned rot, unsigned pivot, double c, double sn) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < half; i += stride) {
+        const u64 b0 = insert_zero(i, pivot), b1 = b0 ^ bf;
+        const double2 v0 = s[b0], v1 = s[b1];
+        const double2 p1 = mul_mi_pow(v1, rot + 2 * par(b0 & pf));
+        const double2 p0 = mul_mi_pow(v0, rot + 2 * par(b1 & pf));
+        // i * sn * p = (-sn p.y, sn p.x)
+        s[b0] = make_double2(fma(c, v0.x, -sn * p1.y), fma(c, v0.y, sn * p1.x));
+        s[b1] = make_double2(fma(c, v1.x, -sn * p0.y), fma(c, v1.y, sn * p0.x));
+    }
+}
+__global__ __launch_bounds__(256) void k_pauli_rot_z(double2* __restrict__ s, u64 dim, u64 pf, double c, double sn) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
+        const double2 f = make_double2(c, par(i & pf) ? -sn : sn);
+        s[i] = cmul(s[i], f);
+    }
+}
+
+void pauli_masks(const u32* t, const u32* p, u32 n, u64* bf, u64* pf, u32* rot, u32* pivot, bool* ok) {
+    *bf = 0; *pf = 0; *rot = 0; *pivot = 0; *ok = true;
+    for (u32 c = 0; c < n; ++c) {
+        switch (p[c]) {
+            case 0: break;
+            case 1: *bf ^= 1ULL << t[c]; *pivot = t[c]; break;
+            case 2: *bf ^= 1ULL << t[c]; *pf ^= 1ULL << t[c]; (*rot)++; *pivot = t[c]; break;
+            case 3: *pf ^= 1ULL << t[c]; break;
+            default: *ok = false;
+        }
+    }
+}
+
+void pauli_gate_masks(u64 bf, u64 pf, u32 rot, u32 pivot, void* state, u64 dim, void* stream, unsigned device,
+                      const char* what) {
+    const int n = log2_dim(dim);
+    QM_CHECK_ARG(state && n >= 0, "%s: null state or bad dim", what);
+    QM_CHECK_ARG(((bf | pf) >> n) == 0, "%s: mask beyond %d qubits", what, n);
+    QM_CHECK_ARG(bf == 0 || ((bf >> pivot) & 1), "%s: pivot %u not in the bit-flip mask", what, pivot);
+    if (!use_device(device)) return;
+    hipStream_t st = stream_of(stream);
+    if (bf == 0)
+        k_pauli_z<<<grid_for(dim, 256 * 4), 256, 0, st>>>((double2*)state, dim, pf);
+    else
+        k_pauli_xz<<<grid_for(dim / 2, 256 * 4), 256, 0, st>>>((double2*)state, dim / 2, bf, pf, rot, pivot);
+    finish_update(st, what);
+}
+
+void pauli_rot_masks(u64 bf, u64 pf, u32 rot, u32 pivot, double angle, void* state, u64 dim, void* stream,
+                     unsigned device, const char* what) {
+    const int n = log2_dim(dim);
+    QM_CHECK_ARG(state && n >= 0, "%s: null state or bad dim", what);
+    QM_CHECK_ARG(((bf | pf) >> n) == 0, "%s: mask beyond %d qubits", what, n);
+    QM_CHECK_ARG(bf == 0 || ((bf >> pivot) & 1), "%s: pivot %u not in the bit-flip mask", what, pivot);
+    if (!use_device(device)) return;
+    hipStream_t st = stream_of(stream);
+    const double c = cos(angle / 2), sn = sin(angle / 2);
+    if (bf == 0)
+        k_pauli_rot_z<<<grid_for(dim, 256 * 4), 256, 0, st>>>((double2*)state, dim, pf, c, sn);
+    else
+        k_pauli_rot_xz<<<grid_for(dim / 2, 256 * 4), 256, 0, st>>>((double2*)state, dim / 2, bf, pf, rot, pivot, c, sn);
+    finish_update(st, what);
+}
+
+}  // namespace qm
+
+using namespace qm;
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+static const qm_complex kSqrtX[4] = {{0.5, 0.5}, {0.5, -0.5}, {0.5, -0.5}, {0.5, 0.5}};
+static const qm_complex kSqrtXdag[4] = {{0.5, -0.5}, {0.5, 0.5}, {0.5, 0.5}, {0.5, -0.5}};
+static const qm_complex kSqrtY[4] = {{0.5, 0.5}, {-0.5, -0.5}, {0.5, 0.5}, {0.5, 0.5}};
+static const qm_complex kSqrtYdag[4] = {{0.5, -0.5}, {0.5, -0.5}, {-0.5, 0.5}, {0.5, -0.5}};
+
+void X_gate_host(unsigned t, void* state, ITYPE dim, void* stream, unsigned dev) {
+    named1(t, OpSwap01{}, state, dim, stream, dev, "X_gate_host");
+}
+void Y_gate_host(unsigned t, void* state, ITYPE dim, void* stream, unsigned dev) {
+    named1(t, OpY{}, state, dim, stream, dev, "Y_gate_host");
+}
+void Z_gate_host(unsigned t, void* state, ITYPE dim, void* stream, unsigned dev) {
+    named1(t, OpNeg1{}, state, dim, stream, dev, "Z_gate_host");
+}
+void H_gate_host(unsigned t, void* state, ITYPE dim, void* stream, unsigned dev) {
+    named1(t, OpH{1.0 / std::sqrt(2.0)}, state, dim, stream, dev, "H_gate_host");
+}
+// src/csim/update_ops_named.cpp:13-26: S/Sdag/T/Tdag are phase gates.
+void S_gate_host(UINT t, void* state, ITYPE dim, void* stream, unsigned dev) {
+    named1(t, OpPhase1{make_double2(0.0, 1.0)}, state, dim, stream, dev, "S_gate_host");
+}
+void Sdag_gate_host(UINT t, void* state, ITYPE dim, void* stream, unsigned dev) {
+    named1(t, OpPhase1{make_double2(0.0, -1.0)}, state, dim, stream, dev, "Sdag_gate_host");
+}
+void T_gate_host(UINT t, void* state, ITYPE dim, void* stream, unsigned dev) {
+    const double r = 1.0 / std::sqrt(2.0);
+    named1(t, OpPhase1{make_double2(r, r)}, state, dim, stream, dev, "T_gate_host");
+}
+void Tdag_gate_host(UINT t, void* state, ITYPE dim, void* stream, unsigned dev) {
+    const double r = 1.0 / std::sqrt(2.0);
+    named1(t, OpPhase1{make_double2(r, -r)}, state, dim, stream, dev, "Tdag_gate_host");
+}
+void sqrtX_gate_host(UINT t, void* state, ITYPE dim, void* stream, unsigned dev) {
+    dense_any(&t, 1, nullptr, nullptr, 0, kSqrtX, state, dim, stream, dev, "sqrtX_gate_host");
+}
+void sqrtXdag_gate_host(UINT t, void* state, ITYPE dim, void* stream, unsigned dev) {
+    dense_any(&t, 1, nullptr, nullptr, 0, kSqrtXdag, state, dim, stream, dev, "sqrtXdag_gate_host");
+}
+void sqrtY_gate_host(UINT t, void* state, ITYPE dim, void* stream, unsigned dev) {
+    dense_any(&t, 1, nullptr, nullptr, 0, kSqrtY, state, dim, stream, dev, "sqrtY_gate_host");
+}
+void sqrtYdag_gate_host(UINT t, void* state, ITYPE dim, void* stream, unsigned dev) {
+    dense_any(&t, 1, nullptr, nullptr, 0, kSqrtYdag, state, dim, stream, dev, "sqrtYdag_gate_host");
+}
+void P0_gate_host(UINT t, void* state, ITYPE dim, void* stream, unsigned dev) {
+    named1(t, OpZero1{}, state, dim, stream, dev, "P0_gate_host");
+}
+void P1_gate_host(UINT t, void* state, ITYPE dim, void* stream, unsigned dev) {
+    named1(t, OpZero0{}, state, dim, stream, dev, "P1_gate_host");
+}
+void CNOT_gate_host(unsigned c, unsigned t, void* state, ITYPE dim, void* stream, unsigned dev) {
+    controlled1(c, t, OpSwap01{}, state, dim, stream, dev, "CNOT_gate_host");
+}
+void CZ_gate_host(unsigned c, unsigned t, void* state, ITYPE dim, void* stream, unsigned dev) {
+    controlled1(c, t, OpNeg1{}, state, dim, stream, dev, "CZ_gate_host");
+}
+void SWAP_gate_host(unsigned t0, unsigned t1, void* state, ITYPE dim, void* stream, unsigned dev) {
+    const u32 tg[2] = {t0, t1};
+    if (check_gate(state, dim, tg, 2, nullptr, 0, "SWAP_gate_host") < 0) return;
+    if (!use_device(dev)) return;
+    hipStream_t st = stream_of(stream);
+    launch_gather<2, 2>((double2*)state, dim, tg, nullptr, nullptr, 0, OpSwap12{}, st);
+    finish_update(st, "SWAP_gate_host");
+}
+
+// src/csim/update_ops_pauli_single.cpp:54-122 (exp(+i angle P / 2))
+void RX_gate_host(UINT t, double angle, void* state, ITYPE dim, void* stream, unsigned dev) {
+    const double c = std::cos(angle / 2), s = std::sin(angle / 2);
+    const qm_complex m[4] = {{c, 0}, {0, s}, {0, s}, {c, 0}};
+    dense_any(&t, 1, nullptr, nullptr, 0, m, state, dim, stream, dev, "RX_gate_host");
+}
+void RY_gate_host(UINT t, double angle, void* state, ITYPE dim, void* stream, unsigned dev) {
+    const double c = std::cos(angle / 2), s = std::sin(angle / 2);
+    const qm_complex m[4] = {{c, 0}, {s, 0}, {-s, 0}, {c, 0}};
+    dense_any(&t, 1, nullptr, nullptr, 0, m, state, dim, stream, dev, "RY_gate_host");
+}
+void RZ_gate_host(UINT t, double angle, void* state, ITYPE dim, void* stream, unsigned dev) {
+    const double c = std::cos(angle / 2), s = std::sin(angle / 2);
+    const qm_complex d[2] = {{c, s}, {c, -s}};
+    single_qubit_diagonal_matrix_gate_host(t, d, state, dim, stream, dev);
+}
+
+void single_qubit_Pauli_gate_host(UINT t, UINT op, void* state, ITYPE dim, void* stream, unsigned dev) {
+    switch (op) {
+        case 0: return;
+        case 1: X_gate_host(t, state, dim, stream, dev); return;
+        case 2: Y_gate_host(t, state, dim, stream, dev); return;
+        case 3: Z_gate_host(t, state, dim, stream, dev); return;
+        default: set_error(QM_ERR_INVALID_ARGUMENT, "single_qubit_Pauli_gate_host: Pauli id %u", op);
+    }
+}
+void single_qubit_Pauli_rotation_gate_host(unsigned t, unsigned op, double angle, void* state, ITYPE dim,
+                                           void* stream, unsigned dev) {
+    switch (op) {
+        case 0: return;
+        case 1: RX_gate_host(t, angle, state, dim, stream, dev); return;
+        case 2: RY_gate_host(t, angle, state, dim, stream, dev); return;
+        case 3: RZ_gate_host(t, angle, state, dim, stream, dev); return;
+        default: set_error(QM_ERR_INVALID_ARGUMENT, "single_qubit_Pauli_rotation_gate_host: Pauli id %u", op);
+    }
+}
+
+void single_qubit_dense_matrix_gate_host(unsigned t, const qm_complex matrix[4], void* state, ITYPE dim,
+                                         void* stream, unsigned dev) {
+    dense_any(&t, 1, nullptr, nullptr, 0, matrix, state, dim, stream, dev, "single_qubit_dense_matrix_gate_host");
+}
+void single_qubit_diagonal_matrix_gate_host(unsigned t, const qm_complex d[2], void* state, ITYPE dim,
+                                            void* stream, unsigned dev) {
+    QM_CHECK_ARG(d, "single_qubit_diagonal_matrix_gate_host: null diagonal");
+    OpDiag<1> op;
+    op.d[0] = make_double2(d[0].re, d[0].im);
+    op.d[1] = make_double2(d[1].re, d[1].im);
+    named1(t, op, state, dim, stream, dev, "single_qubit_diagonal_matrix_gate_host");
+}
+void single_qubit_phase_gate_host(unsigned t, qm_complex phase, void* state, ITYPE dim, void* stream,
+                                  unsigned dev) {
+    named1(t, OpPhase1{make_double2(phase.re, phase.im)}, state, dim, stream, dev, "single_qubit_phase_gate_host");
+}
+void single_qubit_control_single_qubit_dense_matrix_gate_host(unsigned c, unsigned cv, unsigned t,
+                                                              const qm_complex matrix[4], void* state, ITYPE dim,
+                                                              void* stream, unsigned dev) {
+    QM_CHECK_ARG(cv <= 1, "single_qubit_control_single_qubit_dense_matrix_gate_host: control value %u", cv);
+    dense_any(&t, 1, &c, &cv, 1, matrix, state, dim, stream, dev,
+              "single_qubit_control_single_qubit_dense_matrix_gate_host");
+}
+void multi_qubit_control_single_qubit_dense_matrix_gate_host(const UINT* cl, const UINT* cvl, UINT nc, UINT t,
+                                                             const qm_complex matrix[4], void* state, ITYPE dim,
+                                                             void* stream, unsigned dev) {
+    for (UINT i = 0; i < nc; ++i)
+        QM_CHECK_ARG(cvl[i] <= 1, "multi_qubit_control_single_qubit_dense_matrix_gate_host: control value %u", cvl[i]);
+    dense_any(&t, 1, cl, cvl, nc, matrix, state, dim, stream, dev,
+              "multi_qubit_control_single_qubit_dense_matrix_gate_host");
+}
+
+void double_qubit_dense_matrix_gate_host(unsigned t0, unsigned t1, const qm_complex matrix[16], void* state,
+                                         ITYPE dim, void* stream, unsigned dev) {
+    const u32 tg[2] = {t0, t1};
+    dense_any(tg, 2, nullptr, nullptr, 0, matrix, state, dim, stream, dev, "double_qubit_dense_matrix_gate_host");
+}
+void triple_qubit_dense_matrix_gate_host(unsigned t0, unsigned t1, unsigned t2, const qm_complex matrix[64],
+                                         void* state, ITYPE dim, void* stream, unsigned dev) {
+    const u32 tg[3] = {t0, t1, t2};
+    dense_any(tg, 3, nullptr, nullptr, 0, matrix, state, dim, stream, dev, "triple_qubit_dense_matrix_gate_host");
+}
+void quad_qubit_dense_matrix_gate_host(const unsigned tg[4], const qm_complex matrix[256], void* state, ITYPE dim,
+                                       void* stream, unsigned dev) {
+    dense_any(tg, 4, nullptr, nullptr, 0, matrix, state, dim, stream, dev, "quad_qubit_dense_matrix_gate_host");
+}
+void penta_qubit_dense_matrix_gate_host(const unsigned tg[5], const qm_complex matrix[1024], void* state, ITYPE dim,
+                                        void* stream, UINT dev) {
+    dense_any(tg, 5, nullptr, nullptr, 0, matrix, state, dim, stream, dev, "penta_qubit_dense_matrix_gate_host");
+}
+void multi_qubit_dense_matrix_gate_host(const UINT* tl, UINT nt, const qm_complex* matrix, void* state, ITYPE dim,
+                                        void* stream, unsigned dev) {
+    dense_any(tl, nt, nullptr, nullptr, 0, matrix, state, dim, stream, dev, "multi_qubit_dense_matrix_gate_host");
+}
+void single_qubit_control_multi_qubit_dense_matrix_gate_host(UINT c, UINT cv, const UINT* tl, UINT nt,
+                                                             const qm_complex* matrix, void* state, ITYPE dim,
+                                                             void* stream, unsigned dev) {
+    QM_CHECK_ARG(cv <= 1, "single_qubit_control_multi_qubit_dense_matrix_gate_host: control value %u", cv);
+    dense_any(tl, nt, &c, &cv, 1, matrix, state, dim, stream, dev,
+              "single_qubit_control_multi_qubit_dense_matrix_gate_host");
+}
+void multi_qubit_control_multi_qubit_dense_matrix_gate_host(const UINT* cl, const UINT* cvl, UINT nc,
+                                                            const UINT* tl, UINT nt, const qm_complex* matrix,
+                                                            void* state, ITYPE dim, void* stream, unsigned dev) {
+    for (UINT i = 0; i < nc; ++i)
+        QM_CHECK_ARG(cvl[i] <= 1, "multi_qubit_control_multi_qubit_dense_matrix_gate_host: control value %u", cvl[i]);
+    dense_any(tl, nt, cl, cvl, nc, matrix, state, dim, stream, dev,
+              "multi_qubit_control_multi_qubit_dense_matrix_gate_host");
+}
+
+void qm_multi_qubit_control_multi_qubit_diagonal_matrix_gate_host(const UINT* cl, const UINT* cvl, UINT nc,
+                                                                  const UINT* tl, UINT nt, const qm_complex* d,
+                                                                  void* state, ITYPE dim, void* stream,
+                                                                  unsigned dev) {
+    const char* what = "qm_multi_qubit_control_multi_qubit_diagonal_matrix_gate_host";
+    if (check_gate(state, dim, tl, nt, cl, nc, what) < 0) return;
+    QM_CHECK_ARG(d && nt >= 1 && nt <= 24, "%s: null diagonal or %u targets", what, nt);
+    for (UINT i = 0; i < nc; ++i) QM_CHECK_ARG(cvl[i] <= 1, "%s: control value %u", what, cvl[i]);
+    if (!use_device(dev)) return;
+    hipStream_t st = stream_of(stream);
+    Scratch* sc = scratch_for(st);
+    if (!sc) return;
+    const GatherPlan cp = make_plan(nullptr, 0, cl, cvl, nc);
+    unsigned char tg[64];
+    for (UINT i = 0; i < nt; ++i) tg[i] = (unsigned char)tl[i];
+    const size_t nd = (size_t)1 << nt;
+    const double2* dd = (const double2*)upload(sc, d, nd * sizeof(double2));
+    const unsigned char* dcins = (const unsigned char*)upload(sc, cp.ins, 64);
+    const unsigned char* dtg = (const unsigned char*)upload(sc, tg, 64);
+    if (!dd || !dcins || !dtg) return;
+    const u64 n = dim >> nc;
+    k_diag_any<<<grid_for(n, 256 * 4), 256, 0, st>>>((double2*)state, n, dcins, cp.n_ins, cp.control_mask, dtg,
+                                                      (int)nt, dd);
+    finish_update(st, what);
+}
+
+void multi_qubit_diagonal_matrix_gate_host(const qm_complex* d, void* state, ITYPE dim, void* stream, UINT dev) {
+    const char* what = "multi_qubit_diagonal_matrix_gate_host";
+    QM_CHECK_ARG(state && d && log2_dim(dim) >= 0, "%s: bad arguments", what);
+    if (!use_device(dev)) return;
+    hipStream_t st = stream_of(stream);
+    Scratch* sc = scratch_for(st);
+    if (!sc) return;
+    void* buf = big_buffer(sc, (size_t)dim * 16);
+    if (!buf) return;
+    if (!hip_ok(hipMemcpyAsync(buf, d, (size_t)dim * 16, hipMemcpyHostToDevice, st), what)) return;
+    k_diag_whole<<<grid_for(dim, 256 * 4), 256, 0, st>>>((double2*)state, dim, (const double2*)buf);
+    if (!hip_ok(hipGetLastError(), what)) return;
+    // The diagonal was copied from the caller's pageable memory through the
+    // shared buffer; wait so a later call cannot overwrite it early.
+    hip_ok(hipStreamSynchronize(st), what);
+}
+
+void multi_qubit_Pauli_gate_Z_mask_host(ITYPE pf, void* state, ITYPE dim, void* stream, unsigned dev) {
+    pauli_gate_masks(0, pf, 0, 0, state, dim, stream, dev, "multi_qubit_Pauli_gate_Z_mask_host");
+}
+void multi_qubit_Pauli_gate_XZ_mask_host(ITYPE bf, ITYPE pf, UINT rot, UINT pivot, void* state, ITYPE dim,
+                                         void* stream, unsigned dev) {
+    QM_CHECK_ARG(bf != 0, "multi_qubit_Pauli_gate_XZ_mask_host: empty bit-flip mask");
+    pauli_gate_masks(bf, pf, rot, pivot, state, dim, stream, dev, "multi_qubit_Pauli_gate_XZ_mask_host");
+}
+void multi_qubit_Pauli_rotation_gate_XZ_mask_host(ITYPE bf, ITYPE pf, UINT rot, UINT pivot, double angle,
+                                                  void* state, ITYPE dim, void* stream, unsigned dev) {
+    QM_CHECK_ARG(bf != 0, "multi_qubit_Pauli_rotation_gate_XZ_mask_host: empty bit-flip mask");
+    pauli_rot_masks(bf, pf, rot, pivot, angle, state, dim, stream, dev, "multi_qubit_Pauli_rotation_gate_XZ_mask_host");
+}
+void multi_qubit_Pauli_rotation_gate_Z_mask_host(ITYPE pf, double angle, void* state, ITYPE dim, void* stream,
+                                                 unsigned dev) {
+    pauli_rot_masks(0, pf, 0, 0, angle, state, dim, stream, dev, "multi_qubit_Pauli_rotation_gate_Z_mask_host");
+}
+void multi_qubit_Pauli_gate_partial_list_host(const UINT* tl, const UINT* pl, UINT n, void* state, ITYPE dim,
+                                              void* stream, unsigned dev) {
+    u64 bf, pf; u32 rot, pivot; bool ok;
+    pauli_masks(tl, pl, n, &bf, &pf, &rot, &pivot, &ok);
+    QM_CHECK_ARG(ok, "multi_qubit_Pauli_gate_partial_list_host: Pauli id out of 0..3");
+    pauli_gate_masks(bf, pf, rot, pivot, state, dim, stream, dev, "multi_qubit_Pauli_gate_partial_list_host");
+}
+void multi_qubit_Pauli_gate_whole_list_host(const UINT* pl, UINT n, void* state, ITYPE dim, void* stream,
+                                            unsigned dev) {
+    std::vector<u32> tl(n);
+    for (UINT i = 0; i < n; ++i) tl[i] = i;
+    multi_qubit_Pauli_gate_partial_list_host(tl.data(), pl, n, state, dim, stream, dev);
+}
+void multi_qubit_Pauli_rotation_gate_partial_list_host(const UINT* tl, const UINT* pl, UINT n, double angle,
+                                                       void* state, ITYPE dim, void* stream, unsigned dev) {
+    u64 bf, pf; u32 rot, pivot; bool ok;
+    pauli_masks(tl, pl, n, &bf, &pf, &rot, &pivot, &ok);
+    QM_CHECK_ARG(ok, "multi_qubit_Pauli_rotation_gate_partial_list_host: Pauli id out of 0..3");
+    pauli_rot_masks(bf, pf, rot, pivot, angle, state, dim, stream, dev,
+                    "multi_qubit_Pauli_rotation_gate_partial_list_host");
+}
+void multi_qubit_Pauli_rotation_gate_whole_list_host(const UINT* pl, UINT n, double angle, void* state, ITYPE dim,
+                                                     void* stream, unsigned dev) {
+    std::vector<u32> tl(n);
+    for (UINT i = 0; i < n; ++i) tl[i] = i;
+    multi_qubit_Pauli_rotation_gate_partial_list_host(tl.data(), pl, n, angle, state, dim, stream, dev);
+}
+
+}  // extern "C"

@@ -1,0 +1,333 @@
+// Runtime of libqulacs_mi355: error state, device selection, per-stream
+// scratch, state allocation / initialisation / copies.
+//
+// Reference interfaces replaced here: src/gpusim/memory_ops.h:7-20 and
+// src/gpusim/util_func.h:9-25 (implementations memory_ops.cu, util.cu).
+#include <cstdarg>
+#include <cstring>
+#include <map>
+
+#include "qm_internal.hpp"
+
+namespace qm {
+
+namespace {
+thread_local int g_err_code = QM_OK;
+thread_local char g_err_msg[512] = {0};
+int g_synchronous = 1;
+
+std::mutex g_scratch_mu;
+std::map<std::pair<int, hipStream_t>, Scratch*> g_scratch;
+}  // namespace
+
+void set_error(int code, const char* fmt, ...) {
+    g_err_code = code;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err_msg, sizeof(g_err_msg), fmt, ap);
+    va_end(ap);
+}
+
+bool hip_ok(hipError_t e, const char* what) {
+    if (e == hipSuccess) return true;
+    set_error(e == hipErrorOutOfMemory ? QM_ERR_OUT_OF_MEMORY : QM_ERR_HIP, "%s: %s", what,
+              hipGetErrorString(e));
+    return false;
+}
+
+bool synchronous_mode() { return g_synchronous != 0; }
+
+bool use_device(unsigned device) {
+    int cur = -1;
+    if (!hip_ok(hipGetDevice(&cur), "hipGetDevice")) return false;
+    if (cur == (int)device) return true;
+    return hip_ok(hipSetDevice((int)device), "hipSetDevice");
+}
+
+void finish_update(hipStream_t s, const char* what) {
+    if (!hip_ok(hipGetLastError(), what)) return;
+    if (g_synchronous) hip_ok(hipStreamSynchronize(s), what);
+}
+
+Scratch* scratch_for(hipStream_t s) {
+    int dev = 0;
+    if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto key = std::make_pair(dev, s);
+    auto it = g_scratch.find(key);
+    if (it != g_scratch.end()) return it->second;
+    Scratch* sc = new Scratch();
+    sc->device = dev;
+    sc->stream = s;
+    if (!hip_ok(hipMalloc(&sc->partials, kPartialsCap * sizeof(double)), "scratch partials") ||
+        !hip_ok(hipHostMalloc(&sc->host_result, kHostResultCap * sizeof(double), hipHostMallocDefault),
+                "scratch host result") ||
+        !hip_ok(hipMalloc(&sc->ring, kRingBytes), "scratch ring")) {
+        delete sc;
+        return nullptr;
+    }
+    g_scratch[key] = sc;
+    return sc;
+}
+
+void* upload(Scratch* sc, const void* host, size_t bytes) {
+    const size_t b = (bytes + 255) & ~size_t(255);
+    if (b > kRingBytes) {
+        set_error(QM_ERR_UNSUPPORTED, "upload of %zu bytes exceeds the %zu-byte ring", bytes, kRingBytes);
+        return nullptr;
+    }
+    if (sc->ring_head + b > kRingBytes) sc->ring_head = 0;
+    char* dst = sc->ring + sc->ring_head;
+    sc->ring_head += b;
+    // Pageable source: the runtime stages it before returning, so the caller's
+    // buffer is free to reuse; the device copy is ordered on the stream, which
+    // also orders it after every earlier kernel that read this ring slot.
+    if (!hip_ok(hipMemcpyAsync(dst, host, bytes, hipMemcpyHostToDevice, sc->stream), "upload"))
+        return nullptr;
+    return dst;
+}
+
+void* big_buffer(Scratch* sc, size_t bytes) {
+    if (sc->big_bytes >= bytes) return sc->big;
+    if (sc->big) {
+        if (!hip_ok(hipStreamSynchronize(sc->stream), "big buffer sync")) return nullptr;
+        hip_ok(hipFree(sc->big), "big buffer free");
+        sc->big = nullptr;
+        sc->big_bytes = 0;
+    }
+    if (!hip_ok(hipMalloc(&sc->big, bytes), "big buffer")) return nullptr;
+    sc->big_bytes = bytes;
+    return sc->big;
+}
+
+bool check_qubits(const u32* q, u32 n, int nqubits, const char* what) {
+    u64 seen = 0;
+    for (u32 i = 0; i < n; ++i) {
+        if ((int)q[i] >= nqubits) {
+            set_error(QM_ERR_INVALID_ARGUMENT, "%s: qubit %u out of range for %d qubits", what, q[i], nqubits);
+            return false;
+        }
+        if (seen & (1ULL << q[i])) {
+            set_error(QM_ERR_INVALID_ARGUMENT, "%s: qubit %u listed twice", what, q[i]);
+            return false;
+        }
+        seen |= 1ULL << q[i];
+    }
+    return true;
+}
+
+GatherPlan make_plan(const u32* targets, u32 nt, const u32* controls, const u32* cvals, u32 nc) {
+    GatherPlan p;
+    u64 bits = 0;
+    for (u32 i = 0; i < nt; ++i) bits |= 1ULL << targets[i];
+    for (u32 i = 0; i < nc; ++i) {
+        bits |= 1ULL << controls[i];
+        if (cvals[i]) p.control_mask |= 1ULL << controls[i];
+    }
+    p.n_ins = 0;
+    for (int b = 0; b < 64; ++b)
+        if (bits & (1ULL << b)) p.ins[p.n_ins++] = (unsigned char)b;
+    return p;
+}
+
+// ------------------------------------------------------------------ kernels
+__global__ void k_fill_basis(double2* __restrict__ s, u64 dim, u64 basis) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride)
+        s[i] = make_double2(i == basis ? 1.0 : 0.0, 0.0);
+}
+
+// Counter-based generator (SplitMix64 finaliser on seed and index) feeding a
+// Box-Muller transform; each amplitude draws two independent normals.
+__device__ __forceinline__ u64 mix64(u64 z) {
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ double u01(u64 r) { return ((r >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+
+__global__ void k_haar(double2* __restrict__ s, u64 dim, u64 seed) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
+        const double a = u01(mix64(seed ^ mix64(2 * i)));
+        const double b = u01(mix64(seed ^ mix64(2 * i + 1)));
+        const double r = sqrt(-2.0 * log(a));
+        double sn, cs;
+        sincospi(2.0 * b, &sn, &cs);
+        s[i] = make_double2(r * cs, r * sn);
+    }
+}
+
+}  // namespace qm
+
+using namespace qm;
+
+extern "C" {
+
+static const char kVersion[] = "libqulacs_mi355 0.1 (gfx950)";
+
+int qm_get_last_error(char* buf, size_t buf_len) {
+    if (buf && buf_len) {
+        strncpy(buf, g_err_msg, buf_len - 1);
+        buf[buf_len - 1] = 0;
+    }
+    return g_err_code;
+}
+void qm_clear_last_error(void) {
+    g_err_code = QM_OK;
+    g_err_msg[0] = 0;
+}
+void qm_set_synchronous(int on) { g_synchronous = on ? 1 : 0; }
+int qm_get_synchronous(void) { return g_synchronous; }
+const char* qm_version(void) { return kVersion; }
+
+void qm_synchronize(void* stream, UINT device_number) {
+    if (!use_device(device_number)) return;
+    hip_ok(hipStreamSynchronize(stream_of(stream)), "qm_synchronize");
+}
+
+int get_num_device(void) {
+    int n = 0;
+    if (!hip_ok(hipGetDeviceCount(&n), "hipGetDeviceCount")) return 0;
+    return n;
+}
+void set_device(unsigned int device_num) { hip_ok(hipSetDevice((int)device_num), "set_device"); }
+int get_current_device(void) {
+    int d = -1;
+    hip_ok(hipGetDevice(&d), "get_current_device");
+    return d;
+}
+
+// Reference: an array of `max_cuda_stream` streams, returned as void*
+// (src/gpusim/memory_ops.cu:20-30).  All streams use the default priority.
+void* allocate_cuda_stream_host(unsigned int max_cuda_stream, unsigned int device_number) {
+    if (max_cuda_stream == 0) max_cuda_stream = 1;
+    if (!use_device(device_number)) return nullptr;
+    hipStream_t* arr = new hipStream_t[max_cuda_stream];
+    for (unsigned i = 0; i < max_cuda_stream; ++i) {
+        if (!hip_ok(hipStreamCreateWithFlags(&arr[i], hipStreamNonBlocking), "hipStreamCreate")) {
+            for (unsigned j = 0; j < i; ++j) (void)hipStreamDestroy(arr[j]);
+            delete[] arr;
+            return nullptr;
+        }
+    }
+    return arr;
+}
+
+void release_cuda_stream_host(void* cuda_stream, unsigned int max_cuda_stream, unsigned int device_number) {
+    if (!cuda_stream) return;
+    if (!use_device(device_number)) return;
+    hipStream_t* arr = reinterpret_cast<hipStream_t*>(cuda_stream);
+    if (max_cuda_stream == 0) max_cuda_stream = 1;
+    for (unsigned i = 0; i < max_cuda_stream; ++i) {
+        (void)hipStreamSynchronize(arr[i]);
+        {
+            // Drop the scratch tied to this stream.
+            std::lock_guard<std::mutex> lk(g_scratch_mu);
+            int dev = -1;
+            (void)hipGetDevice(&dev);
+            auto it = g_scratch.find(std::make_pair(dev, arr[i]));
+            if (it != g_scratch.end()) {
+                Scratch* sc = it->second;
+                (void)hipFree(sc->partials);
+                (void)hipHostFree(sc->host_result);
+                (void)hipFree(sc->ring);
+                if (sc->big) (void)hipFree(sc->big);
+                delete sc;
+                g_scratch.erase(it);
+            }
+        }
+        (void)hipStreamDestroy(arr[i]);
+    }
+    delete[] arr;
+}
+
+void* allocate_quantum_state_host(ITYPE dim, unsigned int device_number) {
+    if (!use_device(device_number)) return nullptr;
+    void* p = nullptr;
+    if (!hip_ok(hipMalloc(&p, (size_t)dim * 16), "allocate_quantum_state_host")) return nullptr;
+    return p;
+}
+
+void release_quantum_state_host(void* state, unsigned int device_number) {
+    if (!state) return;
+    if (!use_device(device_number)) return;
+    hip_ok(hipFree(state), "release_quantum_state_host");
+}
+
+void set_computational_basis_host(ITYPE comp_basis, void* state, ITYPE dim, void* stream,
+                                  unsigned int device_number) {
+    QM_CHECK_ARG(state && dim > 0, "set_computational_basis_host: null state or zero dim");
+    QM_CHECK_ARG(comp_basis < dim, "set_computational_basis_host: basis %llu >= dim %llu", comp_basis, dim);
+    if (!use_device(device_number)) return;
+    hipStream_t s = stream_of(stream);
+    k_fill_basis<<<grid_for(dim, 256 * 8), 256, 0, s>>>((double2*)state, dim, comp_basis);
+    finish_update(s, "set_computational_basis_host");
+}
+
+void initialize_quantum_state_host(void* state, ITYPE dim, void* stream, unsigned int device_number) {
+    set_computational_basis_host(0, state, dim, stream, device_number);
+}
+
+void initialize_Haar_random_state_with_seed_host(void* state, ITYPE dim, UINT seed, void* stream,
+                                                 unsigned int device_number) {
+    QM_CHECK_ARG(state && log2_dim(dim) >= 0, "initialize_Haar_random_state_with_seed_host: bad state/dim");
+    if (!use_device(device_number)) return;
+    hipStream_t s = stream_of(stream);
+    k_haar<<<grid_for(dim, 256 * 8), 256, 0, s>>>((double2*)state, dim, 0xD1B54A32D192ED03ULL * (u64)(seed + 1));
+    if (!hip_ok(hipGetLastError(), "haar")) return;
+    const double n = state_norm_squared_host(state, dim, stream, device_number);
+    normalize_host(n, state, dim, stream, device_number);
+}
+
+void initialize_Haar_random_state_host(void* state, ITYPE dim, void* stream, unsigned int device_number) {
+    static unsigned counter = 0x5eed;
+    initialize_Haar_random_state_with_seed_host(state, dim, counter++, stream, device_number);
+}
+
+void copy_quantum_state_from_device_to_device(void* dst, const void* src, ITYPE dim, void* stream,
+                                              unsigned int device_number) {
+    QM_CHECK_ARG(dst && src, "copy_quantum_state_from_device_to_device: null pointer");
+    if (!use_device(device_number)) return;
+    hipStream_t s = stream_of(stream);
+    if (!hip_ok(hipMemcpyAsync(dst, src, (size_t)dim * 16, hipMemcpyDeviceToDevice, s), "d2d copy")) return;
+    finish_update(s, "copy_quantum_state_from_device_to_device");
+}
+
+void copy_quantum_state_from_host_to_device(void* dst, const void* src, ITYPE dim, void* stream,
+                                            unsigned int device_number) {
+    QM_CHECK_ARG(dst && src, "copy_quantum_state_from_host_to_device: null pointer");
+    if (!use_device(device_number)) return;
+    hipStream_t s = stream_of(stream);
+    if (!hip_ok(hipMemcpyAsync(dst, src, (size_t)dim * 16, hipMemcpyHostToDevice, s), "h2d copy")) return;
+    // The host buffer belongs to the caller: always wait for the copy.
+    hip_ok(hipStreamSynchronize(s), "h2d copy sync");
+}
+
+void copy_quantum_state_from_cppstate_host(void* dst, const qm_complex* cppstate, ITYPE dim, void* stream,
+                                           UINT device_number) {
+    copy_quantum_state_from_host_to_device(dst, cppstate, dim, stream, device_number);
+}
+
+void copy_quantum_state_from_device_to_host(void* dst, const void* src, ITYPE dim, void* stream,
+                                            unsigned int device_number) {
+    QM_CHECK_ARG(dst && src, "copy_quantum_state_from_device_to_host: null pointer");
+    if (!use_device(device_number)) return;
+    hipStream_t s = stream_of(stream);
+    if (!hip_ok(hipMemcpyAsync(dst, src, (size_t)dim * 16, hipMemcpyDeviceToHost, s), "d2h copy")) return;
+    hip_ok(hipStreamSynchronize(s), "d2h copy sync");
+}
+
+void get_quantum_state_host(void* state_gpu, void* psi_cpu_copy, ITYPE dim, void* stream,
+                            unsigned int device_number) {
+    copy_quantum_state_from_device_to_host(psi_cpu_copy, state_gpu, dim, stream, device_number);
+}
+
+void print_quantum_state_host(void* state, ITYPE dim, unsigned int device_number) {
+    std::vector<double> h((size_t)dim * 2);
+    copy_quantum_state_from_device_to_host(h.data(), state, dim, nullptr, device_number);
+    for (ITYPE i = 0; i < dim; ++i) printf("%llu: %.17g %+.17gi\n", i, h[2 * i], h[2 * i + 1]);
+}
+
+}  // extern "C"
