@@ -75,12 +75,16 @@ class HipEvents:
             self.h.hipEventDestroy(e)
 
 
-KERNEL_OF_KIND = {
-    # gate kind -> kernel that executes it (qulacs_amd/csrc/qm_gates.hip)
-    "dense1": "k_gather<1,OpDenseArg<1>>", "RX": "k_gather<1,OpDenseArg<1>>", "RY": "k_gather<1,OpDenseArg<1>>",
-    "RZ": "k_gather<1,OpDiag<1>>", "dense2": "k_gather<2,OpDenseArg<2>>", "dense5": "k_dense_stream<5>",
-    "CNOT": "k_gather<1,OpSwap01>", "CZ": "k_gather<1,OpNeg1>",
-}
+def kernel_of(g) -> str:
+    """Name of the kernel instantiation that executes gate g (qulacs_amd/csrc/qm_gates.hip): the
+    library streams with non-temporal accesses when no acted-on qubit is below bit 3."""
+    nt = "true" if min(g.targets) >= 3 else "false"
+    return {
+        "dense1": f"k_gather<1, 1, {nt}, OpDenseArg<1>>", "RX": f"k_gather<1, 1, {nt}, OpDenseArg<1>>",
+        "RY": f"k_gather<1, 1, {nt}, OpDenseArg<1>>", "RZ": f"k_gather<1, 1, {nt}, OpDiag<1>>",
+        "dense2": f"k_gather<2, 1, {nt}, OpDenseArg<2>>", "dense5": f"k_dense_stream<5, {nt}, MatInPtr<5>>",
+        "CNOT": f"k_gather<1, 1, {nt}, OpSwap01>", "CZ": f"k_gather<1, 1, {nt}, OpNeg1>",
+    }[g.kind]
 
 
 def sv_sweep(args, lib_mod, hip):
@@ -134,7 +138,7 @@ def sv_sweep(args, lib_mod, hip):
         e["count"] += 1
         e["ms"] += d
         e["bytes"] += alg_bytes(g, n)
-        k = per_kernel.setdefault(KERNEL_OF_KIND[g.kind], {"count": 0, "ms": 0.0, "bytes": 0})
+        k = per_kernel.setdefault(kernel_of(g), {"count": 0, "ms": 0.0, "bytes": 0})
         k["count"] += 1
         k["ms"] += d
         k["bytes"] += alg_bytes(g, n)

@@ -59,8 +59,8 @@ __device__ __forceinline__ u64 base_of(u64 i, const GatherArgs<K>& a) {
 
 // Op interface: READ / WRITE are bit masks over tuple members; apply() maps
 // the registers in place.
-template <int K, int U, class Op>
-__global__ __launch_bounds__(256) void k_gather(double2* __restrict__ s, u64 n_bases, GatherArgs<K> a, Op op) {
+template <int K, int U, bool NT, class Op>
+__global__ __launch_bounds__(512) void k_gather(double2* __restrict__ s, u64 n_bases, GatherArgs<K> a, Op op) {
     constexpr int N = 1 << K;
     const u64 stride = (u64)gridDim.x * blockDim.x * U;
     for (u64 i0 = ((u64)blockIdx.x * blockDim.x) * U + threadIdx.x; i0 < n_bases; i0 += stride) {
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void k_gather(double2* __restrict__ s, u64 n_b
 #pragma unroll
             for (int x = 0; x < N; ++x)
                 if ((Op::READ >> x) & 1)
-                    if (b[u] != ~0ULL) v[u][x] = s[b[u] + a.offs[x]];
+                    if (b[u] != ~0ULL) v[u][x] = ld<NT>(s + b[u] + a.offs[x]);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256) void k_gather(double2* __restrict__ s, u64 n_b
             op.apply(v[u]);
 #pragma unroll
             for (int x = 0; x < N; ++x)
-                if ((Op::WRITE >> x) & 1) s[b[u] + a.offs[x]] = v[u][x];
+                if ((Op::WRITE >> x) & 1) st<NT>(s + b[u] + a.offs[x], v[u][x]);
         }
     }
 }
@@ -126,7 +126,7 @@ struct MatInPtr {  // K = 4, 5: device copy, uniform across the wave (scalar loa
 
 // Dense k-qubit gate (k = 3..5): the tuple sits in registers and each output
 // row is written as soon as it is formed, so live registers stay near 4 * 2^k.
-template <int K, class M>
+template <int K, bool NT, class M>
 __global__ __launch_bounds__(256) void k_dense_stream(double2* __restrict__ s, u64 n_bases, GatherArgs<K> a, M mat) {
     constexpr int N = 1 << K;
     const u64 stride = (u64)gridDim.x * blockDim.x;
@@ -134,13 +134,13 @@ __global__ __launch_bounds__(256) void k_dense_stream(double2* __restrict__ s, u
         const u64 b = base_of<K>(i, a);
         double2 v[N];
 #pragma unroll
-        for (int x = 0; x < N; ++x) v[x] = s[b + a.offs[x]];
+        for (int x = 0; x < N; ++x) v[x] = ld<NT>(s + b + a.offs[x]);
 #pragma unroll
         for (int y = 0; y < N; ++y) {
             double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
             for (int x = 0; x < N; ++x) acc = cmac(acc, mat.at(y, x), v[x]);
-            s[b + a.offs[y]] = acc;
+            st<NT>(s + b + a.offs[y], acc);
         }
     }
 }
@@ -213,8 +213,12 @@ void launch_gather(double2* s, u64 dim, const u32* targets, const u32* controls,
     GatherArgs<K> a;
     if (!fill_args<K>(a, targets, controls, cvals, nc)) return;
     const u64 n_bases = dim >> (K + nc);
-    const unsigned block = 256;
-    k_gather<K, U, Op><<<grid_for(n_bases, (u64)block * U), block, 0, st>>>(s, n_bases, a, op);
+    const unsigned block = 512;
+    const unsigned grid = grid_for(n_bases, (u64)block * U);
+    if (a.n_ins == 0 || a.ins[0] >= kLineQubits)
+        k_gather<K, U, true, Op><<<grid, block, 0, st>>>(s, n_bases, a, op);
+    else
+        k_gather<K, U, false, Op><<<grid, block, 0, st>>>(s, n_bases, a, op);
 }
 
 template <int K, class M>
@@ -223,7 +227,10 @@ void launch_dense_stream(double2* s, u64 dim, const u32* targets, const u32* con
     GatherArgs<K> a;
     if (!fill_args<K>(a, targets, controls, cvals, nc)) return;
     const u64 n_bases = dim >> (K + nc);
-    k_dense_stream<K, M><<<grid_for(n_bases, 256), 256, 0, st>>>(s, n_bases, a, m);
+    if (a.ins[0] >= kLineQubits)
+        k_dense_stream<K, true, M><<<grid_for(n_bases, 256), 256, 0, st>>>(s, n_bases, a, m);
+    else
+        k_dense_stream<K, false, M><<<grid_for(n_bases, 256), 256, 0, st>>>(s, n_bases, a, m);
 }
 
 // Validates state/dim/targets/controls; returns n (qubit count) or -1.
@@ -283,8 +290,8 @@ void dense_any(const u32* targets, u32 nt, const u32* controls, const u32* cvals
     hipStream_t st = stream_of(stream);
     double2* s = (double2*)state;
     switch (nt) {
-        case 1: launch_gather<1, 2>(s, dim, targets, controls, cvals, nc, OpDenseArg<1>{mat_arg<1>(matrix)}, st); break;
-        case 2: launch_gather<2, 2>(s, dim, targets, controls, cvals, nc, OpDenseArg<2>{mat_arg<2>(matrix)}, st); break;
+        case 1: launch_gather<1, 1>(s, dim, targets, controls, cvals, nc, OpDenseArg<1>{mat_arg<1>(matrix)}, st); break;
+        case 2: launch_gather<2, 1>(s, dim, targets, controls, cvals, nc, OpDenseArg<2>{mat_arg<2>(matrix)}, st); break;
         case 3: launch_dense_stream<3>(s, dim, targets, controls, cvals, nc, MatInArg<3>{mat_arg<3>(matrix)}, st); break;
         case 4:
         case 5: {
@@ -347,7 +354,7 @@ void named1(u32 t, const Op& op, void* state, u64 dim, void* stream, unsigned de
     if (check_gate(state, dim, &t, 1, nullptr, 0, what) < 0) return;
     if (!use_device(device)) return;
     hipStream_t st = stream_of(stream);
-    launch_gather<1, 2>((double2*)state, dim, &t, nullptr, nullptr, 0, op, st);
+    launch_gather<1, 1>((double2*)state, dim, &t, nullptr, nullptr, 0, op, st);
     finish_update(st, what);
 }
 
@@ -358,7 +365,7 @@ void controlled1(u32 c, u32 t, const Op& op, void* state, u64 dim, void* stream,
     if (check_gate(state, dim, &t, 1, &c, 1, what) < 0) return;
     if (!use_device(device)) return;
     hipStream_t st = stream_of(stream);
-    launch_gather<1, 2>((double2*)state, dim, &t, &c, &one, 1, op, st);
+    launch_gather<1, 1>((double2*)state, dim, &t, &c, &one, 1, op, st);
     finish_update(st, what);
 }
 
@@ -380,7 +387,7 @@ __global__ __launch_bounds__(256) void k_diag_any(double2* __restrict__ s, u64 n
 
 __global__ __launch_bounds__(256) void k_diag_whole(double2* __restrict__ s, u64 dim, const double2* __restrict__ d) {
     const u64 stride = (u64)gridDim.x * blockDim.x;
-    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) s[i] = cmul(s[i], d[i]);
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) st_stream(s + i, cmul(ld_stream(s + i), d[i]));
 }
 
 // ------------------------------------------------------------------ Pauli gates
@@ -395,14 +402,15 @@ __device__ __forceinline__ double2 mul_mi_pow(double2 v, unsigned j) {
 }
 __device__ __forceinline__ unsigned par(u64 x) { return __popcll(x) & 1; }
 
+template <bool NT>
 __global__ __launch_bounds__(256) void k_pauli_xz(double2* __restrict__ s, u64 half, u64 bf, u64 pf, unsigned rot,
                                                    unsigned pivot) {
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < half; i += stride) {
         const u64 b0 = insert_zero(i, pivot), b1 = b0 ^ bf;
-        const double2 v0 = s[b0], v1 = s[b1];
-        s[b0] = mul_mi_pow(v1, rot + 2 * par(b0 & pf));
-        s[b1] = mul_mi_pow(v0, rot + 2 * par(b1 & pf));
+        const double2 v0 = ld<NT>(s + b0), v1 = ld<NT>(s + b1);
+        st<NT>(s + b0, mul_mi_pow(v1, rot + 2 * par(b0 & pf)));
+        st<NT>(s + b1, mul_mi_pow(v0, rot + 2 * par(b1 & pf)));
     }
 }
 __global__ __launch_bounds__(256) void k_pauli_z(double2* __restrict__ s, u64 dim, u64 pf) {
@@ -414,24 +422,25 @@ __global__ __launch_bounds__(256) void k_pauli_z(double2* __restrict__ s, u64 di
         }
 }
 // exp(+i angle P / 2): v0' = c v0 + i s (phase(b0) v1), v1' = c v1 + i s (phase(b1) v0)
+template <bool NT>
 __global__ __launch_bounds__(256) void k_pauli_rot_xz(double2* __restrict__ s, u64 half, u64 bf, u64 pf,
                                                        unsigned rot, unsigned pivot, double c, double sn) {
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < half; i += stride) {
         const u64 b0 = insert_zero(i, pivot), b1 = b0 ^ bf;
-        const double2 v0 = s[b0], v1 = s[b1];
+        const double2 v0 = ld<NT>(s + b0), v1 = ld<NT>(s + b1);
         const double2 p1 = mul_mi_pow(v1, rot + 2 * par(b0 & pf));
         const double2 p0 = mul_mi_pow(v0, rot + 2 * par(b1 & pf));
         // i * sn * p = (-sn p.y, sn p.x)
-        s[b0] = make_double2(fma(c, v0.x, -sn * p1.y), fma(c, v0.y, sn * p1.x));
-        s[b1] = make_double2(fma(c, v1.x, -sn * p0.y), fma(c, v1.y, sn * p0.x));
+        st<NT>(s + b0, make_double2(fma(c, v0.x, -sn * p1.y), fma(c, v0.y, sn * p1.x)));
+        st<NT>(s + b1, make_double2(fma(c, v1.x, -sn * p0.y), fma(c, v1.y, sn * p0.x)));
     }
 }
 __global__ __launch_bounds__(256) void k_pauli_rot_z(double2* __restrict__ s, u64 dim, u64 pf, double c, double sn) {
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
         const double2 f = make_double2(c, par(i & pf) ? -sn : sn);
-        s[i] = cmul(s[i], f);
+        st_stream(s + i, cmul(ld_stream(s + i), f));
     }
 }
 
@@ -459,7 +468,7 @@ void pauli_gate_masks(u64 bf, u64 pf, u32 rot, u32 pivot, void* state, u64 dim, 
     if (bf == 0)
         k_pauli_z<<<grid_for(dim, 256 * 4), 256, 0, st>>>((double2*)state, dim, pf);
     else
-        k_pauli_xz<<<grid_for(dim / 2, 256 * 4), 256, 0, st>>>((double2*)state, dim / 2, bf, pf, rot, pivot);
+        (pivot >= kLineQubits ? k_pauli_xz<true> : k_pauli_xz<false>)<<<grid_for(dim / 2, 256 * 4), 256, 0, st>>>((double2*)state, dim / 2, bf, pf, rot, pivot);
     finish_update(st, what);
 }
 
@@ -475,7 +484,7 @@ void pauli_rot_masks(u64 bf, u64 pf, u32 rot, u32 pivot, double angle, void* sta
     if (bf == 0)
         k_pauli_rot_z<<<grid_for(dim, 256 * 4), 256, 0, st>>>((double2*)state, dim, pf, c, sn);
     else
-        k_pauli_rot_xz<<<grid_for(dim / 2, 256 * 4), 256, 0, st>>>((double2*)state, dim / 2, bf, pf, rot, pivot, c, sn);
+        (pivot >= kLineQubits ? k_pauli_rot_xz<true> : k_pauli_rot_xz<false>)<<<grid_for(dim / 2, 256 * 4), 256, 0, st>>>((double2*)state, dim / 2, bf, pf, rot, pivot, c, sn);
     finish_update(st, what);
 }
 
@@ -547,7 +556,7 @@ void SWAP_gate_host(unsigned t0, unsigned t1, void* state, ITYPE dim, void* stre
     if (check_gate(state, dim, tg, 2, nullptr, 0, "SWAP_gate_host") < 0) return;
     if (!use_device(dev)) return;
     hipStream_t st = stream_of(stream);
-    launch_gather<2, 2>((double2*)state, dim, tg, nullptr, nullptr, 0, OpSwap12{}, st);
+    launch_gather<2, 1>((double2*)state, dim, tg, nullptr, nullptr, 0, OpSwap12{}, st);
     finish_update(st, "SWAP_gate_host");
 }
 

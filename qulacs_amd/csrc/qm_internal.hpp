@@ -110,6 +110,34 @@ __device__ __forceinline__ double2 cmac(double2 acc, double2 a, double2 b) {
     return acc;
 }
 __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+
+// Streaming (non-temporal) 16-byte accesses for data touched once per pass:
+// measured on MI355X they raise an in-place read-modify-write sweep from
+// ~5.5 to ~6.0 TB/s by not keeping the stream resident in the caches.
+typedef double qm_d2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 ld_stream(const double2* p) {
+    const qm_d2v v = __builtin_nontemporal_load(reinterpret_cast<const qm_d2v*>(p));
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void st_stream(double2* p, double2 v) {
+    const qm_d2v w = {v.x, v.y};
+    __builtin_nontemporal_store(w, reinterpret_cast<qm_d2v*>(p));
+}
+// NT selects streaming accesses.  They pay off only when each 128-byte line is
+// consumed by a single wave instruction, i.e. when no target/control qubit is
+// below bit 3; otherwise the lines are revisited by the partner loads and must
+// stay cached.
+template <bool NT>
+__device__ __forceinline__ double2 ld(const double2* p) {
+    if (NT) return ld_stream(p);
+    return *p;
+}
+template <bool NT>
+__device__ __forceinline__ void st(double2* p, double2 v) {
+    if (NT) st_stream(p, v);
+    else *p = v;
+}
+constexpr unsigned kLineQubits = 3;  // 128-byte line = 8 amplitudes
 __device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 

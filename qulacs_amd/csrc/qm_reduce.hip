@@ -294,19 +294,19 @@ qm_complex transition(const Term& t, void* bra, void* ket, u64 dim, void* stream
 __global__ __launch_bounds__(256) void k_axpy(double2* __restrict__ s, const double2* __restrict__ a, u64 dim,
                                               double2 coef) {
     const u64 stride = (u64)gridDim.x * blockDim.x;
-    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) s[i] = cmac(s[i], coef, a[i]);
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) st_stream(s + i, cmac(ld_stream(s + i), coef, ld_stream(a + i)));
 }
 __global__ __launch_bounds__(256) void k_add(double2* __restrict__ s, const double2* __restrict__ a, u64 dim) {
     const u64 stride = (u64)gridDim.x * blockDim.x;
-    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) s[i] = cadd(s[i], a[i]);
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) st_stream(s + i, cadd(ld_stream(s + i), ld_stream(a + i)));
 }
 __global__ __launch_bounds__(256) void k_scale_c(double2* __restrict__ s, u64 dim, double2 coef) {
     const u64 stride = (u64)gridDim.x * blockDim.x;
-    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) s[i] = cmul(s[i], coef);
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) st_stream(s + i, cmul(ld_stream(s + i), coef));
 }
 __global__ __launch_bounds__(256) void k_scale_r(double2* __restrict__ s, u64 dim, double f) {
     const u64 stride = (u64)gridDim.x * blockDim.x;
-    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) s[i] = cscale(s[i], f);
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) st_stream(s + i, cscale(ld_stream(s + i), f));
 }
 
 }  // namespace qm
