@@ -82,7 +82,7 @@ def kernel_of(g) -> str:
     return {
         "dense1": f"k_gather<1, 1, {nt}, OpDenseArg<1>>", "RX": f"k_gather<1, 1, {nt}, OpDenseArg<1>>",
         "RY": f"k_gather<1, 1, {nt}, OpDenseArg<1>>", "RZ": f"k_gather<1, 1, {nt}, OpDiag<1>>",
-        "dense2": f"k_gather<2, 1, {nt}, OpDenseArg<2>>", "dense5": f"k_dense_stream<5, {nt}, MatInPtr<5>>",
+        "dense2": f"k_gather<2, 1, {nt}, OpDenseArg<2>>", "dense5": f"k_dense_mfma<5, {nt}>",
         "CNOT": f"k_gather<1, 1, {nt}, OpSwap01>", "CZ": f"k_gather<1, 1, {nt}, OpNeg1>",
     }[g.kind]
 

@@ -24,9 +24,12 @@
 namespace qm {
 
 // ------------------------------------------------------------------ launch plumbing
+constexpr int kMaxIns = 24;  // targets + controls per gather gate
+
 template <int K>
 struct GatherArgs {
-    unsigned char ins[24];  // ascending insert positions (targets and controls)
+    u64 low[kMaxIns];       // (1 << p) - 1 for the ascending insert positions p (targets and controls)
+    unsigned char ins0;     // lowest insert position (decides streaming vs cached accesses)
     int n_ins;
     u64 cmask;              // control bits that must be 1
     u64 offs[1 << K];       // offset of tuple member x: sum of 2^targets[i] over set bits i of x
@@ -35,11 +38,12 @@ struct GatherArgs {
 template <int K>
 bool fill_args(GatherArgs<K>& a, const u32* targets, const u32* controls, const u32* cvals, u32 nc) {
     const GatherPlan p = make_plan(targets, K, controls, cvals, nc);
-    if (p.n_ins > 24) {
-        set_error(QM_ERR_UNSUPPORTED, "more than 24 target+control qubits");
+    if (p.n_ins > kMaxIns) {
+        set_error(QM_ERR_UNSUPPORTED, "more than %d target+control qubits", kMaxIns);
         return false;
     }
-    for (int i = 0; i < p.n_ins; ++i) a.ins[i] = p.ins[i];
+    for (int i = 0; i < kMaxIns; ++i) a.low[i] = i < p.n_ins ? (1ULL << p.ins[i]) - 1 : 0;
+    a.ins0 = p.n_ins ? p.ins[0] : 63;
     a.n_ins = p.n_ins;
     a.cmask = p.control_mask;
     for (int x = 0; x < (1 << K); ++x) {
@@ -51,9 +55,15 @@ bool fill_args(GatherArgs<K>& a, const u32* targets, const u32* controls, const 
     return true;
 }
 
+// Constant indices under a wave-uniform guard keep the masks in scalar registers.
 template <int K>
 __device__ __forceinline__ u64 base_of(u64 i, const GatherArgs<K>& a) {
-    for (int k = 0; k < a.n_ins; ++k) i = insert_zero(i, a.ins[k]);
+#pragma unroll
+    for (int k = 0; k < kMaxIns; ++k)
+        if (k < a.n_ins) {
+            const u64 lo = a.low[k];
+            i = (i & lo) | ((i & ~lo) << 1);
+        }
     return i | a.cmask;
 }
 
@@ -145,6 +155,136 @@ __global__ __launch_bounds__(256) void k_dense_stream(double2* __restrict__ s, u
     }
 }
 
+// Dense k-qubit gate (k = 4, 5) on the FP64 matrix cores.
+//
+// For 16 consecutive tuples (the columns of X, 2^k amplitudes each) the update
+// Y = M X is the real product [Yr; Yi] = [[Mr, -Mi], [Mi, Mr]] [Xr; Xi], issued
+// as v_mfma_f64_16x16x4f64 blocks: 16 output rows x 16 tuples per block, 4 rows
+// of X per step.  Lane layout of that instruction (checked on gfx950): lane l
+// supplies A[l % 16][l / 16] and B[l / 16][l % 16] and receives
+// D[l / 16 + 4 v][l % 16] in item v.  So lane l owns tuple l % 16 of the group,
+// loads its amplitudes x = 4 q + l / 16 (q < 2^k / 4) and stores outputs
+// y = 16 r + l / 16 + 4 v.  The matrix fragments stay in VGPRs for the whole
+// kernel; each wave walks groups of 16 tuples.
+typedef double qm_d4 __attribute__((ext_vector_type(4)));
+
+// Compact arguments (few scalar registers): insert positions packed one per
+// byte, target positions, control bits.  Member offsets are formed from the
+// target positions with scalar shifts where they are needed.
+template <int K>
+struct PackedArgs {
+    u64 pos[3];              // ascending insert positions, byte k of pos[k / 8]
+    int n_ins;
+    u64 cmask;
+    unsigned char tg[K];     // target positions, bit i of a member index <-> tg[i]
+};
+
+template <int K>
+__device__ __forceinline__ u64 base_of(u64 i, const PackedArgs<K>& a) {
+#pragma unroll
+    for (int k = 0; k < kMaxIns; ++k)
+        if (k < a.n_ins) {
+            const unsigned p = (unsigned)(a.pos[k >> 3] >> (8 * (k & 7))) & 0xffu;
+            const u64 lo = (1ULL << p) - 1;
+            i = (i & lo) | ((i & ~lo) << 1);
+        }
+    return i | a.cmask;
+}
+// offset of member x (bits of x spread over the targets)
+template <int K>
+__device__ __forceinline__ u64 member_off(unsigned x, const PackedArgs<K>& a) {
+    u64 o = 0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) o |= (u64)((x >> i) & 1u) << a.tg[i];
+    return o;
+}
+
+template <int K, bool NT>
+__global__ __launch_bounds__(256) void k_dense_mfma(double2* __restrict__ s, u64 n_bases, PackedArgs<K> a,
+                                                    const double2* __restrict__ mat) {
+    constexpr int N = 1 << K;
+    constexpr int RT = N / 16;  // 16-row output tiles
+    constexpr int KS = N / 4;   // 4-row steps over one real half of X
+    const int lane = threadIdx.x & 63;
+    const int col = lane & 15;  // A row / tuple column
+    const int kq = lane >> 4;   // k within a step
+    // A fragments in LDS, stored in lane order so each wave's read of one
+    // fragment is 64 consecutive double2 (no bank conflicts); this keeps the
+    // matrix out of VGPRs and leaves room for more resident waves.
+    __shared__ double2 frag[RT * KS * 64];
+    for (int e = threadIdx.x; e < RT * KS * 64; e += blockDim.x) {
+        const int l = e & 63, q = (e >> 6) % KS, rt = (e >> 6) / KS;
+        frag[e] = mat[(16 * rt + (l & 15)) * N + 4 * q + (l >> 4)];
+    }
+    __syncthreads();
+    const u64 groups = (n_bases + 15) >> 4;
+    const u64 wave = (u64)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const u64 nwaves = (u64)gridDim.x * (blockDim.x >> 6);
+    // Member x = 4 q + kq has offset offs[4 q] + offs[kq] (disjoint target bits):
+    // the first term is wave-uniform (scalar), the second is one per-lane value.
+    const u64 lane_off = member_off<K>((unsigned)kq, a);
+    // Software pipeline: the next group's loads are in flight while this
+    // group's MFMAs run (waves are persistent, so the matrix is loaded once).
+    // Lanes past the end load a valid duplicate tuple and do not store.
+    double nr[KS], ni[KS];
+    u64 nb = 0;
+    bool nvalid = false;
+    auto fetch = [&](u64 g) {
+        const u64 i = (g << 4) + col;
+        nvalid = i < n_bases;
+        nb = base_of<K>(nvalid ? i : n_bases - 1, a) + lane_off;
+#pragma unroll
+        for (int q = 0; q < KS; ++q) {
+            const double2 v = ld<NT>(s + nb + member_off<K>(4u * q, a));
+            nr[q] = v.x;
+            ni[q] = v.y;
+        }
+    };
+    if (wave < groups) fetch(wave);
+    for (u64 g = wave; g < groups; g += nwaves) {  // wave-uniform loop: every lane runs every MFMA
+        double br[KS], bi[KS];
+#pragma unroll
+        for (int q = 0; q < KS; ++q) {
+            br[q] = nr[q];
+            bi[q] = ni[q];
+        }
+        const u64 b = nb;
+        const bool valid = nvalid;
+        if (g + nwaves < groups) fetch(g + nwaves);
+        qm_d4 dr[RT], di[RT];
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+            dr[rt] = qm_d4{0.0, 0.0, 0.0, 0.0};
+            di[rt] = qm_d4{0.0, 0.0, 0.0, 0.0};
+        }
+        // 2 * RT independent accumulators between dependent MFMAs
+#pragma unroll
+        for (int q = 0; q < KS; ++q) {
+            const double nbi = -bi[q];  // Yr -= Mi Xi, negated on the B side
+            double2 m[RT];
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) m[rt] = frag[(rt * KS + q) * 64 + lane];  // (Mr, Mi) fragment
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) {
+                dr[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(m[rt].x, br[q], dr[rt], 0, 0, 0);
+                di[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(m[rt].y, br[q], di[rt], 0, 0, 0);
+            }
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) {
+                dr[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(m[rt].y, nbi, dr[rt], 0, 0, 0);
+                di[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(m[rt].x, bi[q], di[rt], 0, 0, 0);
+            }
+        }
+        if (valid) {
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    st<NT>(s + b + member_off<K>(16u * rt + 4u * v, a), make_double2(dr[rt][v], di[rt][v]));
+        }
+    }
+}
+
 struct OpSwap01 {  // X, CNOT
     static constexpr unsigned READ = 3, WRITE = 3;
     __device__ __forceinline__ void apply(double2* v) const {
@@ -215,7 +355,7 @@ void launch_gather(double2* s, u64 dim, const u32* targets, const u32* controls,
     const u64 n_bases = dim >> (K + nc);
     const unsigned block = 512;
     const unsigned grid = grid_for(n_bases, (u64)block * U);
-    if (a.n_ins == 0 || a.ins[0] >= kLineQubits)
+    if (a.ins0 >= kLineQubits)
         k_gather<K, U, true, Op><<<grid, block, 0, st>>>(s, n_bases, a, op);
     else
         k_gather<K, U, false, Op><<<grid, block, 0, st>>>(s, n_bases, a, op);
@@ -227,10 +367,34 @@ void launch_dense_stream(double2* s, u64 dim, const u32* targets, const u32* con
     GatherArgs<K> a;
     if (!fill_args<K>(a, targets, controls, cvals, nc)) return;
     const u64 n_bases = dim >> (K + nc);
-    if (a.ins[0] >= kLineQubits)
+    if (a.ins0 >= kLineQubits)
         k_dense_stream<K, true, M><<<grid_for(n_bases, 256), 256, 0, st>>>(s, n_bases, a, m);
     else
         k_dense_stream<K, false, M><<<grid_for(n_bases, 256), 256, 0, st>>>(s, n_bases, a, m);
+}
+
+template <int K>
+void launch_dense_mfma(double2* s, u64 dim, const u32* targets, const u32* controls, const u32* cvals, u32 nc,
+                       const double2* dmat, hipStream_t st) {
+    const GatherPlan p = make_plan(targets, K, controls, cvals, nc);
+    if (p.n_ins > kMaxIns) {
+        set_error(QM_ERR_UNSUPPORTED, "more than %d target+control qubits", kMaxIns);
+        return;
+    }
+    PackedArgs<K> a;
+    a.pos[0] = a.pos[1] = a.pos[2] = 0;
+    for (int k = 0; k < p.n_ins; ++k) a.pos[k >> 3] |= (u64)p.ins[k] << (8 * (k & 7));
+    a.n_ins = p.n_ins;
+    a.cmask = p.control_mask;
+    for (int i = 0; i < K; ++i) a.tg[i] = (unsigned char)targets[i];
+    const u64 n_bases = dim >> (K + nc);
+    const u64 groups = (n_bases + 15) >> 4;
+    // persistent waves, about one resident set (256 CUs x 4 SIMDs, 4 waves per block)
+    const unsigned grid = grid_for(groups, 4, K == 5 ? 768 : 1792);
+    if (p.ins[0] >= kLineQubits)
+        k_dense_mfma<K, true><<<grid, 256, 0, st>>>(s, n_bases, a, dmat);
+    else
+        k_dense_mfma<K, false><<<grid, 256, 0, st>>>(s, n_bases, a, dmat);
 }
 
 // Validates state/dim/targets/controls; returns n (qubit count) or -1.
@@ -301,9 +465,9 @@ void dense_any(const u32* targets, u32 nt, const u32* controls, const u32* cvals
             const double2* dm = (const double2*)upload(sc, matrix, nel * sizeof(double2));
             if (!dm) return;
             if (nt == 4)
-                launch_dense_stream<4>(s, dim, targets, controls, cvals, nc, MatInPtr<4>{dm}, st);
+                launch_dense_mfma<4>(s, dim, targets, controls, cvals, nc, dm, st);
             else
-                launch_dense_stream<5>(s, dim, targets, controls, cvals, nc, MatInPtr<5>{dm}, st);
+                launch_dense_mfma<5>(s, dim, targets, controls, cvals, nc, dm, st);
             break;
         }
         default: {
