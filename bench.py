@@ -75,6 +75,25 @@ class HipEvents:
             self.h.hipEventDestroy(e)
 
 
+TRAFFIC_PROFILE = ROOT / "profiles" / "hbm_traffic_gate_sweep_30q.md"
+
+
+def measured_traffic(kernel: str, qubits: int):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE / WRITE_SIZE
+    summary (calibrated against a copy of known size), or None when that summary
+    does not cover this kernel at this size."""
+    if qubits != 30 or not TRAFFIC_PROFILE.exists():
+        return None
+    want = kernel.replace(" ", "").replace(">>", ">")
+    for line in TRAFFIC_PROFILE.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 7 and cells[0].startswith("`"):
+            name = cells[0].strip("`").replace(" ", "").replace(">>", ">")
+            if name == want:
+                return round(float(cells[4]) * 1e9)
+    return None
+
+
 def kernel_of(g) -> str:
     """Name of the kernel instantiation that executes gate g (qulacs_amd/csrc/qm_gates.hip): the
     library streams with non-temporal accesses when no acted-on qubit is below bit 3."""
@@ -167,7 +186,8 @@ def sv_sweep(args, lib_mod, hip):
         "value": value, "ms_per_step": ms_per_step, "gates": gates, "per_kind": per_kind, "per_kernel": per_kernel,
         "norm_after": norm,
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                     "traffic": measured_traffic(dom_name, n),
                      "bytes_per_launch": per_launch, "mean_launch_ms": dom["mean_ms"]},
         "hbm_copy_GBps": round(2 * half / (copy_ms * 1e-3) / 1e9, 1),
     }
