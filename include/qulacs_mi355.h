@@ -230,6 +230,7 @@ QM_EXPORT void dm_superoperator_host(const UINT* target_qubit_index_list, UINT t
  * src/cppsim/gate_factory.cpp:454-467) in one pass. */
 QM_EXPORT void dm_depolarizing_host(UINT target_qubit_index, double p, void* state, ITYPE dim, void* stream, unsigned int device_number);
 QM_EXPORT double dm_state_norm_squared_host(void* state, ITYPE dim, void* stream, unsigned int device_number); /* stat_ops_dm.cpp:12-26 */
+QM_EXPORT double dm_measurement_distribution_entropy_host(void* state, ITYPE dim, void* stream, unsigned int device_number); /* stat_ops_dm.cpp:28-47 */
 QM_EXPORT double dm_M0_prob_host(UINT target_qubit_index, void* state, ITYPE dim, void* stream, unsigned int device_number); /* stat_ops_dm.cpp:49-63 */
 QM_EXPORT double dm_marginal_prob_host(const UINT* sorted_target_qubit_index_list, const UINT* measured_value_list, UINT target_qubit_index_count, void* state, ITYPE dim, void* stream, unsigned int device_number); /* stat_ops_dm.cpp:89-114 */
 QM_EXPORT double dm_expectation_value_multi_qubit_Pauli_operator_partial_list_host(const UINT* target_qubit_index_list, const UINT* Pauli_operator_type_list, UINT target_qubit_index_count, void* state, ITYPE dim, void* stream, unsigned int device_number); /* stat_ops_dm.cpp:164-191 */

@@ -30,6 +30,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <random>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -540,6 +541,20 @@ void oracle_sampling(const double* state, ITYPE dim, const double* uniforms, UIN
         auto it = std::lower_bound(cdf.begin(), cdf.end(), uniforms[c]);
         out[c] = (unsigned long long)(std::distance(cdf.begin(), it) - 1);
     }
+}
+
+// The reference's Random (src/cppsim/utility.hpp:54-102): std::mt19937_64
+// seeded with `seed`, uniforms from uniform_real_distribution<double>(0, 1) of
+// the same C++ library.  The raw stream is pinned by the C++ standard's check
+// value (10000th output of the default seed 5489 is 9981545732273789042).
+void oracle_mt64_uniforms(unsigned long long seed, UINT count, double* out) {
+    std::mt19937_64 mt(seed);
+    std::uniform_real_distribution<double> u(0.0, 1.0);
+    for (UINT c = 0; c < count; ++c) out[c] = u(mt);
+}
+void oracle_mt64_raw(unsigned long long seed, UINT count, unsigned long long* out) {
+    std::mt19937_64 mt(seed);
+    for (UINT c = 0; c < count; ++c) out[c] = mt();
 }
 
 // src/cppsim/state_dm.hpp:383-408: same loop over |rho_ii| (std::abs of the

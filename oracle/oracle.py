@@ -235,6 +235,16 @@ class Oracle(_Kernels):
     def dm_depolarizing(self, t: int, p: float, rho: np.ndarray, dim: int) -> None:
         self.f("dm_depolarizing")(_u(t), _d(p), _ptr(rho), _ull(dim))
 
+    def mt64_uniforms(self, seed: int, count: int) -> np.ndarray:
+        out = np.zeros(count, dtype=np.float64)
+        self.f("mt64_uniforms")(_ull(seed), _u(count), _ptr(out))
+        return out
+
+    def mt64_raw(self, seed: int, count: int) -> np.ndarray:
+        out = np.zeros(count, dtype=np.uint64)
+        self.f("mt64_raw")(_ull(seed), _u(count), _ptr(out))
+        return out
+
     def whole_diagonal(self, diag, psi: np.ndarray) -> None:
         d = _c128(diag)
         self.f("whole_state_diagonal_gate")(_ptr(d), _ptr(psi), _ull(psi.size))

@@ -1,0 +1,62 @@
+"""qulacs_amd: qulacs's state-vector / density-matrix simulator API with every
+state resident in MI355X HBM and every update run by HIP kernels (complex128).
+
+Drop-in names (python/cppsim_wrapper.cpp in the reference):
+
+    from qulacs_amd import QuantumState, DensityMatrix, QuantumCircuit, Observable
+    from qulacs_amd import gate
+    from qulacs_amd.state import inner_product
+
+``QuantumState`` and ``QuantumStateGpu`` both name the GPU state vector
+(``QuantumStateMI355``); ``DensityMatrix`` names ``DensityMatrixMI355``.
+Importing the package loads the compiled module ``_core`` and through it
+``libqulacs_mi355.so``; there is no CPU fallback, so a missing build raises
+ImportError here.
+"""
+from __future__ import annotations
+
+import sys as _sys
+
+try:
+    from . import _core
+except ImportError as e:  # pragma: no cover - exercised only without a build
+    raise ImportError(
+        "qulacs_amd: the compiled module is missing; run qulacs_amd/build.py "
+        f"(or __graft_entry__.build()) first ({e})"
+    ) from e
+
+from ._core import (  # noqa: F401
+    ClsPauliGate,
+    DensityMatrixMI355,
+    GeneralQuantumOperator,
+    GradCalculator,
+    Observable,
+    ParametricQuantumCircuit,
+    PauliOperator,
+    QuantumCircuit,
+    QuantumGate_CPTP,
+    QuantumGate_Probabilistic,
+    QuantumGate_SingleParameter,
+    QuantumGateBase,
+    QuantumGateDiagonalMatrix,
+    QuantumGateMatrix,
+    QuantumStateBase,
+    QuantumStateMI355,
+    QulacsError,
+    gate,
+    get_num_device,
+    get_synchronous,
+    set_synchronous,
+    state,
+    synchronize,
+)
+
+QuantumState = QuantumStateMI355
+QuantumStateGpu = QuantumStateMI355
+DensityMatrix = DensityMatrixMI355
+
+# `import qulacs_amd.gate` / `from qulacs_amd.state import inner_product`
+_sys.modules[__name__ + ".gate"] = gate
+_sys.modules[__name__ + ".state"] = state
+
+__version__ = _core.__version__

@@ -115,6 +115,7 @@ _SIGS: dict[str, tuple] = {
     "dm_superoperator_host": (None, [PU, UINT, PC, VP, ITYPE, VP, UINT]),
     "dm_depolarizing_host": (None, [UINT, D, VP, ITYPE, VP, UINT]),
     "dm_state_norm_squared_host": (D, [VP, ITYPE, VP, UINT]),
+    "dm_measurement_distribution_entropy_host": (D, [VP, ITYPE, VP, UINT]),
     "dm_M0_prob_host": (D, [UINT, VP, ITYPE, VP, UINT]),
     "dm_marginal_prob_host": (D, [PU, PU, UINT, VP, ITYPE, VP, UINT]),
     "dm_expectation_value_multi_qubit_Pauli_operator_partial_list_host": (D, [PU, PU, UINT, VP, ITYPE, VP, UINT]),

@@ -1,9 +1,13 @@
-"""Builds libqulacs_mi355.so (HIP kernels + C ABI) for gfx950, in-tree.
+"""Builds libqulacs_mi355.so (HIP kernels + C ABI) for gfx950 and the Python
+module qulacs_amd._core over it, in-tree.
 
 Every translation unit under qulacs_amd/csrc is compiled with hipcc (``.hip``)
 or g++ (``.cpp``) into qulacs_amd/_build/, then linked into
-qulacs_amd/libqulacs_mi355.so, which the package loads with ctypes.  The build
-always recompiles; it never falls back to a prebuilt or CPU library.
+qulacs_amd/libqulacs_mi355.so.  The host-side qulacs API (csrc/host: C++ classes
+over the C ABI plus their pybind11 bindings) is linked into
+qulacs_amd/_core<ext-suffix>.so, which finds the library next to itself through
+an $ORIGIN run path.  The build always recompiles; it never falls back to a
+prebuilt or CPU library.
 """
 from __future__ import annotations
 
@@ -11,6 +15,7 @@ import os
 import shutil
 import subprocess
 import sys
+import sysconfig
 from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
@@ -18,6 +23,8 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libqulacs_mi355.so"
+HOST = CSRC / "host"
+CORE = PKG / ("_core" + sysconfig.get_config_var("EXT_SUFFIX"))
 ROOT = PKG.parent
 
 ARCH = os.environ.get("QM_OFFLOAD_ARCH", "gfx950")
@@ -47,8 +54,15 @@ def _run(cmd: list[str], verbose: bool) -> None:
         print(out, flush=True)
 
 
+def _pybind_includes() -> list[str]:
+    import pybind11
+
+    return [f"-I{pybind11.get_include()}", f"-I{sysconfig.get_paths()['include']}"]
+
+
 def build(verbose: bool = False, jobs: int | None = None) -> Path:
-    """Compile every source and link the shared library.  Returns its path."""
+    """Compile every source, link the shared library and the Python module.
+    Returns the library's path."""
     BUILD.mkdir(exist_ok=True)
     srcs = sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
     if not srcs:
@@ -62,12 +76,22 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
             cmds.append([HIPCC, *HIP_FLAGS, "-c", str(src), "-o", str(obj)])
         else:
             cmds.append([CXX, *CXX_FLAGS, "-c", str(src), "-o", str(obj)])
+    host_objs = []
+    for src in sorted(HOST.glob("*.cpp")):
+        obj = BUILD / ("host_" + src.name + ".o")
+        host_objs.append(obj)
+        cmds.append([CXX, *CXX_FLAGS, *_pybind_includes(), "-c", str(src), "-o", str(obj)])
     jobs = jobs or min(8, os.cpu_count() or 4)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(lambda c: _run(c, verbose), cmds))
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)], verbose)
     os.replace(tmp, LIB)
+    tmp = CORE.with_suffix(".tmp")
+    _run([CXX, "-shared", "-fPIC", "-o", str(tmp), *map(str, host_objs), f"-L{PKG}", "-lqulacs_mi355",
+          "-Wl,-rpath,$ORIGIN", "-Wl,--no-undefined", f"-L{sysconfig.get_config_var('LIBDIR')}",
+          f"-lpython{sysconfig.get_config_var('LDVERSION')}"], verbose)
+    os.replace(tmp, CORE)
     return LIB
 
 

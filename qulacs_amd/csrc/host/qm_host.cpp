@@ -1,0 +1,1263 @@
+// Implementation of the host-side qulacs API over the C ABI (see qm_host.hpp).
+#include "qm_host.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <sstream>
+
+namespace qmh {
+
+namespace {
+inline const qm_complex* qc(const CPPCTYPE* p) { return reinterpret_cast<const qm_complex*>(p); }
+inline qm_complex qc(CPPCTYPE z) { return qm_complex{z.real(), z.imag()}; }
+inline CPPCTYPE cc(qm_complex z) { return CPPCTYPE(z.re, z.im); }
+const CPPCTYPE I_(0.0, 1.0);
+
+ComplexMatrix pauli_matrix(unsigned p) {
+    ComplexMatrix m(2, 2);
+    switch (p) {
+        case 0: m(0, 0) = 1; m(1, 1) = 1; break;
+        case 1: m(0, 1) = 1; m(1, 0) = 1; break;
+        case 2: m(0, 1) = -I_; m(1, 0) = I_; break;
+        case 3: m(0, 0) = 1; m(1, 1) = -1; break;
+        default: throw QulacsError("Pauli id must be 0..3");
+    }
+    return m;
+}
+
+// Kronecker product in qulacs target order: bit i of the index <-> factor i.
+ComplexMatrix pauli_product(const std::vector<unsigned>& paulis) {
+    const size_t k = paulis.size(), d = size_t(1) << k;
+    ComplexMatrix m(d, d);
+    for (size_t y = 0; y < d; ++y)
+        for (size_t x = 0; x < d; ++x) {
+            CPPCTYPE v = 1.0;
+            for (size_t i = 0; i < k; ++i) v *= pauli_matrix(paulis[i])((y >> i) & 1, (x >> i) & 1);
+            m(y, x) = v;
+        }
+    return m;
+}
+
+// Operator of `m` (over `targets`) embedded into the ordered qubit list `onto`.
+ComplexMatrix embed(const ComplexMatrix& m, const std::vector<unsigned>& targets, const std::vector<unsigned>& onto) {
+    const size_t k = onto.size(), d = size_t(1) << k;
+    std::vector<unsigned> pos(targets.size());
+    for (size_t i = 0; i < targets.size(); ++i) {
+        auto it = std::find(onto.begin(), onto.end(), targets[i]);
+        if (it == onto.end()) throw QulacsError("embed: target not in qubit list");
+        pos[i] = (unsigned)(it - onto.begin());
+    }
+    size_t tmask = 0;
+    for (unsigned p : pos) tmask |= size_t(1) << p;
+    ComplexMatrix out(d, d);
+    for (size_t col = 0; col < d; ++col) {
+        size_t x = 0;
+        for (size_t i = 0; i < pos.size(); ++i) x |= ((col >> pos[i]) & 1) << i;
+        const size_t base = col & ~tmask;
+        for (size_t y = 0; y < m.rows; ++y) {
+            size_t row = base;
+            for (size_t i = 0; i < pos.size(); ++i) row |= ((y >> i) & 1) << pos[i];
+            out(row, col) += m(y, x);
+        }
+    }
+    return out;
+}
+
+// S = sum_i w_i K_i (x) conj(K_i); member = row bits * 2^k + column bits.
+void accumulate_superop(ComplexMatrix& S, const ComplexMatrix& K, double w) {
+    const size_t d = K.rows;
+    for (size_t oy = 0; oy < d; ++oy)
+        for (size_t ox = 0; ox < d; ++ox)
+            for (size_t iy = 0; iy < d; ++iy)
+                for (size_t ix = 0; ix < d; ++ix)
+                    S(oy * d + ox, iy * d + ix) += w * K(oy, iy) * std::conj(K(ox, ix));
+}
+
+std::vector<unsigned> sorted_union(const std::vector<std::vector<unsigned>>& lists) {
+    std::vector<unsigned> u;
+    for (const auto& l : lists) u.insert(u.end(), l.begin(), l.end());
+    std::sort(u.begin(), u.end());
+    u.erase(std::unique(u.begin(), u.end()), u.end());
+    return u;
+}
+
+QuantumStateMI355* as_sv(QuantumStateBase* s) { return dynamic_cast<QuantumStateMI355*>(s); }
+DensityMatrixMI355* as_dm(QuantumStateBase* s) { return dynamic_cast<DensityMatrixMI355*>(s); }
+
+void check_targets(const QuantumStateBase* s, const std::vector<unsigned>& qs, const char* what) {
+    for (unsigned q : qs)
+        if (q >= s->qubit_count)
+            throw QulacsError(std::string(what) + ": qubit index " + std::to_string(q) + " >= qubit count " +
+                              std::to_string(s->qubit_count));
+}
+}  // namespace
+
+void check_abi(const char* where) {
+    char buf[512];
+    const int code = qm_get_last_error(buf, sizeof buf);
+    if (code != QM_OK) {
+        qm_clear_last_error();
+        throw QulacsError(std::string(where) + ": " + buf);
+    }
+}
+
+ComplexMatrix ComplexMatrix::identity(size_t n) {
+    ComplexMatrix m(n, n);
+    for (size_t i = 0; i < n; ++i) m(i, i) = 1.0;
+    return m;
+}
+ComplexMatrix ComplexMatrix::operator*(const ComplexMatrix& o) const {
+    if (cols != o.rows) throw QulacsError("matrix shape mismatch");
+    ComplexMatrix r(rows, o.cols);
+    for (size_t i = 0; i < rows; ++i)
+        for (size_t k = 0; k < cols; ++k) {
+            const CPPCTYPE a = (*this)(i, k);
+            if (a == CPPCTYPE(0)) continue;
+            for (size_t j = 0; j < o.cols; ++j) r(i, j) += a * o(k, j);
+        }
+    return r;
+}
+ComplexMatrix ComplexMatrix::adjoint() const {
+    ComplexMatrix r(cols, rows);
+    for (size_t i = 0; i < rows; ++i)
+        for (size_t j = 0; j < cols; ++j) r(j, i) = std::conj((*this)(i, j));
+    return r;
+}
+
+Random::Random() {
+    std::random_device rd;
+    mt_.seed(rd());
+}
+
+// ------------------------------------------------------------------ QuantumStateBase
+QuantumStateBase::QuantumStateBase(unsigned qc_, ITYPE dim_, unsigned device)
+    : qubit_count(qc_), dim(dim_), device_number(device) {
+    if (qc_ == 0 || qc_ > 40) throw QulacsError("qubit count must be in 1..40");
+    stream_ = allocate_cuda_stream_host(1, device);
+    check_abi("allocate_cuda_stream_host");
+}
+
+QuantumStateBase::~QuantumStateBase() {
+    if (data_) release_quantum_state_host(data_, device_number);
+    if (stream_) release_cuda_stream_host(stream_, 1, device_number);
+    qm_clear_last_error();
+}
+
+void QuantumStateBase::load(const QuantumStateBase* o) {
+    if (o->element_count() != element_count() || o->is_state_vector() != is_state_vector())
+        throw QulacsError("load: state kind or size mismatch");
+    qm_synchronize(o->get_cuda_stream(), o->device_number);  // other stream's pending work first
+    copy_quantum_state_from_device_to_device(data_, o->data(), element_count(), stream_, device_number);
+    check_abi("load");
+    classical_register_ = o->classical_register_;
+}
+
+std::vector<CPPCTYPE> QuantumStateBase::duplicate_data_cpp() const {
+    std::vector<CPPCTYPE> h(element_count());
+    get_quantum_state_host(data_, h.data(), element_count(), stream_, device_number);
+    check_abi("duplicate_data_cpp");
+    return h;
+}
+
+void QuantumStateBase::add_state(const QuantumStateBase* o) { add_state_with_coef(1.0, o); }
+
+void QuantumStateBase::add_state_with_coef(CPPCTYPE coef, const QuantumStateBase* o) {
+    if (o->element_count() != element_count()) throw QulacsError("add_state: size mismatch");
+    qm_synchronize(o->get_cuda_stream(), o->device_number);
+    qm_state_add_with_coef_host(qc(coef), o->data(), data_, element_count(), stream_, device_number);
+    check_abi("add_state_with_coef");
+}
+
+void QuantumStateBase::multiply_coef(CPPCTYPE coef) {
+    state_multiply_host(qc(coef), data_, element_count(), stream_, device_number);
+    check_abi("multiply_coef");
+}
+
+std::vector<ITYPE> QuantumStateBase::sampling(unsigned count, unsigned seed) {
+    random_.set_seed(seed);  // src/cppsim/state.hpp:933-936
+    return sampling(count);
+}
+
+unsigned QuantumStateBase::get_classical_value(unsigned index) {
+    if (index >= classical_register_.size()) classical_register_.resize(index + 1, 0);
+    return classical_register_[index];
+}
+void QuantumStateBase::set_classical_value(unsigned index, unsigned value) {
+    if (index >= classical_register_.size()) classical_register_.resize(index + 1, 0);
+    classical_register_[index] = value;
+}
+
+// ------------------------------------------------------------------ state vector
+QuantumStateMI355::QuantumStateMI355(unsigned n, unsigned device) : QuantumStateBase(n, 1ULL << n, device) {
+    data_ = allocate_quantum_state_host(dim, device);
+    check_abi("allocate_quantum_state_host");
+    set_zero_state();
+}
+void QuantumStateMI355::set_zero_state() {
+    initialize_quantum_state_host(data_, dim, stream_, device_number);
+    check_abi("set_zero_state");
+}
+void QuantumStateMI355::set_computational_basis(ITYPE b) {
+    if (b >= dim) throw QulacsError("set_computational_basis: index out of range");
+    set_computational_basis_host(b, data_, dim, stream_, device_number);
+    check_abi("set_computational_basis");
+}
+void QuantumStateMI355::set_Haar_random_state() { set_Haar_random_state(random_.int32()); }
+void QuantumStateMI355::set_Haar_random_state(unsigned seed) {
+    initialize_Haar_random_state_with_seed_host(data_, dim, seed, stream_, device_number);
+    check_abi("set_Haar_random_state");
+}
+double QuantumStateMI355::get_zero_probability(unsigned t) const {
+    if (t >= qubit_count) throw QulacsError("get_zero_probability: qubit index out of range");
+    const double v = M0_prob_host(t, data_, dim, stream_, device_number);
+    check_abi("get_zero_probability");
+    return v;
+}
+double QuantumStateMI355::get_marginal_probability(const std::vector<unsigned>& mv) const {
+    if (mv.size() != qubit_count) throw QulacsError("get_marginal_probability: one value per qubit expected");
+    std::vector<UINT> t, v;
+    for (unsigned q = 0; q < qubit_count; ++q)
+        if (mv[q] == 0 || mv[q] == 1) {
+            t.push_back(q);
+            v.push_back(mv[q]);
+        }
+    const double r = marginal_prob_host(t.data(), v.data(), (UINT)t.size(), data_, dim, stream_, device_number);
+    check_abi("get_marginal_probability");
+    return r;
+}
+double QuantumStateMI355::get_entropy() const {
+    const double v = measurement_distribution_entropy_host(data_, dim, stream_, device_number);
+    check_abi("get_entropy");
+    return v;
+}
+double QuantumStateMI355::get_squared_norm() const {
+    const double v = state_norm_squared_host(data_, dim, stream_, device_number);
+    check_abi("get_squared_norm");
+    return v;
+}
+void QuantumStateMI355::normalize(double sq) {
+    normalize_host(sq, data_, dim, stream_, device_number);
+    check_abi("normalize");
+}
+QuantumStateBase* QuantumStateMI355::allocate_buffer() const { return new QuantumStateMI355(qubit_count, device_number); }
+QuantumStateBase* QuantumStateMI355::copy() const {
+    auto* s = new QuantumStateMI355(qubit_count, device_number);
+    s->load(this);
+    return s;
+}
+void QuantumStateMI355::load(const std::vector<CPPCTYPE>& h) {
+    if (h.size() != dim) throw QulacsError("load: vector size must equal 2^qubit_count");
+    copy_quantum_state_from_host_to_device(data_, h.data(), dim, stream_, device_number);
+    check_abi("load");
+}
+std::vector<ITYPE> QuantumStateMI355::sampling_with_uniforms(const std::vector<double>& u) const {
+    std::vector<ITYPE> out(u.size());
+    if (u.empty()) return out;
+    qm_sampling_host(u.data(), (UINT)u.size(), out.data(), data_, dim, stream_, device_number);
+    check_abi("sampling");
+    return out;
+}
+std::vector<ITYPE> QuantumStateMI355::sampling(unsigned count) {
+    std::vector<double> u(count);
+    for (auto& x : u) x = random_.uniform();  // src/cppsim/state.hpp:924
+    return sampling_with_uniforms(u);
+}
+std::string QuantumStateMI355::to_string() const {
+    std::ostringstream os;
+    os << " *** Quantum State ***\n * Qubit Count : " << qubit_count << "\n * Dimension   : " << dim
+       << "\n * State vector : \n";
+    const auto h = duplicate_data_cpp();
+    for (ITYPE i = 0; i < std::min<ITYPE>(dim, 64); ++i) os << h[i] << "\n";
+    if (dim > 64) os << "...\n";
+    return os.str();
+}
+
+CPPCTYPE inner_product(const QuantumStateMI355* bra, const QuantumStateMI355* ket) {
+    if (bra->dim != ket->dim) throw QulacsError("inner_product: dimension mismatch");
+    qm_synchronize(bra->get_cuda_stream(), bra->device_number);
+    const qm_complex z = inner_product_host(bra->data(), ket->data(), ket->dim, ket->get_cuda_stream(), ket->device_number);
+    check_abi("inner_product");
+    return cc(z);
+}
+
+// ------------------------------------------------------------------ density matrix
+DensityMatrixMI355::DensityMatrixMI355(unsigned n, unsigned device) : QuantumStateBase(n, 1ULL << n, device) {
+    if (n > 20) throw QulacsError("DensityMatrix: at most 20 qubits");
+    data_ = allocate_quantum_state_host(dim * dim, device);
+    check_abi("allocate_quantum_state_host");
+    set_zero_state();
+}
+void DensityMatrixMI355::set_zero_state() {
+    dm_initialize_quantum_state_host(data_, dim, stream_, device_number);
+    check_abi("set_zero_state");
+}
+void DensityMatrixMI355::set_computational_basis(ITYPE b) {
+    if (b >= dim) throw QulacsError("set_computational_basis: index out of range");
+    set_computational_basis_host(b * dim + b, data_, dim * dim, stream_, device_number);
+    check_abi("set_computational_basis");
+}
+void DensityMatrixMI355::set_Haar_random_state() { set_Haar_random_state(random_.int32()); }
+void DensityMatrixMI355::set_Haar_random_state(unsigned seed) {
+    QuantumStateMI355 pure(qubit_count, device_number);
+    pure.set_Haar_random_state(seed);
+    load_pure(&pure);
+}
+void DensityMatrixMI355::load_pure(const QuantumStateMI355* p) {
+    if (p->dim != dim) throw QulacsError("load: dimension mismatch");
+    qm_synchronize(p->get_cuda_stream(), p->device_number);
+    dm_initialize_with_pure_state_host(data_, p->data(), dim, stream_, device_number);
+    check_abi("load");
+}
+double DensityMatrixMI355::get_zero_probability(unsigned t) const {
+    if (t >= qubit_count) throw QulacsError("get_zero_probability: qubit index out of range");
+    const double v = dm_M0_prob_host(t, data_, dim, stream_, device_number);
+    check_abi("get_zero_probability");
+    return v;
+}
+double DensityMatrixMI355::get_marginal_probability(const std::vector<unsigned>& mv) const {
+    if (mv.size() != qubit_count) throw QulacsError("get_marginal_probability: one value per qubit expected");
+    std::vector<UINT> t, v;
+    for (unsigned q = 0; q < qubit_count; ++q)
+        if (mv[q] == 0 || mv[q] == 1) {
+            t.push_back(q);
+            v.push_back(mv[q]);
+        }
+    const double r = dm_marginal_prob_host(t.data(), v.data(), (UINT)t.size(), data_, dim, stream_, device_number);
+    check_abi("get_marginal_probability");
+    return r;
+}
+double DensityMatrixMI355::get_entropy() const {
+    const double v = dm_measurement_distribution_entropy_host(data_, dim, stream_, device_number);
+    check_abi("get_entropy");
+    return v;
+}
+double DensityMatrixMI355::get_squared_norm() const {
+    const double v = dm_state_norm_squared_host(data_, dim, stream_, device_number);
+    check_abi("get_squared_norm");
+    return v;
+}
+void DensityMatrixMI355::normalize(double sq) {
+    dm_normalize_host(sq, data_, dim, stream_, device_number);
+    check_abi("normalize");
+}
+QuantumStateBase* DensityMatrixMI355::allocate_buffer() const { return new DensityMatrixMI355(qubit_count, device_number); }
+QuantumStateBase* DensityMatrixMI355::copy() const {
+    auto* s = new DensityMatrixMI355(qubit_count, device_number);
+    s->load(this);
+    return s;
+}
+void DensityMatrixMI355::load(const std::vector<CPPCTYPE>& h) {
+    if (h.size() == dim) {
+        QuantumStateMI355 pure(qubit_count, device_number);
+        pure.load(h);
+        load_pure(&pure);
+    } else if (h.size() == dim * dim) {
+        copy_quantum_state_from_host_to_device(data_, h.data(), dim * dim, stream_, device_number);
+        check_abi("load");
+    } else {
+        throw QulacsError("load: expected 2^n (pure state) or 4^n (matrix) entries");
+    }
+}
+std::vector<ITYPE> DensityMatrixMI355::sampling_with_uniforms(const std::vector<double>& u) const {
+    std::vector<ITYPE> out(u.size());
+    if (u.empty()) return out;
+    dm_sampling_host(u.data(), (UINT)u.size(), out.data(), data_, dim, stream_, device_number);
+    check_abi("sampling");
+    return out;
+}
+std::vector<ITYPE> DensityMatrixMI355::sampling(unsigned count) {
+    std::vector<double> u(count);
+    for (auto& x : u) x = random_.uniform();
+    return sampling_with_uniforms(u);
+}
+std::string DensityMatrixMI355::to_string() const {
+    std::ostringstream os;
+    os << " *** Density Matrix ***\n * Qubit Count : " << qubit_count << "\n * Dimension   : " << dim
+       << "\n * Density matrix : \n";
+    const auto h = duplicate_data_cpp();
+    const ITYPE show = std::min<ITYPE>(dim, 16);
+    for (ITYPE y = 0; y < show; ++y) {
+        for (ITYPE x = 0; x < show; ++x) os << h[y * dim + x] << " ";
+        os << "\n";
+    }
+    return os.str();
+}
+
+// ------------------------------------------------------------------ gates
+QuantumGateBase* QuantumGateBase::get_inverse() const {
+    return new QuantumGateMatrix(targets_, get_matrix().adjoint(), controls_, cvals_, name_ + "^dagger");
+}
+
+std::string QuantumGateBase::to_string() const {
+    std::ostringstream os;
+    os << " *** gate info *** \n * gate name : " << get_name() << "\n * target    : \n";
+    for (unsigned t : targets_) os << " " << t << " : commute   \n";
+    os << " * control   : \n";
+    for (size_t i = 0; i < controls_.size(); ++i) os << " " << controls_[i] << " : value " << cvals_[i] << "\n";
+    os << " * Pauli     : no \n * Clifford  : no \n * Gaussian  : no \n * Parametric: " << (is_parametric() ? "yes" : "no")
+       << " \n * Diagonal  : no \n";
+    return os.str();
+}
+
+QuantumGateMatrix::QuantumGateMatrix(std::vector<unsigned> t, ComplexMatrix m, std::vector<unsigned> c,
+                                     std::vector<unsigned> v, std::string name)
+    : matrix_(std::move(m)) {
+    targets_ = std::move(t);
+    controls_ = std::move(c);
+    cvals_ = std::move(v);
+    name_ = std::move(name);
+    if (cvals_.size() != controls_.size()) throw QulacsError("one control value per control qubit");
+    const size_t d = size_t(1) << targets_.size();
+    if (matrix_.rows != d || matrix_.cols != d) throw QulacsError("matrix size must be 2^k x 2^k for k targets");
+}
+
+void QuantumGateMatrix::add_control_qubit(unsigned q, unsigned v) {
+    if (std::find(targets_.begin(), targets_.end(), q) != targets_.end() ||
+        std::find(controls_.begin(), controls_.end(), q) != controls_.end())
+        throw QulacsError("add_control_qubit: qubit already used by the gate");
+    controls_.push_back(q);
+    cvals_.push_back(v);
+    kind = GENERIC;
+}
+
+QuantumGateBase* QuantumGateMatrix::get_inverse() const {
+    auto* g = new QuantumGateMatrix(targets_, matrix_.adjoint(), controls_, cvals_, name_);
+    switch (kind) {
+        case RX: case RY: case RZ: g->kind = kind; g->angle = -angle; break;
+        case X: case Y: case Z: case H: case CNOT: case CZ: case SWAP: case IDENTITY: g->kind = kind; break;
+        case S: g->kind = SDAG; break;
+        case SDAG: g->kind = S; break;
+        case T: g->kind = TDAG; break;
+        case TDAG: g->kind = T; break;
+        case SQRTX: g->kind = SQRTXDAG; break;
+        case SQRTXDAG: g->kind = SQRTX; break;
+        case SQRTY: g->kind = SQRTYDAG; break;
+        case SQRTYDAG: g->kind = SQRTY; break;
+        default: break;
+    }
+    return g;
+}
+
+void QuantumGateMatrix::update_quantum_state(QuantumStateBase* state) {
+    check_targets(state, targets_, "update_quantum_state");
+    check_targets(state, controls_, "update_quantum_state");
+    if (auto* s = as_sv(state)) update_sv(s);
+    else if (auto* d = as_dm(state)) update_dm(d);
+    else throw QulacsError("unsupported state type");
+}
+
+void QuantumGateMatrix::update_sv(QuantumStateMI355* s) {
+    void* p = s->data();
+    const ITYPE dim = s->dim;
+    void* st = s->get_cuda_stream();
+    const unsigned dev = s->device_number;
+    const unsigned t0 = targets_.empty() ? 0 : targets_[0];
+    switch (kind) {
+        case IDENTITY: return;
+        case X: X_gate_host(t0, p, dim, st, dev); break;
+        case Y: Y_gate_host(t0, p, dim, st, dev); break;
+        case Z: Z_gate_host(t0, p, dim, st, dev); break;
+        case H: H_gate_host(t0, p, dim, st, dev); break;
+        case S: S_gate_host(t0, p, dim, st, dev); break;
+        case SDAG: Sdag_gate_host(t0, p, dim, st, dev); break;
+        case T: T_gate_host(t0, p, dim, st, dev); break;
+        case TDAG: Tdag_gate_host(t0, p, dim, st, dev); break;
+        case SQRTX: sqrtX_gate_host(t0, p, dim, st, dev); break;
+        case SQRTXDAG: sqrtXdag_gate_host(t0, p, dim, st, dev); break;
+        case SQRTY: sqrtY_gate_host(t0, p, dim, st, dev); break;
+        case SQRTYDAG: sqrtYdag_gate_host(t0, p, dim, st, dev); break;
+        case P0: P0_gate_host(t0, p, dim, st, dev); break;
+        case P1: P1_gate_host(t0, p, dim, st, dev); break;
+        case RX: RX_gate_host(t0, angle, p, dim, st, dev); break;
+        case RY: RY_gate_host(t0, angle, p, dim, st, dev); break;
+        case RZ: RZ_gate_host(t0, angle, p, dim, st, dev); break;
+        case CNOT: CNOT_gate_host(controls_[0], t0, p, dim, st, dev); break;
+        case CZ: CZ_gate_host(controls_[0], t0, p, dim, st, dev); break;
+        case SWAP: SWAP_gate_host(targets_[0], targets_[1], p, dim, st, dev); break;
+        case GENERIC: {
+            const qm_complex* m = qc(matrix_.data.data());
+            const UINT nt = (UINT)targets_.size(), nc = (UINT)controls_.size();
+            if (nc == 0) {
+                if (nt == 1) single_qubit_dense_matrix_gate_host(t0, m, p, dim, st, dev);
+                else if (nt == 2) double_qubit_dense_matrix_gate_host(targets_[0], targets_[1], m, p, dim, st, dev);
+                else multi_qubit_dense_matrix_gate_host(targets_.data(), nt, m, p, dim, st, dev);
+            } else if (nt == 1 && nc == 1) {
+                single_qubit_control_single_qubit_dense_matrix_gate_host(controls_[0], cvals_[0], t0, m, p, dim, st, dev);
+            } else if (nt == 1) {
+                multi_qubit_control_single_qubit_dense_matrix_gate_host(controls_.data(), cvals_.data(), nc, t0, m, p,
+                                                                        dim, st, dev);
+            } else if (nc == 1) {
+                single_qubit_control_multi_qubit_dense_matrix_gate_host(controls_[0], cvals_[0], targets_.data(), nt,
+                                                                        m, p, dim, st, dev);
+            } else {
+                multi_qubit_control_multi_qubit_dense_matrix_gate_host(controls_.data(), cvals_.data(), nc,
+                                                                       targets_.data(), nt, m, p, dim, st, dev);
+            }
+            break;
+        }
+    }
+    check_abi(name_.c_str());
+}
+
+void QuantumGateMatrix::update_dm(DensityMatrixMI355* s) {
+    if (kind == IDENTITY) return;
+    void* p = s->data();
+    const qm_complex* m = qc(matrix_.data.data());
+    if (controls_.empty()) {
+        if (targets_.size() == 1)
+            dm_single_qubit_dense_matrix_gate_host(targets_[0], m, p, s->dim, s->get_cuda_stream(), s->device_number);
+        else
+            dm_multi_qubit_dense_matrix_gate_host(targets_.data(), (UINT)targets_.size(), m, p, s->dim,
+                                                  s->get_cuda_stream(), s->device_number);
+    } else {
+        dm_multi_qubit_control_multi_qubit_dense_matrix_gate_host(controls_.data(), cvals_.data(), (UINT)controls_.size(),
+                                                                  targets_.data(), (UINT)targets_.size(), m, p, s->dim,
+                                                                  s->get_cuda_stream(), s->device_number);
+    }
+    check_abi(name_.c_str());
+}
+
+QuantumGateDiagonalMatrix::QuantumGateDiagonalMatrix(std::vector<unsigned> t, std::vector<CPPCTYPE> d,
+                                                     std::vector<unsigned> c, std::vector<unsigned> v)
+    : diag_(std::move(d)) {
+    targets_ = std::move(t);
+    controls_ = std::move(c);
+    cvals_ = std::move(v);
+    name_ = "DiagonalMatrix";
+    if (diag_.size() != (size_t(1) << targets_.size())) throw QulacsError("diagonal must have 2^k entries");
+}
+ComplexMatrix QuantumGateDiagonalMatrix::get_matrix() const {
+    ComplexMatrix m(diag_.size(), diag_.size());
+    for (size_t i = 0; i < diag_.size(); ++i) m(i, i) = diag_[i];
+    return m;
+}
+void QuantumGateDiagonalMatrix::update_quantum_state(QuantumStateBase* state) {
+    check_targets(state, targets_, "DiagonalMatrix");
+    check_targets(state, controls_, "DiagonalMatrix");
+    if (auto* s = as_sv(state)) {
+        qm_multi_qubit_control_multi_qubit_diagonal_matrix_gate_host(
+            controls_.data(), cvals_.data(), (UINT)controls_.size(), targets_.data(), (UINT)targets_.size(),
+            qc(diag_.data()), s->data(), s->dim, s->get_cuda_stream(), s->device_number);
+        check_abi("DiagonalMatrix");
+    } else {
+        QuantumGateMatrix(targets_, get_matrix(), controls_, cvals_).update_quantum_state(state);
+    }
+}
+
+ClsPauliGate::ClsPauliGate(std::vector<unsigned> t, std::vector<unsigned> p, bool rotation, double angle)
+    : paulis_(std::move(p)), rotation_(rotation), angle_(angle) {
+    targets_ = std::move(t);
+    name_ = rotation ? "Pauli-rotation" : "Pauli";
+    if (targets_.size() != paulis_.size()) throw QulacsError("one Pauli id per target");
+    for (unsigned x : paulis_)
+        if (x > 3) throw QulacsError("Pauli id must be 0..3");
+}
+ComplexMatrix ClsPauliGate::get_matrix() const {
+    ComplexMatrix P = pauli_product(paulis_);
+    if (!rotation_) return P;
+    ComplexMatrix m = ComplexMatrix::identity(P.rows);
+    for (size_t i = 0; i < m.data.size(); ++i) m.data[i] = std::cos(angle_ / 2) * m.data[i] + I_ * std::sin(angle_ / 2) * P.data[i];
+    return m;
+}
+QuantumGateBase* ClsPauliGate::get_inverse() const {
+    if (rotation_) return new ClsPauliGate(targets_, paulis_, true, -angle_);
+    return new ClsPauliGate(targets_, paulis_, false, 0.0);
+}
+void ClsPauliGate::update_quantum_state(QuantumStateBase* state) {
+    check_targets(state, targets_, name_.c_str());
+    if (auto* s = as_sv(state)) {
+        if (rotation_)
+            multi_qubit_Pauli_rotation_gate_partial_list_host(targets_.data(), paulis_.data(), (UINT)targets_.size(),
+                                                              angle_, s->data(), s->dim, s->get_cuda_stream(),
+                                                              s->device_number);
+        else
+            multi_qubit_Pauli_gate_partial_list_host(targets_.data(), paulis_.data(), (UINT)targets_.size(), s->data(),
+                                                     s->dim, s->get_cuda_stream(), s->device_number);
+        check_abi(name_.c_str());
+    } else {
+        QuantumGateMatrix(targets_, get_matrix()).update_quantum_state(state);
+    }
+}
+
+QuantumGate_Probabilistic::QuantumGate_Probabilistic(std::vector<double> dist, std::vector<QuantumGateBase*> gates)
+    : dist_(std::move(dist)) {
+    if (dist_.size() != gates.size()) throw QulacsError("Probabilistic: one probability per gate");
+    cumulative_.push_back(0.0);
+    for (double p : dist_) cumulative_.push_back(cumulative_.back() + p);
+    if (cumulative_.back() > 1.0 + 1e-12) throw QulacsError("Probabilistic: probabilities sum above 1");
+    std::vector<std::vector<unsigned>> lists;
+    for (auto* g : gates) {
+        gates_.push_back(g->copy());
+        lists.push_back(g->get_target_index_list());
+        lists.push_back(g->get_control_index_list());
+    }
+    targets_ = sorted_union(lists);
+    name_ = "Probabilistic";
+}
+QuantumGate_Probabilistic::QuantumGate_Probabilistic(const QuantumGate_Probabilistic& o)
+    : QuantumGateBase(o), dist_(o.dist_), cumulative_(o.cumulative_), random_(o.random_) {
+    gates_.clear();
+    for (auto* g : o.gates_) gates_.push_back(g->copy());
+}
+QuantumGate_Probabilistic::~QuantumGate_Probabilistic() {
+    for (auto* g : gates_) delete g;
+}
+ComplexMatrix QuantumGate_Probabilistic::get_matrix() const {
+    throw QulacsError("Probabilistic gate has no single matrix; use superoperator()");
+}
+ComplexMatrix QuantumGate_Probabilistic::superoperator() const {
+    const size_t d = size_t(1) << targets_.size();
+    ComplexMatrix S(d * d, d * d);
+    const double rest = 1.0 - cumulative_.back();
+    if (rest != 0.0) accumulate_superop(S, ComplexMatrix::identity(d), rest);
+    for (size_t i = 0; i < gates_.size(); ++i) {
+        const QuantumGateBase* g = gates_[i];
+        std::vector<unsigned> on = g->get_target_index_list();
+        ComplexMatrix m = g->get_matrix();
+        if (!g->get_control_index_list().empty()) {
+            // fold controls into the matrix over controls + targets
+            std::vector<unsigned> all = g->get_target_index_list();
+            const auto& cl = g->get_control_index_list();
+            const auto& cv = g->get_control_value_list();
+            all.insert(all.end(), cl.begin(), cl.end());
+            const size_t k = all.size(), dd = size_t(1) << k, nt = g->get_target_index_list().size();
+            ComplexMatrix full = ComplexMatrix::identity(dd);
+            for (size_t col = 0; col < dd; ++col) {
+                bool on_ = true;
+                for (size_t c = 0; c < cl.size(); ++c) on_ = on_ && (((col >> (nt + c)) & 1) == cv[c]);
+                if (!on_) continue;
+                for (size_t row = 0; row < dd; ++row) full(row, col) = 0;
+                const size_t x = col & ((size_t(1) << nt) - 1);
+                for (size_t y = 0; y < (size_t(1) << nt); ++y) full((col & ~((size_t(1) << nt) - 1)) | y, col) = m(y, x);
+            }
+            m = full;
+            on = all;
+        }
+        accumulate_superop(S, embed(m, on, targets_), dist_[i]);
+    }
+    return S;
+}
+void QuantumGate_Probabilistic::update_quantum_state(QuantumStateBase* state) {
+    if (state->is_state_vector()) {
+        // src/cppsim/gate_general.hpp:253-264: draw one branch
+        const double r = random_.uniform();
+        const size_t idx = (size_t)(std::upper_bound(cumulative_.begin(), cumulative_.end(), r) - cumulative_.begin()) - 1;
+        if (idx < gates_.size()) gates_[idx]->update_quantum_state(state);
+        return;
+    }
+    auto* d = as_dm(state);
+    check_targets(state, targets_, "Probabilistic");
+    if (targets_.size() <= 3) {
+        const ComplexMatrix S = superoperator();
+        dm_superoperator_host(targets_.data(), (UINT)targets_.size(), qc(S.data.data()), d->data(), d->dim,
+                              d->get_cuda_stream(), d->device_number);
+        check_abi("Probabilistic");
+        return;
+    }
+    // wide mixtures: the reference's accumulate-over-copies form
+    std::unique_ptr<QuantumStateBase> org(d->copy()), tmp(d->allocate_buffer());
+    d->multiply_coef(1.0 - cumulative_.back());
+    for (size_t i = 0; i < gates_.size(); ++i) {
+        tmp->load(org.get());
+        gates_[i]->update_quantum_state(tmp.get());
+        d->add_state_with_coef(dist_[i], tmp.get());
+    }
+}
+
+QuantumGate_CPTP::QuantumGate_CPTP(std::vector<QuantumGateMatrix> kraus) : kraus_(std::move(kraus)) {
+    std::vector<std::vector<unsigned>> lists;
+    for (auto& k : kraus_) {
+        if (!k.get_control_index_list().empty()) throw QulacsError("CPTP: Kraus operators without controls expected");
+        lists.push_back(k.get_target_index_list());
+    }
+    targets_ = sorted_union(lists);
+    name_ = "CPTP";
+}
+ComplexMatrix QuantumGate_CPTP::get_matrix() const { throw QulacsError("CPTP map has no single matrix"); }
+ComplexMatrix QuantumGate_CPTP::superoperator() const {
+    const size_t d = size_t(1) << targets_.size();
+    ComplexMatrix S(d * d, d * d);
+    for (const auto& k : kraus_) accumulate_superop(S, embed(k.get_matrix(), k.get_target_index_list(), targets_), 1.0);
+    return S;
+}
+void QuantumGate_CPTP::update_quantum_state(QuantumStateBase* state) {
+    check_targets(state, targets_, "CPTP");
+    if (auto* d = as_dm(state)) {
+        if (targets_.size() > 3) throw QulacsError("CPTP on a density matrix: at most 3 target qubits");
+        const ComplexMatrix S = superoperator();
+        dm_superoperator_host(targets_.data(), (UINT)targets_.size(), qc(S.data.data()), d->data(), d->dim,
+                              d->get_cuda_stream(), d->device_number);
+        check_abi("CPTP");
+        return;
+    }
+    // state vector: K_i is kept with probability ||K_i psi||^2 / ||psi||^2 and the
+    // state rescaled to its old norm; identity if the map is not trace preserving
+    // (src/cppsim/gate_general.hpp:433-460)
+    const double r = random_.uniform();
+    const double org = state->get_squared_norm();
+    std::unique_ptr<QuantumStateBase> buf(state->allocate_buffer());
+    double sum = 0.0;
+    for (auto& k : kraus_) {
+        buf->load(state);
+        k.update_quantum_state(buf.get());
+        const double rel = buf->get_squared_norm() / org;
+        sum += rel;
+        if (r < sum) {
+            state->load(buf.get());
+            state->normalize(rel);
+            return;
+        }
+    }
+}
+
+QuantumGate_SingleParameter::QuantumGate_SingleParameter(std::vector<unsigned> t, std::vector<unsigned> p, double angle,
+                                                         std::string name)
+    : paulis_(std::move(p)), angle_(angle) {
+    targets_ = std::move(t);
+    name_ = std::move(name);
+    if (targets_.size() != paulis_.size()) throw QulacsError("one Pauli id per target");
+}
+ComplexMatrix QuantumGate_SingleParameter::get_matrix() const {
+    return ClsPauliGate(targets_, paulis_, true, angle_).get_matrix();
+}
+QuantumGateBase* QuantumGate_SingleParameter::get_inverse() const {
+    return new QuantumGate_SingleParameter(targets_, paulis_, -angle_, name_);
+}
+void QuantumGate_SingleParameter::update_quantum_state(QuantumStateBase* state) {
+    check_targets(state, targets_, name_.c_str());
+    if (auto* s = as_sv(state)) {
+        if (targets_.size() == 1) {
+            const unsigned t = targets_[0];
+            switch (paulis_[0]) {
+                case 1: RX_gate_host(t, angle_, s->data(), s->dim, s->get_cuda_stream(), s->device_number); break;
+                case 2: RY_gate_host(t, angle_, s->data(), s->dim, s->get_cuda_stream(), s->device_number); break;
+                case 3: RZ_gate_host(t, angle_, s->data(), s->dim, s->get_cuda_stream(), s->device_number); break;
+                default: return;
+            }
+        } else {
+            multi_qubit_Pauli_rotation_gate_partial_list_host(targets_.data(), paulis_.data(), (UINT)targets_.size(),
+                                                              angle_, s->data(), s->dim, s->get_cuda_stream(),
+                                                              s->device_number);
+        }
+        check_abi(name_.c_str());
+    } else {
+        QuantumGateMatrix(targets_, get_matrix()).update_quantum_state(state);
+    }
+}
+
+// ------------------------------------------------------------------ gate factory
+namespace gate {
+namespace {
+QuantumGateMatrix* named1(unsigned q, QuantumGateMatrix::Kind k, ComplexMatrix m, const char* name) {
+    auto* g = new QuantumGateMatrix({q}, std::move(m), {}, {}, name);
+    g->kind = k;
+    return g;
+}
+ComplexMatrix m2(CPPCTYPE a, CPPCTYPE b, CPPCTYPE c, CPPCTYPE d) {
+    ComplexMatrix m(2, 2);
+    m(0, 0) = a; m(0, 1) = b; m(1, 0) = c; m(1, 1) = d;
+    return m;
+}
+// exp(+i angle P / 2) (src/csim/update_ops_pauli_single.cpp:54-116)
+QuantumGateMatrix* rotation(unsigned q, unsigned pauli, double angle, const char* name) {
+    const double c = std::cos(angle / 2), s = std::sin(angle / 2);
+    ComplexMatrix m = pauli == 1 ? m2(c, I_ * s, I_ * s, c) : pauli == 2 ? m2(c, s, -s, c) : m2(c + I_ * s, 0, 0, c - I_ * s);
+    auto* g = new QuantumGateMatrix({q}, m, {}, {}, name);
+    g->kind = pauli == 1 ? QuantumGateMatrix::RX : pauli == 2 ? QuantumGateMatrix::RY : QuantumGateMatrix::RZ;
+    g->angle = angle;
+    return g;
+}
+// get_IBMQ_matrix (src/cppsim/gate_factory.cpp:169-181)
+ComplexMatrix ibmq(double theta, double phi, double lambda) {
+    const CPPCTYPE e1 = std::exp(I_ * phi), e2 = std::exp(I_ * lambda);
+    const double c = std::cos(theta / 2), s = std::sin(theta / 2);
+    return m2(c, -e2 * s, e1 * s, e1 * e2 * c);
+}
+}  // namespace
+
+QuantumGateMatrix* Identity(unsigned q) { return named1(q, QuantumGateMatrix::IDENTITY, ComplexMatrix::identity(2), "I"); }
+QuantumGateMatrix* X(unsigned q) { return named1(q, QuantumGateMatrix::X, m2(0, 1, 1, 0), "X"); }
+QuantumGateMatrix* Y(unsigned q) { return named1(q, QuantumGateMatrix::Y, m2(0, -I_, I_, 0), "Y"); }
+QuantumGateMatrix* Z(unsigned q) { return named1(q, QuantumGateMatrix::Z, m2(1, 0, 0, -1), "Z"); }
+QuantumGateMatrix* H(unsigned q) {
+    const double r = 1.0 / std::sqrt(2.0);
+    return named1(q, QuantumGateMatrix::H, m2(r, r, r, -r), "H");
+}
+QuantumGateMatrix* S(unsigned q) { return named1(q, QuantumGateMatrix::S, m2(1, 0, 0, I_), "S"); }
+QuantumGateMatrix* Sdag(unsigned q) { return named1(q, QuantumGateMatrix::SDAG, m2(1, 0, 0, -I_), "Sdag"); }
+QuantumGateMatrix* T(unsigned q) {
+    return named1(q, QuantumGateMatrix::T, m2(1, 0, 0, (1.0 + I_) / std::sqrt(2.0)), "T");
+}
+QuantumGateMatrix* Tdag(unsigned q) {
+    return named1(q, QuantumGateMatrix::TDAG, m2(1, 0, 0, (1.0 - I_) / std::sqrt(2.0)), "Tdag");
+}
+QuantumGateMatrix* sqrtX(unsigned q) {
+    return named1(q, QuantumGateMatrix::SQRTX, m2(0.5 + 0.5 * I_, 0.5 - 0.5 * I_, 0.5 - 0.5 * I_, 0.5 + 0.5 * I_), "sqrtX");
+}
+QuantumGateMatrix* sqrtXdag(unsigned q) {
+    return named1(q, QuantumGateMatrix::SQRTXDAG, m2(0.5 - 0.5 * I_, 0.5 + 0.5 * I_, 0.5 + 0.5 * I_, 0.5 - 0.5 * I_), "sqrtXdag");
+}
+QuantumGateMatrix* sqrtY(unsigned q) {
+    return named1(q, QuantumGateMatrix::SQRTY, m2(0.5 + 0.5 * I_, -0.5 - 0.5 * I_, 0.5 + 0.5 * I_, 0.5 + 0.5 * I_), "sqrtY");
+}
+QuantumGateMatrix* sqrtYdag(unsigned q) {
+    return named1(q, QuantumGateMatrix::SQRTYDAG, m2(0.5 - 0.5 * I_, 0.5 - 0.5 * I_, -0.5 + 0.5 * I_, 0.5 - 0.5 * I_), "sqrtYdag");
+}
+QuantumGateMatrix* P0(unsigned q) { return named1(q, QuantumGateMatrix::P0, m2(1, 0, 0, 0), "Projection-0"); }
+QuantumGateMatrix* P1(unsigned q) { return named1(q, QuantumGateMatrix::P1, m2(0, 0, 0, 1), "Projection-1"); }
+QuantumGateMatrix* RX(unsigned q, double a) { return rotation(q, 1, a, "X-rotation"); }
+QuantumGateMatrix* RY(unsigned q, double a) { return rotation(q, 2, a, "Y-rotation"); }
+QuantumGateMatrix* RZ(unsigned q, double a) { return rotation(q, 3, a, "Z-rotation"); }
+// RotX/RotY/RotZ use the opposite sign (src/cppsim/gate_factory.cpp:152-165)
+QuantumGateMatrix* RotX(unsigned q, double a) { return rotation(q, 1, -a, "X-rotation"); }
+QuantumGateMatrix* RotY(unsigned q, double a) { return rotation(q, 2, -a, "Y-rotation"); }
+QuantumGateMatrix* RotZ(unsigned q, double a) { return rotation(q, 3, -a, "Z-rotation"); }
+QuantumGateMatrix* RotInvX(unsigned q, double a) { return rotation(q, 1, a, "X-rotation"); }
+QuantumGateMatrix* RotInvY(unsigned q, double a) { return rotation(q, 2, a, "Y-rotation"); }
+QuantumGateMatrix* RotInvZ(unsigned q, double a) { return rotation(q, 3, a, "Z-rotation"); }
+QuantumGateMatrix* U1(unsigned q, double l) { return new QuantumGateMatrix({q}, ibmq(0, 0, l), {}, {}, "DenseMatrix"); }
+QuantumGateMatrix* U2(unsigned q, double p, double l) {
+    return new QuantumGateMatrix({q}, ibmq(M_PI / 2, p, l), {}, {}, "DenseMatrix");
+}
+QuantumGateMatrix* U3(unsigned q, double t, double p, double l) {
+    return new QuantumGateMatrix({q}, ibmq(t, p, l), {}, {}, "DenseMatrix");
+}
+QuantumGateMatrix* CNOT(unsigned c, unsigned t) {
+    if (c == t) throw QulacsError("CNOT: control and target must differ");
+    auto* g = new QuantumGateMatrix({t}, m2(0, 1, 1, 0), {c}, {1}, "CNOT");
+    g->kind = QuantumGateMatrix::CNOT;
+    return g;
+}
+QuantumGateMatrix* CZ(unsigned c, unsigned t) {
+    if (c == t) throw QulacsError("CZ: control and target must differ");
+    auto* g = new QuantumGateMatrix({t}, m2(1, 0, 0, -1), {c}, {1}, "CZ");
+    g->kind = QuantumGateMatrix::CZ;
+    return g;
+}
+QuantumGateMatrix* SWAP(unsigned a, unsigned b) {
+    if (a == b) throw QulacsError("SWAP: targets must differ");
+    ComplexMatrix m(4, 4);
+    m(0, 0) = 1; m(1, 2) = 1; m(2, 1) = 1; m(3, 3) = 1;
+    auto* g = new QuantumGateMatrix({a, b}, m, {}, {}, "SWAP");
+    g->kind = QuantumGateMatrix::SWAP;
+    return g;
+}
+QuantumGateMatrix* DenseMatrix(const std::vector<unsigned>& t, const ComplexMatrix& m) {
+    std::vector<unsigned> s(t);
+    std::sort(s.begin(), s.end());
+    if (std::adjacent_find(s.begin(), s.end()) != s.end()) throw QulacsError("DenseMatrix: duplicated target");
+    return new QuantumGateMatrix(t, m);
+}
+QuantumGateDiagonalMatrix* DiagonalMatrix(const std::vector<unsigned>& t, const std::vector<CPPCTYPE>& d) {
+    return new QuantumGateDiagonalMatrix(t, d);
+}
+ClsPauliGate* Pauli(const std::vector<unsigned>& t, const std::vector<unsigned>& p) { return new ClsPauliGate(t, p, false, 0.0); }
+ClsPauliGate* PauliRotation(const std::vector<unsigned>& t, const std::vector<unsigned>& p, double a) {
+    return new ClsPauliGate(t, p, true, a);
+}
+// Haar-random unitary: QR of a complex Gaussian matrix (modified Gram-Schmidt)
+// with the phases of R's diagonal divided out, as gate::RandomUnitary
+// (src/cppsim/gate_factory.cpp:325-380); the Gaussian stream is this library's.
+QuantumGateMatrix* RandomUnitary(const std::vector<unsigned>& t, uint64_t seed) {
+    Random rnd;
+    rnd.set_seed(seed);
+    const size_t d = size_t(1) << t.size();
+    ComplexMatrix a(d, d);
+    for (auto& z : a.data) z = CPPCTYPE(rnd.normal(), rnd.normal()) / std::sqrt(2.0);
+    ComplexMatrix q(d, d);
+    for (size_t j = 0; j < d; ++j) {
+        std::vector<CPPCTYPE> v(d);
+        for (size_t i = 0; i < d; ++i) v[i] = a(i, j);
+        for (size_t k = 0; k < j; ++k) {
+            CPPCTYPE dot = 0;
+            for (size_t i = 0; i < d; ++i) dot += std::conj(q(i, k)) * v[i];
+            for (size_t i = 0; i < d; ++i) v[i] -= dot * q(i, k);
+        }
+        double nrm = 0;
+        for (auto& z : v) nrm += std::norm(z);
+        nrm = std::sqrt(nrm);
+        // R_jj = nrm (real positive here), so the phase fix is the identity
+        for (size_t i = 0; i < d; ++i) q(i, j) = v[i] / nrm;
+    }
+    return new QuantumGateMatrix(t, q);
+}
+QuantumGate_Probabilistic* DepolarizingNoise(unsigned q, double p) {
+    // probabilities {p/3, p/3, p/3, 1-p} on {X, Z, Y, I} (src/cppsim/gate_factory.cpp:454-467)
+    std::unique_ptr<QuantumGateBase> x(X(q)), z(Z(q)), y(Y(q)), i(Identity(q));
+    auto* g = new QuantumGate_Probabilistic({p / 3, p / 3, p / 3, 1 - p}, {x.get(), z.get(), y.get(), i.get()});
+    return g;
+}
+QuantumGate_Probabilistic* TwoQubitDepolarizingNoise(unsigned a, unsigned b, double p) {
+    if (a == b) throw QulacsError("TwoQubitDepolarizingNoise: target list contains duplicated values");
+    std::vector<std::unique_ptr<QuantumGateBase>> own;
+    std::vector<QuantumGateBase*> gates;
+    std::vector<double> dist;
+    for (unsigned i = 0; i < 16; ++i) {
+        const unsigned pa = i % 4, pb = i / 4;
+        own.emplace_back(i == 0 ? static_cast<QuantumGateBase*>(Identity(a)) : Pauli({a, b}, {pa, pb}));
+        gates.push_back(own.back().get());
+        dist.push_back(i == 0 ? 1 - p : p / 15);
+    }
+    return new QuantumGate_Probabilistic(dist, gates);
+}
+QuantumGate_Probabilistic* BitFlipNoise(unsigned q, double p) {
+    std::unique_ptr<QuantumGateBase> x(X(q)), i(Identity(q));
+    return new QuantumGate_Probabilistic({p, 1 - p}, {x.get(), i.get()});
+}
+QuantumGate_Probabilistic* DephasingNoise(unsigned q, double p) {
+    std::unique_ptr<QuantumGateBase> z(Z(q)), i(Identity(q));
+    return new QuantumGate_Probabilistic({p, 1 - p}, {z.get(), i.get()});
+}
+QuantumGate_Probabilistic* IndependentXZNoise(unsigned q, double p) {
+    std::unique_ptr<QuantumGateBase> x(X(q)), z(Z(q)), y(Y(q)), i(Identity(q));
+    const double p1 = p * (1 - p), p2 = p * p;
+    return new QuantumGate_Probabilistic({p1, p1, p2, 1 - 2 * p1 - p2}, {x.get(), z.get(), y.get(), i.get()});
+}
+QuantumGate_CPTP* AmplitudeDampingNoise(unsigned q, double p) {
+    std::vector<QuantumGateMatrix> k;
+    k.emplace_back(std::vector<unsigned>{q}, m2(1, 0, 0, std::sqrt(1 - p)));
+    k.emplace_back(std::vector<unsigned>{q}, m2(0, std::sqrt(p), 0, 0));
+    return new QuantumGate_CPTP(k);
+}
+QuantumGate_Probabilistic* Probabilistic(const std::vector<double>& dist, const std::vector<QuantumGateBase*>& gates) {
+    return new QuantumGate_Probabilistic(dist, gates);
+}
+QuantumGate_CPTP* CPTP(const std::vector<QuantumGateBase*>& kraus) {
+    std::vector<QuantumGateMatrix> k;
+    for (auto* g : kraus) k.emplace_back(g->get_target_index_list(), g->get_matrix());
+    return new QuantumGate_CPTP(k);
+}
+QuantumGateMatrix* to_matrix_gate(const QuantumGateBase* g) {
+    return new QuantumGateMatrix(g->get_target_index_list(), g->get_matrix(), g->get_control_index_list(),
+                                 g->get_control_value_list());
+}
+// second * first over the ascending union of targets (src/cppsim/gate_merge.cpp:153-357)
+QuantumGateMatrix* merge(const QuantumGateBase* first, const QuantumGateBase* second) {
+    if (!first->get_control_index_list().empty() || !second->get_control_index_list().empty())
+        throw QulacsError("merge: controlled gates are not supported");
+    const auto u = sorted_union({first->get_target_index_list(), second->get_target_index_list()});
+    const ComplexMatrix a = embed(first->get_matrix(), first->get_target_index_list(), u);
+    const ComplexMatrix b = embed(second->get_matrix(), second->get_target_index_list(), u);
+    return new QuantumGateMatrix(u, b * a);
+}
+QuantumGate_SingleParameter* ParametricRX(unsigned q, double a) { return new QuantumGate_SingleParameter({q}, {1}, a, "ParametricRX"); }
+QuantumGate_SingleParameter* ParametricRY(unsigned q, double a) { return new QuantumGate_SingleParameter({q}, {2}, a, "ParametricRY"); }
+QuantumGate_SingleParameter* ParametricRZ(unsigned q, double a) { return new QuantumGate_SingleParameter({q}, {3}, a, "ParametricRZ"); }
+QuantumGate_SingleParameter* ParametricPauliRotation(const std::vector<unsigned>& t, const std::vector<unsigned>& p, double a) {
+    return new QuantumGate_SingleParameter(t, p, a, "ParametricPauliRotation");
+}
+}  // namespace gate
+
+// ------------------------------------------------------------------ circuit
+QuantumCircuit::QuantumCircuit(const QuantumCircuit& o) : qubit_count_(o.qubit_count_) {
+    for (auto* g : o.gates_) gates_.push_back(g->copy());
+}
+QuantumCircuit::~QuantumCircuit() {
+    for (auto* g : gates_) delete g;
+}
+const QuantumGateBase* QuantumCircuit::get_gate(size_t i) const {
+    if (i >= gates_.size()) throw QulacsError("get_gate: index out of range");
+    return gates_[i];
+}
+void QuantumCircuit::add_gate(const QuantumGateBase& g) { add_gate_consume(g.copy()); }
+void QuantumCircuit::add_gate(const QuantumGateBase& g, size_t pos) {
+    if (pos > gates_.size()) throw QulacsError("add_gate: position out of range");
+    gates_.insert(gates_.begin() + pos, g.copy());
+}
+void QuantumCircuit::add_gate_consume(QuantumGateBase* g) {
+    for (unsigned q : g->get_target_index_list())
+        if (q >= qubit_count_) {
+            delete g;
+            throw QulacsError("add_gate: target index exceeds the circuit's qubit count");
+        }
+    for (unsigned q : g->get_control_index_list())
+        if (q >= qubit_count_) {
+            delete g;
+            throw QulacsError("add_gate: control index exceeds the circuit's qubit count");
+        }
+    gates_.push_back(g);
+}
+void QuantumCircuit::remove_gate(size_t i) {
+    if (i >= gates_.size()) throw QulacsError("remove_gate: index out of range");
+    delete gates_[i];
+    gates_.erase(gates_.begin() + i);
+}
+// src/cppsim/circuit.cpp:46-57
+void QuantumCircuit::update_quantum_state(QuantumStateBase* s) { update_quantum_state(s, 0, gates_.size()); }
+void QuantumCircuit::update_quantum_state(QuantumStateBase* s, size_t start, size_t end) {
+    if (s->qubit_count != qubit_count_) throw QulacsError("update_quantum_state: qubit count mismatch");
+    if (start > end || end > gates_.size()) throw QulacsError("update_quantum_state: bad gate range");
+    for (size_t i = start; i < end; ++i) gates_[i]->update_quantum_state(s);
+}
+unsigned QuantumCircuit::calculate_depth() const {
+    std::vector<unsigned> depth(qubit_count_, 0);
+    unsigned best = 0;
+    for (auto* g : gates_) {
+        unsigned d = 0;
+        auto qs = g->get_target_index_list();
+        const auto& cl = g->get_control_index_list();
+        qs.insert(qs.end(), cl.begin(), cl.end());
+        for (unsigned q : qs) d = std::max(d, depth[q]);
+        for (unsigned q : qs) depth[q] = d + 1;
+        best = std::max(best, d + 1);
+    }
+    return best;
+}
+std::string QuantumCircuit::to_string() const {
+    std::ostringstream os;
+    os << "*** Quantum Circuit Info ***\n# of qubit: " << qubit_count_ << "\n# of step : " << calculate_depth()
+       << "\n# of gate : " << gates_.size() << "\n";
+    return os.str();
+}
+QuantumCircuit* QuantumCircuit::get_inverse() const {
+    auto* c = new QuantumCircuit(qubit_count_);
+    for (auto it = gates_.rbegin(); it != gates_.rend(); ++it) c->add_gate_consume((*it)->get_inverse());
+    return c;
+}
+
+// ------------------------------------------------------------------ operators
+// src/cppsim/pauli_operator.cpp:16-46
+PauliOperator::PauliOperator(const std::string& s, CPPCTYPE coef) : coef_(coef) {
+    std::istringstream ss(s);
+    std::string p;
+    unsigned idx;
+    while (ss >> p) {
+        if (!(ss >> idx)) throw QulacsError("Pauli string without index: " + s);
+        unsigned id;
+        if (p == "I" || p == "i") id = 0;
+        else if (p == "X" || p == "x") id = 1;
+        else if (p == "Y" || p == "y") id = 2;
+        else if (p == "Z" || p == "z") id = 3;
+        else throw QulacsError("invalid Pauli string is given : " + p);
+        if (id != 0) add_single_Pauli(idx, id);
+    }
+}
+PauliOperator::PauliOperator(const std::vector<unsigned>& t, const std::vector<unsigned>& p, CPPCTYPE coef) : coef_(coef) {
+    if (t.size() != p.size()) throw QulacsError("one Pauli id per target");
+    for (size_t i = 0; i < t.size(); ++i)
+        if (p[i] != 0) add_single_Pauli(t[i], p[i]);
+}
+void PauliOperator::add_single_Pauli(unsigned q, unsigned p) {
+    if (p > 3) throw QulacsError("Pauli id must be 0..3");
+    targets_.push_back(q);
+    paulis_.push_back(p);
+}
+std::string PauliOperator::get_pauli_string() const {
+    std::ostringstream os;
+    const char* names = "IXYZ";
+    for (size_t i = 0; i < targets_.size(); ++i) os << (i ? " " : "") << names[paulis_[i]] << " " << targets_[i];
+    return os.str();
+}
+// src/csim/utility.cpp:12-43
+void PauliOperator::masks(ITYPE* bf, ITYPE* pf, unsigned* rot) const {
+    *bf = 0; *pf = 0; *rot = 0;
+    for (size_t i = 0; i < targets_.size(); ++i) {
+        const ITYPE b = 1ULL << targets_[i];
+        switch (paulis_[i]) {
+            case 1: *bf ^= b; break;
+            case 2: *bf ^= b; *pf ^= b; ++*rot; break;
+            case 3: *pf ^= b; break;
+            default: break;
+        }
+    }
+}
+CPPCTYPE PauliOperator::get_expectation_value(const QuantumStateBase* state) const {
+    for (unsigned q : targets_)
+        if (q >= state->qubit_count) throw QulacsError("PauliOperator acts beyond the state's qubits");
+    double v;
+    if (state->is_state_vector())
+        v = expectation_value_multi_qubit_Pauli_operator_partial_list_host(
+            targets_.data(), paulis_.data(), (UINT)targets_.size(), state->data(), state->dim, state->get_cuda_stream(),
+            state->device_number);
+    else
+        v = dm_expectation_value_multi_qubit_Pauli_operator_partial_list_host(
+            targets_.data(), paulis_.data(), (UINT)targets_.size(), state->data(), state->dim, state->get_cuda_stream(),
+            state->device_number);
+    check_abi("PauliOperator::get_expectation_value");
+    return coef_ * v;
+}
+CPPCTYPE PauliOperator::get_transition_amplitude(const QuantumStateBase* bra, const QuantumStateBase* ket) const {
+    if (!bra->is_state_vector() || !ket->is_state_vector()) throw QulacsError("transition amplitude needs state vectors");
+    qm_synchronize(bra->get_cuda_stream(), bra->device_number);
+    const qm_complex z = transition_amplitude_multi_qubit_Pauli_operator_partial_list_host(
+        targets_.data(), paulis_.data(), (UINT)targets_.size(), bra->data(), ket->data(), ket->dim,
+        ket->get_cuda_stream(), ket->device_number);
+    check_abi("PauliOperator::get_transition_amplitude");
+    return coef_ * cc(z);
+}
+
+void GeneralQuantumOperator::add_operator(const PauliOperator& p) {
+    for (unsigned q : p.get_index_list())
+        if (q >= qubit_count_) throw QulacsError("add_operator: Pauli operator acts beyond qubit_count");
+    terms_.push_back(p);
+}
+void GeneralQuantumOperator::add_operator(CPPCTYPE coef, const std::string& s) { add_operator(PauliOperator(s, coef)); }
+void GeneralQuantumOperator::add_operator(const std::vector<unsigned>& t, const std::vector<unsigned>& p, CPPCTYPE coef) {
+    add_operator(PauliOperator(t, p, coef));
+}
+bool GeneralQuantumOperator::is_hermitian() const {
+    for (const auto& t : terms_)
+        if (t.get_coef().imag() != 0.0) return false;
+    return true;
+}
+CPPCTYPE GeneralQuantumOperator::get_expectation_value(const QuantumStateBase* state) const {
+    if (qubit_count_ > state->qubit_count) throw QulacsError("get_expectation_value: invalid qubit count");
+    if (terms_.empty()) return 0.0;
+    if (!state->is_state_vector()) {
+        CPPCTYPE sum = 0;
+        for (const auto& t : terms_) sum += t.get_expectation_value(state);
+        return sum;
+    }
+    std::vector<ITYPE> bf(terms_.size()), pf(terms_.size());
+    std::vector<UINT> rot(terms_.size());
+    std::vector<qm_complex> co(terms_.size());
+    for (size_t i = 0; i < terms_.size(); ++i) {
+        unsigned r;
+        terms_[i].masks(&bf[i], &pf[i], &r);
+        rot[i] = r;
+        co[i] = qc(terms_[i].get_coef());
+    }
+    const qm_complex z = qm_observable_expectation_host(bf.data(), pf.data(), rot.data(), co.data(), (UINT)terms_.size(),
+                                                        state->data(), state->dim, state->get_cuda_stream(),
+                                                        state->device_number);
+    check_abi("get_expectation_value");
+    return cc(z);
+}
+CPPCTYPE GeneralQuantumOperator::get_transition_amplitude(const QuantumStateBase* bra, const QuantumStateBase* ket) const {
+    CPPCTYPE sum = 0;
+    for (const auto& t : terms_) sum += t.get_transition_amplitude(bra, ket);
+    return sum;
+}
+// src/cppsim/general_quantum_operator.cpp:330-348
+void GeneralQuantumOperator::apply_to_state(QuantumStateMI355* work, const QuantumStateMI355& src,
+                                            QuantumStateMI355* dst) const {
+    dst->multiply_coef(0.0);
+    for (const auto& t : terms_) {
+        work->load(&src);
+        if (!t.get_index_list().empty()) {
+            multi_qubit_Pauli_gate_partial_list_host(t.get_index_list().data(), t.get_pauli_id_list().data(),
+                                                     (UINT)t.get_index_list().size(), work->data(), work->dim,
+                                                     work->get_cuda_stream(), work->device_number);
+            check_abi("apply_to_state");
+        }
+        dst->add_state_with_coef(t.get_coef(), work);
+    }
+}
+
+}  // namespace qmh
+
+namespace qmh {
+
+// ------------------------------------------------------------------ parametric circuit
+QuantumGate_SingleParameter* ParametricQuantumCircuit::param(size_t i) const {
+    if (i >= param_pos_.size()) throw QulacsError("parameter index out of range");
+    return static_cast<QuantumGate_SingleParameter*>(gates_[param_pos_[i]]);
+}
+void ParametricQuantumCircuit::add_parametric_gate(const QuantumGate_SingleParameter& g) {
+    add_gate_consume(g.copy());
+    param_pos_.push_back(gates_.size() - 1);
+}
+void ParametricQuantumCircuit::add_parametric_gate(const QuantumGate_SingleParameter& g, size_t position) {
+    add_gate(g, position);
+    param_pos_.push_back(position);
+}
+void ParametricQuantumCircuit::add_gate(const QuantumGateBase& g, size_t position) {
+    QuantumCircuit::add_gate(g, position);
+    for (auto& p : param_pos_)
+        if (p >= position) ++p;
+}
+void ParametricQuantumCircuit::remove_gate(size_t i) {
+    QuantumCircuit::remove_gate(i);
+    std::vector<size_t> kept;
+    for (size_t p : param_pos_) {
+        if (p == i) continue;
+        kept.push_back(p > i ? p - 1 : p);
+    }
+    param_pos_ = kept;
+}
+void ParametricQuantumCircuit::add_parametric_RX_gate(unsigned q, double a) {
+    std::unique_ptr<QuantumGate_SingleParameter> g(gate::ParametricRX(q, a));
+    add_parametric_gate(*g);
+}
+void ParametricQuantumCircuit::add_parametric_RY_gate(unsigned q, double a) {
+    std::unique_ptr<QuantumGate_SingleParameter> g(gate::ParametricRY(q, a));
+    add_parametric_gate(*g);
+}
+void ParametricQuantumCircuit::add_parametric_RZ_gate(unsigned q, double a) {
+    std::unique_ptr<QuantumGate_SingleParameter> g(gate::ParametricRZ(q, a));
+    add_parametric_gate(*g);
+}
+void ParametricQuantumCircuit::add_parametric_multi_Pauli_rotation_gate(const std::vector<unsigned>& t,
+                                                                        const std::vector<unsigned>& p, double a) {
+    std::unique_ptr<QuantumGate_SingleParameter> g(gate::ParametricPauliRotation(t, p, a));
+    add_parametric_gate(*g);
+}
+double ParametricQuantumCircuit::get_parameter(size_t i) const { return param(i)->get_parameter_value(); }
+void ParametricQuantumCircuit::set_parameter(size_t i, double v) { param(i)->set_parameter_value(v); }
+size_t ParametricQuantumCircuit::get_parametric_gate_position(size_t i) const {
+    if (i >= param_pos_.size()) throw QulacsError("parameter index out of range");
+    return param_pos_[i];
+}
+
+// Walks the circuit backwards holding psi_i (state after gate i) and the
+// adjoint-propagated bistate; d/d theta of exp(i theta P / 2) is
+// exp(i (theta + pi) P / 2) / 2, so each parameter costs one extra rotation
+// and one inner product on the device.
+std::vector<double> ParametricQuantumCircuit::backprop_inner_product(QuantumStateMI355* bistate) {
+    if (bistate->qubit_count != qubit_count_) throw QulacsError("backprop: qubit count mismatch");
+    QuantumStateMI355 state(qubit_count_, bistate->device_number), work(qubit_count_, bistate->device_number);
+    update_quantum_state(&state);
+    std::vector<long> which(gates_.size(), -1);
+    for (size_t i = 0; i < param_pos_.size(); ++i) which[param_pos_[i]] = (long)i;
+    std::vector<double> grad(param_pos_.size(), 0.0);
+    for (size_t gi = gates_.size(); gi-- > 0;) {
+        QuantumGateBase* g = gates_[gi];
+        if (which[gi] >= 0) {
+            auto* pg = static_cast<QuantumGate_SingleParameter*>(g);
+            work.load(&state);
+            QuantumGate_SingleParameter shift(pg->get_target_index_list(), pg->get_pauli_id_list(), M_PI, "shift");
+            shift.update_quantum_state(&work);
+            grad[which[gi]] = inner_product(bistate, &work).real() / 2.0;
+        }
+        std::unique_ptr<QuantumGateBase> inv(g->get_inverse());
+        inv->update_quantum_state(bistate);
+        inv->update_quantum_state(&state);
+    }
+    return grad;
+}
+
+std::vector<double> ParametricQuantumCircuit::backprop(const GeneralQuantumOperator& obs) {
+    QuantumStateMI355 state(qubit_count_), bistate(qubit_count_), work(qubit_count_);
+    update_quantum_state(&state);
+    obs.apply_to_state(&work, state, &bistate);
+    bistate.multiply_coef(2.0);
+    return backprop_inner_product(&bistate);
+}
+
+std::vector<CPPCTYPE> GradCalculator::calculate_grad(ParametricQuantumCircuit& c, const GeneralQuantumOperator& obs,
+                                                     const std::vector<double>& theta) {
+    const size_t np = c.get_parameter_count();
+    if (theta.size() != np) throw QulacsError("calculate_grad: one angle per parameter expected");
+    std::unique_ptr<ParametricQuantumCircuit> cc(static_cast<ParametricQuantumCircuit*>(c.copy()));
+    QuantumStateMI355 state(c.get_qubit_count());
+    std::vector<CPPCTYPE> grad(np);
+    for (size_t k = 0; k < np; ++k) {
+        CPPCTYPE e[2];
+        for (int s = 0; s < 2; ++s) {
+            for (size_t q = 0; q < np; ++q) cc->set_parameter(q, theta[q] + (q == k ? (s ? -M_PI_2 : M_PI_2) : 0.0));
+            state.set_zero_state();
+            cc->update_quantum_state(&state);
+            e[s] = obs.get_expectation_value(&state);
+        }
+        grad[k] = (e[0] - e[1]) / 2.0;
+    }
+    return grad;
+}
+std::vector<CPPCTYPE> GradCalculator::calculate_grad(ParametricQuantumCircuit& c, const GeneralQuantumOperator& obs) {
+    std::vector<double> theta;
+    for (size_t i = 0; i < c.get_parameter_count(); ++i) theta.push_back(c.get_parameter(i));
+    return calculate_grad(c, obs, theta);
+}
+
+}  // namespace qmh

@@ -1,0 +1,534 @@
+// Host-side C++ API over libqulacs_mi355: the qulacs cppsim interface for the
+// state-vector / density-matrix hot path, with every state resident on the GPU.
+//
+// Mirrors (names, argument meaning, conventions):
+//   QuantumStateBase / QuantumState       src/cppsim/state.hpp:18-417, state_gpu.hpp:13-415
+//   DensityMatrix                         src/cppsim/state_dm.hpp
+//   QuantumGateBase::update_quantum_state src/cppsim/gate.hpp:83-200
+//   named / rotation / matrix / diagonal / Pauli gates
+//                                         src/cppsim/gate_named_*.hpp, gate_matrix*.cpp
+//   QuantumGate_Probabilistic / CPTP      src/cppsim/gate_general.hpp
+//   QuantumCircuit                        src/cppsim/circuit.hpp:29
+//   PauliOperator / GeneralQuantumOperator / Observable
+//                                         src/cppsim/pauli_operator.hpp, general_quantum_operator.hpp,
+//                                         observable.hpp
+// Every device operation goes through the C ABI in include/qulacs_mi355.h.
+#pragma once
+
+#include <complex>
+#include <cstdint>
+#include <memory>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/qulacs_mi355.h"
+
+namespace qmh {
+
+using CPPCTYPE = std::complex<double>;
+
+// Raised for invalid arguments and for errors recorded by the C ABI.
+class QulacsError : public std::runtime_error {
+   public:
+    using std::runtime_error::runtime_error;
+};
+
+// Throws if the last C ABI call on this thread recorded an error.
+void check_abi(const char* where);
+
+// Row-major dense matrix.
+struct ComplexMatrix {
+    size_t rows = 0, cols = 0;
+    std::vector<CPPCTYPE> data;
+    ComplexMatrix() = default;
+    ComplexMatrix(size_t r, size_t c) : rows(r), cols(c), data(r * c) {}
+    CPPCTYPE& operator()(size_t r, size_t c) { return data[r * cols + c]; }
+    const CPPCTYPE& operator()(size_t r, size_t c) const { return data[r * cols + c]; }
+    static ComplexMatrix identity(size_t n);
+    ComplexMatrix operator*(const ComplexMatrix& o) const;
+    ComplexMatrix adjoint() const;
+};
+
+// std::mt19937_64 + uniform_real_distribution<double>(0, 1), as the reference's
+// Random (src/cppsim/utility.hpp:54-102), so a given seed draws the same uniforms.
+class Random {
+   public:
+    Random();
+    void set_seed(uint64_t seed) { mt_.seed(seed); }
+    double uniform() { return uniform_(mt_); }
+    double normal() { return normal_(mt_); }
+    uint64_t int64() { return mt_(); }
+    uint32_t int32() { return (uint32_t)(mt_() % 0xffffffffULL); }
+
+   private:
+    std::uniform_real_distribution<double> uniform_{0.0, 1.0};
+    std::normal_distribution<double> normal_{0.0, 1.0};
+    std::mt19937_64 mt_;
+};
+
+// ------------------------------------------------------------------ states
+class QuantumStateBase {
+   public:
+    const unsigned qubit_count;
+    const ITYPE dim;
+    const unsigned device_number;
+
+    QuantumStateBase(unsigned qubit_count, ITYPE dim, unsigned device);
+    virtual ~QuantumStateBase();
+    QuantumStateBase(const QuantumStateBase&) = delete;
+    QuantumStateBase& operator=(const QuantumStateBase&) = delete;
+
+    virtual bool is_state_vector() const = 0;
+    std::string get_device_name() const { return "gpu"; }
+    // Number of complex entries in device memory (dim for a state vector, dim^2 for a DM).
+    virtual ITYPE element_count() const = 0;
+
+    virtual void set_zero_state() = 0;
+    virtual void set_computational_basis(ITYPE comp_basis) = 0;
+    virtual void set_Haar_random_state() = 0;
+    virtual void set_Haar_random_state(unsigned seed) = 0;
+    virtual double get_zero_probability(unsigned target_qubit_index) const = 0;
+    // measured_values[q] in {0, 1, 2}; 2 means "not measured" (src/cppsim/state.hpp).
+    virtual double get_marginal_probability(const std::vector<unsigned>& measured_values) const = 0;
+    virtual double get_entropy() const = 0;
+    virtual double get_squared_norm() const = 0;
+    virtual void normalize(double squared_norm) = 0;
+    virtual QuantumStateBase* allocate_buffer() const = 0;
+    virtual QuantumStateBase* copy() const = 0;
+    void load(const QuantumStateBase* other);
+    virtual void load(const std::vector<CPPCTYPE>& host) = 0;
+    void* data() const { return data_; }
+    std::vector<CPPCTYPE> duplicate_data_cpp() const;
+    void add_state(const QuantumStateBase* other);
+    void add_state_with_coef(CPPCTYPE coef, const QuantumStateBase* other);
+    void multiply_coef(CPPCTYPE coef);
+    virtual std::vector<ITYPE> sampling(unsigned count) = 0;
+    std::vector<ITYPE> sampling(unsigned count, unsigned seed);
+    virtual std::string to_string() const = 0;
+
+    void* get_cuda_stream() const { return stream_; }
+    unsigned get_classical_value(unsigned index);
+    void set_classical_value(unsigned index, unsigned value);
+    const std::vector<unsigned>& get_classical_register() const { return classical_register_; }
+
+    Random& random() { return random_; }
+
+   protected:
+    void* data_ = nullptr;
+    void* stream_ = nullptr;  // hipStream_t[1] from allocate_cuda_stream_host
+    std::vector<unsigned> classical_register_;
+    Random random_;
+};
+
+// State vector of `qubit_count` qubits in GPU memory; get_device_name() == "gpu".
+class QuantumStateMI355 : public QuantumStateBase {
+   public:
+    explicit QuantumStateMI355(unsigned qubit_count, unsigned device = 0);
+    bool is_state_vector() const override { return true; }
+    ITYPE element_count() const override { return dim; }
+    void set_zero_state() override;
+    void set_computational_basis(ITYPE comp_basis) override;
+    void set_Haar_random_state() override;
+    void set_Haar_random_state(unsigned seed) override;
+    double get_zero_probability(unsigned target_qubit_index) const override;
+    double get_marginal_probability(const std::vector<unsigned>& measured_values) const override;
+    double get_entropy() const override;
+    double get_squared_norm() const override;
+    void normalize(double squared_norm) override;
+    QuantumStateBase* allocate_buffer() const override;
+    QuantumStateBase* copy() const override;
+    using QuantumStateBase::load;
+    void load(const std::vector<CPPCTYPE>& host) override;
+    using QuantumStateBase::sampling;
+    std::vector<ITYPE> sampling(unsigned count) override;
+    // Sampling with caller-supplied uniforms (same indices as the sequential CPU loop).
+    std::vector<ITYPE> sampling_with_uniforms(const std::vector<double>& uniforms) const;
+    std::string to_string() const override;
+};
+
+// Density matrix of `qubit_count` qubits (dim x dim, row-major) in GPU memory.
+class DensityMatrixMI355 : public QuantumStateBase {
+   public:
+    explicit DensityMatrixMI355(unsigned qubit_count, unsigned device = 0);
+    bool is_state_vector() const override { return false; }
+    ITYPE element_count() const override { return dim * dim; }
+    void set_zero_state() override;
+    void set_computational_basis(ITYPE comp_basis) override;
+    void set_Haar_random_state() override;
+    void set_Haar_random_state(unsigned seed) override;
+    double get_zero_probability(unsigned target_qubit_index) const override;
+    double get_marginal_probability(const std::vector<unsigned>& measured_values) const override;
+    double get_entropy() const override;
+    double get_squared_norm() const override;
+    void normalize(double squared_norm) override;
+    QuantumStateBase* allocate_buffer() const override;
+    QuantumStateBase* copy() const override;
+    using QuantumStateBase::load;
+    void load(const std::vector<CPPCTYPE>& host) override;  // a dim vector (pure) or dim^2 matrix
+    void load_pure(const QuantumStateMI355* pure);
+    using QuantumStateBase::sampling;
+    std::vector<ITYPE> sampling(unsigned count) override;
+    std::vector<ITYPE> sampling_with_uniforms(const std::vector<double>& uniforms) const;
+    std::string to_string() const override;
+};
+
+CPPCTYPE inner_product(const QuantumStateMI355* bra, const QuantumStateMI355* ket);
+
+// ------------------------------------------------------------------ gates
+class QuantumGateBase {
+   public:
+    virtual ~QuantumGateBase() = default;
+    virtual void update_quantum_state(QuantumStateBase* state) = 0;
+    virtual QuantumGateBase* copy() const = 0;
+    // Dense matrix over the target list (bit i <-> target i); throws for non-unitary channels.
+    virtual ComplexMatrix get_matrix() const = 0;
+    virtual QuantumGateBase* get_inverse() const;
+    virtual std::string get_name() const { return name_; }
+    virtual bool is_parametric() const { return false; }
+    virtual bool is_noise() const { return false; }
+    std::string to_string() const;
+    const std::vector<unsigned>& get_target_index_list() const { return targets_; }
+    const std::vector<unsigned>& get_control_index_list() const { return controls_; }
+    const std::vector<unsigned>& get_control_value_list() const { return cvals_; }
+
+   protected:
+    std::string name_;
+    std::vector<unsigned> targets_, controls_, cvals_;
+};
+
+// Dense (optionally controlled) matrix gate; also the base of every unitary
+// gate with a fast path (named, rotation, CNOT/CZ/SWAP).
+class QuantumGateMatrix : public QuantumGateBase {
+   public:
+    QuantumGateMatrix(std::vector<unsigned> targets, ComplexMatrix m, std::vector<unsigned> controls = {},
+                      std::vector<unsigned> cvals = {}, std::string name = "DenseMatrix");
+    void update_quantum_state(QuantumStateBase* state) override;
+    QuantumGateBase* copy() const override { return new QuantumGateMatrix(*this); }
+    ComplexMatrix get_matrix() const override { return matrix_; }
+    QuantumGateBase* get_inverse() const override;
+    void add_control_qubit(unsigned qubit, unsigned value);
+    // kind selects a dedicated entry point (0 = generic dense)
+    enum Kind { GENERIC, X, Y, Z, H, S, SDAG, T, TDAG, SQRTX, SQRTXDAG, SQRTY, SQRTYDAG, P0, P1, RX, RY, RZ,
+                CNOT, CZ, SWAP, IDENTITY };
+    Kind kind = GENERIC;
+    double angle = 0.0;  // RX/RY/RZ in the reference's exp(+i angle P / 2) convention
+
+   protected:
+    ComplexMatrix matrix_;
+    void update_sv(QuantumStateMI355* s);
+    void update_dm(DensityMatrixMI355* s);
+};
+
+class QuantumGateDiagonalMatrix : public QuantumGateBase {
+   public:
+    QuantumGateDiagonalMatrix(std::vector<unsigned> targets, std::vector<CPPCTYPE> diag,
+                              std::vector<unsigned> controls = {}, std::vector<unsigned> cvals = {});
+    void update_quantum_state(QuantumStateBase* state) override;
+    QuantumGateBase* copy() const override { return new QuantumGateDiagonalMatrix(*this); }
+    ComplexMatrix get_matrix() const override;
+
+   private:
+    std::vector<CPPCTYPE> diag_;
+};
+
+// Pauli product (rotation = false) or exp(+i angle P / 2) (rotation = true).
+class ClsPauliGate : public QuantumGateBase {
+   public:
+    ClsPauliGate(std::vector<unsigned> targets, std::vector<unsigned> paulis, bool rotation, double angle);
+    void update_quantum_state(QuantumStateBase* state) override;
+    QuantumGateBase* copy() const override { return new ClsPauliGate(*this); }
+    ComplexMatrix get_matrix() const override;
+    QuantumGateBase* get_inverse() const override;
+    const std::vector<unsigned>& get_pauli_id_list() const { return paulis_; }
+    double get_angle() const { return angle_; }
+    void set_angle(double a) { angle_ = a; }
+
+   protected:
+    std::vector<unsigned> paulis_;
+    bool rotation_;
+    double angle_;
+};
+
+// Mixture sum_i p_i G_i rho G_i^dagger (src/cppsim/gate_general.hpp:230-290).
+// On a state vector one G_i is drawn; on a density matrix the whole map is applied
+// as one superoperator pass when every G_i is a matrix gate.
+class QuantumGate_Probabilistic : public QuantumGateBase {
+   public:
+    QuantumGate_Probabilistic(std::vector<double> distribution, std::vector<QuantumGateBase*> gates);
+    QuantumGate_Probabilistic(const QuantumGate_Probabilistic& o);
+    ~QuantumGate_Probabilistic() override;
+    void update_quantum_state(QuantumStateBase* state) override;
+    QuantumGateBase* copy() const override { return new QuantumGate_Probabilistic(*this); }
+    ComplexMatrix get_matrix() const override;
+    bool is_noise() const override { return true; }
+    void set_seed(uint64_t seed) { random_.set_seed(seed); }
+    const std::vector<double>& get_distribution() const { return dist_; }
+    // superoperator over targets_ (4^k x 4^k, member = row bits * 2^k + column bits)
+    ComplexMatrix superoperator() const;
+
+   private:
+    std::vector<double> dist_, cumulative_;
+    std::vector<QuantumGateBase*> gates_;
+    Random random_;
+};
+
+// Kraus channel sum_i K_i rho K_i^dagger (src/cppsim/gate_general.hpp QuantumGate_CPTP).
+// On a state vector one K_i is chosen with probability ||K_i psi||^2 and the
+// state renormalised; on a density matrix it is one superoperator pass.
+class QuantumGate_CPTP : public QuantumGateBase {
+   public:
+    QuantumGate_CPTP(std::vector<QuantumGateMatrix> kraus);
+    void update_quantum_state(QuantumStateBase* state) override;
+    QuantumGateBase* copy() const override { return new QuantumGate_CPTP(*this); }
+    ComplexMatrix get_matrix() const override;
+    bool is_noise() const override { return true; }
+    ComplexMatrix superoperator() const;
+    void set_seed(uint64_t seed) { random_.set_seed(seed); }
+
+   private:
+    std::vector<QuantumGateMatrix> kraus_;
+    Random random_;
+};
+
+// Parametric rotation gates (src/vqcsim/parametric_gate.hpp:31-212).
+class QuantumGate_SingleParameter : public QuantumGateBase {
+   public:
+    // kind: 1 = RX, 2 = RY, 3 = RZ (single target) or Pauli rotation when paulis given
+    QuantumGate_SingleParameter(std::vector<unsigned> targets, std::vector<unsigned> paulis, double angle,
+                                std::string name);
+    void update_quantum_state(QuantumStateBase* state) override;
+    QuantumGateBase* copy() const override { return new QuantumGate_SingleParameter(*this); }
+    ComplexMatrix get_matrix() const override;
+    QuantumGateBase* get_inverse() const override;
+    bool is_parametric() const override { return true; }
+    double get_parameter_value() const { return angle_; }
+    void set_parameter_value(double v) { angle_ = v; }
+    const std::vector<unsigned>& get_pauli_id_list() const { return paulis_; }
+
+   private:
+    std::vector<unsigned> paulis_;
+    double angle_;
+};
+
+namespace gate {
+QuantumGateMatrix* Identity(unsigned q);
+QuantumGateMatrix* X(unsigned q);
+QuantumGateMatrix* Y(unsigned q);
+QuantumGateMatrix* Z(unsigned q);
+QuantumGateMatrix* H(unsigned q);
+QuantumGateMatrix* S(unsigned q);
+QuantumGateMatrix* Sdag(unsigned q);
+QuantumGateMatrix* T(unsigned q);
+QuantumGateMatrix* Tdag(unsigned q);
+QuantumGateMatrix* sqrtX(unsigned q);
+QuantumGateMatrix* sqrtXdag(unsigned q);
+QuantumGateMatrix* sqrtY(unsigned q);
+QuantumGateMatrix* sqrtYdag(unsigned q);
+QuantumGateMatrix* P0(unsigned q);
+QuantumGateMatrix* P1(unsigned q);
+QuantumGateMatrix* RX(unsigned q, double angle);
+QuantumGateMatrix* RY(unsigned q, double angle);
+QuantumGateMatrix* RZ(unsigned q, double angle);
+QuantumGateMatrix* RotX(unsigned q, double angle);
+QuantumGateMatrix* RotY(unsigned q, double angle);
+QuantumGateMatrix* RotZ(unsigned q, double angle);
+QuantumGateMatrix* RotInvX(unsigned q, double angle);
+QuantumGateMatrix* RotInvY(unsigned q, double angle);
+QuantumGateMatrix* RotInvZ(unsigned q, double angle);
+QuantumGateMatrix* U1(unsigned q, double lambda);
+QuantumGateMatrix* U2(unsigned q, double phi, double lambda);
+QuantumGateMatrix* U3(unsigned q, double theta, double phi, double lambda);
+QuantumGateMatrix* CNOT(unsigned control, unsigned target);
+QuantumGateMatrix* CZ(unsigned control, unsigned target);
+QuantumGateMatrix* SWAP(unsigned a, unsigned b);
+QuantumGateMatrix* DenseMatrix(const std::vector<unsigned>& targets, const ComplexMatrix& m);
+QuantumGateDiagonalMatrix* DiagonalMatrix(const std::vector<unsigned>& targets, const std::vector<CPPCTYPE>& diag);
+ClsPauliGate* Pauli(const std::vector<unsigned>& targets, const std::vector<unsigned>& paulis);
+ClsPauliGate* PauliRotation(const std::vector<unsigned>& targets, const std::vector<unsigned>& paulis, double angle);
+QuantumGateMatrix* RandomUnitary(const std::vector<unsigned>& targets, uint64_t seed);
+QuantumGate_Probabilistic* DepolarizingNoise(unsigned q, double p);
+QuantumGate_Probabilistic* TwoQubitDepolarizingNoise(unsigned a, unsigned b, double p);
+QuantumGate_Probabilistic* BitFlipNoise(unsigned q, double p);
+QuantumGate_Probabilistic* DephasingNoise(unsigned q, double p);
+QuantumGate_Probabilistic* IndependentXZNoise(unsigned q, double p);
+QuantumGate_CPTP* AmplitudeDampingNoise(unsigned q, double p);
+QuantumGate_Probabilistic* Probabilistic(const std::vector<double>& dist, const std::vector<QuantumGateBase*>& gates);
+QuantumGate_CPTP* CPTP(const std::vector<QuantumGateBase*>& kraus);
+QuantumGateMatrix* to_matrix_gate(const QuantumGateBase* g);
+QuantumGateMatrix* merge(const QuantumGateBase* first, const QuantumGateBase* second);
+QuantumGate_SingleParameter* ParametricRX(unsigned q, double angle);
+QuantumGate_SingleParameter* ParametricRY(unsigned q, double angle);
+QuantumGate_SingleParameter* ParametricRZ(unsigned q, double angle);
+QuantumGate_SingleParameter* ParametricPauliRotation(const std::vector<unsigned>& targets,
+                                                     const std::vector<unsigned>& paulis, double angle);
+}  // namespace gate
+
+// ------------------------------------------------------------------ circuits
+class QuantumCircuit {
+   public:
+    explicit QuantumCircuit(unsigned qubit_count) : qubit_count_(qubit_count) {}
+    QuantumCircuit(const QuantumCircuit& o);
+    virtual ~QuantumCircuit();
+    unsigned get_qubit_count() const { return qubit_count_; }
+    size_t get_gate_count() const { return gates_.size(); }
+    const QuantumGateBase* get_gate(size_t i) const;
+    void add_gate(const QuantumGateBase& g);
+    void add_gate(const QuantumGateBase& g, size_t position);
+    void add_gate_consume(QuantumGateBase* g);
+    void remove_gate(size_t i);
+    virtual QuantumCircuit* copy() const { return new QuantumCircuit(*this); }
+    void update_quantum_state(QuantumStateBase* state);
+    void update_quantum_state(QuantumStateBase* state, size_t start, size_t end);
+    unsigned calculate_depth() const;
+    std::string to_string() const;
+    QuantumCircuit* get_inverse() const;
+
+    // add_<name>_gate helpers (src/cppsim/circuit.hpp)
+    void add_X_gate(unsigned q) { add_gate_consume(gate::X(q)); }
+    void add_Y_gate(unsigned q) { add_gate_consume(gate::Y(q)); }
+    void add_Z_gate(unsigned q) { add_gate_consume(gate::Z(q)); }
+    void add_H_gate(unsigned q) { add_gate_consume(gate::H(q)); }
+    void add_S_gate(unsigned q) { add_gate_consume(gate::S(q)); }
+    void add_Sdag_gate(unsigned q) { add_gate_consume(gate::Sdag(q)); }
+    void add_T_gate(unsigned q) { add_gate_consume(gate::T(q)); }
+    void add_Tdag_gate(unsigned q) { add_gate_consume(gate::Tdag(q)); }
+    void add_sqrtX_gate(unsigned q) { add_gate_consume(gate::sqrtX(q)); }
+    void add_sqrtXdag_gate(unsigned q) { add_gate_consume(gate::sqrtXdag(q)); }
+    void add_sqrtY_gate(unsigned q) { add_gate_consume(gate::sqrtY(q)); }
+    void add_sqrtYdag_gate(unsigned q) { add_gate_consume(gate::sqrtYdag(q)); }
+    void add_P0_gate(unsigned q) { add_gate_consume(gate::P0(q)); }
+    void add_P1_gate(unsigned q) { add_gate_consume(gate::P1(q)); }
+    void add_CNOT_gate(unsigned c, unsigned t) { add_gate_consume(gate::CNOT(c, t)); }
+    void add_CZ_gate(unsigned c, unsigned t) { add_gate_consume(gate::CZ(c, t)); }
+    void add_SWAP_gate(unsigned a, unsigned b) { add_gate_consume(gate::SWAP(a, b)); }
+    void add_RX_gate(unsigned q, double a) { add_gate_consume(gate::RX(q, a)); }
+    void add_RY_gate(unsigned q, double a) { add_gate_consume(gate::RY(q, a)); }
+    void add_RZ_gate(unsigned q, double a) { add_gate_consume(gate::RZ(q, a)); }
+    void add_RotX_gate(unsigned q, double a) { add_gate_consume(gate::RotX(q, a)); }
+    void add_RotY_gate(unsigned q, double a) { add_gate_consume(gate::RotY(q, a)); }
+    void add_RotZ_gate(unsigned q, double a) { add_gate_consume(gate::RotZ(q, a)); }
+    void add_RotInvX_gate(unsigned q, double a) { add_gate_consume(gate::RotInvX(q, a)); }
+    void add_RotInvY_gate(unsigned q, double a) { add_gate_consume(gate::RotInvY(q, a)); }
+    void add_RotInvZ_gate(unsigned q, double a) { add_gate_consume(gate::RotInvZ(q, a)); }
+    void add_U1_gate(unsigned q, double l) { add_gate_consume(gate::U1(q, l)); }
+    void add_U2_gate(unsigned q, double p, double l) { add_gate_consume(gate::U2(q, p, l)); }
+    void add_U3_gate(unsigned q, double t, double p, double l) { add_gate_consume(gate::U3(q, t, p, l)); }
+    void add_dense_matrix_gate(const std::vector<unsigned>& t, const ComplexMatrix& m) {
+        add_gate_consume(gate::DenseMatrix(t, m));
+    }
+    void add_diagonal_matrix_gate(const std::vector<unsigned>& t, const std::vector<CPPCTYPE>& d) {
+        add_gate_consume(gate::DiagonalMatrix(t, d));
+    }
+    void add_multi_Pauli_gate(const std::vector<unsigned>& t, const std::vector<unsigned>& p) {
+        add_gate_consume(gate::Pauli(t, p));
+    }
+    void add_multi_Pauli_rotation_gate(const std::vector<unsigned>& t, const std::vector<unsigned>& p, double a) {
+        add_gate_consume(gate::PauliRotation(t, p, a));
+    }
+    void add_random_unitary_gate(const std::vector<unsigned>& t, uint64_t seed) {
+        add_gate_consume(gate::RandomUnitary(t, seed));
+    }
+
+   protected:
+    unsigned qubit_count_;
+    std::vector<QuantumGateBase*> gates_;
+};
+
+class GeneralQuantumOperator;
+
+// Circuit with registered parametric rotations (src/vqcsim/parametric_circuit.hpp).
+// A parametric gate added through add_gate is a plain gate; only the
+// add_parametric_* calls register a parameter.
+class ParametricQuantumCircuit : public QuantumCircuit {
+   public:
+    using QuantumCircuit::QuantumCircuit;
+    QuantumCircuit* copy() const override { return new ParametricQuantumCircuit(*this); }
+    void add_parametric_gate(const QuantumGate_SingleParameter& g);
+    void add_parametric_gate(const QuantumGate_SingleParameter& g, size_t position);
+    void add_parametric_RX_gate(unsigned q, double a);
+    void add_parametric_RY_gate(unsigned q, double a);
+    void add_parametric_RZ_gate(unsigned q, double a);
+    void add_parametric_multi_Pauli_rotation_gate(const std::vector<unsigned>& t, const std::vector<unsigned>& p,
+                                                  double a);
+    void add_gate(const QuantumGateBase& g) { QuantumCircuit::add_gate(g); }
+    void add_gate(const QuantumGateBase& g, size_t position);
+    void remove_gate(size_t i);
+    size_t get_parameter_count() const { return param_pos_.size(); }
+    double get_parameter(size_t i) const;
+    void set_parameter(size_t i, double v);
+    size_t get_parametric_gate_position(size_t i) const;
+    // d<O>/d(theta_i) by the adjoint sweep (src/vqcsim/parametric_circuit.cpp:205-335)
+    std::vector<double> backprop(const GeneralQuantumOperator& obs);
+    // Re <bistate|d psi/d theta_i> with psi = circuit |0> (bistate is overwritten)
+    std::vector<double> backprop_inner_product(QuantumStateMI355* bistate);
+
+   private:
+    QuantumGate_SingleParameter* param(size_t i) const;
+    std::vector<size_t> param_pos_;  // gate index of parameter i
+};
+
+// Parameter-shift gradient (src/vqcsim/GradCalculator.cpp:8-53).
+struct GradCalculator {
+    std::vector<CPPCTYPE> calculate_grad(ParametricQuantumCircuit& c, const GeneralQuantumOperator& obs,
+                                         const std::vector<double>& theta);
+    std::vector<CPPCTYPE> calculate_grad(ParametricQuantumCircuit& c, const GeneralQuantumOperator& obs);
+};
+
+// ------------------------------------------------------------------ operators
+class PauliOperator {
+   public:
+    explicit PauliOperator(CPPCTYPE coef = 1.0) : coef_(coef) {}
+    PauliOperator(const std::string& pauli_string, CPPCTYPE coef = 1.0);
+    PauliOperator(const std::vector<unsigned>& targets, const std::vector<unsigned>& paulis, CPPCTYPE coef = 1.0);
+    void add_single_Pauli(unsigned qubit, unsigned pauli);
+    CPPCTYPE get_coef() const { return coef_; }
+    void change_coef(CPPCTYPE c) { coef_ = c; }
+    const std::vector<unsigned>& get_index_list() const { return targets_; }
+    const std::vector<unsigned>& get_pauli_id_list() const { return paulis_; }
+    std::string get_pauli_string() const;
+    // coef * <psi|P|psi> (complex because coef is)
+    CPPCTYPE get_expectation_value(const QuantumStateBase* state) const;
+    CPPCTYPE get_transition_amplitude(const QuantumStateBase* bra, const QuantumStateBase* ket) const;
+    void masks(ITYPE* bit_flip, ITYPE* phase_flip, unsigned* rot90) const;
+    PauliOperator* copy() const { return new PauliOperator(*this); }
+
+   private:
+    std::vector<unsigned> targets_, paulis_;
+    CPPCTYPE coef_;
+};
+
+class GeneralQuantumOperator {
+   public:
+    explicit GeneralQuantumOperator(unsigned qubit_count) : qubit_count_(qubit_count) {}
+    virtual ~GeneralQuantumOperator() = default;
+    void add_operator(const PauliOperator& p);
+    void add_operator(CPPCTYPE coef, const std::string& pauli_string);
+    void add_operator(const std::vector<unsigned>& targets, const std::vector<unsigned>& paulis, CPPCTYPE coef);
+    size_t get_term_count() const { return terms_.size(); }
+    unsigned get_qubit_count() const { return qubit_count_; }
+    ITYPE get_state_dim() const { return 1ULL << qubit_count_; }
+    const PauliOperator& get_term(size_t i) const { return terms_.at(i); }
+    bool is_hermitian() const;
+    // sum_k coef_k <P_k>: one device call for a state vector
+    CPPCTYPE get_expectation_value(const QuantumStateBase* state) const;
+    CPPCTYPE get_transition_amplitude(const QuantumStateBase* bra, const QuantumStateBase* ket) const;
+    // dst = (sum_k coef_k P_k) src, using `work` as scratch (same size)
+    void apply_to_state(QuantumStateMI355* work, const QuantumStateMI355& src, QuantumStateMI355* dst) const;
+
+   protected:
+    unsigned qubit_count_;
+    std::vector<PauliOperator> terms_;
+};
+
+// HermitianQuantumOperator: expectation values are real (src/cppsim/observable.cpp:56-58).
+class Observable : public GeneralQuantumOperator {
+   public:
+    using GeneralQuantumOperator::GeneralQuantumOperator;
+    double get_expectation_value_real(const QuantumStateBase* state) const {
+        return GeneralQuantumOperator::get_expectation_value(state).real();
+    }
+};
+
+}  // namespace qmh

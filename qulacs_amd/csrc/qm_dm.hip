@@ -223,6 +223,9 @@ void dm_depolarizing_host(UINT t, double p, void* state, ITYPE dim, void* stream
 double dm_state_norm_squared_host(void* state, ITYPE dim, void* stream, unsigned dev) {
     return dm_diag_sum(false, 0, 0, state, dim, stream, dev, "dm_state_norm_squared_host");
 }
+double dm_measurement_distribution_entropy_host(void* state, ITYPE dim, void* stream, unsigned dev) {
+    return dm_diag_sum(true, 0, 0, state, dim, stream, dev, "dm_measurement_distribution_entropy_host");
+}
 double dm_M0_prob_host(UINT t, void* state, ITYPE dim, void* stream, unsigned dev) {
     QM_CHECK_ARG_RET(log2_dim(dim) > (int)t, 0.0, "dm_M0_prob_host: qubit %u out of range", t);
     return dm_diag_sum(false, 1ULL << t, 0, state, dim, stream, dev, "dm_M0_prob_host");
