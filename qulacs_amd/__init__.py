@@ -34,6 +34,7 @@ from ._core import (  # noqa: F401
     ParametricQuantumCircuit,
     PauliOperator,
     QuantumCircuit,
+    QuantumCircuitSimulator,
     QuantumGate_CPTP,
     QuantumGate_Probabilistic,
     QuantumGate_SingleParameter,
@@ -43,6 +44,7 @@ from ._core import (  # noqa: F401
     QuantumStateBase,
     QuantumStateMI355,
     QulacsError,
+    circuit,
     gate,
     get_num_device,
     get_synchronous,
@@ -53,10 +55,13 @@ from ._core import (  # noqa: F401
 
 QuantumState = QuantumStateMI355
 QuantumStateGpu = QuantumStateMI355
+StateVector = QuantumStateMI355
+StateVectorGpu = QuantumStateMI355
 DensityMatrix = DensityMatrixMI355
 
 # `import qulacs_amd.gate` / `from qulacs_amd.state import inner_product`
 _sys.modules[__name__ + ".gate"] = gate
 _sys.modules[__name__ + ".state"] = state
+_sys.modules[__name__ + ".circuit"] = circuit
 
 __version__ = _core.__version__

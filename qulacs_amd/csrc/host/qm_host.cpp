@@ -82,6 +82,28 @@ std::vector<unsigned> sorted_union(const std::vector<std::vector<unsigned>>& lis
     return u;
 }
 
+// Matrix of g over its targets followed by its controls (control qubits become
+// targets; the matrix is the identity where a control value does not match).
+ComplexMatrix with_controls(const QuantumGateBase* g, std::vector<unsigned>* on) {
+    const auto& tl = g->get_target_index_list();
+    const auto& cl = g->get_control_index_list();
+    const auto& cv = g->get_control_value_list();
+    ComplexMatrix m = g->get_matrix();
+    *on = tl;
+    if (cl.empty()) return m;
+    on->insert(on->end(), cl.begin(), cl.end());
+    const size_t nt = tl.size(), dt = size_t(1) << nt, dd = size_t(1) << on->size();
+    ComplexMatrix full = ComplexMatrix::identity(dd);
+    for (size_t col = 0; col < dd; ++col) {
+        bool active = true;
+        for (size_t c = 0; c < cl.size(); ++c) active = active && (((col >> (nt + c)) & 1) == cv[c]);
+        if (!active) continue;
+        const size_t high = col & ~(dt - 1), x = col & (dt - 1);
+        for (size_t y = 0; y < dt; ++y) full(high | y, col) = m(y, x);
+    }
+    return full;
+}
+
 QuantumStateMI355* as_sv(QuantumStateBase* s) { return dynamic_cast<QuantumStateMI355*>(s); }
 DensityMatrixMI355* as_dm(QuantumStateBase* s) { return dynamic_cast<DensityMatrixMI355*>(s); }
 
@@ -613,27 +635,8 @@ ComplexMatrix QuantumGate_Probabilistic::superoperator() const {
     if (rest != 0.0) accumulate_superop(S, ComplexMatrix::identity(d), rest);
     for (size_t i = 0; i < gates_.size(); ++i) {
         const QuantumGateBase* g = gates_[i];
-        std::vector<unsigned> on = g->get_target_index_list();
-        ComplexMatrix m = g->get_matrix();
-        if (!g->get_control_index_list().empty()) {
-            // fold controls into the matrix over controls + targets
-            std::vector<unsigned> all = g->get_target_index_list();
-            const auto& cl = g->get_control_index_list();
-            const auto& cv = g->get_control_value_list();
-            all.insert(all.end(), cl.begin(), cl.end());
-            const size_t k = all.size(), dd = size_t(1) << k, nt = g->get_target_index_list().size();
-            ComplexMatrix full = ComplexMatrix::identity(dd);
-            for (size_t col = 0; col < dd; ++col) {
-                bool on_ = true;
-                for (size_t c = 0; c < cl.size(); ++c) on_ = on_ && (((col >> (nt + c)) & 1) == cv[c]);
-                if (!on_) continue;
-                for (size_t row = 0; row < dd; ++row) full(row, col) = 0;
-                const size_t x = col & ((size_t(1) << nt) - 1);
-                for (size_t y = 0; y < (size_t(1) << nt); ++y) full((col & ~((size_t(1) << nt) - 1)) | y, col) = m(y, x);
-            }
-            m = full;
-            on = all;
-        }
+        std::vector<unsigned> on;
+        const ComplexMatrix m = with_controls(g, &on);
         accumulate_superop(S, embed(m, on, targets_), dist_[i]);
     }
     return S;
@@ -932,14 +935,13 @@ QuantumGateMatrix* to_matrix_gate(const QuantumGateBase* g) {
     return new QuantumGateMatrix(g->get_target_index_list(), g->get_matrix(), g->get_control_index_list(),
                                  g->get_control_value_list());
 }
-// second * first over the ascending union of targets (src/cppsim/gate_merge.cpp:153-357)
+// second * first over the ascending union of target and control qubits
+// (src/cppsim/gate_merge.cpp:153-357); controls are folded into the matrix.
 QuantumGateMatrix* merge(const QuantumGateBase* first, const QuantumGateBase* second) {
-    if (!first->get_control_index_list().empty() || !second->get_control_index_list().empty())
-        throw QulacsError("merge: controlled gates are not supported");
-    const auto u = sorted_union({first->get_target_index_list(), second->get_target_index_list()});
-    const ComplexMatrix a = embed(first->get_matrix(), first->get_target_index_list(), u);
-    const ComplexMatrix b = embed(second->get_matrix(), second->get_target_index_list(), u);
-    return new QuantumGateMatrix(u, b * a);
+    std::vector<unsigned> on1, on2;
+    const ComplexMatrix m1 = with_controls(first, &on1), m2_ = with_controls(second, &on2);
+    const auto u = sorted_union({on1, on2});
+    return new QuantumGateMatrix(u, embed(m2_, on2, u) * embed(m1, on1, u));
 }
 QuantumGate_SingleParameter* ParametricRX(unsigned q, double a) { return new QuantumGate_SingleParameter({q}, {1}, a, "ParametricRX"); }
 QuantumGate_SingleParameter* ParametricRY(unsigned q, double a) { return new QuantumGate_SingleParameter({q}, {2}, a, "ParametricRY"); }
@@ -1260,4 +1262,138 @@ std::vector<CPPCTYPE> GradCalculator::calculate_grad(ParametricQuantumCircuit& c
     return calculate_grad(c, obs, theta);
 }
 
+}  // namespace qmh
+
+namespace qmh {
+
+// ------------------------------------------------------------------ circuit optimizer
+namespace {
+bool mergeable(const QuantumGateBase* g) { return !g->is_parametric() && !g->is_noise(); }
+
+uint64_t qubit_mask(const QuantumGateBase* g) {
+    uint64_t m = 0;
+    for (unsigned q : g->get_target_index_list()) m |= 1ULL << q;
+    for (unsigned q : g->get_control_index_list()) m |= 1ULL << q;
+    return m;
+}
+
+bool is_diagonal(const QuantumGateBase* g) {
+    if (g->is_noise()) return false;
+    const ComplexMatrix m = g->get_matrix();
+    for (size_t y = 0; y < m.rows; ++y)
+        for (size_t x = 0; x < m.cols; ++x)
+            if (x != y && m(y, x) != CPPCTYPE(0)) return false;
+    return true;  // a controlled diagonal is diagonal too
+}
+}  // namespace
+
+bool QuantumCircuitOptimizer::commute(size_t a, size_t b) const {
+    const QuantumGateBase* ga = c_->gates_[a];
+    const QuantumGateBase* gb = c_->gates_[b];
+    if ((qubit_mask(ga) & qubit_mask(gb)) == 0) return true;
+    return is_diagonal(ga) && is_diagonal(gb);
+}
+// src/cppsim/circuit_optimizer.cpp:35-54
+size_t QuantumCircuitOptimizer::rightmost_commute(size_t i) const {
+    size_t cur = i + 1;
+    for (; cur < c_->gates_.size(); ++cur)
+        if (!commute(i, cur)) break;
+    return cur - 1;
+}
+size_t QuantumCircuitOptimizer::leftmost_commute(size_t i) const {
+    long cur = (long)i - 1;
+    for (; cur >= 0; --cur)
+        if (!commute(i, (size_t)cur)) break;
+    return (size_t)(cur + 1);
+}
+
+// src/cppsim/circuit_optimizer.cpp:113-164
+void QuantumCircuitOptimizer::optimize(QuantumCircuit* circuit, unsigned max_block_size, unsigned swap_level) {
+    if (swap_level != 0) throw QulacsError("optimize: swap_level > 0 needs a distributed state");
+    if (circuit->get_qubit_count() > 64) throw QulacsError("optimize: at most 64 qubits");
+    c_ = circuit;
+    auto& gl = circuit->gates_;
+    bool merged = true;
+    while (merged) {
+        merged = false;
+        for (size_t i1 = 0; i1 < gl.size(); ++i1) {
+            for (size_t i2 = i1 + 1; i2 < gl.size(); ++i2) {
+                if (!mergeable(gl[i1]) || !mergeable(gl[i2])) continue;
+                if ((unsigned)__builtin_popcountll(qubit_mask(gl[i1]) | qubit_mask(gl[i2])) > max_block_size) continue;
+                if (leftmost_commute(i2) > rightmost_commute(i1) + 1) continue;  // not neighbouring
+                QuantumGateMatrix* m = gate::merge(gl[i1], gl[i2]);
+                const size_t insert = std::max(leftmost_commute(i2), i1 + 1);
+                delete gl[i2];
+                gl.erase(gl.begin() + i2);
+                gl.insert(gl.begin() + insert, m);
+                delete gl[i1];
+                gl.erase(gl.begin() + i1);
+                --i2;
+                merged = true;
+            }
+        }
+    }
+    c_ = nullptr;
+}
+
+// src/cppsim/circuit_optimizer.cpp:166-216
+void QuantumCircuitOptimizer::optimize_light(QuantumCircuit* circuit, unsigned swap_level) {
+    if (swap_level != 0) throw QulacsError("optimize_light: swap_level > 0 needs a distributed state");
+    auto& gl = circuit->gates_;
+    const unsigned n = circuit->get_qubit_count();
+    std::vector<std::pair<long, std::vector<unsigned>>> step(n, {-1, {}});
+    for (size_t i = 0; i < gl.size(); ++i) {
+        QuantumGateBase* g = gl[i];
+        std::vector<unsigned> qs = g->get_target_index_list();
+        for (unsigned c : g->get_control_index_list()) qs.push_back(c);
+        std::sort(qs.begin(), qs.end());
+        long pos = -1;
+        long hit = -1;
+        for (unsigned qb : qs)
+            if (step[qb].first > pos) {
+                pos = step[qb].first;
+                hit = qb;
+            }
+        std::vector<unsigned> parent;
+        if (hit != -1) parent = step[hit].second;
+        // A parametric or noise gate that cannot merge starts a new step here; the
+        // reference skips it without recording it, which lets a later gate merge
+        // across it (src/cppsim/circuit_optimizer.cpp:196-200).
+        if (std::includes(parent.begin(), parent.end(), qs.begin(), qs.end()) && mergeable(gl[pos]) &&
+            mergeable(g)) {
+            QuantumGateMatrix* m = gate::merge(gl[pos], g);
+            delete gl[i];
+            gl.erase(gl.begin() + i);
+            gl.insert(gl.begin() + pos + 1, m);
+            delete gl[pos];
+            gl.erase(gl.begin() + pos);
+            --i;
+        } else {
+            for (unsigned qb : qs) step[qb] = {(long)i, qs};
+        }
+    }
+}
+
+// src/cppsim/circuit_optimizer.cpp:218-235
+QuantumGateMatrix* QuantumCircuitOptimizer::merge_all(const QuantumCircuit* circuit) {
+    std::unique_ptr<QuantumGateBase> cur(gate::to_matrix_gate(std::unique_ptr<QuantumGateBase>(gate::Identity(0)).get()));
+    for (auto* g : circuit->gates_) cur.reset(gate::merge(cur.get(), g));
+    return static_cast<QuantumGateMatrix*>(cur.release());
+}
+
+}  // namespace qmh
+
+namespace qmh {
+CPPCTYPE QuantumCircuitSimulator::get_expectation_value(const GeneralQuantumOperator& observable) {
+    return observable.get_expectation_value(state_);
+}
+void QuantumCircuitSimulator::swap_state_and_buffer() {
+    if (!buffer_) {
+        buffer_ = state_->allocate_buffer();
+        buffer_->set_zero_state();
+    }
+    std::unique_ptr<QuantumStateBase> tmp(state_->copy());
+    state_->load(buffer_);
+    buffer_->load(tmp.get());
+}
 }  // namespace qmh

@@ -432,11 +432,58 @@ class QuantumCircuit {
     }
 
    protected:
+    friend class QuantumCircuitOptimizer;
     unsigned qubit_count_;
     std::vector<QuantumGateBase*> gates_;
 };
 
+// Host-side gate fusion (src/cppsim/circuit_optimizer.cpp:113-235).  Two gates
+// commute when they act on disjoint qubits or are both diagonal; parametric and
+// noise gates are never merged.  Only swap_level 0 (no distributed-qubit swap
+// insertion) exists on one device.
+class QuantumCircuitOptimizer {
+   public:
+    void optimize(QuantumCircuit* circuit, unsigned max_block_size, unsigned swap_level = 0);
+    void optimize_light(QuantumCircuit* circuit, unsigned swap_level = 0);
+    QuantumGateMatrix* merge_all(const QuantumCircuit* circuit);
+
+   private:
+    QuantumCircuit* c_ = nullptr;
+    bool commute(size_t a, size_t b) const;
+    size_t rightmost_commute(size_t i) const;
+    size_t leftmost_commute(size_t i) const;
+};
+
 class GeneralQuantumOperator;
+
+// Circuit + state + one buffer (src/cppsim/simulator.hpp).
+class QuantumCircuitSimulator {
+   public:
+    QuantumCircuitSimulator(QuantumCircuit* circuit, QuantumStateBase* state)
+        : circuit_(circuit), state_(state) {}
+    ~QuantumCircuitSimulator() { delete buffer_; }
+    void initialize_state(ITYPE computational_basis = 0) { state_->set_computational_basis(computational_basis); }
+    void initialize_random_state() { state_->set_Haar_random_state(); }
+    void initialize_random_state(unsigned seed) { state_->set_Haar_random_state(seed); }
+    void simulate() { circuit_->update_quantum_state(state_); }
+    void simulate_range(size_t start, size_t end) { circuit_->update_quantum_state(state_, start, end); }
+    CPPCTYPE get_expectation_value(const GeneralQuantumOperator& observable);
+    size_t get_gate_count() const { return circuit_->get_gate_count(); }
+    void copy_state_to_buffer() {
+        if (!buffer_) buffer_ = state_->allocate_buffer();
+        buffer_->load(state_);
+    }
+    void copy_state_from_buffer() {
+        if (!buffer_) throw QulacsError("copy_state_from_buffer: the buffer is empty");
+        state_->load(buffer_);
+    }
+    void swap_state_and_buffer();
+
+   private:
+    QuantumCircuit* circuit_;
+    QuantumStateBase* state_;
+    QuantumStateBase* buffer_ = nullptr;
+};
 
 // Circuit with registered parametric rotations (src/vqcsim/parametric_circuit.hpp).
 // A parametric gate added through add_gate is a plain gate; only the

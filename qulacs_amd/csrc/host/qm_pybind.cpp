@@ -444,6 +444,34 @@ PYBIND11_MODULE(_core, m) {
                  &GradCalculator::calculate_grad),
              py::arg("parametric_circuit"), py::arg("observable"), py::arg("angles_of_gates"));
 
+    auto mcircuit = m.def_submodule("circuit");
+    py::class_<QuantumCircuitOptimizer>(mcircuit, "QuantumCircuitOptimizer")
+        .def(py::init([](unsigned mpi_size) {
+                 if (mpi_size > 1) throw QulacsError("QuantumCircuitOptimizer: one device only");
+                 return new QuantumCircuitOptimizer();
+             }),
+             py::arg("mpi_size") = 0)
+        .def("optimize", &QuantumCircuitOptimizer::optimize, py::arg("circuit"), py::arg("block_size"),
+             py::arg("swap_level") = 0)
+        .def("optimize_light", &QuantumCircuitOptimizer::optimize_light, py::arg("circuit"), py::arg("swap_level") = 0)
+        .def("merge_all", &QuantumCircuitOptimizer::merge_all, py::return_value_policy::take_ownership,
+             py::arg("circuit"));
+
+    py::class_<QuantumCircuitSimulator>(m, "QuantumCircuitSimulator")
+        .def(py::init<QuantumCircuit*, QuantumStateBase*>(), py::arg("circuit"), py::arg("state"),
+             py::keep_alive<1, 2>(), py::keep_alive<1, 3>())
+        .def("initialize_state", &QuantumCircuitSimulator::initialize_state, py::arg("computational_basis") = 0)
+        .def("initialize_random_state", py::overload_cast<>(&QuantumCircuitSimulator::initialize_random_state))
+        .def("initialize_random_state", py::overload_cast<unsigned>(&QuantumCircuitSimulator::initialize_random_state),
+             py::arg("seed"))
+        .def("simulate", &QuantumCircuitSimulator::simulate)
+        .def("simulate_range", &QuantumCircuitSimulator::simulate_range, py::arg("start"), py::arg("end"))
+        .def("get_expectation_value", &QuantumCircuitSimulator::get_expectation_value, py::arg("observable"))
+        .def("get_gate_count", &QuantumCircuitSimulator::get_gate_count)
+        .def("copy_state_to_buffer", &QuantumCircuitSimulator::copy_state_to_buffer)
+        .def("copy_state_from_buffer", &QuantumCircuitSimulator::copy_state_from_buffer)
+        .def("swap_state_and_buffer", &QuantumCircuitSimulator::swap_state_and_buffer);
+
     // ---------------------------------------------------------------- runtime controls
     m.def("set_synchronous", [](bool on) { qm_set_synchronous(on ? 1 : 0); }, py::arg("on"));
     m.def("get_synchronous", []() { return qm_get_synchronous() != 0; });

@@ -254,3 +254,52 @@ def test_aliases():
     import qulacs_amd.gate as g2
 
     assert g2 is gate
+
+
+def _random_circuit(rng, n, count):
+    c = q.QuantumCircuit(n)
+    for _ in range(count):
+        k = int(rng.integers(6))
+        t = [int(x) for x in rng.choice(n, 2, replace=False)]
+        if k == 0:
+            c.add_gate(gate.RandomUnitary([t[0]], int(rng.integers(1 << 30))))
+        elif k == 1:
+            c.add_CNOT_gate(t[0], t[1])
+        elif k == 2:
+            c.add_CZ_gate(t[0], t[1])
+        elif k == 3:
+            c.add_RZ_gate(t[0], float(rng.uniform(0, 6)))
+        elif k == 4:
+            c.add_gate(gate.RandomUnitary(t, int(rng.integers(1 << 30))))
+        else:
+            c.add_gate(gate.ParametricRY(t[0], 0.4))
+    return c
+
+
+@pytest.mark.parametrize("block", [1, 2, 3, 4])
+def test_optimizer_preserves_the_circuit_unitary(block):
+    """merge_all of the optimised circuit equals merge_all of the original
+    (src/cppsim/circuit_optimizer.cpp:113-235); host-only arithmetic."""
+    from qulacs_amd.circuit import QuantumCircuitOptimizer
+
+    rng = np.random.default_rng(block)
+    n = 5
+    c = _random_circuit(rng, n, 40)
+    want = QuantumCircuitOptimizer().merge_all(c)
+    assert want.get_target_index_list() == list(range(n))
+    for light in (False, True):
+        c2 = c.copy()
+        opt = QuantumCircuitOptimizer()
+        if light:
+            opt.optimize_light(c2)
+        else:
+            opt.optimize(c2, block)
+        assert c2.get_gate_count() <= c.get_gate_count()
+        if not light and block >= 2:
+            assert c2.get_gate_count() < c.get_gate_count()
+        for i in range(c2.get_gate_count()):
+            g = c2.get_gate(i)
+            if not light:
+                assert len(g.get_target_index_list()) + len(g.get_control_index_list()) <= max(block, 2)
+        got = QuantumCircuitOptimizer().merge_all(c2)
+        assert np.allclose(got.get_matrix(), want.get_matrix(), atol=1e-12), (block, light)

@@ -402,3 +402,61 @@ def test_backprop_matches_parameter_shift_and_finite_differences():
         em = expectation(c, obs)
         c.set_parameter(i, th)
         assert abs((ep - em) / (2 * h) - grad[i]) < 1e-6, i
+
+
+# ------------------------------------------------------------------ optimizer / simulator
+@pytest.mark.parametrize("block", [2, 3, 5])
+def test_optimized_circuit_gives_the_same_state(block):
+    from qulacs_amd.circuit import QuantumCircuitOptimizer
+
+    n = 8
+    rng = np.random.default_rng(block)
+    c = q.QuantumCircuit(n)
+    for _ in range(60):
+        t = [int(x) for x in rng.choice(n, 2, replace=False)]
+        k = int(rng.integers(5))
+        if k == 0:
+            c.add_gate(gate.RandomUnitary([t[0]], int(rng.integers(1 << 30))))
+        elif k == 1:
+            c.add_CNOT_gate(*t)
+        elif k == 2:
+            c.add_CZ_gate(*t)
+        elif k == 3:
+            c.add_gate(gate.ParametricRX(t[0], 0.3))
+        else:
+            c.add_RZ_gate(t[0], float(rng.uniform(0, 6)))
+    s0 = q.QuantumState(n)
+    s0.set_Haar_random_state(3)
+    s1 = s0.copy()
+    c.update_quantum_state(s0)
+    c2 = c.copy()
+    QuantumCircuitOptimizer().optimize(c2, block)
+    assert c2.get_gate_count() < c.get_gate_count()
+    c2.update_quantum_state(s1)
+    assert rel(s1.get_vector(), s0.get_vector()) < TOL
+    s2 = q.QuantumState(n)
+    s2.set_Haar_random_state(3)
+    QuantumCircuitOptimizer().merge_all(c).update_quantum_state(s2)
+    assert rel(s2.get_vector(), s0.get_vector()) < TOL
+
+
+def test_circuit_simulator():
+    n = 6
+    c = q.QuantumCircuit(n)
+    for i in range(n):
+        c.add_H_gate(i)
+    c.add_CNOT_gate(0, 5)
+    st = q.QuantumState(n)
+    sim = q.QuantumCircuitSimulator(c, st)
+    sim.initialize_state(0)
+    sim.simulate()
+    obs = q.Observable(n)
+    obs.add_operator(1.0, "X 0 X 5")
+    assert abs(sim.get_expectation_value(obs) - 1.0) < TOL
+    sim.copy_state_to_buffer()
+    sim.initialize_state(3)
+    sim.swap_state_and_buffer()
+    assert abs(st.get_vector()[0] - 1 / 8) < TOL
+    sim.copy_state_from_buffer()
+    assert st.get_vector()[3] == 1
+    assert sim.get_gate_count() == n + 1
