@@ -238,6 +238,54 @@ QM_EXPORT void dm_normalize_host(double squared_norm, void* state, ITYPE dim, vo
 /* DM sampling over |rho_ii| (src/cppsim/state_dm.hpp:383-408), same contract as qm_sampling_host. */
 QM_EXPORT void dm_sampling_host(const double* uniforms, UINT count, ITYPE* out, void* state, ITYPE dim, void* stream, unsigned int device_number);
 
+/* ------------------------------------------------------------------ batched circuits
+ * Many small circuits in one launch.  Replaces, for each circuit of a batch,
+ * QuantumCircuit::update_quantum_state on |0...0> (src/cppsim/circuit.cpp:46-57)
+ * followed by Observable::get_expectation_value (src/cppsim/observable.cpp:56-58,
+ * general_quantum_operator.cpp:105-141): no reference counterpart runs more than
+ * one circuit per call.  A batch is built on the host (add circuits, compile),
+ * then run any number of times; each run is one kernel launch in which every
+ * work-group takes circuits from a queue and keeps the state in LDS (up to 13
+ * qubits; 14..20 qubits in 13-qubit tiles staged through device memory).
+ *
+ * Gate kinds: QM_BATCH_DENSE (1..4 qubits, row-major 2^k x 2^k matrix, bit i of
+ * the matrix index <-> qubits[i]), QM_BATCH_DIAG (1..4 qubits, 2^k diagonal),
+ * QM_BATCH_CZ (qubits[0], qubits[1]), QM_BATCH_CNOT (qubits[0] = control,
+ * qubits[1] = target).  `matrix` is host memory read during add_circuit. */
+#define QM_BATCH_DENSE 0u
+#define QM_BATCH_DIAG 1u
+#define QM_BATCH_CZ 2u
+#define QM_BATCH_CNOT 3u
+#define QM_BATCH_MAX_QUBITS 20u
+typedef struct qm_batch_gate {
+    UINT kind;
+    UINT qubit_count;
+    UINT qubits[4];
+    const qm_complex* matrix;
+} qm_batch_gate;
+typedef struct qm_batch qm_batch;
+QM_EXPORT qm_batch* qm_batch_create(void);
+QM_EXPORT void qm_batch_destroy(qm_batch* batch);
+/* Adds one circuit on `qubit_count` (1..20) qubits starting from |0...0> and its
+ * observable sum_k coefs[k] P_k (P_k given by masks as in
+ * qm_observable_expectation_host; real coefficients).  Returns its index, or -1. */
+QM_EXPORT int qm_batch_add_circuit(qm_batch* batch, UINT qubit_count, const qm_batch_gate* gates, UINT gate_count, const ITYPE* bit_flip_masks, const ITYPE* phase_flip_masks, const UINT* global_phase_90rot_counts, const double* coefs, UINT term_count);
+/* Schedules every circuit into register-blocked passes of `block_qubits` (2..4;
+ * 0 = 4) and uploads the programs.  keep_states != 0 keeps every final state
+ * in device memory (qm_batch_get_state).  Returns QM_OK or an error code. */
+QM_EXPORT int qm_batch_compile(qm_batch* batch, UINT block_qubits, int keep_states, unsigned int device_number);
+/* Runs every circuit; energies[i] (host array) receives circuit i's expectation
+ * value.  Returns QM_OK or an error code. */
+QM_EXPORT int qm_batch_run(qm_batch* batch, double* energies, void* stream, unsigned int device_number);
+/* As qm_batch_run but leaves the energies in device memory (returned pointer,
+ * valid until the next compile or destroy) without synchronising the stream. */
+QM_EXPORT const double* qm_batch_run_async(qm_batch* batch, void* stream, unsigned int device_number);
+/* Copies circuit i's final state (2^n complex128) to host memory after a run
+ * compiled with keep_states. */
+QM_EXPORT int qm_batch_get_state(qm_batch* batch, UINT index, qm_complex* out, void* stream, unsigned int device_number);
+/* Schedule statistics: number of circuits, stages, passes and sub-operations. */
+QM_EXPORT void qm_batch_stats(const qm_batch* batch, ITYPE* circuits, ITYPE* stages, ITYPE* passes, ITYPE* subops);
+
 #ifdef __cplusplus
 }
 #endif

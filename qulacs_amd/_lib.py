@@ -25,6 +25,13 @@ class QmComplex(C.Structure):
 
 PC = C.POINTER(QmComplex)
 
+
+class QmBatchGate(C.Structure):
+    _fields_ = [("kind", C.c_uint), ("qubit_count", C.c_uint), ("qubits", C.c_uint * 4), ("matrix", PC)]
+
+
+BATCH_DENSE, BATCH_DIAG, BATCH_CZ, BATCH_CNOT = 0, 1, 2, 3
+
 # name -> (restype, argtypes)
 _SIGS: dict[str, tuple] = {
     "qm_get_last_error": (C.c_int, [C.c_char_p, C.c_size_t]),
@@ -121,6 +128,15 @@ _SIGS: dict[str, tuple] = {
     "dm_expectation_value_multi_qubit_Pauli_operator_partial_list_host": (D, [PU, PU, UINT, VP, ITYPE, VP, UINT]),
     "dm_normalize_host": (None, [D, VP, ITYPE, VP, UINT]),
     "dm_sampling_host": (None, [PD, UINT, PI, VP, ITYPE, VP, UINT]),
+    # batched circuits
+    "qm_batch_create": (VP, []),
+    "qm_batch_destroy": (None, [VP]),
+    "qm_batch_add_circuit": (C.c_int, [VP, UINT, C.POINTER(QmBatchGate), UINT, PI, PI, PU, PD, UINT]),
+    "qm_batch_compile": (C.c_int, [VP, UINT, C.c_int, UINT]),
+    "qm_batch_run": (C.c_int, [VP, PD, VP, UINT]),
+    "qm_batch_run_async": (VP, [VP, VP, UINT]),
+    "qm_batch_get_state": (C.c_int, [VP, UINT, PC, VP, UINT]),
+    "qm_batch_stats": (None, [VP, PI, PI, PI, PI]),
 }
 
 _lib = None

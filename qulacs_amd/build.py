@@ -40,6 +40,11 @@ HIP_FLAGS = [
     "-Wno-unknown-warning-option",
     f"-I{ROOT / 'include'}",
 ]
+# Per-file device flags.  qm_batch.hip dispatches register-array updates through
+# a switch on a runtime bit position; LLVM's common-code sinking would merge the
+# cases into one access with a computed index and move the array to private
+# (scratch) memory, so it is switched off for that file.
+EXTRA_HIP_FLAGS = {"qm_batch.hip": ["-mllvm", "-simplifycfg-sink-common=false"]}
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fvisibility=hidden", "-Wall", f"-I{ROOT / 'include'}"]
 
 
@@ -73,7 +78,7 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
         obj = BUILD / (src.name + ".o")
         objs.append(obj)
         if src.suffix == ".hip":
-            cmds.append([HIPCC, *HIP_FLAGS, "-c", str(src), "-o", str(obj)])
+            cmds.append([HIPCC, *HIP_FLAGS, *EXTRA_HIP_FLAGS.get(src.name, []), "-c", str(src), "-o", str(obj)])
         else:
             cmds.append([CXX, *CXX_FLAGS, "-c", str(src), "-o", str(obj)])
     host_objs = []

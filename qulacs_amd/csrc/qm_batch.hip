@@ -1,0 +1,851 @@
+// Batched executor: many small circuits, each run from |0...0> and reduced to
+// the expectation value of its observable, in one launch.
+//
+// What it replaces: for each circuit, QuantumCircuit::update_quantum_state
+// (src/cppsim/circuit.cpp:46-57, one kernel launch and synchronisation per gate
+// on the GPU backend) and Observable::get_expectation_value
+// (src/cppsim/observable.cpp:56-58 -> general_quantum_operator.cpp:105-141, one
+// reduction per term).
+//
+// Design:
+//   * one work-group (512 threads) per circuit, circuits taken from a queue
+//     (an atomic counter) in descending-cost order, so one launch covers a batch
+//     of uneven sizes; every work-group leaves the loop once the queue is empty;
+//   * the state lives in LDS: 2^lq complex128 with lq = min(n, 13) (128 KiB at
+//     13 qubits); circuits of 14..20 qubits keep the state in device memory and
+//     are processed in 2^13-amplitude tiles, one "stage" at a time: a stage is a
+//     set of gates whose qubits all lie in one 13-qubit window (qubits 0..3 are
+//     always in the window, so a tile is made of 256-byte contiguous runs);
+//   * inside a stage the gates are grouped into passes: each pass names K
+//     "block" qubits; a thread loads the 2^K amplitudes that differ in those
+//     bits into registers, applies every gate of the pass (dense gates on block
+//     qubits, diagonal gates and CZ on any local qubits) and writes them back,
+//     so one LDS round trip serves several gates;
+//   * LDS slots are XOR-swizzled (slot = i ^ fold3(i)) so that the 16-byte
+//     accesses of a wave spread over the banks whichever block bits a pass uses;
+//     fold3 is linear over GF(2), so slot(base | off) = slot(base) ^ slot(off);
+//   * the observable sum_k c_k <P_k> is reduced per work-group in a fixed order
+//     (deterministic), with P_k in the mask form of the reference
+//     (src/csim/utility.cpp:12-43): P|x> = i^rot (-1)^popcount(x & pf) |x ^ bf>.
+//
+// Scheduling (host): gates are scanned in order; a gate joins the current
+// stage / pass unless it touches a qubit of a gate skipped earlier in the same
+// scan (then it is skipped too and its qubits are blocked) or it does not fit
+// the window / block.  A gate is therefore only moved ahead of gates it
+// commutes with trivially (disjoint qubits), and every scan schedules at least
+// its first gate.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <vector>
+
+#include "qm_internal.hpp"
+
+namespace qm {
+namespace {
+
+using cd = std::complex<double>;
+
+constexpr u32 kMaxLocal = 13;   // qubits resident in LDS
+constexpr u32 kLowFixed = 4;    // qubits 0..3 are local in every stage
+constexpr u32 kBatchThreads = 512;
+constexpr u32 kMinQubits = 4;   // smaller circuits run on 4 qubits (extra qubits stay |0>)
+
+enum : u32 { SUB_D1 = 0, SUB_D2 = 1, SUB_DK = 2, SUB_DIAG = 3, SUB_CZ = 4 };
+
+struct DCirc {
+    u32 n, lq, nstage, stage0, term0, nterm;
+    u64 state_off;  // complex128 offset of the circuit's state in the scratch buffer
+};
+// Qubit lists are packed one byte per entry into 64-bit words so that the
+// kernel indexes them with shifts (a runtime index into an array would put it
+// in private memory).
+struct DStage {
+    u64 win[2];  // local bit 4 + i <-> qubit byte i of win (i < lq - 4 <= 9)
+    u64 comp;    // tile bit i <-> qubit byte i of comp (i < n - lq <= 7)
+    u32 npass, pass0;
+};
+__host__ __device__ __forceinline__ u32 byte_at(u64 w, u32 i) { return (u32)(w >> (8 * i)) & 0xffu; }
+__host__ __forceinline__ void set_byte(u64& w, u32 i, u32 v) { w = (w & ~(0xffULL << (8 * i))) | ((u64)v << (8 * i)); }
+struct DPass {
+    unsigned char pos[4];  // block bits (local positions, ascending)
+    u32 nsub, sub0;
+};
+struct DSub {
+    u32 type;
+    u32 a;     // packed bytes: D1/D2 block-bit indices; DIAG/CZ local positions
+    u32 m;     // DIAG: qubit count
+    u32 data;  // offset into the complex pool
+};
+struct DTerm {
+    u64 bf, pf;
+    u32 rot, pad;
+    double coef;
+};
+
+struct HOp {
+    u32 type;  // 0 dense, 1 diag, 2 cz
+    std::vector<u32> q;
+    std::vector<cd> m;
+    u64 mask() const {
+        u64 r = 0;
+        for (u32 x : q) r |= 1ULL << x;
+        return r;
+    }
+};
+struct HCirc {
+    u32 n = 0;  // as given
+    u32 ne = 0; // as run (>= kMinQubits)
+    std::vector<HOp> ops;
+    std::vector<DTerm> terms;
+};
+
+struct BatchArgs {
+    const DCirc* circs;
+    const DStage* stages;
+    const DPass* passes;
+    const DSub* subs;
+    const double2* pool;
+    const DTerm* terms;
+    const u32* order;
+    double2* scratch;
+    double* energies;
+    int* counter;
+    u32 ncirc;
+    int keep;
+};
+
+__host__ __device__ __forceinline__ u32 fold3(u32 x) { return ((x >> 3) ^ (x >> 6) ^ (x >> 9) ^ (x >> 12)) & 7u; }
+__host__ __device__ __forceinline__ u32 slot_of(u32 x) { return x ^ fold3(x); }
+
+template <int K, int P>
+__device__ __forceinline__ void sub_d1(double2* v, const double2* mp) {
+    const double2 m0 = mp[0], m1 = mp[1], m2 = mp[2], m3 = mp[3];
+#pragma unroll
+    for (int j = 0; j < (1 << K); ++j) {
+        if (j & (1 << P)) continue;
+        const double2 x = v[j], y = v[j | (1 << P)];
+        v[j] = cmac(cmul(m0, x), m1, y);
+        v[j | (1 << P)] = cmac(cmul(m2, x), m3, y);
+    }
+}
+
+template <int K, int P0, int P1>
+__device__ __forceinline__ void sub_d2(double2* v, const double2* mp) {
+    double2 m[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) m[i] = mp[i];
+#pragma unroll
+    for (int j = 0; j < (1 << K); ++j) {
+        if (j & ((1 << P0) | (1 << P1))) continue;
+        const double2 i0 = v[j], i1 = v[j | (1 << P0)], i2 = v[j | (1 << P1)], i3 = v[j | (1 << P0) | (1 << P1)];
+        v[j] = cmac(cmac(cmac(cmul(m[0], i0), m[1], i1), m[2], i2), m[3], i3);
+        v[j | (1 << P0)] = cmac(cmac(cmac(cmul(m[4], i0), m[5], i1), m[6], i2), m[7], i3);
+        v[j | (1 << P1)] = cmac(cmac(cmac(cmul(m[8], i0), m[9], i1), m[10], i2), m[11], i3);
+        v[j | (1 << P0) | (1 << P1)] = cmac(cmac(cmac(cmul(m[12], i0), m[13], i1), m[14], i2), m[15], i3);
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void sub_dk(double2* v, const double2* mp) {
+    constexpr int D = 1 << K;
+    double2 out[D];
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+        double2 acc = cmul(mp[r * D], v[0]);
+#pragma unroll
+        for (int c = 1; c < D; ++c) acc = cmac(acc, mp[r * D + c], v[c]);
+        out[r] = acc;
+    }
+#pragma unroll
+    for (int r = 0; r < D; ++r) v[r] = out[r];
+}
+
+template <int K>
+__device__ __forceinline__ void apply_d1(double2* v, u32 p, const double2* mp) {
+    switch (p) {
+        case 0: sub_d1<K, 0>(v, mp); break;
+        case 1: sub_d1<K, 1>(v, mp); break;
+        case 2: if constexpr (K > 2) sub_d1<K, 2>(v, mp); break;
+        case 3: if constexpr (K > 3) sub_d1<K, 3>(v, mp); break;
+        default: break;
+    }
+}
+
+#define QM_D2_CASE(A, B) \
+    case (A) * 4 + (B):  \
+        if constexpr (K > (A) && K > (B)) sub_d2<K, A, B>(v, mp); \
+        break;
+template <int K>
+__device__ __forceinline__ void apply_d2(double2* v, u32 p0, u32 p1, const double2* mp) {
+    switch (p0 * 4 + p1) {
+        QM_D2_CASE(0, 1) QM_D2_CASE(0, 2) QM_D2_CASE(0, 3)
+        QM_D2_CASE(1, 0) QM_D2_CASE(1, 2) QM_D2_CASE(1, 3)
+        QM_D2_CASE(2, 0) QM_D2_CASE(2, 1) QM_D2_CASE(2, 3)
+        QM_D2_CASE(3, 0) QM_D2_CASE(3, 1) QM_D2_CASE(3, 2)
+        default: break;
+    }
+}
+#undef QM_D2_CASE
+
+// This thread's share of sum_k c_k Re <psi|P_k|psi>, reading the state from LDS
+// (swizzled slots) or from device memory.
+template <bool FROM_LDS>
+__device__ __forceinline__ double energy_sum(const double2* __restrict__ gs, const double2* st, const DCirc& c,
+                                             const DTerm* __restrict__ terms) {
+    double acc = 0.0;
+    const u64 dim = 1ULL << c.n;
+    for (u64 x = threadIdx.x; x < dim; x += blockDim.x) {
+        const double2 px = FROM_LDS ? st[slot_of((u32)x)] : gs[x];
+        for (u32 k = 0; k < c.nterm; ++k) {
+            const DTerm tm = terms[c.term0 + k];
+            const u64 y = x ^ tm.bf;
+            const double2 py = FROM_LDS ? st[slot_of((u32)y)] : gs[y];  // y == x when bf == 0
+            // psi_x * conj(psi_y) * i^rot * (-1)^popcount(x & pf)
+            const double re0 = px.x * py.x + px.y * py.y, im0 = px.y * py.x - px.x * py.y;
+            double re;
+            switch (tm.rot & 3u) {
+                case 0: re = re0; break;
+                case 1: re = -im0; break;
+                case 2: re = -re0; break;
+                default: re = im0; break;
+            }
+            if (__popcll(x & tm.pf) & 1) re = -re;
+            acc = fma(tm.coef, re, acc);
+        }
+    }
+    return acc;
+}
+
+__global__ void k_batch_reset(int* counter) {
+    if (threadIdx.x == 0) *counter = 0;
+}
+
+template <int K>
+__global__ __launch_bounds__(kBatchThreads) void k_batch(BatchArgs a) {
+    constexpr int D = 1 << K;
+    extern __shared__ double2 st[];
+    __shared__ u32 hi_tab[1u << (kMaxLocal - kLowFixed)];
+    __shared__ double red[16];
+    __shared__ int s_slot;
+    const u32 tid = threadIdx.x, bd = blockDim.x;
+
+    for (;;) {
+        if (tid == 0) s_slot = atomicAdd(a.counter, 1);
+        __syncthreads();
+        const int slot = s_slot;
+        __syncthreads();
+        if (slot >= (int)a.ncirc) break;  // uniform: every thread read the same slot
+        const u32 ci = a.order[slot];
+        const DCirc c = a.circs[ci];
+        const u32 lq = c.lq, L = 1u << lq, tl = c.n - lq;
+        double2* gs = a.scratch + c.state_off;
+
+        for (u32 s = 0; s < c.nstage; ++s) {
+            const DStage sg = a.stages[c.stage0 + s];
+            const u64 win0 = sg.win[0], win1 = sg.win[1];
+            if (tl > 0) {
+                for (u32 j = tid; j < (L >> kLowFixed); j += bd) {
+                    u32 o = 0;
+                    for (u32 i = 0; i + kLowFixed < lq; ++i) o |= ((j >> i) & 1u) << byte_at(i < 8 ? win0 : win1, i & 7);
+                    hi_tab[j] = o;
+                }
+                __syncthreads();
+            }
+            for (u32 t = 0; t < (1u << tl); ++t) {
+                u64 tb = 0;
+                for (u32 i = 0; i < tl; ++i) tb |= (u64)((t >> i) & 1u) << byte_at(sg.comp, i);
+                if (s == 0) {
+                    // |0...0>: slot_of(0) == 0 and amplitude 0 lies in tile 0
+                    for (u32 i = tid; i < L; i += bd) st[i] = make_double2((i == 0 && t == 0) ? 1.0 : 0.0, 0.0);
+                } else {
+                    for (u32 i = tid; i < L; i += bd)
+                        st[slot_of(i)] = gs[tb + hi_tab[i >> kLowFixed] + (i & ((1u << kLowFixed) - 1))];
+                }
+                __syncthreads();
+
+                for (u32 p = 0; p < sg.npass; ++p) {
+                    const DPass ps = a.passes[sg.pass0 + p];
+                    u32 off[D];
+#pragma unroll
+                    for (int j = 0; j < D; ++j) {
+                        u32 o = 0;
+#pragma unroll
+                        for (int i = 0; i < K; ++i) o |= (u32)((j >> i) & 1) << ps.pos[i];
+                        off[j] = o;
+                    }
+                    for (u32 g = tid; g < (L >> K); g += bd) {
+                        u32 base = g;
+#pragma unroll
+                        for (int i = 0; i < K; ++i) {
+                            const u32 q = ps.pos[i];
+                            base = (base & ((1u << q) - 1)) | ((base >> q) << (q + 1));
+                        }
+                        const u32 sb = slot_of(base);
+                        double2 v[D];
+#pragma unroll
+                        for (int j = 0; j < D; ++j) v[j] = st[sb ^ slot_of(off[j])];
+                        for (u32 k = 0; k < ps.nsub; ++k) {
+                            const DSub sub = a.subs[ps.sub0 + k];
+                            const double2* mp = a.pool + sub.data;
+                            switch (sub.type) {
+                                case SUB_D1: apply_d1<K>(v, byte_at(sub.a, 0), mp); break;
+                                case SUB_D2: apply_d2<K>(v, byte_at(sub.a, 0), byte_at(sub.a, 1), mp); break;
+                                case SUB_DK: sub_dk<K>(v, mp); break;
+                                case SUB_DIAG:
+#pragma unroll
+                                    for (int j = 0; j < D; ++j) {
+                                        const u32 li = base | off[j];
+                                        u32 idx = 0;
+                                        for (u32 i = 0; i < sub.m; ++i) idx |= ((li >> byte_at(sub.a, i)) & 1u) << i;
+                                        v[j] = cmul(v[j], mp[idx]);
+                                    }
+                                    break;
+                                case SUB_CZ:
+#pragma unroll
+                                    for (int j = 0; j < D; ++j) {
+                                        const u32 li = base | off[j];
+                                        if ((li >> byte_at(sub.a, 0)) & (li >> byte_at(sub.a, 1)) & 1u)
+                                            v[j] = make_double2(-v[j].x, -v[j].y);
+                                    }
+                                    break;
+                                default: break;
+                            }
+                        }
+#pragma unroll
+                        for (int j = 0; j < D; ++j) st[sb ^ slot_of(off[j])] = v[j];
+                    }
+                    __syncthreads();
+                }
+
+                if (tl > 0) {
+                    for (u32 i = tid; i < L; i += bd)
+                        gs[tb + hi_tab[i >> kLowFixed] + (i & ((1u << kLowFixed) - 1))] = st[slot_of(i)];
+                } else if (a.keep) {
+                    for (u32 i = tid; i < L; i += bd) gs[i] = st[slot_of(i)];
+                }
+                __syncthreads();
+            }
+        }
+
+        // sum_k c_k Re <psi| P_k |psi>
+        const double acc = tl ? energy_sum<false>(gs, st, c, a.terms) : energy_sum<true>(gs, st, c, a.terms);
+        const double total = block_sum(acc, red);
+        if (tid == 0) a.energies[ci] = total;
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ host side
+void mat_mul(const std::vector<cd>& a, const std::vector<cd>& b, std::vector<cd>& out, int d) {
+    out.assign((size_t)d * d, 0.0);
+    for (int i = 0; i < d; ++i)
+        for (int k = 0; k < d; ++k)
+            for (int j = 0; j < d; ++j) out[i * d + j] += a[i * d + k] * b[k * d + j];
+}
+
+template <class T>
+bool dev_upload(T** dst, const std::vector<T>& src) {
+    *dst = nullptr;
+    if (src.empty()) return true;
+    if (!hip_ok(hipMalloc((void**)dst, src.size() * sizeof(T)), "qm_batch_compile: hipMalloc")) return false;
+    return hip_ok(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice),
+                  "qm_batch_compile: hipMemcpy");
+}
+
+}  // namespace
+}  // namespace qm
+
+struct qm_batch {
+    std::vector<qm::HCirc> circs;
+    // compiled
+    int device = -1;
+    unsigned K = 0;
+    int keep = 0;
+    unsigned lq_max = 0;
+    std::vector<qm::DCirc> h_circs;
+    qm::u64 n_stages = 0, n_passes = 0, n_subs = 0;
+    qm::DCirc* d_circs = nullptr;
+    qm::DStage* d_stages = nullptr;
+    qm::DPass* d_passes = nullptr;
+    qm::DSub* d_subs = nullptr;
+    double2* d_pool = nullptr;
+    qm::DTerm* d_terms = nullptr;
+    qm::u32* d_order = nullptr;
+    double2* d_scratch = nullptr;
+    double* d_energies = nullptr;
+    int* d_counter = nullptr;
+
+    void release() {
+        if (device >= 0) {
+            (void)hipSetDevice(device);
+            void* ptrs[] = {d_circs, d_stages, d_passes, d_subs, d_pool, d_terms, d_order, d_scratch, d_energies, d_counter};
+            for (void* p : ptrs)
+                if (p) (void)hipFree(p);
+        }
+        d_circs = nullptr; d_stages = nullptr; d_passes = nullptr; d_subs = nullptr; d_pool = nullptr;
+        d_terms = nullptr; d_order = nullptr; d_scratch = nullptr; d_energies = nullptr; d_counter = nullptr;
+        device = -1;
+        h_circs.clear();
+    }
+};
+
+namespace qm {
+namespace {
+
+// Schedules one circuit; appends to the flat arrays.
+struct Flat {
+    std::vector<DStage> stages;
+    std::vector<DPass> passes;
+    std::vector<DSub> subs;
+    std::vector<double2> pool;
+};
+
+u32 pool_put(Flat& f, const std::vector<cd>& m) {
+    const u32 off = (u32)f.pool.size();
+    for (const cd& z : m) f.pool.push_back(make_double2(z.real(), z.imag()));
+    return off;
+}
+
+// Passes of one stage: ops (already in local positions) -> DPass/DSub.
+void schedule_passes(Flat& f, std::vector<HOp> ops, u32 lq, u32 K, DStage& sg) {
+    sg.pass0 = (u32)f.passes.size();
+    sg.npass = 0;
+    std::vector<bool> done(ops.size(), false);
+    size_t left = ops.size();
+    while (left > 0) {
+        u64 blocked = 0, block = 0;
+        std::vector<size_t> sel;
+        for (size_t i = 0; i < ops.size(); ++i) {
+            if (done[i]) continue;
+            const HOp& o = ops[i];
+            const u64 m = o.mask();
+            if (m & blocked) {
+                blocked |= m;
+                continue;
+            }
+            if (o.type != 0) {
+                sel.push_back(i);
+                continue;
+            }
+            if ((u32)__builtin_popcountll(block | m) <= K) {
+                block |= m;
+                sel.push_back(i);
+            } else {
+                blocked |= m;
+            }
+        }
+        // pad the block to K local positions
+        for (u32 p = 0; p < lq && (u32)__builtin_popcountll(block) < K; ++p) block |= 1ULL << p;
+        DPass ps{};
+        u32 bpos[4] = {0, 0, 0, 0};
+        {
+            u32 k = 0;
+            for (u32 p = 0; p < lq; ++p)
+                if ((block >> p) & 1) bpos[k++] = p;
+        }
+        for (u32 i = 0; i < 4; ++i) ps.pos[i] = (unsigned char)bpos[i];
+        ps.sub0 = (u32)f.subs.size();
+        auto bit_of = [&](u32 p) -> u32 {
+            for (u32 i = 0; i < K; ++i)
+                if (bpos[i] == p) return i;
+            return 0;
+        };
+        for (size_t i : sel) {
+            const HOp& o = ops[i];
+            DSub sb{};
+            if (o.type == 2) {
+                sb.type = SUB_CZ;
+                sb.a = o.q[0] | (o.q[1] << 8);
+            } else if (o.type == 1) {
+                sb.type = SUB_DIAG;
+                sb.m = (u32)o.q.size();
+                for (size_t k = 0; k < o.q.size(); ++k) sb.a |= o.q[k] << (8 * k);
+                sb.data = pool_put(f, o.m);
+            } else if (o.q.size() == 1) {
+                sb.type = SUB_D1;
+                sb.a = bit_of(o.q[0]);
+                sb.data = pool_put(f, o.m);
+            } else if (o.q.size() == 2) {
+                sb.type = SUB_D2;
+                sb.a = bit_of(o.q[0]) | (bit_of(o.q[1]) << 8);
+                sb.data = pool_put(f, o.m);
+            } else {
+                // embed into the K block bits (ascending) with the identity elsewhere
+                const u32 k = (u32)o.q.size(), D = 1u << K;
+                std::vector<u32> bits(k);
+                u32 omask = 0;
+                for (u32 t = 0; t < k; ++t) {
+                    bits[t] = bit_of(o.q[t]);
+                    omask |= 1u << bits[t];
+                }
+                std::vector<cd> M((size_t)D * D, 0.0);
+                for (u32 col = 0; col < D; ++col) {
+                    u32 x = 0;
+                    for (u32 t = 0; t < k; ++t) x |= ((col >> bits[t]) & 1u) << t;
+                    for (u32 y = 0; y < (1u << k); ++y) {
+                        u32 row = col & ~omask;
+                        for (u32 t = 0; t < k; ++t) row |= ((y >> t) & 1u) << bits[t];
+                        M[(size_t)row * D + col] += o.m[(size_t)y * (1u << k) + x];
+                    }
+                }
+                sb.type = SUB_DK;
+                sb.data = pool_put(f, M);
+            }
+            f.subs.push_back(sb);
+            done[i] = true;
+            --left;
+        }
+        ps.nsub = (u32)f.subs.size() - ps.sub0;
+        f.passes.push_back(ps);
+        ++sg.npass;
+    }
+}
+
+// Stages (windows of 13 qubits including 0..3) for circuits above 13 qubits; a
+// single identity stage otherwise.  Returns the number of stages appended.
+u32 schedule_circuit(Flat& f, const HCirc& c, u32 K) {
+    const u32 n = c.ne, lq = std::min(n, kMaxLocal);
+    if (n <= kMaxLocal) {
+        DStage sg{};
+        for (u32 i = 0; i + kLowFixed < lq; ++i) set_byte(sg.win[i >> 3], i & 7, kLowFixed + i);
+        schedule_passes(f, c.ops, lq, K, sg);
+        f.stages.push_back(sg);
+        return 1;
+    }
+    std::vector<bool> done(c.ops.size(), false);
+    size_t left = c.ops.size();
+    u32 count = 0;
+    const u64 low = (1ULL << kLowFixed) - 1;
+    while (left > 0) {
+        u64 win = low, blocked = 0;
+        std::vector<size_t> sel;
+        for (size_t i = 0; i < c.ops.size(); ++i) {
+            if (done[i]) continue;
+            const u64 m = c.ops[i].mask();
+            if (m & blocked) {
+                blocked |= m;
+                continue;
+            }
+            if ((u32)__builtin_popcountll(win | m) <= lq) {
+                win |= m;
+                sel.push_back(i);
+            } else {
+                blocked |= m;
+            }
+        }
+        for (u32 q = 0; q < n && (u32)__builtin_popcountll(win) < lq; ++q) win |= 1ULL << q;
+        DStage sg{};
+        u32 local_of[64];
+        u32 wi = 0, ci = 0;
+        for (u32 q = 0; q < n; ++q) {
+            if (q < kLowFixed) {
+                local_of[q] = q;
+            } else if ((win >> q) & 1) {
+                local_of[q] = kLowFixed + wi;
+                set_byte(sg.win[wi >> 3], wi & 7, q);
+                ++wi;
+            } else {
+                set_byte(sg.comp, ci++, q);
+            }
+        }
+        std::vector<HOp> ops;
+        ops.reserve(sel.size());
+        for (size_t i : sel) {
+            HOp o = c.ops[i];
+            for (u32& q : o.q) q = local_of[q];
+            ops.push_back(std::move(o));
+            done[i] = true;
+            --left;
+        }
+        schedule_passes(f, ops, lq, K, sg);
+        f.stages.push_back(sg);
+        ++count;
+    }
+    if (count == 0) {  // a circuit without gates still needs one stage to initialise |0>
+        DStage sg{};
+        u32 wi = 0, ci = 0;
+        for (u32 q = kLowFixed; q < n; ++q) {
+            if (wi + kLowFixed < lq) {
+                set_byte(sg.win[wi >> 3], wi & 7, q);
+                ++wi;
+            } else {
+                set_byte(sg.comp, ci++, q);
+            }
+        }
+        sg.npass = 0;
+        sg.pass0 = (u32)f.passes.size();
+        f.stages.push_back(sg);
+        count = 1;
+    }
+    return count;
+}
+
+bool batch_ok(qm_batch* b, const char* what) {
+    if (!b) {
+        set_error(QM_ERR_INVALID_ARGUMENT, "%s: null batch", what);
+        return false;
+    }
+    return true;
+}
+
+}  // namespace
+}  // namespace qm
+
+using namespace qm;
+
+extern "C" {
+
+qm_batch* qm_batch_create(void) { return new qm_batch(); }
+
+void qm_batch_destroy(qm_batch* b) {
+    if (!b) return;
+    b->release();
+    delete b;
+}
+
+int qm_batch_add_circuit(qm_batch* b, UINT n, const qm_batch_gate* gates, UINT gate_count, const ITYPE* bf,
+                         const ITYPE* pf, const UINT* rot, const double* coefs, UINT term_count) {
+    const char* what = "qm_batch_add_circuit";
+    if (!batch_ok(b, what)) return -1;
+    if (n < 1 || n > QM_BATCH_MAX_QUBITS) {
+        set_error(QM_ERR_INVALID_ARGUMENT, "%s: qubit count %u outside 1..%u", what, n, QM_BATCH_MAX_QUBITS);
+        return -1;
+    }
+    if ((gate_count && !gates) || (term_count && (!bf || !pf || !rot || !coefs))) {
+        set_error(QM_ERR_INVALID_ARGUMENT, "%s: null gate or term array", what);
+        return -1;
+    }
+    HCirc c;
+    c.n = n;
+    c.ne = std::max<u32>(n, kMinQubits);
+    std::vector<long> last(c.ne, -1);  // last op touching each qubit
+    for (UINT gi = 0; gi < gate_count; ++gi) {
+        const qm_batch_gate& g = gates[gi];
+        const u32 k = g.qubit_count;
+        const bool two = g.kind == QM_BATCH_CZ || g.kind == QM_BATCH_CNOT;
+        if (g.kind > QM_BATCH_CNOT || k < 1 || k > 4 || (two && k != 2)) {
+            set_error(QM_ERR_INVALID_ARGUMENT, "%s: gate %u: kind %u with %u qubits is not supported", what, gi, g.kind, k);
+            return -1;
+        }
+        u64 mask = 0;
+        for (u32 i = 0; i < k; ++i) {
+            if (g.qubits[i] >= n || ((mask >> g.qubits[i]) & 1)) {
+                set_error(QM_ERR_INVALID_ARGUMENT, "%s: gate %u: qubit %u out of range or repeated", what, gi, g.qubits[i]);
+                return -1;
+            }
+            mask |= 1ULL << g.qubits[i];
+        }
+        if ((g.kind == QM_BATCH_DENSE || g.kind == QM_BATCH_DIAG) && !g.matrix) {
+            set_error(QM_ERR_INVALID_ARGUMENT, "%s: gate %u: null matrix", what, gi);
+            return -1;
+        }
+        HOp o;
+        o.q.assign(g.qubits, g.qubits + k);
+        const u32 d = 1u << k;
+        if (g.kind == QM_BATCH_CZ) {
+            o.type = 2;
+        } else if (g.kind == QM_BATCH_CNOT) {
+            // bit 0 <-> control, bit 1 <-> target: |c t> -> |c, t xor c>
+            o.type = 0;
+            o.m.assign(16, 0.0);
+            for (u32 col = 0; col < 4; ++col) {
+                const u32 cbit = col & 1, tbit = (col >> 1) & 1;
+                o.m[(cbit | ((tbit ^ cbit) << 1)) * 4 + col] = 1.0;
+            }
+        } else if (g.kind == QM_BATCH_DIAG) {
+            o.type = 1;
+            for (u32 i = 0; i < d; ++i) o.m.push_back(cd(g.matrix[i].re, g.matrix[i].im));
+        } else {
+            o.type = 0;
+            for (u32 i = 0; i < d * d; ++i) o.m.push_back(cd(g.matrix[i].re, g.matrix[i].im));
+        }
+        // fuse consecutive single-qubit dense / diagonal gates on the same qubit
+        if (k == 1 && o.type != 2) {
+            const long p = last[o.q[0]];
+            if (p >= 0 && c.ops[p].q.size() == 1 && c.ops[p].type != 2) {
+                HOp& prev = c.ops[p];
+                auto full = [](const HOp& h) {
+                    if (h.type == 0) return h.m;
+                    return std::vector<cd>{h.m[0], 0.0, 0.0, h.m[1]};
+                };
+                if (prev.type == 1 && o.type == 1) {
+                    prev.m[0] *= o.m[0];
+                    prev.m[1] *= o.m[1];
+                } else {
+                    std::vector<cd> prod;
+                    mat_mul(full(o), full(prev), prod, 2);  // new after old
+                    prev.m = prod;
+                    prev.type = 0;
+                }
+                continue;
+            }
+        }
+        c.ops.push_back(std::move(o));
+        for (u32 q : c.ops.back().q) last[q] = (long)c.ops.size() - 1;
+    }
+    for (UINT t = 0; t < term_count; ++t) {
+        if ((bf[t] >> n) || (pf[t] >> n)) {
+            set_error(QM_ERR_INVALID_ARGUMENT, "%s: term %u acts beyond %u qubits", what, t, n);
+            return -1;
+        }
+        DTerm tm{};
+        tm.bf = bf[t];
+        tm.pf = pf[t];
+        tm.rot = rot[t];
+        tm.coef = coefs[t];
+        c.terms.push_back(tm);
+    }
+    b->circs.push_back(std::move(c));
+    return (int)b->circs.size() - 1;
+}
+
+int qm_batch_compile(qm_batch* b, UINT block_qubits, int keep_states, unsigned dev) {
+    const char* what = "qm_batch_compile";
+    if (!batch_ok(b, what)) return QM_ERR_INVALID_ARGUMENT;
+    const u32 K = block_qubits ? block_qubits : 4;
+    if (K < 2 || K > 4) {
+        set_error(QM_ERR_INVALID_ARGUMENT, "%s: block_qubits %u outside 2..4", what, K);
+        return QM_ERR_INVALID_ARGUMENT;
+    }
+    for (const HCirc& c : b->circs)
+        for (const HOp& o : c.ops)
+            if (o.type == 0 && o.q.size() > K) {
+                set_error(QM_ERR_INVALID_ARGUMENT, "%s: a %zu-qubit dense gate needs block_qubits >= %zu", what,
+                          o.q.size(), o.q.size());
+                return QM_ERR_INVALID_ARGUMENT;
+            }
+    b->release();
+    if (!use_device(dev)) return QM_ERR_HIP;
+    Flat f;
+    std::vector<DTerm> terms;
+    std::vector<DCirc> dc;
+    u64 state_off = 0;
+    u32 lq_max = kMinQubits;
+    for (const HCirc& c : b->circs) {
+        DCirc d{};
+        d.n = c.ne;
+        d.lq = std::min(c.ne, kMaxLocal);
+        d.stage0 = (u32)f.stages.size();
+        d.nstage = schedule_circuit(f, c, K);
+        d.term0 = (u32)terms.size();
+        d.nterm = (u32)c.terms.size();
+        terms.insert(terms.end(), c.terms.begin(), c.terms.end());
+        d.state_off = state_off;
+        if (c.ne > kMaxLocal || keep_states) state_off += 1ULL << c.ne;
+        lq_max = std::max(lq_max, d.lq);
+        dc.push_back(d);
+    }
+    // descending cost: amplitudes x (passes + terms)
+    std::vector<u32> order(dc.size());
+    std::vector<double> cost(dc.size());
+    for (size_t i = 0; i < dc.size(); ++i) {
+        u64 np = 0;
+        for (u32 s = 0; s < dc[i].nstage; ++s) np += f.stages[dc[i].stage0 + s].npass;
+        cost[i] = std::ldexp(1.0, (int)dc[i].n) * (double)(np + dc[i].nterm + 1);
+        order[i] = (u32)i;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](u32 x, u32 y) { return cost[x] > cost[y]; });
+    if (f.pool.empty()) f.pool.push_back(make_double2(0, 0));
+    if (f.subs.empty()) f.subs.push_back(DSub{});
+    if (f.passes.empty()) f.passes.push_back(DPass{});
+    if (terms.empty()) terms.push_back(DTerm{});
+    b->device = (int)dev;
+    bool ok = dev_upload(&b->d_circs, dc) && dev_upload(&b->d_stages, f.stages) && dev_upload(&b->d_passes, f.passes) &&
+              dev_upload(&b->d_subs, f.subs) && dev_upload(&b->d_pool, f.pool) && dev_upload(&b->d_terms, terms) &&
+              dev_upload(&b->d_order, order);
+    if (ok && state_off) {
+        ok = hip_ok(hipMalloc((void**)&b->d_scratch, state_off * sizeof(double2)), "qm_batch_compile: state memory");
+        if (!ok) set_error(QM_ERR_OUT_OF_MEMORY, "%s: %llu bytes of state memory", what, state_off * 16ULL);
+    }
+    if (ok) ok = hip_ok(hipMalloc((void**)&b->d_energies, std::max<size_t>(1, dc.size()) * sizeof(double)), what);
+    if (ok) ok = hip_ok(hipMalloc((void**)&b->d_counter, sizeof(int)), what);
+    if (!ok) {
+        b->release();
+        return QM_ERR_HIP;
+    }
+    b->K = K;
+    b->keep = keep_states ? 1 : 0;
+    b->lq_max = lq_max;
+    b->h_circs = dc;
+    b->n_stages = f.stages.size();
+    b->n_passes = 0;
+    for (const DStage& s : f.stages) b->n_passes += s.npass;
+    b->n_subs = 0;
+    for (const DPass& p : f.passes) b->n_subs += p.nsub;
+    return QM_OK;
+}
+
+const double* qm_batch_run_async(qm_batch* b, void* stream, unsigned dev) {
+    const char* what = "qm_batch_run";
+    if (!batch_ok(b, what)) return nullptr;
+    if (b->device < 0 || b->device != (int)dev) {
+        set_error(QM_ERR_INVALID_ARGUMENT, "%s: batch not compiled for device %u", what, dev);
+        return nullptr;
+    }
+    if (!use_device(dev)) return nullptr;
+    hipStream_t st = stream_of(stream);
+    const u32 ncirc = (u32)b->h_circs.size();
+    if (ncirc == 0) return b->d_energies;
+    const size_t lds = sizeof(double2) << b->lq_max;
+    const void* fn = b->K == 2 ? (const void*)k_batch<2> : b->K == 3 ? (const void*)k_batch<3> : (const void*)k_batch<4>;
+    if (!hip_ok(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), what)) return nullptr;
+    int per_cu = 0, cus = 0;
+    if (!hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBatchThreads, lds), what)) return nullptr;
+    if (!hip_ok(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, (int)dev), what)) return nullptr;
+    if (per_cu < 1) {
+        set_error(QM_ERR_HIP, "%s: the kernel does not fit a compute unit with %zu bytes of LDS", what, lds);
+        return nullptr;
+    }
+    const u32 grid = std::min<u32>(ncirc, (u32)(per_cu * cus));
+    BatchArgs a{b->d_circs, b->d_stages, b->d_passes, b->d_subs, b->d_pool, b->d_terms, b->d_order,
+                b->d_scratch, b->d_energies, b->d_counter, ncirc, b->keep};
+    k_batch_reset<<<1, 64, 0, st>>>(b->d_counter);
+    switch (b->K) {
+        case 2: k_batch<2><<<grid, kBatchThreads, lds, st>>>(a); break;
+        case 3: k_batch<3><<<grid, kBatchThreads, lds, st>>>(a); break;
+        default: k_batch<4><<<grid, kBatchThreads, lds, st>>>(a); break;
+    }
+    if (!hip_ok(hipGetLastError(), what)) return nullptr;
+    return b->d_energies;
+}
+
+int qm_batch_run(qm_batch* b, double* energies, void* stream, unsigned dev) {
+    const double* d = qm_batch_run_async(b, stream, dev);
+    if (!d) return QM_ERR_HIP;
+    hipStream_t st = stream_of(stream);
+    const size_t n = b->h_circs.size();
+    if (n && energies) {
+        if (!hip_ok(hipMemcpyAsync(energies, d, n * sizeof(double), hipMemcpyDeviceToHost, st), "qm_batch_run"))
+            return QM_ERR_HIP;
+    }
+    if (!hip_ok(hipStreamSynchronize(st), "qm_batch_run")) return QM_ERR_HIP;
+    return QM_OK;
+}
+
+int qm_batch_get_state(qm_batch* b, UINT index, qm_complex* out, void* stream, unsigned dev) {
+    const char* what = "qm_batch_get_state";
+    if (!batch_ok(b, what)) return QM_ERR_INVALID_ARGUMENT;
+    if (!b->keep || index >= b->h_circs.size() || !out || b->device != (int)dev) {
+        set_error(QM_ERR_INVALID_ARGUMENT, "%s: no kept state %u (compile with keep_states, then run)", what, index);
+        return QM_ERR_INVALID_ARGUMENT;
+    }
+    if (!use_device(dev)) return QM_ERR_HIP;
+    hipStream_t st = stream_of(stream);
+    const DCirc& c = b->h_circs[index];
+    const u32 n = b->circs[index].n;  // the circuit's own size: extra qubits stay |0>, so the state is the prefix
+    if (!hip_ok(hipMemcpyAsync(out, b->d_scratch + c.state_off, sizeof(double2) << n, hipMemcpyDeviceToHost, st), what))
+        return QM_ERR_HIP;
+    if (!hip_ok(hipStreamSynchronize(st), what)) return QM_ERR_HIP;
+    return QM_OK;
+}
+
+void qm_batch_stats(const qm_batch* b, ITYPE* circuits, ITYPE* stages, ITYPE* passes, ITYPE* subops) {
+    if (circuits) *circuits = b ? b->circs.size() : 0;
+    if (stages) *stages = b ? b->n_stages : 0;
+    if (passes) *passes = b ? b->n_passes : 0;
+    if (subops) *subops = b ? b->n_subs : 0;
+}
+
+}  // extern "C"

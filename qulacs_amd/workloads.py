@@ -156,3 +156,52 @@ class GateLauncher:
     def launch(self, i: int, state_ptr, dim: int, stream, device: int = 0) -> None:
         fn, args = self.calls[i]
         fn(*args, state_ptr, dim, stream, device)
+
+
+# ---------------------------------------------------------------- many small circuits
+@dataclass
+class SmallCircuit:
+    """One circuit of the many-small-circuits workload: gates in batch form
+    (kind, qubits, matrix) and the observable as (bit_flip, phase_flip, rot90, coef)."""
+
+    n: int
+    layers: int
+    gates: list
+    terms: list
+    field: float
+
+
+def tfim_terms(n: int, h: float) -> list:
+    """H = sum_{i<n-1} Z_i Z_{i+1} + h sum_i X_i in mask form."""
+    terms = [(0, (1 << i) | (1 << (i + 1)), 0, 1.0) for i in range(n - 1)]
+    terms += [(1 << i, 0, 0, h) for i in range(n)]
+    return terms
+
+
+def many_small_circuits(count: int = 4096, seed: int = 3003, n_range=(8, 16), layer_range=(4, 12)) -> list:
+    """count circuits with n ~ U{n_range}, L ~ U{layer_range} layers; a layer is
+    RY(theta) then RZ(phi) on every qubit (angles U[0, 2 pi)), then CZ(i, i+1)
+    for i = 0..n-2; observable: transverse-field Ising H with h ~ U[0.5, 1.5]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(rng.integers(n_range[0], n_range[1] + 1))
+        L = int(rng.integers(layer_range[0], layer_range[1] + 1))
+        gates = []
+        for _ in range(L):
+            th = rng.uniform(0, 2 * np.pi, n)
+            ph = rng.uniform(0, 2 * np.pi, n)
+            for q in range(n):
+                gates.append(("dense", [q], rotation_matrix("RY", th[q])))
+                gates.append(("dense", [q], rotation_matrix("RZ", ph[q])))
+            for q in range(n - 1):
+                gates.append(("CZ", [q, q + 1], None))
+        h = float(rng.uniform(0.5, 1.5))
+        out.append(SmallCircuit(n, L, gates, tfim_terms(n, h), h))
+    return out
+
+
+def small_circuit_alg_flops(c: SmallCircuit) -> float:
+    """FP64 flops of the fused form: one 2x2 complex matrix per qubit per layer
+    (8 real multiply-adds = 16 flops per amplitude)."""
+    return 16.0 * c.layers * c.n * (1 << c.n)

@@ -1,0 +1,46 @@
+"""Host-side validation of the batched-circuit entry points (no device work)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from qulacs_amd._lib import QulacsMI355Error
+from qulacs_amd.batch import Batch, pauli_masks
+
+
+def test_pauli_masks_match_reference_form():
+    # X on 0, Y on 2, Z on 3 -> bit flip {0, 2}, phase flip {2, 3}, one Y
+    assert pauli_masks([0, 2, 3], [1, 2, 3]) == (0b101, 0b1100, 1)
+
+
+@pytest.mark.parametrize(
+    "n,gates",
+    [
+        (0, []),
+        (21, []),
+        (3, [("dense", [3], np.eye(2))]),
+        (3, [("dense", [1, 1], np.eye(4))]),
+        (6, [("dense", [0, 1, 2, 3, 4], np.eye(32))]),
+        (3, [("CZ", [0], None)]),
+    ],
+)
+def test_add_circuit_rejects_bad_input(n, gates):
+    b = Batch()
+    with pytest.raises(QulacsMI355Error):
+        b.add(n, gates, [])
+    assert len(b) == 0
+
+
+def test_term_beyond_qubits_rejected():
+    b = Batch()
+    with pytest.raises(QulacsMI355Error):
+        b.add(3, [], [(1 << 3, 0, 0, 1.0)])
+
+
+def test_compile_argument_checks_before_device_work():
+    b = Batch()
+    b.add(5, [("dense", [0, 1, 2, 3], np.eye(16))], [])
+    with pytest.raises(QulacsMI355Error):
+        b.compile(block_qubits=3)  # 4-qubit gate needs a 4-qubit block
+    with pytest.raises(QulacsMI355Error):
+        b.compile(block_qubits=5)

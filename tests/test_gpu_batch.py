@@ -1,0 +1,81 @@
+"""Batched circuits (qm_batch_*) on the GPU against the oracle.
+
+Bar: final states within 1e-10 relative L2 of the oracle; energies within
+1e-10 * max(1, sum |c_k|); identical energies on repeated runs (the per-circuit
+reduction has a fixed order)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from qulacs_amd.batch import Batch
+from qulacs_amd.workloads import many_small_circuits
+from tests.batch_util import oracle_run, random_batch_circuit, random_terms
+from tests.gpu_util import rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(oracle, b, circuits, states=True):
+    e = b.run()
+    for i, (n, gates, terms) in enumerate(circuits):
+        psi, want = oracle_run(oracle, n, gates, terms)
+        scale = max(1.0, sum(abs(t[3]) for t in terms))
+        if states:
+            assert rel(b.state(i), psi) < 1e-10, (i, n, len(gates))
+        assert abs(e[i] - want) < 1e-10 * scale, (i, n, e[i], want)
+    return e
+
+
+@pytest.mark.parametrize("block", [4, 3, 2])
+def test_random_circuits_all_sizes(oracle, block):
+    rng = np.random.default_rng(100 + block)
+    b = Batch()
+    circuits = []
+    for n in [1, 2, 3, 4, 5, 7, 9, 12, 13, 14, 15, 16, 17]:
+        gates = random_batch_circuit(rng, n, 40 if n < 14 else 60, max_dense=block)
+        terms = random_terms(rng, n, 6)
+        circuits.append((n, gates, terms))
+        b.add(n, gates, terms)
+    b.compile(block_qubits=block, keep_states=True)
+    print("block", block, b.stats(), flush=True)
+    e1 = _check(oracle, b, circuits)
+    e2 = b.run()
+    assert np.array_equal(e1, e2)
+    b.close()
+
+
+def test_empty_and_trivial_circuits(oracle):
+    b = Batch()
+    circuits = [(3, [], [(0, 1, 0, 0.5), (1, 0, 0, 2.0)]), (15, [], [(0, 1 << 14, 0, 1.0)]), (6, [], [])]
+    for c in circuits:
+        b.add(*c)
+    b.compile(keep_states=True)
+    e = _check(oracle, b, circuits)
+    assert e[0] == 0.5 and e[1] == 1.0 and e[2] == 0.0
+
+
+def test_many_small_circuits_workload_subset(oracle):
+    cs = many_small_circuits(48, seed=77, n_range=(8, 16), layer_range=(2, 5))
+    b = Batch()
+    circuits = []
+    for c in cs:
+        b.add(c.n, c.gates, c.terms)
+        circuits.append((c.n, c.gates, c.terms))
+    times = b.autotune(repeats=1, keep_states=True)
+    print("autotune", times, b.stats(), flush=True)
+    _check(oracle, b, circuits)
+
+
+def test_batch_errors():
+    from qulacs_amd._lib import QulacsMI355Error
+
+    b = Batch()
+    b.add(4, [("dense", [0, 1, 2], np.eye(8))], [])
+    with pytest.raises(QulacsMI355Error):
+        b.compile(block_qubits=2)
+    with pytest.raises(QulacsMI355Error):
+        b.state(0)  # not compiled with keep_states
+    b.compile(block_qubits=3)
+    e = b.run()
+    assert e.shape == (1,) and e[0] == 0.0
