@@ -250,6 +250,253 @@ def dm_sweep(args, lib_mod, hip):
             "alg_GBps": round(alg / (ms * 1e-3) / 1e9, 1), "trace_after": tr}
 
 
+FP64_PEAK_TFLOPS = 78.6
+
+
+def many_small(args, lib_mod, hip):
+    """Many small circuits (n ~ U{8..16}, 4..12 layers of RY RZ + CZ chain, TFIM
+    energy each) through the batched executor: one launch per step."""
+    from oracle import oracle as O
+    from qulacs_amd.batch import Batch
+    from qulacs_amd.workloads import many_small_circuits, small_circuit_alg_flops
+
+    cs = many_small_circuits(args.small_count, seed=args.seed + 1001)
+    b = Batch()
+    t0 = time.perf_counter()
+    for c in cs:
+        b.add(c.n, c.gates, c.terms)
+    t1 = time.perf_counter()
+    tune = b.autotune(repeats=2)
+    t2 = time.perf_counter()
+    stream = lib_mod.call("allocate_cuda_stream_host", 1, 0)
+    hstream = C.c_void_p.from_address(stream).value
+    b.run(stream)
+    ev = hip.create(2)
+    hip.record(ev[0], hstream)
+    for _ in range(args.small_steps):
+        b.run_async(stream)
+    hip.record(ev[1], hstream)
+    ms = hip.elapsed_ms(ev[0], ev[1]) / args.small_steps
+    hip.destroy(ev)
+    energies = b.run(stream)
+    flops = sum(small_circuit_alg_flops(c) for c in cs)
+    stats = b.stats()
+    b.close()
+
+    # the per-gate path on an evenly spaced subset: one C ABI call per gate and
+    # one observable reduction per circuit, stream-ordered, as a plain loop would
+    L = lib_mod.lib()
+    stride = max(1, len(cs) // args.small_plain)
+    sub = cs[::stride]
+    lib_mod.call("qm_set_synchronous", 0)
+    t3 = time.perf_counter()
+    plain_gates = 0
+    for c in sub:
+        dim = 1 << c.n
+        st = lib_mod.call("allocate_quantum_state_host", dim, 0)
+        lib_mod.call("initialize_quantum_state_host", st, dim, stream, 0)
+        for kind, q, m in c.gates:
+            if kind == "CZ":
+                L.CZ_gate_host(q[0], q[1], st, dim, stream, 0)
+            else:
+                L.single_qubit_dense_matrix_gate_host(q[0], _cm(lib_mod, m), st, dim, stream, 0)
+        plain_gates += len(c.gates)
+        t = len(c.terms)
+        bf = (C.c_ulonglong * t)(*[x[0] for x in c.terms])
+        pf = (C.c_ulonglong * t)(*[x[1] for x in c.terms])
+        rot = (C.c_uint * t)(*[x[2] for x in c.terms])
+        co = (lib_mod.QmComplex * t)(*[lib_mod.QmComplex(x[3], 0.0) for x in c.terms])
+        L.qm_observable_expectation_host(bf, pf, rot, co, t, st, dim, stream, 0)
+        lib_mod.call("release_quantum_state_host", st, 0)
+    lib_mod.check()
+    t4 = time.perf_counter()
+    lib_mod.call("qm_set_synchronous", 1)
+    lib_mod.call("release_cuda_stream_host", stream, 1, 0)
+    total_gates = sum(len(c.gates) for c in cs)
+    plain_s = (t4 - t3) * total_gates / plain_gates
+
+    out = {
+        "circuits": len(cs), "qubits": "U{8..16}", "layers": "U{4..12}", "gates": total_gates,
+        "ms_per_step": round(ms, 3), "circuits_per_s": round(len(cs) / (ms * 1e-3), 1),
+        "fp64": {"bound": "fp64-valu", "achieved": round(flops / (ms * 1e-3) / 1e12, 2), "peak": FP64_PEAK_TFLOPS,
+                 "unit": "TFLOP/s", "frac": round(flops / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
+                 "flops_per_step": flops},
+        "autotune_s": {str(k): round(v, 5) for k, v in tune.items()}, "schedule": stats,
+        "host_add_s": round(t1 - t0, 2), "host_compile_and_tune_s": round(t2 - t1, 2),
+        "plain_gpu": {"circuits_per_s": round(len(cs) / plain_s, 1),
+                      "sample": f"{len(sub)} evenly spaced circuits ({plain_gates} gates), one C ABI call per gate, "
+                                f"scaled by gate count to all {len(cs)}"},
+        "energy_checksum": float(np.sum(energies)),
+    }
+    if not args.no_cpu_baseline:
+        k, kind = (O.Reference(), "reference") if O.reference_available() else (O.Oracle(), "port")
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        threads = min(threads, os.cpu_count() or threads)
+        k.set_num_threads(threads)
+        spent, done_circ, done_gates = 0.0, 0, 0
+        for c in sub:
+            t0 = time.perf_counter()
+            psi = np.zeros(1 << c.n, dtype=np.complex128)
+            psi[0] = 1.0
+            for kind, q, m in c.gates:
+                if kind == "CZ":
+                    k.two("CZ", q[0], q[1], psi)
+                else:
+                    k.dense1(q[0], m, psi)
+            e = 0.0
+            for bf, pf, rot, coef in c.terms:
+                tl, pl = _mask_lists(bf, pf, c.n)
+                e += coef * k.expectation(tl, pl, psi)
+            spent += time.perf_counter() - t0
+            done_circ += 1
+            done_gates += len(c.gates)
+            if spent >= args.cpu_seconds / 2:
+                break
+        cpu_s = spent * total_gates / done_gates
+        out["cpu_baseline"] = {"circuits_per_s": round(len(cs) / cpu_s, 1), "cores": threads, "kind": kind,
+                               "sample": f"first {done_circ} of the {len(sub)} evenly spaced circuits "
+                                         f"({done_gates} gates, {spent:.1f} s, {threads} OpenMP threads), scaled by "
+                                         f"gate count to all {len(cs)}"}
+    return out
+
+
+def brickwork(args, lib_mod, hip):
+    """20-qubit brickwork (10 layers of Haar 1q on every qubit + a CNOT brick),
+    16-term observable and 1024 samples: the per-gate path end to end, the
+    batched executor on the same circuit, and the reference csim on the host."""
+    from oracle import oracle as O
+    from qulacs_amd.batch import Batch
+    from qulacs_amd.workloads import brickwork_circuit
+
+    n = 20
+    dim = 1 << n
+    gates, obs, u = brickwork_circuit(n, 10, 16, 1024, seed=args.seed + 2)
+    L = lib_mod.lib()
+    stream = lib_mod.call("allocate_cuda_stream_host", 1, 0)
+    st = lib_mod.call("allocate_quantum_state_host", dim, 0)
+    calls = []
+    for kind, q, m in gates:
+        if kind == "CNOT":
+            calls.append((L.CNOT_gate_host, (q[0], q[1])))
+        else:
+            calls.append((L.single_qubit_dense_matrix_gate_host, (q[0], _cm(lib_mod, m))))
+    t = len(obs)
+    bf = (C.c_ulonglong * t)(*[x[0] for x in obs])
+    pf = (C.c_ulonglong * t)(*[x[1] for x in obs])
+    rot = (C.c_uint * t)(*[x[2] for x in obs])
+    co = (lib_mod.QmComplex * t)(*[lib_mod.QmComplex(x[3], 0.0) for x in obs])
+    uu = np.ascontiguousarray(u)
+    samples = np.zeros(u.size, dtype=np.uint64)
+
+    def run():
+        L.initialize_quantum_state_host(st, dim, stream, 0)
+        for fn, a in calls:
+            fn(*a, st, dim, stream, 0)
+        e = L.qm_observable_expectation_host(bf, pf, rot, co, t, st, dim, stream, 0)
+        L.qm_sampling_host(uu.ctypes.data_as(lib_mod.PD), uu.size, samples.ctypes.data_as(lib_mod.PI), st, dim,
+                           stream, 0)
+        return e.re
+
+    from qulacs_amd.batch import FusedCircuit
+
+    fused = FusedCircuit(n, gates)
+    fused_samples = np.zeros(u.size, dtype=np.uint64)
+
+    def run_fused():
+        L.initialize_quantum_state_host(st, dim, stream, 0)
+        L.qm_circuit_apply(fused.h, st, dim, stream, 0)
+        e = L.qm_observable_expectation_host(bf, pf, rot, co, t, st, dim, stream, 0)
+        L.qm_sampling_host(uu.ctypes.data_as(lib_mod.PD), uu.size, fused_samples.ctypes.data_as(lib_mod.PI), st,
+                           dim, stream, 0)
+        return e.re
+
+    lib_mod.call("qm_set_synchronous", 0)
+    run()
+    run_fused()
+    lib_mod.check()
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        e_fused = run_fused()
+    t1 = time.perf_counter()
+    fused_ms = (t1 - t0) * 1e3 / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        e_gpu = run()
+    t1 = time.perf_counter()
+    lib_mod.check()
+    lib_mod.call("qm_set_synchronous", 1)
+    gpu_ms = (t1 - t0) * 1e3 / reps
+    fused_stats = fused.stats()
+    fused.close()
+    psi_gpu = np.zeros(dim, dtype=np.complex128)
+    lib_mod.call("copy_quantum_state_from_device_to_host", psi_gpu.ctypes.data, st, dim, stream, 0)
+    lib_mod.call("release_quantum_state_host", st, 0)
+    lib_mod.call("release_cuda_stream_host", stream, 1, 0)
+
+    b = Batch()
+    b.add(n, gates, obs)
+    b.compile(4)
+    e_batch = b.run()[0]
+    t2 = time.perf_counter()
+    for _ in range(reps):
+        b.run()
+    batch_ms = (time.perf_counter() - t2) * 1e3 / reps
+    b.close()
+
+    scale = sum(abs(x[3]) for x in obs)
+    out = {"workload": "20-qubit brickwork, 10 layers (200 Haar 1q + 95 CNOT), 16-term observable, 1024 samples",
+           "gpu_ms": round(gpu_ms, 3), "fused_ms": round(fused_ms, 3), "fused_schedule": fused_stats,
+           "batched_executor_ms": round(batch_ms, 3), "energy": e_gpu,
+           "fused_matches": bool(abs(e_fused - e_gpu) <= 1e-10 * scale and np.array_equal(fused_samples, samples))}
+    out["batched_energy_matches"] = bool(abs(e_batch - e_gpu) <= 1e-10 * scale)
+    if not args.no_cpu_baseline:
+        k, kind = (O.Reference(), "reference") if O.reference_available() else (O.Oracle(), "port")
+        o = O.Oracle()
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        threads = min(threads, os.cpu_count() or threads)
+        k.set_num_threads(threads)
+        best = float("inf")
+        for _ in range(3):
+            t0 = time.perf_counter()
+            psi = np.zeros(dim, dtype=np.complex128)
+            psi[0] = 1.0
+            for kind_g, q, m in gates:
+                if kind_g == "CNOT":
+                    k.two("CNOT", q[0], q[1], psi)
+                else:
+                    k.dense1(q[0], m, psi)
+            e_cpu = 0.0
+            for x in obs:
+                tl, pl = _mask_lists(x[0], x[1], n)
+                e_cpu += x[3] * k.expectation(tl, pl, psi)
+            s_cpu = o.sampling(psi, uu)
+            best = min(best, time.perf_counter() - t0)
+        out["cpu_baseline"] = {"ms": round(best * 1e3, 2), "cores": threads, "kind": kind,
+                               "sample": "whole workload, best of 3 (sampling by the in-repo port)"}
+        out["energy_matches"] = bool(abs(e_cpu - e_gpu) <= 1e-10 * scale)
+        out["samples_identical"] = bool(np.array_equal(s_cpu, samples))
+        out["state_rel_l2"] = float(np.linalg.norm(psi_gpu - psi) / np.linalg.norm(psi))
+    return out
+
+
+def _mask_lists(bf, pf, n):
+    t, p = [], []
+    for q in range(n):
+        x, z = (bf >> q) & 1, (pf >> q) & 1
+        if x or z:
+            t.append(q)
+            p.append(2 if (x and z) else (1 if x else 3))
+    return t, p
+
+
+def _cm(lib_mod, m):
+    a = np.ascontiguousarray(m, dtype=np.complex128).ravel()
+    arr = (lib_mod.QmComplex * a.size)()
+    C.memmove(arr, a.ctypes.data, a.nbytes)
+    return arr
+
+
 def cpu_baseline(args, gates, n):
     """Reference csim on the host cores, on a time-boxed prefix of the same sweep."""
     from oracle import oracle as O
@@ -305,6 +552,10 @@ def main() -> None:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-dm", action="store_true")
+    ap.add_argument("--small-count", type=int, default=4096)
+    ap.add_argument("--small-steps", type=int, default=5)
+    ap.add_argument("--small-plain", type=int, default=64)
+    ap.add_argument("--no-small", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the sweep is a single 30-qubit "
@@ -322,6 +573,13 @@ def main() -> None:
     if not args.no_dm:
         dm = dm_sweep(args, _lib, hip)
         print(f"[bench] dm sweep {dm['ms']} ms", file=sys.stderr, flush=True)
+    c20 = brickwork(args, _lib, hip)
+    print(f"[bench] 20q brickwork {c20['gpu_ms']} ms", file=sys.stderr, flush=True)
+    small = None
+    if not args.no_small:
+        small = many_small(args, _lib, hip)
+        print(f"[bench] many small circuits {small['circuits_per_s']} circuits/s ({small['ms_per_step']} ms)",
+              file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args, sv["gates"], args.qubits)
@@ -348,6 +606,8 @@ def main() -> None:
         "per_kind": sv["per_kind"],
         "per_kernel": sv["per_kernel"],
         "dm_sweep": dm,
+        "brickwork_20q": c20,
+        "many_small_circuits": small,
         "norm_after": sv["norm_after"],
     }
     print(json.dumps(line), flush=True)

@@ -286,6 +286,23 @@ QM_EXPORT int qm_batch_get_state(qm_batch* batch, UINT index, qm_complex* out, v
 /* Schedule statistics: number of circuits, stages, passes and sub-operations. */
 QM_EXPORT void qm_batch_stats(const qm_batch* batch, ITYPE* circuits, ITYPE* stages, ITYPE* passes, ITYPE* subops);
 
+/* ------------------------------------------------------------------ fused circuits
+ * One circuit applied in place to an existing state with the batch scheduler:
+ * gates are grouped into stages whose qubits fit a window of `local_qubits`
+ * (8..13; qubits 0..3 always included), one launch per stage, each work-group
+ * holding 2^local_qubits-amplitude tiles in LDS and applying every gate of the
+ * stage to a tile before writing it back.  Replaces QuantumCircuit::update_quantum_state's
+ * one launch per gate (src/cppsim/circuit.cpp:46-57) for circuits of 4..40
+ * qubits made of the batch gate kinds.  create() does host work only. */
+#define QM_CIRCUIT_MAX_QUBITS 40u
+typedef struct qm_circuit qm_circuit;
+/* block_qubits 2..4 (0 = 4), local_qubits 8..13 (0 = 13); NULL on error. */
+QM_EXPORT qm_circuit* qm_circuit_create(UINT qubit_count, const qm_batch_gate* gates, UINT gate_count, UINT block_qubits, UINT local_qubits);
+QM_EXPORT void qm_circuit_destroy(qm_circuit* circuit);
+/* Applies the circuit to `state` (dim = 2^qubit_count) in stream order. */
+QM_EXPORT int qm_circuit_apply(qm_circuit* circuit, void* state, ITYPE dim, void* stream, unsigned int device_number);
+QM_EXPORT void qm_circuit_stats(const qm_circuit* circuit, ITYPE* stages, ITYPE* passes, ITYPE* subops);
+
 #ifdef __cplusplus
 }
 #endif

@@ -137,6 +137,10 @@ _SIGS: dict[str, tuple] = {
     "qm_batch_run_async": (VP, [VP, VP, UINT]),
     "qm_batch_get_state": (C.c_int, [VP, UINT, PC, VP, UINT]),
     "qm_batch_stats": (None, [VP, PI, PI, PI, PI]),
+    "qm_circuit_create": (VP, [UINT, C.POINTER(QmBatchGate), UINT, UINT, UINT]),
+    "qm_circuit_destroy": (None, [VP]),
+    "qm_circuit_apply": (C.c_int, [VP, VP, ITYPE, VP, UINT]),
+    "qm_circuit_stats": (None, [VP, PI, PI, PI]),
 }
 
 _lib = None

@@ -66,20 +66,7 @@ class Batch:
     def add(self, n: int, gates, terms) -> int:
         """Add a circuit on n qubits; returns its index."""
         gates = list(gates)
-        arr = (_lib.QmBatchGate * max(1, len(gates)))()
-        keep = []  # matrices must stay alive until the call returns
-        for i, (kind, qubits, matrix) in enumerate(gates):
-            if len(qubits) > 4 or kind not in _KIND:
-                raise _lib.QulacsMI355Error(f"batch gate {i}: {kind} on {len(qubits)} qubits is not supported")
-            g = arr[i]
-            g.kind = _KIND[kind]
-            g.qubit_count = len(qubits)
-            for j, q in enumerate(qubits):
-                g.qubits[j] = int(q)
-            if matrix is not None:
-                m = np.ascontiguousarray(np.asarray(matrix, dtype=np.complex128)).ravel()
-                keep.append(m)
-                g.matrix = C.cast(m.ctypes.data, _lib.PC)
+        arr, keep = _marshal(gates)  # keep: the matrices stay alive until the call returns
         terms = list(terms)
         t = max(1, len(terms))
         bf = (C.c_ulonglong * t)(*[int(x[0]) for x in terms])
@@ -136,3 +123,54 @@ class Batch:
             raise _lib.QulacsMI355Error("autotune: no block size can hold the batch's gates")
         self.compile(min(times, key=times.get), keep_states)
         return times
+
+
+class FusedCircuit:
+    """One circuit applied in place to a device state by the stage scheduler
+    (qm_circuit_*): one launch per stage of gates that fit a window of
+    `local_qubits` qubits, register-blocked passes inside each stage."""
+
+    def __init__(self, n: int, gates, block_qubits: int = 4, local_qubits: int = 13):
+        gates = list(gates)
+        arr, self._keep = _marshal(gates)
+        self.n = n
+        self.h = _lib.call("qm_circuit_create", int(n), arr, len(gates), int(block_qubits), int(local_qubits))
+        if not self.h:
+            raise _lib.QulacsMI355Error("qm_circuit_create failed")
+
+    def apply(self, state_ptr, stream=None, device: int = 0) -> None:
+        _lib.call("qm_circuit_apply", self.h, state_ptr, 1 << self.n, stream, device)
+
+    def stats(self) -> dict:
+        v = [C.c_ulonglong() for _ in range(3)]
+        _lib.call("qm_circuit_stats", self.h, *[C.byref(x) for x in v])
+        return dict(zip(("stages", "passes", "subops"), (int(x.value) for x in v)))
+
+    def close(self) -> None:
+        if self.h:
+            _lib.call("qm_circuit_destroy", self.h)
+            self.h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _marshal(gates):
+    arr = (_lib.QmBatchGate * max(1, len(gates)))()
+    keep = []
+    for i, (kind, qubits, matrix) in enumerate(gates):
+        if len(qubits) > 4 or kind not in _KIND:
+            raise _lib.QulacsMI355Error(f"batch gate {i}: {kind} on {len(qubits)} qubits is not supported")
+        g = arr[i]
+        g.kind = _KIND[kind]
+        g.qubit_count = len(qubits)
+        for j, q in enumerate(qubits):
+            g.qubits[j] = int(q)
+        if matrix is not None:
+            m = np.ascontiguousarray(np.asarray(matrix, dtype=np.complex128)).ravel()
+            keep.append(m)
+            g.matrix = C.cast(m.ctypes.data, _lib.PC)
+    return arr, keep

@@ -62,8 +62,8 @@ struct DCirc {
 // kernel indexes them with shifts (a runtime index into an array would put it
 // in private memory).
 struct DStage {
-    u64 win[2];  // local bit 4 + i <-> qubit byte i of win (i < lq - 4 <= 9)
-    u64 comp;    // tile bit i <-> qubit byte i of comp (i < n - lq <= 7)
+    u64 win[2];   // local bit 4 + i <-> qubit byte i of win (i < lq - 4 <= 9)
+    u64 comp[4];  // tile bit i <-> qubit byte i of comp (i < n - lq <= 32)
     u32 npass, pass0;
 };
 __host__ __device__ __forceinline__ u32 byte_at(u64 w, u32 i) { return (u32)(w >> (8 * i)) & 0xffu; }
@@ -222,9 +222,96 @@ __global__ void k_batch_reset(int* counter) {
     if (threadIdx.x == 0) *counter = 0;
 }
 
+// hi_tab[j] = global offset of local index j << 4 (the window qubits 4.. of the stage)
+__device__ __forceinline__ void build_hi_tab(u32* hi_tab, u32 lq, const DStage& sg) {
+    const u64 win0 = sg.win[0], win1 = sg.win[1];
+    for (u32 j = threadIdx.x; j < (1u << (lq - kLowFixed)); j += blockDim.x) {
+        u32 o = 0;
+        for (u32 i = 0; i + kLowFixed < lq; ++i) o |= ((j >> i) & 1u) << byte_at(i < 8 ? win0 : win1, i & 7);
+        hi_tab[j] = o;
+    }
+}
+
+// global index of the first amplitude of tile t (tile bits deposited on the non-window qubits)
+__device__ __forceinline__ u64 tile_base(u64 t, u32 tl, const DStage& sg) {
+    const u64 c0 = sg.comp[0], c1 = sg.comp[1], c2 = sg.comp[2], c3 = sg.comp[3];
+    u64 tb = 0;
+    for (u32 i = 0; i < tl; ++i) {
+        const u64 w = i < 8 ? c0 : i < 16 ? c1 : i < 24 ? c2 : c3;
+        tb |= ((t >> i) & 1ULL) << byte_at(w, i & 7);
+    }
+    return tb;
+}
+
+__device__ __forceinline__ u64 tile_index(u64 tb, const u32* hi_tab, u32 i) {
+    return tb + hi_tab[i >> kLowFixed] + (i & ((1u << kLowFixed) - 1));
+}
+
+// Applies passes [pass0, pass0 + npass) to the 2^lq amplitudes in LDS; ends
+// with a barrier after every pass.
+template <int K>
+__device__ __forceinline__ void run_passes(double2* st, u32 lq, const DPass* __restrict__ passes, u32 pass0,
+                                           u32 npass, const DSub* __restrict__ subs,
+                                           const double2* __restrict__ pool) {
+    constexpr int D = 1 << K;
+    const u32 L = 1u << lq;
+    for (u32 p = 0; p < npass; ++p) {
+        const DPass ps = passes[pass0 + p];
+        u32 off[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            u32 o = 0;
+#pragma unroll
+            for (int i = 0; i < K; ++i) o |= (u32)((j >> i) & 1) << ps.pos[i];
+            off[j] = o;
+        }
+        for (u32 g = threadIdx.x; g < (L >> K); g += blockDim.x) {
+            u32 base = g;
+#pragma unroll
+            for (int i = 0; i < K; ++i) {
+                const u32 q = ps.pos[i];
+                base = (base & ((1u << q) - 1)) | ((base >> q) << (q + 1));
+            }
+            const u32 sb = slot_of(base);
+            double2 v[D];
+#pragma unroll
+            for (int j = 0; j < D; ++j) v[j] = st[sb ^ slot_of(off[j])];
+            for (u32 k = 0; k < ps.nsub; ++k) {
+                const DSub sub = subs[ps.sub0 + k];
+                const double2* mp = pool + sub.data;
+                switch (sub.type) {
+                    case SUB_D1: apply_d1<K>(v, byte_at(sub.a, 0), mp); break;
+                    case SUB_D2: apply_d2<K>(v, byte_at(sub.a, 0), byte_at(sub.a, 1), mp); break;
+                    case SUB_DK: sub_dk<K>(v, mp); break;
+                    case SUB_DIAG:
+#pragma unroll
+                        for (int j = 0; j < D; ++j) {
+                            const u32 li = base | off[j];
+                            u32 idx = 0;
+                            for (u32 i = 0; i < sub.m; ++i) idx |= ((li >> byte_at(sub.a, i)) & 1u) << i;
+                            v[j] = cmul(v[j], mp[idx]);
+                        }
+                        break;
+                    case SUB_CZ:
+#pragma unroll
+                        for (int j = 0; j < D; ++j) {
+                            const u32 li = base | off[j];
+                            if ((li >> byte_at(sub.a, 0)) & (li >> byte_at(sub.a, 1)) & 1u)
+                                v[j] = make_double2(-v[j].x, -v[j].y);
+                        }
+                        break;
+                    default: break;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < D; ++j) st[sb ^ slot_of(off[j])] = v[j];
+        }
+        __syncthreads();
+    }
+}
+
 template <int K>
 __global__ __launch_bounds__(kBatchThreads) void k_batch(BatchArgs a) {
-    constexpr int D = 1 << K;
     extern __shared__ double2 st[];
     __shared__ u32 hi_tab[1u << (kMaxLocal - kLowFixed)];
     __shared__ double red[16];
@@ -244,84 +331,22 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(BatchArgs a) {
 
         for (u32 s = 0; s < c.nstage; ++s) {
             const DStage sg = a.stages[c.stage0 + s];
-            const u64 win0 = sg.win[0], win1 = sg.win[1];
             if (tl > 0) {
-                for (u32 j = tid; j < (L >> kLowFixed); j += bd) {
-                    u32 o = 0;
-                    for (u32 i = 0; i + kLowFixed < lq; ++i) o |= ((j >> i) & 1u) << byte_at(i < 8 ? win0 : win1, i & 7);
-                    hi_tab[j] = o;
-                }
+                build_hi_tab(hi_tab, lq, sg);
                 __syncthreads();
             }
             for (u32 t = 0; t < (1u << tl); ++t) {
-                u64 tb = 0;
-                for (u32 i = 0; i < tl; ++i) tb |= (u64)((t >> i) & 1u) << byte_at(sg.comp, i);
+                const u64 tb = tile_base(t, tl, sg);
                 if (s == 0) {
                     // |0...0>: slot_of(0) == 0 and amplitude 0 lies in tile 0
                     for (u32 i = tid; i < L; i += bd) st[i] = make_double2((i == 0 && t == 0) ? 1.0 : 0.0, 0.0);
                 } else {
-                    for (u32 i = tid; i < L; i += bd)
-                        st[slot_of(i)] = gs[tb + hi_tab[i >> kLowFixed] + (i & ((1u << kLowFixed) - 1))];
+                    for (u32 i = tid; i < L; i += bd) st[slot_of(i)] = gs[tile_index(tb, hi_tab, i)];
                 }
                 __syncthreads();
-
-                for (u32 p = 0; p < sg.npass; ++p) {
-                    const DPass ps = a.passes[sg.pass0 + p];
-                    u32 off[D];
-#pragma unroll
-                    for (int j = 0; j < D; ++j) {
-                        u32 o = 0;
-#pragma unroll
-                        for (int i = 0; i < K; ++i) o |= (u32)((j >> i) & 1) << ps.pos[i];
-                        off[j] = o;
-                    }
-                    for (u32 g = tid; g < (L >> K); g += bd) {
-                        u32 base = g;
-#pragma unroll
-                        for (int i = 0; i < K; ++i) {
-                            const u32 q = ps.pos[i];
-                            base = (base & ((1u << q) - 1)) | ((base >> q) << (q + 1));
-                        }
-                        const u32 sb = slot_of(base);
-                        double2 v[D];
-#pragma unroll
-                        for (int j = 0; j < D; ++j) v[j] = st[sb ^ slot_of(off[j])];
-                        for (u32 k = 0; k < ps.nsub; ++k) {
-                            const DSub sub = a.subs[ps.sub0 + k];
-                            const double2* mp = a.pool + sub.data;
-                            switch (sub.type) {
-                                case SUB_D1: apply_d1<K>(v, byte_at(sub.a, 0), mp); break;
-                                case SUB_D2: apply_d2<K>(v, byte_at(sub.a, 0), byte_at(sub.a, 1), mp); break;
-                                case SUB_DK: sub_dk<K>(v, mp); break;
-                                case SUB_DIAG:
-#pragma unroll
-                                    for (int j = 0; j < D; ++j) {
-                                        const u32 li = base | off[j];
-                                        u32 idx = 0;
-                                        for (u32 i = 0; i < sub.m; ++i) idx |= ((li >> byte_at(sub.a, i)) & 1u) << i;
-                                        v[j] = cmul(v[j], mp[idx]);
-                                    }
-                                    break;
-                                case SUB_CZ:
-#pragma unroll
-                                    for (int j = 0; j < D; ++j) {
-                                        const u32 li = base | off[j];
-                                        if ((li >> byte_at(sub.a, 0)) & (li >> byte_at(sub.a, 1)) & 1u)
-                                            v[j] = make_double2(-v[j].x, -v[j].y);
-                                    }
-                                    break;
-                                default: break;
-                            }
-                        }
-#pragma unroll
-                        for (int j = 0; j < D; ++j) st[sb ^ slot_of(off[j])] = v[j];
-                    }
-                    __syncthreads();
-                }
-
+                run_passes<K>(st, lq, a.passes, sg.pass0, sg.npass, a.subs, a.pool);
                 if (tl > 0) {
-                    for (u32 i = tid; i < L; i += bd)
-                        gs[tb + hi_tab[i >> kLowFixed] + (i & ((1u << kLowFixed) - 1))] = st[slot_of(i)];
+                    for (u32 i = tid; i < L; i += bd) gs[tile_index(tb, hi_tab, i)] = st[slot_of(i)];
                 } else if (a.keep) {
                     for (u32 i = tid; i < L; i += bd) gs[i] = st[slot_of(i)];
                 }
@@ -333,6 +358,31 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(BatchArgs a) {
         const double acc = tl ? energy_sum<false>(gs, st, c, a.terms) : energy_sum<true>(gs, st, c, a.terms);
         const double total = block_sum(acc, red);
         if (tid == 0) a.energies[ci] = total;
+        __syncthreads();
+    }
+}
+
+// One stage of a single-state circuit: every work-group takes tiles of 2^lq
+// amplitudes (grid-stride), runs the stage's passes on each in LDS and writes
+// it back in place.  Tiles of one stage are disjoint, so work-groups never
+// touch the same amplitudes.
+template <int K>
+__global__ __launch_bounds__(kBatchThreads) void k_stage(double2* __restrict__ state, u32 lq, u32 tl, DStage sg,
+                                                         const DPass* __restrict__ passes,
+                                                         const DSub* __restrict__ subs,
+                                                         const double2* __restrict__ pool) {
+    extern __shared__ double2 st[];
+    __shared__ u32 hi_tab[1u << (kMaxLocal - kLowFixed)];
+    const u32 L = 1u << lq;
+    build_hi_tab(hi_tab, lq, sg);
+    __syncthreads();
+    const u64 ntiles = 1ULL << tl;
+    for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const u64 tb = tile_base(t, tl, sg);
+        for (u32 i = threadIdx.x; i < L; i += blockDim.x) st[slot_of(i)] = state[tile_index(tb, hi_tab, i)];
+        __syncthreads();
+        run_passes<K>(st, lq, passes, sg.pass0, sg.npass, subs, pool);
+        for (u32 i = threadIdx.x; i < L; i += blockDim.x) state[tile_index(tb, hi_tab, i)] = st[slot_of(i)];
         __syncthreads();
     }
 }
@@ -388,6 +438,30 @@ struct qm_batch {
         d_terms = nullptr; d_order = nullptr; d_scratch = nullptr; d_energies = nullptr; d_counter = nullptr;
         device = -1;
         h_circs.clear();
+    }
+};
+
+struct qm_circuit {
+    unsigned n = 0, K = 4, lq = 13;
+    std::vector<qm::DStage> stages;
+    std::vector<qm::DPass> passes;
+    std::vector<qm::DSub> subs;
+    std::vector<double2> pool;
+    int device = -1;
+    int per_cu = 0, cus = 0;
+    qm::DPass* d_passes = nullptr;
+    qm::DSub* d_subs = nullptr;
+    double2* d_pool = nullptr;
+    void release() {
+        if (device >= 0) {
+            (void)hipSetDevice(device);
+            void* ptrs[] = {d_passes, d_subs, d_pool};
+            for (void* p : ptrs)
+                if (p) (void)hipFree(p);
+        }
+        d_passes = nullptr; d_subs = nullptr; d_pool = nullptr;
+        device = -1;
+        per_cu = 0;
     }
 };
 
@@ -503,11 +577,11 @@ void schedule_passes(Flat& f, std::vector<HOp> ops, u32 lq, u32 K, DStage& sg) {
     }
 }
 
-// Stages (windows of 13 qubits including 0..3) for circuits above 13 qubits; a
-// single identity stage otherwise.  Returns the number of stages appended.
-u32 schedule_circuit(Flat& f, const HCirc& c, u32 K) {
-    const u32 n = c.ne, lq = std::min(n, kMaxLocal);
-    if (n <= kMaxLocal) {
+// Stages (windows of lq_cap qubits including 0..3) for circuits above lq_cap
+// qubits; a single identity stage otherwise.  Returns the number of stages appended.
+u32 schedule_circuit(Flat& f, const HCirc& c, u32 K, u32 lq_cap = kMaxLocal) {
+    const u32 n = c.ne, lq = std::min(n, lq_cap);
+    if (n <= lq_cap) {
         DStage sg{};
         for (u32 i = 0; i + kLowFixed < lq; ++i) set_byte(sg.win[i >> 3], i & 7, kLowFixed + i);
         schedule_passes(f, c.ops, lq, K, sg);
@@ -547,7 +621,8 @@ u32 schedule_circuit(Flat& f, const HCirc& c, u32 K) {
                 set_byte(sg.win[wi >> 3], wi & 7, q);
                 ++wi;
             } else {
-                set_byte(sg.comp, ci++, q);
+                set_byte(sg.comp[ci >> 3], ci & 7, q);
+            ++ci;
             }
         }
         std::vector<HOp> ops;
@@ -571,7 +646,8 @@ u32 schedule_circuit(Flat& f, const HCirc& c, u32 K) {
                 set_byte(sg.win[wi >> 3], wi & 7, q);
                 ++wi;
             } else {
-                set_byte(sg.comp, ci++, q);
+                set_byte(sg.comp[ci >> 3], ci & 7, q);
+            ++ci;
             }
         }
         sg.npass = 0;
@@ -582,42 +658,9 @@ u32 schedule_circuit(Flat& f, const HCirc& c, u32 K) {
     return count;
 }
 
-bool batch_ok(qm_batch* b, const char* what) {
-    if (!b) {
-        set_error(QM_ERR_INVALID_ARGUMENT, "%s: null batch", what);
-        return false;
-    }
-    return true;
-}
-
-}  // namespace
-}  // namespace qm
-
-using namespace qm;
-
-extern "C" {
-
-qm_batch* qm_batch_create(void) { return new qm_batch(); }
-
-void qm_batch_destroy(qm_batch* b) {
-    if (!b) return;
-    b->release();
-    delete b;
-}
-
-int qm_batch_add_circuit(qm_batch* b, UINT n, const qm_batch_gate* gates, UINT gate_count, const ITYPE* bf,
-                         const ITYPE* pf, const UINT* rot, const double* coefs, UINT term_count) {
-    const char* what = "qm_batch_add_circuit";
-    if (!batch_ok(b, what)) return -1;
-    if (n < 1 || n > QM_BATCH_MAX_QUBITS) {
-        set_error(QM_ERR_INVALID_ARGUMENT, "%s: qubit count %u outside 1..%u", what, n, QM_BATCH_MAX_QUBITS);
-        return -1;
-    }
-    if ((gate_count && !gates) || (term_count && (!bf || !pf || !rot || !coefs))) {
-        set_error(QM_ERR_INVALID_ARGUMENT, "%s: null gate or term array", what);
-        return -1;
-    }
-    HCirc c;
+// Gates in the C ABI form -> internal ops, with consecutive single-qubit dense /
+// diagonal gates on the same qubit multiplied together.
+bool convert_gates(u32 n, const qm_batch_gate* gates, UINT gate_count, HCirc& c, const char* what) {
     c.n = n;
     c.ne = std::max<u32>(n, kMinQubits);
     std::vector<long> last(c.ne, -1);  // last op touching each qubit
@@ -627,19 +670,19 @@ int qm_batch_add_circuit(qm_batch* b, UINT n, const qm_batch_gate* gates, UINT g
         const bool two = g.kind == QM_BATCH_CZ || g.kind == QM_BATCH_CNOT;
         if (g.kind > QM_BATCH_CNOT || k < 1 || k > 4 || (two && k != 2)) {
             set_error(QM_ERR_INVALID_ARGUMENT, "%s: gate %u: kind %u with %u qubits is not supported", what, gi, g.kind, k);
-            return -1;
+            return false;
         }
         u64 mask = 0;
         for (u32 i = 0; i < k; ++i) {
             if (g.qubits[i] >= n || ((mask >> g.qubits[i]) & 1)) {
                 set_error(QM_ERR_INVALID_ARGUMENT, "%s: gate %u: qubit %u out of range or repeated", what, gi, g.qubits[i]);
-                return -1;
+                return false;
             }
             mask |= 1ULL << g.qubits[i];
         }
         if ((g.kind == QM_BATCH_DENSE || g.kind == QM_BATCH_DIAG) && !g.matrix) {
             set_error(QM_ERR_INVALID_ARGUMENT, "%s: gate %u: null matrix", what, gi);
-            return -1;
+            return false;
         }
         HOp o;
         o.q.assign(g.qubits, g.qubits + k);
@@ -685,6 +728,46 @@ int qm_batch_add_circuit(qm_batch* b, UINT n, const qm_batch_gate* gates, UINT g
         c.ops.push_back(std::move(o));
         for (u32 q : c.ops.back().q) last[q] = (long)c.ops.size() - 1;
     }
+    return true;
+}
+
+bool batch_ok(qm_batch* b, const char* what) {
+    if (!b) {
+        set_error(QM_ERR_INVALID_ARGUMENT, "%s: null batch", what);
+        return false;
+    }
+    return true;
+}
+
+}  // namespace
+}  // namespace qm
+
+using namespace qm;
+
+extern "C" {
+
+qm_batch* qm_batch_create(void) { return new qm_batch(); }
+
+void qm_batch_destroy(qm_batch* b) {
+    if (!b) return;
+    b->release();
+    delete b;
+}
+
+int qm_batch_add_circuit(qm_batch* b, UINT n, const qm_batch_gate* gates, UINT gate_count, const ITYPE* bf,
+                         const ITYPE* pf, const UINT* rot, const double* coefs, UINT term_count) {
+    const char* what = "qm_batch_add_circuit";
+    if (!batch_ok(b, what)) return -1;
+    if (n < 1 || n > QM_BATCH_MAX_QUBITS) {
+        set_error(QM_ERR_INVALID_ARGUMENT, "%s: qubit count %u outside 1..%u", what, n, QM_BATCH_MAX_QUBITS);
+        return -1;
+    }
+    if ((gate_count && !gates) || (term_count && (!bf || !pf || !rot || !coefs))) {
+        set_error(QM_ERR_INVALID_ARGUMENT, "%s: null gate or term array", what);
+        return -1;
+    }
+    HCirc c;
+    if (!convert_gates(n, gates, gate_count, c, what)) return -1;
     for (UINT t = 0; t < term_count; ++t) {
         if ((bf[t] >> n) || (pf[t] >> n)) {
             set_error(QM_ERR_INVALID_ARGUMENT, "%s: term %u acts beyond %u qubits", what, t, n);
@@ -839,6 +922,113 @@ int qm_batch_get_state(qm_batch* b, UINT index, qm_complex* out, void* stream, u
         return QM_ERR_HIP;
     if (!hip_ok(hipStreamSynchronize(st), what)) return QM_ERR_HIP;
     return QM_OK;
+}
+
+qm_circuit* qm_circuit_create(UINT n, const qm_batch_gate* gates, UINT gate_count, UINT block_qubits,
+                              UINT local_qubits) {
+    const char* what = "qm_circuit_create";
+    const u32 K = block_qubits ? block_qubits : 4;
+    const u32 lq = local_qubits ? local_qubits : kMaxLocal;
+    if (n < kMinQubits || n > QM_CIRCUIT_MAX_QUBITS || K < 2 || K > 4 || lq < 8 || lq > kMaxLocal ||
+        (gate_count && !gates)) {
+        set_error(QM_ERR_INVALID_ARGUMENT,
+                  "%s: need %u <= qubits (%u) <= %u, block_qubits (%u) in 2..4, local_qubits (%u) in 8..%u and a gate array",
+                  what, kMinQubits, n, QM_CIRCUIT_MAX_QUBITS, K, lq, kMaxLocal);
+        return nullptr;
+    }
+    auto* c = new qm_circuit();
+    HCirc hc;
+    if (!convert_gates(n, gates, gate_count, hc, what)) {
+        delete c;
+        return nullptr;
+    }
+    for (const HOp& o : hc.ops)
+        if (o.type == 0 && o.q.size() > K) {
+            set_error(QM_ERR_INVALID_ARGUMENT, "%s: a %zu-qubit dense gate needs block_qubits >= %zu", what, o.q.size(),
+                      o.q.size());
+            delete c;
+            return nullptr;
+        }
+    c->n = n;
+    c->K = K;
+    c->lq = std::min(n, lq);
+    Flat f;
+    schedule_circuit(f, hc, K, lq);
+    c->stages = f.stages;
+    c->passes = f.passes;
+    c->subs = f.subs;
+    c->pool = f.pool;
+    if (c->pool.empty()) c->pool.push_back(make_double2(0, 0));
+    if (c->subs.empty()) c->subs.push_back(DSub{});
+    if (c->passes.empty()) c->passes.push_back(DPass{});
+    return c;
+}
+
+void qm_circuit_destroy(qm_circuit* c) {
+    if (!c) return;
+    c->release();
+    delete c;
+}
+
+int qm_circuit_apply(qm_circuit* c, void* state, ITYPE dim, void* stream, unsigned dev) {
+    const char* what = "qm_circuit_apply";
+    if (!c || !state || dim != (1ULL << c->n)) {
+        set_error(QM_ERR_INVALID_ARGUMENT, "%s: null circuit/state or dim is not 2^%u", what, c ? c->n : 0u);
+        return QM_ERR_INVALID_ARGUMENT;
+    }
+    if (!use_device(dev)) return QM_ERR_HIP;
+    if (c->device != (int)dev) {
+        c->release();
+        bool ok = dev_upload(&c->d_passes, c->passes) && dev_upload(&c->d_subs, c->subs) &&
+                  dev_upload(&c->d_pool, c->pool);
+        if (!ok) {
+            c->release();
+            return QM_ERR_HIP;
+        }
+        c->device = (int)dev;
+    }
+    hipStream_t st = stream_of(stream);
+    const size_t lds = sizeof(double2) << c->lq;
+    const void* fn = c->K == 2 ? (const void*)k_stage<2> : c->K == 3 ? (const void*)k_stage<3> : (const void*)k_stage<4>;
+    if (c->per_cu == 0) {
+        int per_cu = 0, cus = 0;
+        if (!hip_ok(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), what) ||
+            !hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBatchThreads, lds), what) ||
+            !hip_ok(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, (int)dev), what))
+            return QM_ERR_HIP;
+        if (per_cu < 1) {
+            set_error(QM_ERR_HIP, "%s: the kernel does not fit a compute unit with %zu bytes of LDS", what, lds);
+            return QM_ERR_HIP;
+        }
+        c->per_cu = per_cu;
+        c->cus = cus;
+    }
+    const u32 tl = c->n - c->lq;
+    const u64 ntiles = 1ULL << tl;
+    const u32 grid = (u32)std::min<u64>(ntiles, (u64)c->per_cu * c->cus);
+    for (const DStage& sg : c->stages) {
+        if (sg.npass == 0) continue;
+        switch (c->K) {
+            case 2: k_stage<2><<<grid, kBatchThreads, lds, st>>>((double2*)state, c->lq, tl, sg, c->d_passes, c->d_subs, c->d_pool); break;
+            case 3: k_stage<3><<<grid, kBatchThreads, lds, st>>>((double2*)state, c->lq, tl, sg, c->d_passes, c->d_subs, c->d_pool); break;
+            default: k_stage<4><<<grid, kBatchThreads, lds, st>>>((double2*)state, c->lq, tl, sg, c->d_passes, c->d_subs, c->d_pool); break;
+        }
+    }
+    if (!hip_ok(hipGetLastError(), what)) return QM_ERR_HIP;
+    finish_update(st, what);
+    return QM_OK;
+}
+
+void qm_circuit_stats(const qm_circuit* c, ITYPE* stages, ITYPE* passes, ITYPE* subops) {
+    ITYPE np = 0, ns = 0;
+    if (c)
+        for (const DStage& sg : c->stages) {
+            np += sg.npass;
+            for (u32 p = 0; p < sg.npass; ++p) ns += c->passes[sg.pass0 + p].nsub;
+        }
+    if (stages) *stages = c ? c->stages.size() : 0;
+    if (passes) *passes = np;
+    if (subops) *subops = ns;
 }
 
 void qm_batch_stats(const qm_batch* b, ITYPE* circuits, ITYPE* stages, ITYPE* passes, ITYPE* subops) {

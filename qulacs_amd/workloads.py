@@ -205,3 +205,31 @@ def small_circuit_alg_flops(c: SmallCircuit) -> float:
     """FP64 flops of the fused form: one 2x2 complex matrix per qubit per layer
     (8 real multiply-adds = 16 flops per amplitude)."""
     return 16.0 * c.layers * c.n * (1 << c.n)
+
+
+# ---------------------------------------------------------------- 20-qubit brickwork
+def brickwork_circuit(n: int = 20, layers: int = 10, terms: int = 16, samples: int = 1024, seed: int = 1001):
+    """n-qubit |0> circuit of `layers` brickwork layers: a Haar 2x2 on every qubit,
+    then CNOT(i, i+1) for even i (even layers) or odd i (odd layers); observable of
+    `terms` Pauli strings (uniform I/X/Y/Z per qubit, coef ~ N(0, 1)) and
+    `samples` uniforms in [0, 1).  Gates are (kind, qubits, matrix) with kinds
+    "dense" and "CNOT" (control first); terms are (bit_flip, phase_flip, rot90, coef)."""
+    rng = np.random.default_rng(seed)
+    gates = []
+    for layer in range(layers):
+        for q in range(n):
+            gates.append(("dense", [q], haar_unitary(rng, 1)))
+        for q in range(layer % 2, n - 1, 2):
+            gates.append(("CNOT", [q, q + 1], None))
+    obs = []
+    for _ in range(terms):
+        p = rng.integers(0, 4, size=n)
+        bf = pf = rot = 0
+        for q in range(n):
+            if p[q] in (1, 2):
+                bf |= 1 << q
+            if p[q] in (2, 3):
+                pf |= 1 << q
+            rot += int(p[q] == 2)
+        obs.append((bf, pf, rot, float(rng.standard_normal())))
+    return gates, obs, rng.random(samples)

@@ -44,3 +44,18 @@ def test_compile_argument_checks_before_device_work():
         b.compile(block_qubits=3)  # 4-qubit gate needs a 4-qubit block
     with pytest.raises(QulacsMI355Error):
         b.compile(block_qubits=5)
+
+
+def test_fused_circuit_rejects_bad_arguments():
+    from qulacs_amd.batch import FusedCircuit
+
+    with pytest.raises(QulacsMI355Error):
+        FusedCircuit(3, [])  # below 4 qubits
+    with pytest.raises(QulacsMI355Error):
+        FusedCircuit(10, [], local_qubits=7)
+    with pytest.raises(QulacsMI355Error):
+        FusedCircuit(10, [("dense", [0, 1, 2], np.eye(8))], block_qubits=2)
+    fc = FusedCircuit(24, [("dense", [q], np.eye(2)) for q in range(24)] + [("CZ", [0, 23], None)])
+    st = fc.stats()
+    assert st["stages"] >= 2 and st["passes"] >= st["stages"]
+    fc.close()

@@ -79,3 +79,31 @@ def test_batch_errors():
     b.compile(block_qubits=3)
     e = b.run()
     assert e.shape == (1,) and e[0] == 0.0
+
+
+@pytest.mark.parametrize("n,block,local", [(4, 4, 13), (8, 2, 8), (13, 3, 13), (14, 4, 13), (16, 4, 10),
+                                           (20, 3, 12), (22, 4, 13)])
+def test_fused_circuit_matches_oracle(oracle, n, block, local):
+    from qulacs_amd import _lib
+    from qulacs_amd.batch import FusedCircuit
+    from tests.golden.gen import random_state
+    from tests.gpu_util import DevState
+
+    rng = np.random.default_rng(1000 + n)
+    gates = random_batch_circuit(rng, n, 80, max_dense=block)
+    psi = random_state(rng, n)
+    want = psi.copy()
+    for kind, q, m in gates:
+        if kind == "dense":
+            oracle.dense(q, m, want)
+        elif kind == "diag":
+            oracle.controlled_diag([], [], q, m, want)
+        else:
+            oracle.two(kind, q[0], q[1], want)
+    fc = FusedCircuit(n, gates, block_qubits=block, local_qubits=local)
+    print("fused", n, block, local, fc.stats(), flush=True)
+    with DevState.from_host(psi) as s:
+        fc.apply(s.ptr)
+        got = s.download()
+    assert rel(got, want) < 1e-10
+    fc.close()
