@@ -212,6 +212,9 @@ QM_EXPORT qm_complex qm_observable_expectation_host(const ITYPE* bit_flip_masks,
  * sequence of IEEE roundings, so the indices equal the sequential CPU loop's.
  * uniforms and out are host arrays of `count` entries. */
 QM_EXPORT void qm_sampling_host(const double* uniforms, UINT count, ITYPE* out, void* state, ITYPE dim, void* stream, unsigned int device_number);
+/* The CDF qm_sampling_host searches: cdf_out[0..dim] (host, dim + 1 doubles),
+ * equal bit for bit to the sequential loop's stacked probabilities. */
+QM_EXPORT void qm_sampling_cdf_host(double* cdf_out, void* state, ITYPE dim, void* stream, unsigned int device_number);
 
 /* ------------------------------------------------------------------ density matrix
  * `state` holds dim*dim complex128 values, row-major rho[y*dim + x]; `dim` is

@@ -128,6 +128,7 @@ _SIGS: dict[str, tuple] = {
     "dm_expectation_value_multi_qubit_Pauli_operator_partial_list_host": (D, [PU, PU, UINT, VP, ITYPE, VP, UINT]),
     "dm_normalize_host": (None, [D, VP, ITYPE, VP, UINT]),
     "dm_sampling_host": (None, [PD, UINT, PI, VP, ITYPE, VP, UINT]),
+    "qm_sampling_cdf_host": (None, [PD, VP, ITYPE, VP, UINT]),
     # batched circuits
     "qm_batch_create": (VP, []),
     "qm_batch_destroy": (None, [VP]),

@@ -17,6 +17,7 @@
 // the first element that is a tie or crosses the binade a single IEEE addition
 // exactly as the CPU performs it, after which the chunk continues from there.
 // Zero prefixes (s = 0, where fl(0 + p) = p) are handled the same way.
+#include <climits>
 #include <cmath>
 
 #include "qm_internal.hpp"
@@ -80,6 +81,125 @@ __device__ __forceinline__ int block_min_int(int v, int* lds /* >= 16 */) {
     return r;
 }
 
+// Sums chunk [c0, c0 + len) onto the running sum s exactly as the sequential
+// loop does and writes cdf[c0 + 1 .. c0 + len].  Called by every thread of a
+// 1024-thread block with the same s; returns with the same s in every thread.
+template <bool DM>
+__device__ __forceinline__ void exact_chunk(const double2* __restrict__ a, u64 dim, u64 c0, int len, double& s,
+                                            double* __restrict__ cdf, double* P, u64* scan_lds, int* min_lds,
+                                            double* s_bcast_p) {
+    double& s_bcast = *s_bcast_p;
+    for (int j = threadIdx.x; j < len; j += kCdfThreads)
+        P[j] = DM ? prob_dm(a, c0 + j, dim) : prob_sv(a, c0 + j);
+    __syncthreads();
+    int pos = 0;  // first element of the chunk not yet summed
+    while (pos < len) {
+        const int lo = threadIdx.x * kCdfPer;  // this thread's elements: lo .. lo+kCdfPer-1
+        if (!isfinite(s)) {
+            // inf/NaN: finish the chunk with plain sequential additions
+            // (only reachable from a non-finite amplitude).
+            if (threadIdx.x == 0) {
+                double t = s;
+                for (int j = pos; j < len; ++j) {
+                    t = t + P[j];
+                    cdf[c0 + j + 1] = t;
+                }
+                s_bcast = t;
+            }
+            __syncthreads();
+            s = s_bcast;
+            __syncthreads();
+            break;
+        }
+        if (s == 0.0) {
+            // fl(0 + p) = p: the sum stays 0 up to the first positive p.
+            int first = INT_MAX;
+            for (int k = 0; k < kCdfPer; ++k) {
+                const int j = lo + k;
+                if (j >= pos && j < len && P[j] != 0.0) { first = j; break; }
+            }
+            first = block_min_int(first, min_lds);
+            const int stop = first == INT_MAX ? len : first;
+            for (int k = 0; k < kCdfPer; ++k) {
+                const int j = lo + k;
+                if (j >= pos && j < stop) cdf[c0 + j + 1] = 0.0;
+            }
+            if (first == INT_MAX) { pos = len; break; }
+            s = 0.0 + P[first];
+            if (threadIdx.x == 0) cdf[c0 + first + 1] = s;
+            pos = first + 1;
+            continue;
+        }
+        int e = ilogb(s);
+        if (e < -1022) e = -1022;
+        const double u = ldexp(1.0, e - 52);
+        const u64 S = (u64)(s / u);  // exact: s is a multiple of u below 2^53 u
+        const u64 LIM = 1ULL << 53;
+        // per-element integer increments; ties are marked special
+        u64 R[kCdfPer];
+        bool tie[kCdfPer];
+        u64 mine = 0;
+#pragma unroll
+        for (int k = 0; k < kCdfPer; ++k) {
+            const int j = lo + k;
+            R[k] = 0;
+            tie[k] = false;
+            if (j >= pos && j < len) {
+                double q = ldexp(P[j], 52 - e);  // exact scaling by a power of two (or +inf)
+                if (!(q < 0x1p62)) q = 0x1p62;   // certain binade crossing; keeps sums in range
+                const double fl = floor(q);
+                const double fr = q - fl;        // exact for q < 2^53; 0 above
+                R[k] = (u64)fl + (fr > 0.5 ? 1 : 0);
+                tie[k] = (fr == 0.5);
+            }
+            mine += R[k];
+        }
+        u64 total;
+        const u64 excl = block_excl_scan(mine, scan_lds, &total);
+        // first special element: a tie or the first partial sum reaching 2^53
+        int first = INT_MAX;
+        u64 run = S + excl;
+#pragma unroll
+        for (int k = 0; k < kCdfPer; ++k) {
+            const int j = lo + k;
+            if (j >= pos && j < len && first == INT_MAX) {
+                if (tie[k] || run + R[k] >= LIM) first = j;
+                run += R[k];
+            }
+        }
+        first = block_min_int(first, min_lds);
+        const int stop = first == INT_MAX ? len : first;
+        run = S + excl;
+        u64 before_first = 0;  // S + increments strictly before `first` (owned by its thread)
+#pragma unroll
+        for (int k = 0; k < kCdfPer; ++k) {
+            const int j = lo + k;
+            if (j >= pos && j < len) {
+                if (j == first) before_first = run;
+                run += R[k];
+                if (j < stop) cdf[c0 + j + 1] = (double)run * u;
+            }
+        }
+        if (first == INT_MAX) {
+            s = (double)(S + total) * u;
+            pos = len;
+        } else {
+            // The owning thread performs the one IEEE addition exactly as the CPU does.
+            if (lo <= first && first < lo + kCdfPer) {
+                const double sb = (double)before_first * u;
+                const double sn = sb + P[first];
+                cdf[c0 + first + 1] = sn;
+                s_bcast = sn;
+            }
+            __syncthreads();
+            s = s_bcast;
+            __syncthreads();
+            pos = first + 1;
+        }
+    }
+    __syncthreads();
+}
+
 template <bool DM>
 __global__ __launch_bounds__(kCdfThreads) void k_cdf_exact(const double2* __restrict__ a, u64 dim,
                                                            double* __restrict__ cdf) {
@@ -91,116 +211,259 @@ __global__ __launch_bounds__(kCdfThreads) void k_cdf_exact(const double2* __rest
     if (threadIdx.x == 0) cdf[0] = 0.0;
     for (u64 c0 = 0; c0 < dim; c0 += kCdfChunk) {
         const int len = (int)min((u64)kCdfChunk, dim - c0);
-        for (int j = threadIdx.x; j < len; j += kCdfThreads)
-            P[j] = DM ? prob_dm(a, c0 + j, dim) : prob_sv(a, c0 + j);
-        __syncthreads();
-        int pos = 0;  // first element of the chunk not yet summed
-        while (pos < len) {
-            const int lo = threadIdx.x * kCdfPer;  // this thread's elements: lo .. lo+kCdfPer-1
-            if (!isfinite(s)) {
-                // inf/NaN: finish the chunk with plain sequential additions
-                // (only reachable from a non-finite amplitude).
-                if (threadIdx.x == 0) {
-                    double t = s;
-                    for (int j = pos; j < len; ++j) {
-                        t = t + P[j];
-                        cdf[c0 + j + 1] = t;
-                    }
-                    s_bcast = t;
-                }
-                __syncthreads();
-                s = s_bcast;
-                __syncthreads();
-                break;
-            }
-            if (s == 0.0) {
-                // fl(0 + p) = p: the sum stays 0 up to the first positive p.
-                int first = INT_MAX;
-                for (int k = 0; k < kCdfPer; ++k) {
-                    const int j = lo + k;
-                    if (j >= pos && j < len && P[j] != 0.0) { first = j; break; }
-                }
-                first = block_min_int(first, min_lds);
-                const int stop = first == INT_MAX ? len : first;
-                for (int k = 0; k < kCdfPer; ++k) {
-                    const int j = lo + k;
-                    if (j >= pos && j < stop) cdf[c0 + j + 1] = 0.0;
-                }
-                if (first == INT_MAX) { pos = len; break; }
-                s = 0.0 + P[first];
-                if (threadIdx.x == 0) cdf[c0 + first + 1] = s;
-                pos = first + 1;
-                continue;
-            }
-            int e = ilogb(s);
-            if (e < -1022) e = -1022;
-            const double u = ldexp(1.0, e - 52);
-            const u64 S = (u64)(s / u);  // exact: s is a multiple of u below 2^53 u
-            const u64 LIM = 1ULL << 53;
-            // per-element integer increments; ties are marked special
-            u64 R[kCdfPer];
-            bool tie[kCdfPer];
-            u64 mine = 0;
+        exact_chunk<DM>(a, dim, c0, len, s, cdf, P, scan_lds, min_lds, &s_bcast);
+    }
+}
+
+// ------------------------------------------------------------------ multi-block CDF
+// The same sums for large states, spread over the whole GPU:
+//   A (grid): approximate chunk totals T[c] (used only to predict);
+//   B (1 block): predicted binade E[c] of the running sum at each chunk start;
+//   C (grid): per chunk, the integer increments in the predicted binade, their
+//      total N[c], and slow[c] if any increment is a tie or would cross the binade;
+//   D (1 block): the exact running sum walks the chunks 1024 at a time: a chunk
+//      is fast when it is all zeros or when it is not slow, its predicted binade
+//      equals the running sum's binade and the sum stays below 2^53 units; fast
+//      chunks only record their exact start value, any other chunk is summed
+//      element by element (exact_chunk) before the walk continues;
+//   E (grid): every fast chunk writes its CDF values from its exact start.
+// A wrong prediction only turns a chunk slow; every value written equals the
+// sequential loop's.
+constexpr int kPredictThreads = 1024;
+constexpr int kIncrThreads = 256;
+constexpr u64 kMultiBlockMinDim = 1ULL << 16;
+
+template <bool DM>
+__device__ __forceinline__ double prob_at(const double2* __restrict__ a, u64 i, u64 dim) {
+    return DM ? prob_dm(a, i, dim) : prob_sv(a, i);
+}
+
+template <bool DM>
+__global__ __launch_bounds__(kIncrThreads) void k_chunk_total(const double2* __restrict__ a, u64 dim,
+                                                              double* __restrict__ T) {
+    __shared__ double lds[32];
+    const u64 c0 = (u64)blockIdx.x * kCdfChunk;
+    const int len = (int)min((u64)kCdfChunk, dim - c0);
+    double acc = 0.0;
+    for (int j = threadIdx.x; j < len; j += blockDim.x) acc += prob_at<DM>(a, c0 + j, dim);
+    acc = block_sum(acc, lds);
+    if (threadIdx.x == 0) T[blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(kPredictThreads) void k_chunk_predict(const double* __restrict__ T, u64 M,
+                                                                   int* __restrict__ E) {
+    __shared__ double part[kPredictThreads];
+    const u64 seg = (M + kPredictThreads - 1) / kPredictThreads;
+    const u64 lo = (u64)threadIdx.x * seg, hi = min(lo + seg, M);
+    double local = 0.0;
+    for (u64 c = lo; c < hi; ++c) local += T[c];
+    part[threadIdx.x] = local;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double run = 0.0;
+        for (int t = 0; t < kPredictThreads; ++t) {
+            const double v = part[t];
+            part[t] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    double run = part[threadIdx.x];
+    for (u64 c = lo; c < hi; ++c) {
+        E[c] = run == 0.0 ? INT_MIN : max(ilogb(run), -1022);
+        run += T[c];
+    }
+}
+
+template <bool DM>
+__global__ __launch_bounds__(kIncrThreads) void k_chunk_incr(const double2* __restrict__ a, u64 dim,
+                                                             const int* __restrict__ E, const double* __restrict__ T,
+                                                             u64* __restrict__ N, int* __restrict__ slow) {
+    __shared__ u64 lds_u[kIncrThreads / 64];
+    __shared__ int lds_i[kIncrThreads / 64];
+    const u64 c = blockIdx.x;
+    const double tc = T[c];
+    const int e = E[c];
+    if (tc == 0.0 || !isfinite(tc) || e == INT_MIN) {
+        if (threadIdx.x == 0) {
+            N[c] = 0;
+            slow[c] = tc == 0.0 ? 0 : 1;
+        }
+        return;
+    }
+    const u64 c0 = c * kCdfChunk;
+    const int len = (int)min((u64)kCdfChunk, dim - c0);
+    const u64 LIM = 1ULL << 53;
+    u64 acc = 0;
+    int bad = 0;
+    for (int j = threadIdx.x; j < len; j += blockDim.x) {
+        const double q = ldexp(prob_at<DM>(a, c0 + j, dim), 52 - e);
+        if (!(q < 0x1p53)) {
+            bad = 1;
+            continue;
+        }
+        const double fl = floor(q), fr = q - fl;
+        if (fr == 0.5) bad = 1;
+        acc += (u64)fl + (fr > 0.5 ? 1 : 0);
+    }
+    if (acc >= LIM) bad = 1;  // keeps every per-thread value below 2^53, the block sum below 2^61
 #pragma unroll
-            for (int k = 0; k < kCdfPer; ++k) {
-                const int j = lo + k;
-                R[k] = 0;
-                tie[k] = false;
-                if (j >= pos && j < len) {
-                    double q = ldexp(P[j], 52 - e);  // exact scaling by a power of two (or +inf)
-                    if (!(q < 0x1p62)) q = 0x1p62;   // certain binade crossing; keeps sums in range
-                    const double fl = floor(q);
-                    const double fr = q - fl;        // exact for q < 2^53; 0 above
-                    R[k] = (u64)fl + (fr > 0.5 ? 1 : 0);
-                    tie[k] = (fr == 0.5);
-                }
-                mine += R[k];
-            }
-            u64 total;
-            const u64 excl = block_excl_scan(mine, scan_lds, &total);
-            // first special element: a tie or the first partial sum reaching 2^53
-            int first = INT_MAX;
-            u64 run = S + excl;
-#pragma unroll
-            for (int k = 0; k < kCdfPer; ++k) {
-                const int j = lo + k;
-                if (j >= pos && j < len && first == INT_MAX) {
-                    if (tie[k] || run + R[k] >= LIM) first = j;
-                    run += R[k];
-                }
-            }
-            first = block_min_int(first, min_lds);
-            const int stop = first == INT_MAX ? len : first;
-            run = S + excl;
-            u64 before_first = 0;  // S + increments strictly before `first` (owned by its thread)
-#pragma unroll
-            for (int k = 0; k < kCdfPer; ++k) {
-                const int j = lo + k;
-                if (j >= pos && j < len) {
-                    if (j == first) before_first = run;
-                    run += R[k];
-                    if (j < stop) cdf[c0 + j + 1] = (double)run * u;
-                }
-            }
-            if (first == INT_MAX) {
-                s = (double)(S + total) * u;
-                pos = len;
-            } else {
-                // The owning thread performs the one IEEE addition exactly as the CPU does.
-                if (lo <= first && first < lo + kCdfPer) {
-                    const double sb = (double)before_first * u;
-                    const double sn = sb + P[first];
-                    cdf[c0 + first + 1] = sn;
-                    s_bcast = sn;
-                }
-                __syncthreads();
-                s = s_bcast;
-                __syncthreads();
-                pos = first + 1;
+    for (int off = 32; off > 0; off >>= 1) {
+        acc += __shfl_xor(acc, off, 64);
+        bad |= __shfl_xor(bad, off, 64);
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        lds_u[wid] = acc;
+        lds_i[wid] = bad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        u64 tot = 0;
+        int b = 0;
+        for (int w = 0; w < kIncrThreads / 64; ++w) {
+            tot += lds_u[w];
+            b |= lds_i[w];
+        }
+        N[c] = tot;
+        slow[c] = (b || tot >= LIM) ? 1 : 0;
+    }
+}
+
+template <bool DM>
+__global__ __launch_bounds__(kCdfThreads) void k_cdf_walk(const double2* __restrict__ a, u64 dim, u64 M,
+                                                          const int* __restrict__ E, const u64* __restrict__ N,
+                                                          const int* __restrict__ slow, const double* __restrict__ T,
+                                                          double* __restrict__ start, int* __restrict__ exact,
+                                                          double* __restrict__ cdf) {
+    __shared__ double P[kCdfChunk];
+    __shared__ u64 scan_lds[17];
+    __shared__ int min_lds[17];
+    __shared__ double s_bcast;
+    const u64 LIM = 1ULL << 53;
+    double s = 0.0;  // exact running sum, identical in every thread
+    if (threadIdx.x == 0) cdf[0] = 0.0;
+    u64 c = 0;
+    while (c < M) {
+        const u64 ci = c + threadIdx.x;
+        int es = INT_MIN;
+        double u = 0.0;
+        u64 S = 0;
+        if (s != 0.0 && isfinite(s)) {
+            es = max(ilogb(s), -1022);
+            u = ldexp(1.0, es - 52);
+            S = (u64)(s / u);
+        }
+        bool ok = false;
+        u64 v = 0;
+        if (ci < M && isfinite(s)) {
+            if (T[ci] == 0.0) {
+                ok = true;
+            } else if (s != 0.0 && !slow[ci] && E[ci] == es) {
+                ok = true;
+                v = N[ci];
             }
         }
-        __syncthreads();
+        const int first_bad = block_min_int(ok ? INT_MAX : (int)threadIdx.x, min_lds);
+        if ((int)threadIdx.x >= first_bad) v = 0;
+        u64 total;
+        const u64 excl = block_excl_scan(v, scan_lds, &total);
+        const int cross =
+            ((int)threadIdx.x < first_bad && S + excl + v >= LIM) ? (int)threadIdx.x : INT_MAX;
+        const int first_cross = block_min_int(cross, min_lds);
+        const int stop = min(first_bad, first_cross);  // chunks c .. c + stop - 1 are fast
+        if ((int)threadIdx.x < stop) {
+            start[ci] = s == 0.0 ? 0.0 : (double)(S + excl) * u;
+            exact[ci] = 0;
+        }
+        if (stop > 0) {
+            if ((int)threadIdx.x == stop - 1) s_bcast = s == 0.0 ? 0.0 : (double)(S + excl + v) * u;
+            __syncthreads();
+            s = s_bcast;
+            __syncthreads();
+        }
+        c += (u64)stop;
+        if (stop < kCdfThreads && c < M) {
+            if (threadIdx.x == 0) {
+                start[c] = s;
+                exact[c] = 1;
+            }
+            const int len = (int)min((u64)kCdfChunk, dim - c * kCdfChunk);
+            exact_chunk<DM>(a, dim, c * kCdfChunk, len, s, cdf, P, scan_lds, min_lds, &s_bcast);
+            c += 1;
+        }
     }
+}
+
+template <bool DM>
+__global__ __launch_bounds__(kCdfThreads) void k_cdf_fill(const double2* __restrict__ a, u64 dim,
+                                                          const int* __restrict__ E, const double* __restrict__ T,
+                                                          const double* __restrict__ start,
+                                                          const int* __restrict__ exact, double* __restrict__ cdf) {
+    __shared__ u64 scan_lds[17];
+    const u64 c = blockIdx.x;
+    if (exact[c]) return;
+    const u64 c0 = c * kCdfChunk;
+    const int len = (int)min((u64)kCdfChunk, dim - c0);
+    const int lo = threadIdx.x * kCdfPer;
+    const double s0 = start[c];
+    if (T[c] == 0.0) {
+        for (int k = 0; k < kCdfPer; ++k)
+            if (lo + k < len) cdf[c0 + lo + k + 1] = s0;
+        return;
+    }
+    const int e = E[c];
+    const double u = ldexp(1.0, e - 52);
+    const u64 S = (u64)(s0 / u);
+    u64 R[kCdfPer];
+    u64 mine = 0;
+#pragma unroll
+    for (int k = 0; k < kCdfPer; ++k) {
+        R[k] = 0;
+        if (lo + k < len) {
+            const double q = ldexp(prob_at<DM>(a, c0 + lo + k, dim), 52 - e);
+            const double fl = floor(q), fr = q - fl;
+            R[k] = (u64)fl + (fr > 0.5 ? 1 : 0);
+        }
+        mine += R[k];
+    }
+    u64 total;
+    u64 run = S + block_excl_scan(mine, scan_lds, &total);
+#pragma unroll
+    for (int k = 0; k < kCdfPer; ++k) {
+        if (lo + k < len) {
+            run += R[k];
+            cdf[c0 + lo + k + 1] = (double)run * u;
+        }
+    }
+}
+
+// Enqueues the CDF of `a` into cdf[0..dim] on st.  `work` is device scratch of
+// cdf_work_bytes(dim) bytes.
+size_t cdf_work_bytes(u64 dim) {
+    const u64 M = (dim + kCdfChunk - 1) / kCdfChunk;
+    return (size_t)M * (sizeof(double) * 2 + sizeof(u64) + sizeof(int) * 3) + 1024;
+}
+
+template <bool DM>
+void enqueue_cdf(const double2* a, u64 dim, double* cdf, void* work, hipStream_t st) {
+    if (dim < kMultiBlockMinDim) {
+        k_cdf_exact<DM><<<1, kCdfThreads, 0, st>>>(a, dim, cdf);
+        return;
+    }
+    const u64 M = (dim + kCdfChunk - 1) / kCdfChunk;
+    char* w = (char*)work;
+    double* T = (double*)w;
+    double* start = T + M;
+    u64* N = (u64*)(start + M);
+    int* E = (int*)(N + M);
+    int* slow = E + M;
+    int* exact = slow + M;
+    k_chunk_total<DM><<<(unsigned)M, kIncrThreads, 0, st>>>(a, dim, T);
+    k_chunk_predict<<<1, kPredictThreads, 0, st>>>(T, M, E);
+    k_chunk_incr<DM><<<(unsigned)M, kIncrThreads, 0, st>>>(a, dim, E, T, N, slow);
+    k_cdf_walk<DM><<<1, kCdfThreads, 0, st>>>(a, dim, M, E, N, slow, T, start, exact, cdf);
+    k_cdf_fill<DM><<<(unsigned)M, kCdfThreads, 0, st>>>(a, dim, E, T, start, exact, cdf);
 }
 
 // lower_bound(cdf[0..dim], u) - 1, as an unsigned 64-bit index (the reference's
@@ -235,17 +498,19 @@ void sample(const double* uniforms, unsigned count, ITYPE* out, void* state, u64
     hipStream_t st = stream_of(stream);
     Scratch* sc = scratch_for(st);
     if (!sc) return;
-    const size_t cdf_bytes = (size_t)(dim + 1) * sizeof(double);
-    const size_t io_bytes = (size_t)count * 8;
-    char* buf = (char*)big_buffer(sc, cdf_bytes + 2 * io_bytes + 256);
+    const size_t cdf_bytes = ((size_t)(dim + 1) * sizeof(double) + 255) & ~size_t(255);
+    const size_t work_bytes = (cdf_work_bytes(dim) + 255) & ~size_t(255);
+    const size_t io_bytes = ((size_t)count * 8 + 255) & ~size_t(255);
+    char* buf = (char*)big_buffer(sc, cdf_bytes + work_bytes + 2 * io_bytes);
     if (!buf) return;
     double* cdf = (double*)buf;
-    double* dus = (double*)(buf + ((cdf_bytes + 255) & ~size_t(255)));
+    void* work = buf + cdf_bytes;
+    double* dus = (double*)(buf + cdf_bytes + work_bytes);
     u64* dout = (u64*)((char*)dus + io_bytes);
-    if (!hip_ok(hipMemcpyAsync(dus, uniforms, io_bytes, hipMemcpyHostToDevice, st), what)) return;
-    k_cdf_exact<DM><<<1, kCdfThreads, 0, st>>>((const double2*)state, dim, cdf);
+    if (!hip_ok(hipMemcpyAsync(dus, uniforms, (size_t)count * 8, hipMemcpyHostToDevice, st), what)) return;
+    enqueue_cdf<DM>((const double2*)state, dim, cdf, work, st);
     k_search<<<(count + 255) / 256, 256, 0, st>>>(cdf, dim, dus, count, dout);
-    if (!hip_ok(hipMemcpyAsync(out, dout, io_bytes, hipMemcpyDeviceToHost, st), what)) return;
+    if (!hip_ok(hipMemcpyAsync(out, dout, (size_t)count * 8, hipMemcpyDeviceToHost, st), what)) return;
     hip_ok(hipStreamSynchronize(st), what);
 }
 
@@ -258,6 +523,25 @@ extern "C" {
 void qm_sampling_host(const double* uniforms, UINT count, ITYPE* out, void* state, ITYPE dim, void* stream,
                       unsigned dev) {
     sample<false>(uniforms, count, out, state, dim, stream, dev, "qm_sampling_host");
+}
+
+void qm_sampling_cdf_host(double* cdf_out, void* state, ITYPE dim, void* stream, unsigned dev) {
+    const char* what = "qm_sampling_cdf_host";
+    if (!state || !cdf_out || log2_dim(dim) < 0) {
+        set_error(QM_ERR_INVALID_ARGUMENT, "%s: null state/output or bad dim", what);
+        return;
+    }
+    if (!use_device(dev)) return;
+    hipStream_t st = stream_of(stream);
+    Scratch* sc = scratch_for(st);
+    if (!sc) return;
+    const size_t cdf_bytes = ((size_t)(dim + 1) * sizeof(double) + 255) & ~size_t(255);
+    char* buf = (char*)big_buffer(sc, cdf_bytes + cdf_work_bytes(dim));
+    if (!buf) return;
+    enqueue_cdf<false>((const double2*)state, dim, (double*)buf, buf + cdf_bytes, st);
+    if (!hip_ok(hipMemcpyAsync(cdf_out, buf, (size_t)(dim + 1) * sizeof(double), hipMemcpyDeviceToHost, st), what))
+        return;
+    hip_ok(hipStreamSynchronize(st), what);
 }
 
 void dm_sampling_host(const double* uniforms, UINT count, ITYPE* out, void* state, ITYPE dim, void* stream,

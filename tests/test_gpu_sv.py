@@ -245,6 +245,25 @@ def test_sampling_identical_to_sequential_cpu(oracle, n):
         assert np.array_equal(out, want), (label, np.nonzero(out != want)[0][:5])
 
 
+@pytest.mark.parametrize("n", [15, 16, 17, 20])
+def test_sampling_cdf_bit_identical(oracle, n):
+    """The device CDF (single work-group below 2^16 amplitudes, the multi-block
+    chunk walk from 2^16 up) equals the sequential loop's stacked sums bit for bit."""
+    rng = np.random.default_rng(300 + n)
+    dim = 1 << n
+    states = list(_sampling_states(rng, n))
+    # a state whose running sum crosses many binades inside single chunks
+    g = random_state(rng, n) * (2.0 ** (-np.arange(dim) % 97))
+    states.append(("binades", g / np.linalg.norm(g)))
+    for label, psi in states:
+        want = oracle.sampling_cdf(psi)
+        got = np.zeros(dim + 1)
+        with DevState.from_host(psi) as s:
+            _lib.call("qm_sampling_cdf_host", got.ctypes.data_as(_lib.PD), s.ptr, dim, None, 0)
+        bad = np.nonzero(got.view(np.uint64) != want.view(np.uint64))[0]
+        assert bad.size == 0, (label, bad[:5])
+
+
 def test_state_arithmetic_and_init(oracle):
     rng = np.random.default_rng(5)
     n = 11
