@@ -286,6 +286,10 @@ QM_EXPORT const double* qm_batch_run_async(qm_batch* batch, void* stream, unsign
 /* Copies circuit i's final state (2^n complex128) to host memory after a run
  * compiled with keep_states. */
 QM_EXPORT int qm_batch_get_state(qm_batch* batch, UINT index, qm_complex* out, void* stream, unsigned int device_number);
+/* Removes every circuit (device memory is kept for the next compile). */
+QM_EXPORT void qm_batch_clear(qm_batch* batch);
+/* Device address of circuit i's final state (compiled with keep_states). */
+QM_EXPORT void* qm_batch_state_device_ptr(qm_batch* batch, UINT index);
 /* Schedule statistics: number of circuits, stages, passes and sub-operations. */
 QM_EXPORT void qm_batch_stats(const qm_batch* batch, ITYPE* circuits, ITYPE* stages, ITYPE* passes, ITYPE* subops);
 
@@ -305,6 +309,26 @@ QM_EXPORT void qm_circuit_destroy(qm_circuit* circuit);
 /* Applies the circuit to `state` (dim = 2^qubit_count) in stream order. */
 QM_EXPORT int qm_circuit_apply(qm_circuit* circuit, void* state, ITYPE dim, void* stream, unsigned int device_number);
 QM_EXPORT void qm_circuit_stats(const qm_circuit* circuit, ITYPE* stages, ITYPE* passes, ITYPE* subops);
+
+/* ------------------------------------------------------------------ request server
+ * Circuits of mixed size (1..40 qubits) submitted from any thread and run on
+ * one device with continuous batching: each request is a circuit from
+ * |0...0> (batch gate kinds), a real-coefficient Pauli observable and uniforms;
+ * it returns <H> and the sampled indices (as qm_sampling_host).  Requests are
+ * admitted while the sum of 16 * 2^n bytes stays within `hbm_budget_bytes`;
+ * requests of up to `small_max_qubits` (<= 16) qubits are batched into one
+ * launch, larger ones run on up to `large_slots` streams at once.  No reference
+ * counterpart (the reference runs one circuit per call, src/cppsim/circuit.cpp:46-57). */
+typedef struct qm_server qm_server;
+QM_EXPORT qm_server* qm_server_create(ITYPE hbm_budget_bytes, UINT large_slots, UINT small_max_qubits, unsigned int device_number);
+/* Copies the request; returns its id (>= 0) or -1 on an invalid request. */
+QM_EXPORT long long qm_server_submit(qm_server* server, UINT qubit_count, const qm_batch_gate* gates, UINT gate_count, const ITYPE* bit_flip_masks, const ITYPE* phase_flip_masks, const UINT* global_phase_90rot_counts, const double* coefs, UINT term_count, const double* uniforms, UINT sample_count);
+/* Waits for request `id` (once per id): energy, samples (sample_count entries)
+ * and the time from submission to completion. */
+QM_EXPORT int qm_server_wait(qm_server* server, long long id, double* energy, ITYPE* samples, double* latency_seconds);
+QM_EXPORT void qm_server_stats(qm_server* server, ITYPE* admitted, ITYPE* completed, ITYPE* failed, ITYPE* batches, ITYPE* max_batch, ITYPE* peak_bytes);
+/* Finishes every submitted request, then frees the server. */
+QM_EXPORT void qm_server_destroy(qm_server* server);
 
 #ifdef __cplusplus
 }

@@ -138,10 +138,17 @@ _SIGS: dict[str, tuple] = {
     "qm_batch_run_async": (VP, [VP, VP, UINT]),
     "qm_batch_get_state": (C.c_int, [VP, UINT, PC, VP, UINT]),
     "qm_batch_stats": (None, [VP, PI, PI, PI, PI]),
+    "qm_batch_clear": (None, [VP]),
+    "qm_batch_state_device_ptr": (VP, [VP, UINT]),
     "qm_circuit_create": (VP, [UINT, C.POINTER(QmBatchGate), UINT, UINT, UINT]),
     "qm_circuit_destroy": (None, [VP]),
     "qm_circuit_apply": (C.c_int, [VP, VP, ITYPE, VP, UINT]),
     "qm_circuit_stats": (None, [VP, PI, PI, PI]),
+    "qm_server_create": (VP, [ITYPE, UINT, UINT, UINT]),
+    "qm_server_submit": (C.c_longlong, [VP, UINT, C.POINTER(QmBatchGate), UINT, PI, PI, PU, PD, UINT, PD, UINT]),
+    "qm_server_wait": (C.c_int, [VP, C.c_longlong, PD, PI, PD]),
+    "qm_server_stats": (None, [VP, PI, PI, PI, PI, PI, PI]),
+    "qm_server_destroy": (None, [VP]),
 }
 
 _lib = None

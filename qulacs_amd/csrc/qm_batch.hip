@@ -404,6 +404,25 @@ bool dev_upload(T** dst, const std::vector<T>& src) {
                   "qm_batch_compile: hipMemcpy");
 }
 
+// Grows *dst to hold `bytes` (contents not kept); capacity in *cap.
+bool dev_reserve(void** dst, size_t* cap, size_t bytes, const char* what) {
+    if (*cap >= bytes && *dst) return true;
+    if (*dst) {
+        if (!hip_ok(hipFree(*dst), what)) return false;
+        *dst = nullptr;
+        *cap = 0;
+    }
+    if (!hip_ok(hipMalloc(dst, std::max<size_t>(bytes, 256)), what)) return false;
+    *cap = std::max<size_t>(bytes, 256);
+    return true;
+}
+
+template <class T>
+bool dev_put(T** dst, size_t* cap, const std::vector<T>& src, const char* what) {
+    if (!dev_reserve((void**)dst, cap, src.size() * sizeof(T), what)) return false;
+    return src.empty() || hip_ok(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice), what);
+}
+
 }  // namespace
 }  // namespace qm
 
@@ -426,6 +445,7 @@ struct qm_batch {
     double2* d_scratch = nullptr;
     double* d_energies = nullptr;
     int* d_counter = nullptr;
+    size_t cap[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // bytes held by each device buffer above
 
     void release() {
         if (device >= 0) {
@@ -436,6 +456,7 @@ struct qm_batch {
         }
         d_circs = nullptr; d_stages = nullptr; d_passes = nullptr; d_subs = nullptr; d_pool = nullptr;
         d_terms = nullptr; d_order = nullptr; d_scratch = nullptr; d_energies = nullptr; d_counter = nullptr;
+        for (size_t& c : cap) c = 0;
         device = -1;
         h_circs.clear();
     }
@@ -799,10 +820,10 @@ int qm_batch_compile(qm_batch* b, UINT block_qubits, int keep_states, unsigned d
                           o.q.size(), o.q.size());
                 return QM_ERR_INVALID_ARGUMENT;
             }
-    b->release();
+    if (b->device >= 0 && b->device != (int)dev) b->release();
+    b->h_circs.clear();
     if (!use_device(dev)) return QM_ERR_HIP;
-    Flat f;
-    std::vector<DTerm> terms;
+    Flat f;    std::vector<DTerm> terms;
     std::vector<DCirc> dc;
     u64 state_off = 0;
     u32 lq_max = kMinQubits;
@@ -835,15 +856,16 @@ int qm_batch_compile(qm_batch* b, UINT block_qubits, int keep_states, unsigned d
     if (f.passes.empty()) f.passes.push_back(DPass{});
     if (terms.empty()) terms.push_back(DTerm{});
     b->device = (int)dev;
-    bool ok = dev_upload(&b->d_circs, dc) && dev_upload(&b->d_stages, f.stages) && dev_upload(&b->d_passes, f.passes) &&
-              dev_upload(&b->d_subs, f.subs) && dev_upload(&b->d_pool, f.pool) && dev_upload(&b->d_terms, terms) &&
-              dev_upload(&b->d_order, order);
+    bool ok = dev_put(&b->d_circs, &b->cap[0], dc, what) && dev_put(&b->d_stages, &b->cap[1], f.stages, what) &&
+              dev_put(&b->d_passes, &b->cap[2], f.passes, what) && dev_put(&b->d_subs, &b->cap[3], f.subs, what) &&
+              dev_put(&b->d_pool, &b->cap[4], f.pool, what) && dev_put(&b->d_terms, &b->cap[5], terms, what) &&
+              dev_put(&b->d_order, &b->cap[6], order, what);
     if (ok && state_off) {
-        ok = hip_ok(hipMalloc((void**)&b->d_scratch, state_off * sizeof(double2)), "qm_batch_compile: state memory");
+        ok = dev_reserve((void**)&b->d_scratch, &b->cap[7], state_off * sizeof(double2), what);
         if (!ok) set_error(QM_ERR_OUT_OF_MEMORY, "%s: %llu bytes of state memory", what, state_off * 16ULL);
     }
-    if (ok) ok = hip_ok(hipMalloc((void**)&b->d_energies, std::max<size_t>(1, dc.size()) * sizeof(double)), what);
-    if (ok) ok = hip_ok(hipMalloc((void**)&b->d_counter, sizeof(int)), what);
+    if (ok) ok = dev_reserve((void**)&b->d_energies, &b->cap[8], std::max<size_t>(1, dc.size()) * sizeof(double), what);
+    if (ok) ok = dev_reserve((void**)&b->d_counter, &b->cap[9], sizeof(int), what);
     if (!ok) {
         b->release();
         return QM_ERR_HIP;
@@ -924,6 +946,20 @@ int qm_batch_get_state(qm_batch* b, UINT index, qm_complex* out, void* stream, u
     return QM_OK;
 }
 
+void qm_batch_clear(qm_batch* b) {
+    if (!b) return;
+    b->circs.clear();
+    b->h_circs.clear();
+}
+
+void* qm_batch_state_device_ptr(qm_batch* b, UINT index) {
+    if (!b || !b->keep || index >= b->h_circs.size()) {
+        set_error(QM_ERR_INVALID_ARGUMENT, "qm_batch_state_device_ptr: no kept state %u", index);
+        return nullptr;
+    }
+    return b->d_scratch + b->h_circs[index].state_off;
+}
+
 qm_circuit* qm_circuit_create(UINT n, const qm_batch_gate* gates, UINT gate_count, UINT block_qubits,
                               UINT local_qubits) {
     const char* what = "qm_circuit_create";
@@ -977,17 +1013,31 @@ int qm_circuit_apply(qm_circuit* c, void* state, ITYPE dim, void* stream, unsign
         return QM_ERR_INVALID_ARGUMENT;
     }
     if (!use_device(dev)) return QM_ERR_HIP;
+    hipStream_t st = stream_of(stream);
+    if (!qm::circuit_enqueue(c, state, dim, st, dev)) return QM_ERR_HIP;
+    finish_update(st, what);
+    return QM_OK;
+}
+
+}  // extern "C"
+
+namespace qm {
+bool circuit_enqueue(qm_circuit* c, void* state, u64 dim, hipStream_t st, unsigned dev) {
+    const char* what = "qm_circuit_apply";
+    if (!c || !state || dim != (1ULL << c->n)) {
+        set_error(QM_ERR_INVALID_ARGUMENT, "%s: null circuit/state or dim is not 2^%u", what, c ? c->n : 0u);
+        return false;
+    }
     if (c->device != (int)dev) {
         c->release();
         bool ok = dev_upload(&c->d_passes, c->passes) && dev_upload(&c->d_subs, c->subs) &&
                   dev_upload(&c->d_pool, c->pool);
         if (!ok) {
             c->release();
-            return QM_ERR_HIP;
+            return false;
         }
         c->device = (int)dev;
     }
-    hipStream_t st = stream_of(stream);
     const size_t lds = sizeof(double2) << c->lq;
     const void* fn = c->K == 2 ? (const void*)k_stage<2> : c->K == 3 ? (const void*)k_stage<3> : (const void*)k_stage<4>;
     if (c->per_cu == 0) {
@@ -995,10 +1045,10 @@ int qm_circuit_apply(qm_circuit* c, void* state, ITYPE dim, void* stream, unsign
         if (!hip_ok(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), what) ||
             !hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBatchThreads, lds), what) ||
             !hip_ok(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, (int)dev), what))
-            return QM_ERR_HIP;
+            return false;
         if (per_cu < 1) {
             set_error(QM_ERR_HIP, "%s: the kernel does not fit a compute unit with %zu bytes of LDS", what, lds);
-            return QM_ERR_HIP;
+            return false;
         }
         c->per_cu = per_cu;
         c->cus = cus;
@@ -1014,10 +1064,11 @@ int qm_circuit_apply(qm_circuit* c, void* state, ITYPE dim, void* stream, unsign
             default: k_stage<4><<<grid, kBatchThreads, lds, st>>>((double2*)state, c->lq, tl, sg, c->d_passes, c->d_subs, c->d_pool); break;
         }
     }
-    if (!hip_ok(hipGetLastError(), what)) return QM_ERR_HIP;
-    finish_update(st, what);
-    return QM_OK;
+    return hip_ok(hipGetLastError(), what);
 }
+}  // namespace qm
+
+extern "C" {
 
 void qm_circuit_stats(const qm_circuit* c, ITYPE* stages, ITYPE* passes, ITYPE* subops) {
     ITYPE np = 0, ns = 0;

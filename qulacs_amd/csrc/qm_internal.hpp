@@ -207,4 +207,31 @@ GatherPlan make_plan(const u32* targets, u32 nt, const u32* controls, const u32*
 void dense_any(const u32* targets, u32 nt, const u32* controls, const u32* cvals, u32 nc, const qm_complex* matrix,
                void* state, u64 dim, void* stream, unsigned device, const char* what);
 
+// ------------------------------------------------------------------ enqueue-only
+// Stream-ordered building blocks that never wait for the stream (the request
+// server chains them and waits on events).  The current device must already
+// be selected.  Host pointers named h_* should be pinned for the copies to be
+// asynchronous.
+
+// state <- |basis> (qm_runtime.hip)
+void enqueue_basis(void* state, u64 dim, u64 basis, hipStream_t st);
+// sum_k coef_k <P_k>: the Z-only terms' total is copied to h_out[0..1] and the
+// flip terms' total to h_out[2..3] (re, im); the caller adds the two.
+// (qm_reduce.hip)
+bool enqueue_observable(const u64* bf, const u64* pf, const u32* rot, const double* coef, u32 nterms,
+                        const void* state, u64 dim, hipStream_t st, double* h_out);
+// lower_bound sampling of one state vector with h_uniforms -> h_out (qm_sampling.hip)
+bool enqueue_sampling(const void* state, u64 dim, const double* h_uniforms, u32 count, u64* h_out, hipStream_t st);
+// Many small state vectors (dim <= 2^16 each), one work-group per CDF.
+struct SampleJob {
+    const double2* state;
+    u64 dim;
+    const double* h_uniforms;
+    u64* h_out;
+    u32 count;
+};
+bool enqueue_sampling_jobs(const SampleJob* jobs, u32 njobs, hipStream_t st);
+// qm_circuit_apply without the final synchronisation (qm_batch.hip)
+bool circuit_enqueue(qm_circuit* c, void* state, u64 dim, hipStream_t st, unsigned dev);
+
 }  // namespace qm

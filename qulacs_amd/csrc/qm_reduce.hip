@@ -174,16 +174,17 @@ __global__ __launch_bounds__(kRBlock) void k_finish(const double* __restrict__ p
 
 unsigned reduce_blocks(u64 work) { return grid_for(work, (u64)kRBlock * 16, kMaxRBlocks); }
 
-// Runs one reduction family.  `nterms` > 0 for term-indexed kinds.  Returns
-// false on error; result (re, im) of the ordered total in res.
+// Enqueues one reduction family; on success *dout points at the device result
+// (2 + 2 * max(nterms, 1) doubles: the ordered total, then per-term values),
+// valid until the next reduction on the same stream.
 template <int KIND>
-bool run_reduce(RArgs r, u64 work, const Term* host_terms, unsigned nterms, const qm_complex* coefs, hipStream_t st,
-                double res[2], std::vector<double>* per_term = nullptr) {
+bool enqueue_reduce(RArgs r, u64 work, const Term* host_terms, unsigned nterms, const qm_complex* coefs, hipStream_t st,
+                    double** dout, unsigned* ny_out) {
     Scratch* sc = scratch_for(st);
     if (!sc) return false;
     const unsigned nb = reduce_blocks(work);
     const unsigned ny = nterms ? nterms : 1;
-    if ((u64)nb * ny * 2 > kPartialsCap || 2 + 2 * (u64)ny > kHostResultCap) {
+    if ((u64)nb * ny * 2 + 2 + 2 * (u64)ny > kPartialsCap || 2 + 2 * (u64)ny > kHostResultCap) {
         set_error(QM_ERR_UNSUPPORTED, "too many terms in one reduction (%u)", ny);
         return false;
     }
@@ -197,13 +198,23 @@ bool run_reduce(RArgs r, u64 work, const Term* host_terms, unsigned nterms, cons
         if (!dcoefs) return false;
     }
     r.partials = sc->partials;
-    double* dout = sc->partials + (u64)nb * ny * 2;
-    if ((u64)nb * ny * 2 + 2 + 2 * (u64)ny > kPartialsCap) {
-        set_error(QM_ERR_UNSUPPORTED, "too many terms in one reduction (%u)", ny);
-        return false;
-    }
+    double* out = sc->partials + (u64)nb * ny * 2;
     k_reduce<KIND><<<dim3(nb, ny), kRBlock, 0, st>>>(r);
-    k_finish<<<1, kRBlock, 0, st>>>(sc->partials, nb, ny, dcoefs, dout);
+    k_finish<<<1, kRBlock, 0, st>>>(sc->partials, nb, ny, dcoefs, out);
+    *dout = out;
+    *ny_out = ny;
+    return true;
+}
+
+// Runs one reduction family and waits for it.  `nterms` > 0 for term-indexed
+// kinds.  Returns false on error; result (re, im) of the ordered total in res.
+template <int KIND>
+bool run_reduce(RArgs r, u64 work, const Term* host_terms, unsigned nterms, const qm_complex* coefs, hipStream_t st,
+                double res[2], std::vector<double>* per_term = nullptr) {
+    double* dout = nullptr;
+    unsigned ny = 0;
+    if (!enqueue_reduce<KIND>(r, work, host_terms, nterms, coefs, st, &dout, &ny)) return false;
+    Scratch* sc = scratch_for(st);
     const size_t nout = 2 + 2 * (size_t)ny;
     if (!hip_ok(hipMemcpyAsync(sc->host_result, dout, nout * sizeof(double), hipMemcpyDeviceToHost, st), "reduce"))
         return false;
@@ -211,6 +222,44 @@ bool run_reduce(RArgs r, u64 work, const Term* host_terms, unsigned nterms, cons
     res[0] = sc->host_result[0];
     res[1] = sc->host_result[1];
     if (per_term) per_term->assign(sc->host_result + 2, sc->host_result + nout);
+    return true;
+}
+
+bool enqueue_observable(const u64* bf, const u64* pf, const u32* rot, const double* coef, u32 nterms,
+                        const void* state, u64 dim, hipStream_t st, double* h_out) {
+    std::vector<Term> zt, xt;
+    std::vector<qm_complex> zc, xc;
+    for (u32 k = 0; k < nterms; ++k) {
+        const unsigned pivot = bf[k] ? 63 - __builtin_clzll(bf[k]) : 0;
+        Term t{bf[k], pf[k], rot[k], pivot};
+        if (bf[k]) {
+            xt.push_back(t);
+            xc.push_back(qm_complex{coef[k], 0.0});
+        } else {
+            zt.push_back(t);
+            zc.push_back(qm_complex{coef[k], 0.0});
+        }
+    }
+    RArgs r{};
+    r.a = (const double2*)state;
+    r.dim = dim;
+    for (int g = 0; g < 2; ++g) {
+        const std::vector<Term>& tv = g ? xt : zt;
+        const std::vector<qm_complex>& cv = g ? xc : zc;
+        if (tv.empty()) {
+            h_out[2 * g] = 0.0;
+            h_out[2 * g + 1] = 0.0;
+            continue;
+        }
+        double* dout = nullptr;
+        unsigned ny = 0;
+        if (!enqueue_reduce<R_PAULI>(r, g ? dim / 2 : dim, tv.data(), (unsigned)tv.size(), cv.data(), st, &dout, &ny))
+            return false;
+        // copied before the next reduction on this stream can reuse the scratch
+        if (!hip_ok(hipMemcpyAsync(h_out + 2 * g, dout, 2 * sizeof(double), hipMemcpyDeviceToHost, st),
+                    "enqueue_observable"))
+            return false;
+    }
     return true;
 }
 

@@ -137,6 +137,10 @@ __global__ void k_fill_basis(double2* __restrict__ s, u64 dim, u64 basis) {
         s[i] = make_double2(i == basis ? 1.0 : 0.0, 0.0);
 }
 
+void enqueue_basis(void* state, u64 dim, u64 basis, hipStream_t st) {
+    k_fill_basis<<<grid_for(dim, 256 * 8), 256, 0, st>>>((double2*)state, dim, basis);
+}
+
 // Counter-based generator (SplitMix64 finaliser on seed and index) feeding a
 // Box-Muller transform; each amplitude draws two independent normals.
 __device__ __forceinline__ u64 mix64(u64 z) {

@@ -19,6 +19,7 @@
 // Zero prefixes (s = 0, where fl(0 + p) = p) are handled the same way.
 #include <climits>
 #include <cmath>
+#include <vector>
 
 #include "qm_internal.hpp"
 
@@ -480,6 +481,119 @@ __global__ void k_search(const double* __restrict__ cdf, u64 dim, const double* 
         else hi = mid;
     }
     out[c] = lo - 1;
+}
+
+bool enqueue_sampling(const void* state, u64 dim, const double* h_uniforms, u32 count, u64* h_out, hipStream_t st) {
+    if (count == 0) return true;
+    Scratch* sc = scratch_for(st);
+    if (!sc) return false;
+    const size_t cdf_bytes = ((size_t)(dim + 1) * sizeof(double) + 255) & ~size_t(255);
+    const size_t work_bytes = (cdf_work_bytes(dim) + 255) & ~size_t(255);
+    const size_t io_bytes = ((size_t)count * 8 + 255) & ~size_t(255);
+    char* buf = (char*)big_buffer(sc, cdf_bytes + work_bytes + 2 * io_bytes);
+    if (!buf) return false;
+    double* cdf = (double*)buf;
+    double* dus = (double*)(buf + cdf_bytes + work_bytes);
+    u64* dout = (u64*)((char*)dus + io_bytes);
+    if (!hip_ok(hipMemcpyAsync(dus, h_uniforms, (size_t)count * 8, hipMemcpyHostToDevice, st), "enqueue_sampling"))
+        return false;
+    enqueue_cdf<false>((const double2*)state, dim, cdf, buf + cdf_bytes, st);
+    k_search<<<(count + 255) / 256, 256, 0, st>>>(cdf, dim, dus, count, dout);
+    return hip_ok(hipMemcpyAsync(h_out, dout, (size_t)count * 8, hipMemcpyDeviceToHost, st), "enqueue_sampling");
+}
+
+// One work-group per job: the exact sequential CDF of a small state (the
+// single-block walk of k_cdf_exact), then the job's searches.
+struct DevSampleJob {
+    const double2* state;
+    u64 dim;
+    double* cdf;
+    const double* uniforms;
+    u64* out;
+    u32 count, pad;
+};
+
+__global__ __launch_bounds__(kCdfThreads) void k_cdf_jobs(const DevSampleJob* __restrict__ jobs) {
+    __shared__ double P[kCdfChunk];
+    __shared__ u64 scan_lds[17];
+    __shared__ int min_lds[17];
+    __shared__ double s_bcast;
+    const DevSampleJob jb = jobs[blockIdx.x];
+    double s = 0.0;
+    if (threadIdx.x == 0) jb.cdf[0] = 0.0;
+    for (u64 c0 = 0; c0 < jb.dim; c0 += kCdfChunk) {
+        const int len = (int)min((u64)kCdfChunk, jb.dim - c0);
+        exact_chunk<false>(jb.state, jb.dim, c0, len, s, jb.cdf, P, scan_lds, min_lds, &s_bcast);
+    }
+    __syncthreads();
+    for (u32 c = threadIdx.x; c < jb.count; c += blockDim.x) {
+        const double u = jb.uniforms[c];
+        u64 lo = 0, hi = jb.dim + 1;
+        while (lo < hi) {
+            const u64 mid = lo + (hi - lo) / 2;
+            if (jb.cdf[mid] < u) lo = mid + 1;
+            else hi = mid;
+        }
+        jb.out[c] = lo - 1;
+    }
+}
+
+bool enqueue_sampling_jobs(const SampleJob* jobs, u32 njobs, hipStream_t st) {
+    if (njobs == 0) return true;
+    Scratch* sc = scratch_for(st);
+    if (!sc) return false;
+    size_t cdf_bytes = 0, total = 0;
+    bool contiguous = true;  // host uniforms and outputs laid out job after job
+    for (u32 j = 0; j < njobs; ++j) {
+        if (jobs[j].dim > kMultiBlockMinDim) {
+            set_error(QM_ERR_INVALID_ARGUMENT, "enqueue_sampling_jobs: state of %llu amplitudes is too large", jobs[j].dim);
+            return false;
+        }
+        cdf_bytes += ((size_t)(jobs[j].dim + 1) * 8 + 255) & ~size_t(255);
+        if (j && (jobs[j].h_uniforms != jobs[j - 1].h_uniforms + jobs[j - 1].count ||
+                  jobs[j].h_out != jobs[j - 1].h_out + jobs[j - 1].count))
+            contiguous = false;
+        total += jobs[j].count;
+    }
+    const size_t desc_bytes = ((size_t)njobs * sizeof(DevSampleJob) + 255) & ~size_t(255);
+    const size_t io_bytes = (total * 8 + 255) & ~size_t(255);
+    char* buf = (char*)big_buffer(sc, desc_bytes + cdf_bytes + 2 * io_bytes);
+    if (!buf) return false;
+    double* dus = (double*)(buf + desc_bytes + cdf_bytes);
+    u64* dout = (u64*)((char*)dus + io_bytes);
+    std::vector<DevSampleJob> d(njobs);
+    size_t off = desc_bytes, io = 0;
+    for (u32 j = 0; j < njobs; ++j) {
+        d[j].state = jobs[j].state;
+        d[j].dim = jobs[j].dim;
+        d[j].count = jobs[j].count;
+        d[j].cdf = (double*)(buf + off);
+        off += ((size_t)(jobs[j].dim + 1) * 8 + 255) & ~size_t(255);
+        d[j].uniforms = dus + io;
+        d[j].out = dout + io;
+        if (!contiguous && jobs[j].count &&
+            !hip_ok(hipMemcpyAsync(dus + io, jobs[j].h_uniforms, (size_t)jobs[j].count * 8, hipMemcpyHostToDevice, st),
+                    "enqueue_sampling_jobs"))
+            return false;
+        io += jobs[j].count;
+    }
+    if (contiguous && total &&
+        !hip_ok(hipMemcpyAsync(dus, jobs[0].h_uniforms, total * 8, hipMemcpyHostToDevice, st), "enqueue_sampling_jobs"))
+        return false;
+    if (!hip_ok(hipMemcpyAsync(buf, d.data(), (size_t)njobs * sizeof(DevSampleJob), hipMemcpyHostToDevice, st),
+                "enqueue_sampling_jobs"))
+        return false;
+    k_cdf_jobs<<<njobs, kCdfThreads, 0, st>>>((const DevSampleJob*)buf);
+    if (contiguous) {
+        return !total ||
+               hip_ok(hipMemcpyAsync(jobs[0].h_out, dout, total * 8, hipMemcpyDeviceToHost, st), "enqueue_sampling_jobs");
+    }
+    for (u32 j = 0; j < njobs; ++j)
+        if (jobs[j].count && !hip_ok(hipMemcpyAsync(jobs[j].h_out, d[j].out, (size_t)jobs[j].count * 8,
+                                                    hipMemcpyDeviceToHost, st),
+                                     "enqueue_sampling_jobs"))
+            return false;
+    return true;
 }
 
 template <bool DM>

@@ -233,3 +233,48 @@ def brickwork_circuit(n: int = 20, layers: int = 10, terms: int = 16, samples: i
             rot += int(p[q] == 2)
         obs.append((bf, pf, rot, float(rng.standard_normal())))
     return gates, obs, rng.random(samples)
+
+
+# ---------------------------------------------------------------- serving stream
+@dataclass
+class ServingRequest:
+    arrival: float
+    n: int
+    depth: int
+    gates: list
+    terms: list
+    uniforms: np.ndarray
+
+
+def serving_requests(count: int = 64, rate: float = 50.0, seed: int = 5005, n_range=(10, 28),
+                     depth_range=(20, 200), terms: int = 8, samples: int = 1000) -> list:
+    """Poisson arrivals at `rate` per second; each request is a brickwork circuit
+    (a Haar 2x2 on every qubit, then CNOT(i, i+1) on even/odd i by layer) with
+    n = round(exp(U[ln lo, ln hi])) qubits and depth ~ U{depth_range}, a
+    `terms`-term random Pauli observable (coef ~ N(0, 1)) and `samples` uniforms."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for _ in range(count):
+        t += float(rng.exponential(1.0 / rate))
+        n = int(round(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1])))))
+        depth = int(rng.integers(depth_range[0], depth_range[1] + 1))
+        gates = []
+        for layer in range(depth):
+            for q in range(n):
+                gates.append(("dense", [q], haar_unitary(rng, 1)))
+            for q in range(layer % 2, n - 1, 2):
+                gates.append(("CNOT", [q, q + 1], None))
+        obs = []
+        for _ in range(terms):
+            p = rng.integers(0, 4, size=n)
+            bf = pf = rot = 0
+            for q in range(n):
+                if p[q] in (1, 2):
+                    bf |= 1 << q
+                if p[q] in (2, 3):
+                    pf |= 1 << q
+                rot += int(p[q] == 2)
+            obs.append((bf, pf, rot, float(rng.standard_normal())))
+        out.append(ServingRequest(t, n, depth, gates, obs, rng.random(samples)))
+    return out

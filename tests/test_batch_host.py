@@ -59,3 +59,14 @@ def test_fused_circuit_rejects_bad_arguments():
     st = fc.stats()
     assert st["stages"] >= 2 and st["passes"] >= st["stages"]
     fc.close()
+
+
+def test_server_rejects_bad_configuration():
+    from qulacs_amd.serving import Server
+
+    with pytest.raises(QulacsMI355Error):
+        Server(hbm_budget_bytes=100)
+    with pytest.raises(QulacsMI355Error):
+        Server(hbm_budget_bytes=1 << 30, large_slots=0)
+    with pytest.raises(QulacsMI355Error):
+        Server(hbm_budget_bytes=1 << 30, small_max_qubits=17)
