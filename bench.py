@@ -480,6 +480,119 @@ def brickwork(args, lib_mod, hip):
     return out
 
 
+def serving(args, lib_mod, hip):
+    """Poisson stream of mixed-size brickwork circuits through the request
+    server, against the same requests run one after another through the
+    per-gate path (FIFO, same arrival times)."""
+    import threading
+
+    from qulacs_amd.batch import _marshal
+    from qulacs_amd.serving import Server
+    from qulacs_amd.workloads import serving_requests
+
+    reqs = serving_requests(args.serve_count, rate=args.serve_rate, seed=args.seed + 5,
+                            n_range=(10, args.serve_max_qubits))
+    L = lib_mod.lib()
+    # pre-marshal so submission costs only the call
+    prepared = []
+    for r in reqs:
+        arr, keep = _marshal(r.gates)
+        t = len(r.terms)
+        prepared.append((arr, keep, (C.c_ulonglong * t)(*[x[0] for x in r.terms]),
+                         (C.c_ulonglong * t)(*[x[1] for x in r.terms]), (C.c_uint * t)(*[x[2] for x in r.terms]),
+                         (C.c_double * t)(*[x[3] for x in r.terms]), np.ascontiguousarray(r.uniforms)))
+    srv = Server(hbm_budget_bytes=args.serve_budget_gib << 30, large_slots=4, small_max_qubits=16)
+    # warm-up: one small and one large request (kernel loading, allocator pools)
+    for r in (min(reqs, key=lambda x: x.n), max(reqs, key=lambda x: x.n)):
+        srv.wait(srv.submit(r.n, r.gates[:8], r.terms, r.uniforms[:4]))
+    ids = [None] * len(reqs)
+    submit_at = [0.0] * len(reqs)
+
+    def submitter():
+        t0 = time.perf_counter()
+        for i, (r, p) in enumerate(zip(reqs, prepared)):
+            delay = r.arrival - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            submit_at[i] = time.perf_counter() - t0
+            ids[i] = L.qm_server_submit(srv.h, r.n, p[0], len(r.gates), p[2], p[3], p[4], p[5], len(r.terms),
+                                        p[6].ctypes.data_as(lib_mod.PD), p[6].size)
+
+    t_start = time.perf_counter()
+    th = threading.Thread(target=submitter)
+    th.start()
+    th.join()
+    lib_mod.check()
+    served = []
+    for i, r in enumerate(reqs):
+        e = C.c_double()
+        lat = C.c_double()
+        out = np.zeros(r.uniforms.size, dtype=np.uint64)
+        L.qm_server_wait(srv.h, ids[i], C.byref(e), out.ctypes.data_as(lib_mod.PI), C.byref(lat))
+        served.append((e.value, out, lat.value))
+    t_end = time.perf_counter()
+    lib_mod.check()
+    stats = srv.stats()
+    srv.close()
+    lat_served = np.array([x[2] for x in served])
+    makespan = max(submit_at[i] + served[i][2] for i in range(len(reqs)))
+
+    # the per-gate path, one request at a time
+    stream = lib_mod.call("allocate_cuda_stream_host", 1, 0)
+    lib_mod.call("qm_set_synchronous", 0)
+    service = []
+    match_e, match_s = 0, 0
+    for r, p in zip(reqs, prepared):
+        t0 = time.perf_counter()
+        dim = 1 << r.n
+        st = lib_mod.call("allocate_quantum_state_host", dim, 0)
+        L.initialize_quantum_state_host(st, dim, stream, 0)
+        for kind, q, m in r.gates:
+            if kind == "CNOT":
+                L.CNOT_gate_host(q[0], q[1], st, dim, stream, 0)
+            else:
+                L.single_qubit_dense_matrix_gate_host(q[0], _cm(lib_mod, m), st, dim, stream, 0)
+        co = (lib_mod.QmComplex * len(r.terms))(*[lib_mod.QmComplex(x[3], 0.0) for x in r.terms])
+        e = L.qm_observable_expectation_host(p[2], p[3], p[4], co, len(r.terms), st, dim, stream, 0).re
+        out = np.zeros(r.uniforms.size, dtype=np.uint64)
+        L.qm_sampling_host(p[6].ctypes.data_as(lib_mod.PD), p[6].size, out.ctypes.data_as(lib_mod.PI), st, dim,
+                           stream, 0)
+        lib_mod.call("release_quantum_state_host", st, 0)
+        service.append(time.perf_counter() - t0)
+        i = len(service) - 1
+        scale = sum(abs(x[3]) for x in r.terms)
+        match_e += int(abs(e - served[i][0]) <= 1e-10 * scale)
+        match_s += int(np.array_equal(out, served[i][1]))
+    lib_mod.check()
+    lib_mod.call("qm_set_synchronous", 1)
+    lib_mod.call("release_cuda_stream_host", stream, 1, 0)
+    finish = 0.0
+    lat_plain = []
+    for r, sv in zip(reqs, service):
+        start = max(r.arrival, finish)
+        finish = start + sv
+        lat_plain.append(finish - r.arrival)
+    lat_plain = np.array(lat_plain)
+    plain_makespan = finish - reqs[0].arrival
+
+    def pct(a, q):
+        return round(float(np.percentile(a, q)) * 1e3, 2)
+
+    return {
+        "workload": f"{len(reqs)} Poisson requests at {args.serve_rate}/s, brickwork circuits, n = round(exp(U[ln 10, "
+                    f"ln {args.serve_max_qubits}])), depth U{{20..200}}, 8-term observable, 1000 samples each",
+        "qubits": [r.n for r in reqs], "gates": int(sum(len(r.gates) for r in reqs)),
+        "served": {"circuits_per_s": round(len(reqs) / makespan, 2), "p50_ms": pct(lat_served, 50),
+                   "p99_ms": pct(lat_served, 99), "makespan_s": round(makespan, 3), "stats": stats},
+        "plain_sequential": {"circuits_per_s": round(len(reqs) / plain_makespan, 2), "p50_ms": pct(lat_plain, 50),
+                             "p99_ms": pct(lat_plain, 99), "service_s_total": round(float(np.sum(service)), 3),
+                             "method": "each request alone through the per-gate C ABI path; latency from the same "
+                                       "arrival times with first-in first-out queueing"},
+        "speedup_circuits_per_s": round(plain_makespan / makespan, 2),
+        "energies_match": f"{match_e}/{len(reqs)}", "samples_identical": f"{match_s}/{len(reqs)}",
+    }
+
+
 def _mask_lists(bf, pf, n):
     t, p = [], []
     for q in range(n):
@@ -556,6 +669,11 @@ def main() -> None:
     ap.add_argument("--small-steps", type=int, default=5)
     ap.add_argument("--small-plain", type=int, default=64)
     ap.add_argument("--no-small", action="store_true")
+    ap.add_argument("--serve-count", type=int, default=48)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-max-qubits", type=int, default=28)
+    ap.add_argument("--serve-budget-gib", type=int, default=64)
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the sweep is a single 30-qubit "
@@ -580,6 +698,10 @@ def main() -> None:
         small = many_small(args, _lib, hip)
         print(f"[bench] many small circuits {small['circuits_per_s']} circuits/s ({small['ms_per_step']} ms)",
               file=sys.stderr, flush=True)
+    serve = None
+    if not args.no_serve:
+        serve = serving(args, _lib, hip)
+        print(f"[bench] serving {serve['served']} vs plain {serve['plain_sequential']}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args, sv["gates"], args.qubits)
@@ -608,6 +730,7 @@ def main() -> None:
         "dm_sweep": dm,
         "brickwork_20q": c20,
         "many_small_circuits": small,
+        "serving": serve,
         "norm_after": sv["norm_after"],
     }
     print(json.dumps(line), flush=True)

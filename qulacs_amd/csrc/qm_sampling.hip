@@ -372,7 +372,8 @@ __global__ __launch_bounds__(kCdfThreads) void k_cdf_walk(const double2* __restr
         const int cross =
             ((int)threadIdx.x < first_bad && S + excl + v >= LIM) ? (int)threadIdx.x : INT_MAX;
         const int first_cross = block_min_int(cross, min_lds);
-        const int stop = min(first_bad, first_cross);  // chunks c .. c + stop - 1 are fast
+        // chunks c .. c + stop - 1 are fast (at most the window's kCdfThreads)
+        const int stop = min(min(first_bad, first_cross), kCdfThreads);
         if ((int)threadIdx.x < stop) {
             start[ci] = s == 0.0 ? 0.0 : (double)(S + excl) * u;
             exact[ci] = 0;

@@ -245,7 +245,7 @@ def test_sampling_identical_to_sequential_cpu(oracle, n):
         assert np.array_equal(out, want), (label, np.nonzero(out != want)[0][:5])
 
 
-@pytest.mark.parametrize("n", [15, 16, 17, 20])
+@pytest.mark.parametrize("n", [15, 16, 17, 20, 22, 23])
 def test_sampling_cdf_bit_identical(oracle, n):
     """The device CDF (single work-group below 2^16 amplitudes, the multi-block
     chunk walk from 2^16 up) equals the sequential loop's stacked sums bit for bit."""
