@@ -593,6 +593,92 @@ def serving(args, lib_mod, hip):
     }
 
 
+def adjoint(args, lib_mod, hip):
+    """Adjoint gradients: 20-qubit hardware-efficient ansatz (12 layers of RY, RZ
+    on every qubit + CZ chain = 480 parameters), Heisenberg XXX open chain (57
+    terms), `adj_batch` parameter sets in one forward/backward."""
+    import torch
+
+    from oracle import oracle as O
+    from qulacs_amd.adjoint import AdjointCircuit, hardware_efficient_ansatz, heisenberg_terms
+
+    n, layers, B = 20, 12, args.adj_batch
+    gates, P = hardware_efficient_ansatz(n, layers)
+    terms = heisenberg_terms(n)
+    rng = np.random.default_rng(args.seed + 7)
+    theta_np = rng.uniform(0, 2 * np.pi, (B, P))
+    theta = torch.tensor(theta_np, device="cuda:0")
+    circ = AdjointCircuit(n, gates, P)
+    circ.run(theta[:2], terms)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    energy, grad = circ.run(theta, terms)
+    torch.cuda.synchronize()
+    gpu_s = time.perf_counter() - t0
+    energy, grad = energy.cpu().numpy(), grad.cpu().numpy()
+    # central differences on two parameters of set 0 (step 1e-6), through the same forward
+    fd = []
+    for p in (0, P - 1):
+        tp = theta[:1].clone()
+        tm = theta[:1].clone()
+        tp[0, p] += 1e-6
+        tm[0, p] -= 1e-6
+        ep, _ = circ.run(tp, terms)
+        em, _ = circ.run(tm, terms)
+        fd.append(abs((ep.item() - em.item()) / 2e-6 - grad[0, p]))
+    scale = sum(abs(t[3]) for t in terms)
+    out = {"workload": f"{n}-qubit ansatz, {layers} layers, {P} parameters, {len(terms)}-term Heisenberg chain, "
+                       f"{B} parameter sets", "gates": len(gates),
+           "gpu_s": round(gpu_s, 3), "gradients_per_s": round(B / gpu_s, 2),
+           "fd_max_abs_diff": max(fd), "fd_within_1e-8": bool(max(fd) <= 1e-8 * scale)}
+    if not args.no_cpu_baseline and O.reference_available():
+        # the reference's backprop loop (src/vqcsim/parametric_circuit.cpp:205-335) driven
+        # through the compiled csim kernels, one parameter set
+        k = O.Reference()
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        threads = min(threads, os.cpu_count() or threads)
+        k.set_num_threads(threads)
+        th = theta_np[0]
+        t0 = time.perf_counter()
+
+        def apply(g, psi, inverse=False):
+            if g[0] == "rot":
+                _, qs, ps, p, c = g
+                k.pauli_rotation(qs, ps, -c * th[p] if inverse else c * th[p], psi)
+            else:
+                k.two("CZ", g[1][0], g[1][1], psi)
+
+        psi = np.zeros(1 << n, dtype=np.complex128)
+        psi[0] = 1
+        for g in gates:
+            apply(g, psi)
+        lam = np.zeros_like(psi)
+        work = np.empty_like(psi)
+        for bf, pf, rot, c in terms:
+            work[:] = psi
+            tl, pl = _mask_lists(bf, pf, n)
+            k.pauli(tl, pl, work)
+            k.add_with_coef(c, work, lam)
+        k.multiply(2.0, lam)
+        g_cpu = np.zeros(P)
+        for g in reversed(gates):
+            if g[0] == "rot":
+                work[:] = psi
+                k.pauli_rotation(g[1], g[2], np.pi, work)
+                g_cpu[g[3]] += g[4] * k.inner_product(lam, work).real / 2
+            apply(g, lam, inverse=True)
+            apply(g, psi, inverse=True)
+        cpu_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"s_per_gradient": round(cpu_s, 3), "cores": threads, "kind": "reference",
+                               "sample": "one parameter set, the reference backprop loop over the compiled csim "
+                                         "kernels"}
+        out["gpu_speedup_per_gradient"] = round(cpu_s / (gpu_s / B), 1)
+        out["cpu_gpu_max_abs_diff"] = float(np.abs(g_cpu - grad[0]).max())
+        out["cpu_gpu_within_1e-10"] = bool(np.abs(g_cpu - grad[0]).max() <= 1e-10 * scale)
+    circ.close()
+    return out
+
+
 def _mask_lists(bf, pf, n):
     t, p = [], []
     for q in range(n):
@@ -674,6 +760,8 @@ def main() -> None:
     ap.add_argument("--serve-max-qubits", type=int, default=28)
     ap.add_argument("--serve-budget-gib", type=int, default=64)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--adj-batch", type=int, default=256)
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the sweep is a single 30-qubit "
@@ -698,6 +786,10 @@ def main() -> None:
         small = many_small(args, _lib, hip)
         print(f"[bench] many small circuits {small['circuits_per_s']} circuits/s ({small['ms_per_step']} ms)",
               file=sys.stderr, flush=True)
+    adj = None
+    if not args.no_adjoint:
+        adj = adjoint(args, _lib, hip)
+        print(f"[bench] adjoint {adj}", file=sys.stderr, flush=True)
     serve = None
     if not args.no_serve:
         serve = serving(args, _lib, hip)
@@ -731,6 +823,7 @@ def main() -> None:
         "brickwork_20q": c20,
         "many_small_circuits": small,
         "serving": serve,
+        "adjoint": adj,
         "norm_after": sv["norm_after"],
     }
     print(json.dumps(line), flush=True)

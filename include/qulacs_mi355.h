@@ -310,6 +310,33 @@ QM_EXPORT void qm_circuit_destroy(qm_circuit* circuit);
 QM_EXPORT int qm_circuit_apply(qm_circuit* circuit, void* state, ITYPE dim, void* stream, unsigned int device_number);
 QM_EXPORT void qm_circuit_stats(const qm_circuit* circuit, ITYPE* stages, ITYPE* passes, ITYPE* subops);
 
+/* ------------------------------------------------------------------ adjoint gradients
+ * d<psi(theta)|H|psi(theta)>/d theta for a circuit of fixed gates (batch kinds
+ * on 1..2 qubits) and parametric rotations exp(i coef theta[param] P / 2) with
+ * P a Pauli string on 1..4 qubits, for `batch` parameter sets at once.
+ * Restates ParametricQuantumCircuit::backprop (src/vqcsim/parametric_circuit.cpp:205-335)
+ * with one fused pass per gate on the way back (gradient term, then the
+ * inverse applied to both states). */
+#define QM_ADJ_ROT 4u
+typedef struct qm_adjoint_gate {
+    UINT kind;                 /* QM_BATCH_DENSE/DIAG/CZ/CNOT (fixed) or QM_ADJ_ROT */
+    UINT qubit_count;
+    UINT qubits[4];
+    const qm_complex* matrix;  /* fixed dense / diagonal gates (host) */
+    UINT paulis[4];            /* QM_ADJ_ROT: 1 = X, 2 = Y, 3 = Z on qubits[i] */
+    UINT param;                /* QM_ADJ_ROT: index into theta */
+    double coef;               /* QM_ADJ_ROT: angle = coef * theta[param] */
+} qm_adjoint_gate;
+typedef struct qm_adjoint qm_adjoint;
+QM_EXPORT qm_adjoint* qm_adjoint_create(UINT qubit_count, const qm_adjoint_gate* gates, UINT gate_count, UINT param_count);
+QM_EXPORT void qm_adjoint_destroy(qm_adjoint* circuit);
+/* Device memory qm_adjoint_run needs for `batch` parameter sets. */
+QM_EXPORT ITYPE qm_adjoint_workspace_bytes(const qm_adjoint* circuit, UINT batch);
+/* theta: device, batch x param_count (row-major); energy: device, batch;
+ * grad: device, batch x param_count; workspace: device, qm_adjoint_workspace_bytes.
+ * The observable is sum_k coefs[k] P_k in mask form (host arrays, real coefficients). */
+QM_EXPORT int qm_adjoint_run(qm_adjoint* circuit, UINT batch, const double* theta, const ITYPE* bit_flip_masks, const ITYPE* phase_flip_masks, const UINT* global_phase_90rot_counts, const double* coefs, UINT term_count, double* energy, double* grad, void* workspace, void* stream, unsigned int device_number);
+
 /* ------------------------------------------------------------------ request server
  * Circuits of mixed size (1..40 qubits) submitted from any thread and run on
  * one device with continuous batching: each request is a circuit from

@@ -31,6 +31,12 @@ class QmBatchGate(C.Structure):
 
 
 BATCH_DENSE, BATCH_DIAG, BATCH_CZ, BATCH_CNOT = 0, 1, 2, 3
+ADJ_ROT = 4
+
+
+class QmAdjointGate(C.Structure):
+    _fields_ = [("kind", C.c_uint), ("qubit_count", C.c_uint), ("qubits", C.c_uint * 4), ("matrix", PC),
+                ("paulis", C.c_uint * 4), ("param", C.c_uint), ("coef", C.c_double)]
 
 # name -> (restype, argtypes)
 _SIGS: dict[str, tuple] = {
@@ -144,6 +150,10 @@ _SIGS: dict[str, tuple] = {
     "qm_circuit_destroy": (None, [VP]),
     "qm_circuit_apply": (C.c_int, [VP, VP, ITYPE, VP, UINT]),
     "qm_circuit_stats": (None, [VP, PI, PI, PI]),
+    "qm_adjoint_create": (VP, [UINT, C.POINTER(QmAdjointGate), UINT, UINT]),
+    "qm_adjoint_destroy": (None, [VP]),
+    "qm_adjoint_workspace_bytes": (ITYPE, [VP, UINT]),
+    "qm_adjoint_run": (C.c_int, [VP, UINT, VP, PI, PI, PU, PD, UINT, VP, VP, VP, VP, UINT]),
     "qm_server_create": (VP, [ITYPE, UINT, UINT, UINT]),
     "qm_server_submit": (C.c_longlong, [VP, UINT, C.POINTER(QmBatchGate), UINT, PI, PI, PU, PD, UINT, PD, UINT]),
     "qm_server_wait": (C.c_int, [VP, C.c_longlong, PD, PI, PD]),

@@ -70,3 +70,14 @@ def test_server_rejects_bad_configuration():
         Server(hbm_budget_bytes=1 << 30, large_slots=0)
     with pytest.raises(QulacsMI355Error):
         Server(hbm_budget_bytes=1 << 30, small_max_qubits=17)
+
+
+def test_adjoint_circuit_validation_is_host_only():
+    from qulacs_amd.adjoint import AdjointCircuit, hardware_efficient_ansatz, heisenberg_terms
+
+    gates, P = hardware_efficient_ansatz(20, 12)
+    assert P == 480 and len(heisenberg_terms(20)) == 57
+    c = AdjointCircuit(20, gates, P)
+    c.close()
+    with pytest.raises(QulacsMI355Error):
+        AdjointCircuit(3, [("rot", [0], [4], 0, 1.0)], 1)  # Pauli id 4
