@@ -279,6 +279,17 @@ def many_small(args, lib_mod, hip):
     ms = hip.elapsed_ms(ev[0], ev[1]) / args.small_steps
     hip.destroy(ev)
     energies = b.run(stream)
+    # the same launch captured in a HIP graph and replayed
+    b.capture(stream)
+    g_e = b.run_graph(stream)
+    ev = hip.create(2)
+    hip.record(ev[0], hstream)
+    for _ in range(args.small_steps):
+        lib_mod.lib().qm_batch_run_graph(b.h, None, stream, 0)
+    hip.record(ev[1], hstream)
+    graph_ms = hip.elapsed_ms(ev[0], ev[1]) / args.small_steps
+    hip.destroy(ev)
+    graph_same = bool(np.array_equal(g_e, energies))
     flops = sum(small_circuit_alg_flops(c) for c in cs)
     stats = b.stats()
     b.close()
@@ -321,7 +332,8 @@ def many_small(args, lib_mod, hip):
         "fp64": {"bound": "fp64-valu", "achieved": round(flops / (ms * 1e-3) / 1e12, 2), "peak": FP64_PEAK_TFLOPS,
                  "unit": "TFLOP/s", "frac": round(flops / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
                  "flops_per_step": flops},
-        "autotune_s": {str(k): round(v, 5) for k, v in tune.items()}, "schedule": stats,
+        "graph_ms_per_step": round(graph_ms, 3), "graph_matches_eager": graph_same,
+        "autotune_s": {f"K{k[0]}/window{k[1]}": round(v, 5) for k, v in tune.items()}, "schedule": stats,
         "host_add_s": round(t1 - t0, 2), "host_compile_and_tune_s": round(t2 - t1, 2),
         "plain_gpu": {"circuits_per_s": round(len(cs) / plain_s, 1),
                       "sample": f"{len(sub)} evenly spaced circuits ({plain_gates} gates), one C ABI call per gate, "
@@ -756,12 +768,14 @@ def main() -> None:
     ap.add_argument("--small-plain", type=int, default=64)
     ap.add_argument("--no-small", action="store_true")
     ap.add_argument("--serve-count", type=int, default=48)
-    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-rate", type=float, default=100.0)
     ap.add_argument("--serve-max-qubits", type=int, default=28)
     ap.add_argument("--serve-budget-gib", type=int, default=64)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--adj-batch", type=int, default=256)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--only", choices=["dm", "brickwork", "small", "adjoint", "serve"],
+                    help="run one secondary leg alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the sweep is a single 30-qubit "
@@ -771,6 +785,11 @@ def main() -> None:
     from qulacs_amd import _lib
 
     hip = HipEvents()
+    if args.only:
+        leg = {"dm": dm_sweep, "brickwork": brickwork, "small": many_small, "adjoint": adjoint,
+               "serve": serving}[args.only]
+        print(json.dumps({args.only: leg(args, _lib, hip)}, default=str), flush=True)
+        return
     print(f"[bench] {args.qubits}-qubit sweep, {args.gates} gates, warmup {args.warmup}, steps {args.steps}",
           file=sys.stderr, flush=True)
     sv = sv_sweep(args, _lib, hip)

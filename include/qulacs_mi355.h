@@ -286,6 +286,15 @@ QM_EXPORT const double* qm_batch_run_async(qm_batch* batch, void* stream, unsign
 /* Copies circuit i's final state (2^n complex128) to host memory after a run
  * compiled with keep_states. */
 QM_EXPORT int qm_batch_get_state(qm_batch* batch, UINT index, qm_complex* out, void* stream, unsigned int device_number);
+/* Qubits held in LDS per tile (8..13, default 13); takes effect at the next
+ * compile.  A smaller window halves LDS per work-group (two work-groups per CU
+ * at 12) and tiles larger circuits. */
+QM_EXPORT int qm_batch_set_local_qubits(qm_batch* batch, UINT local_qubits);
+/* Captures one run (queue reset + the launch) on `stream` (a created stream,
+ * not NULL) into a HIP graph; qm_batch_run_graph replays it.  A compile drops
+ * the graph.  The graph holds kernel nodes only. */
+QM_EXPORT int qm_batch_capture(qm_batch* batch, void* stream, unsigned int device_number);
+QM_EXPORT int qm_batch_run_graph(qm_batch* batch, double* energies, void* stream, unsigned int device_number);
 /* Removes every circuit (device memory is kept for the next compile). */
 QM_EXPORT void qm_batch_clear(qm_batch* batch);
 /* Device address of circuit i's final state (compiled with keep_states). */

@@ -145,6 +145,9 @@ _SIGS: dict[str, tuple] = {
     "qm_batch_get_state": (C.c_int, [VP, UINT, PC, VP, UINT]),
     "qm_batch_stats": (None, [VP, PI, PI, PI, PI]),
     "qm_batch_clear": (None, [VP]),
+    "qm_batch_set_local_qubits": (C.c_int, [VP, UINT]),
+    "qm_batch_capture": (C.c_int, [VP, VP, UINT]),
+    "qm_batch_run_graph": (C.c_int, [VP, PD, VP, UINT]),
     "qm_batch_state_device_ptr": (VP, [VP, UINT]),
     "qm_circuit_create": (VP, [UINT, C.POINTER(QmBatchGate), UINT, UINT, UINT]),
     "qm_circuit_destroy": (None, [VP]),

@@ -79,14 +79,25 @@ class Batch:
         self.sizes.append(int(n))
         return idx
 
-    def compile(self, block_qubits: int = 4, keep_states: bool = False) -> None:
+    def compile(self, block_qubits: int = 4, keep_states: bool = False, local_qubits: int = 13) -> None:
+        _lib.call("qm_batch_set_local_qubits", self.h, int(local_qubits))
         _lib.call("qm_batch_compile", self.h, int(block_qubits), 1 if keep_states else 0, self.device)
         self.block_qubits = block_qubits
+        self.local_qubits = local_qubits
         self.keep_states = keep_states
 
     def run(self, stream=None) -> np.ndarray:
         out = np.zeros(max(1, len(self.sizes)), dtype=np.float64)
         _lib.call("qm_batch_run", self.h, out.ctypes.data_as(_lib.PD), stream, self.device)
+        return out[: len(self.sizes)]
+
+    def capture(self, stream) -> None:
+        """Capture one run into a HIP graph on `stream` (from allocate_cuda_stream_host)."""
+        _lib.call("qm_batch_capture", self.h, stream, self.device)
+
+    def run_graph(self, stream) -> np.ndarray:
+        out = np.zeros(max(1, len(self.sizes)), dtype=np.float64)
+        _lib.call("qm_batch_run_graph", self.h, out.ctypes.data_as(_lib.PD), stream, self.device)
         return out[: len(self.sizes)]
 
     def run_async(self, stream=None) -> int:
@@ -103,25 +114,28 @@ class Batch:
         _lib.call("qm_batch_stats", self.h, *[C.byref(x) for x in v])
         return dict(zip(("circuits", "stages", "passes", "subops"), (int(x.value) for x in v)))
 
-    def autotune(self, candidates=(2, 3, 4), repeats: int = 2, keep_states: bool = False) -> dict:
-        """Compile with each register-block size, time `repeats` runs of each and
-        keep the fastest.  Returns {block_qubits: best seconds}."""
+    def autotune(self, candidates=(2, 3, 4), local_candidates=(12, 13), repeats: int = 2,
+                 keep_states: bool = False) -> dict:
+        """Compile with each (register-block size, LDS window) pair, time `repeats`
+        runs of each and keep the fastest.  Returns {(block, window): best seconds}."""
         times = {}
         for k in candidates:
-            try:
-                self.compile(k, keep_states)
-            except _lib.QulacsMI355Error:
-                continue  # a dense gate wider than k
-            self.run()
-            best = float("inf")
-            for _ in range(repeats):
-                t0 = time.perf_counter()
+            for lq in local_candidates:
+                try:
+                    self.compile(k, keep_states, lq)
+                except _lib.QulacsMI355Error:
+                    continue  # a dense gate wider than k
                 self.run()
-                best = min(best, time.perf_counter() - t0)
-            times[k] = best
+                best = float("inf")
+                for _ in range(repeats):
+                    t0 = time.perf_counter()
+                    self.run()
+                    best = min(best, time.perf_counter() - t0)
+                times[(k, lq)] = best
         if not times:
             raise _lib.QulacsMI355Error("autotune: no block size can hold the batch's gates")
-        self.compile(min(times, key=times.get), keep_states)
+        k, lq = min(times, key=times.get)
+        self.compile(k, keep_states, lq)
         return times
 
 

@@ -101,20 +101,6 @@ struct HCirc {
     std::vector<DTerm> terms;
 };
 
-struct BatchArgs {
-    const DCirc* circs;
-    const DStage* stages;
-    const DPass* passes;
-    const DSub* subs;
-    const double2* pool;
-    const DTerm* terms;
-    const u32* order;
-    double2* scratch;
-    double* energies;
-    int* counter;
-    u32 ncirc;
-    int keep;
-};
 
 __host__ __device__ __forceinline__ u32 fold3(u32 x) { return ((x >> 3) ^ (x >> 6) ^ (x >> 9) ^ (x >> 12)) & 7u; }
 __host__ __device__ __forceinline__ u32 slot_of(u32 x) { return x ^ fold3(x); }
@@ -310,8 +296,15 @@ __device__ __forceinline__ void run_passes(double2* st, u32 lq, const DPass* __r
     }
 }
 
+// The program arrays arrive as separate read-only, non-aliasing arguments so the
+// compiler reads them through the scalar cache (inside a by-value struct they
+// became per-lane vector loads).
 template <int K>
-__global__ __launch_bounds__(kBatchThreads) void k_batch(BatchArgs a) {
+__global__ __launch_bounds__(kBatchThreads) void k_batch(
+    const DCirc* __restrict__ circs, const DStage* __restrict__ stages, const DPass* __restrict__ passes,
+    const DSub* __restrict__ subs, const double2* __restrict__ pool, const DTerm* __restrict__ terms,
+    const u32* __restrict__ order, double2* __restrict__ scratch, double* __restrict__ energies, int* counter,
+    u32 ncirc, int keep) {
     extern __shared__ double2 st[];
     __shared__ u32 hi_tab[1u << (kMaxLocal - kLowFixed)];
     __shared__ double red[16];
@@ -319,18 +312,18 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(BatchArgs a) {
     const u32 tid = threadIdx.x, bd = blockDim.x;
 
     for (;;) {
-        if (tid == 0) s_slot = atomicAdd(a.counter, 1);
+        if (tid == 0) s_slot = atomicAdd(counter, 1);
         __syncthreads();
         const int slot = s_slot;
         __syncthreads();
-        if (slot >= (int)a.ncirc) break;  // uniform: every thread read the same slot
-        const u32 ci = a.order[slot];
-        const DCirc c = a.circs[ci];
+        if (slot >= (int)ncirc) break;  // uniform: every thread read the same slot
+        const u32 ci = order[slot];
+        const DCirc c = circs[ci];
         const u32 lq = c.lq, L = 1u << lq, tl = c.n - lq;
-        double2* gs = a.scratch + c.state_off;
+        double2* gs = scratch + c.state_off;
 
         for (u32 s = 0; s < c.nstage; ++s) {
-            const DStage sg = a.stages[c.stage0 + s];
+            const DStage sg = stages[c.stage0 + s];
             if (tl > 0) {
                 build_hi_tab(hi_tab, lq, sg);
                 __syncthreads();
@@ -344,10 +337,10 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(BatchArgs a) {
                     for (u32 i = tid; i < L; i += bd) st[slot_of(i)] = gs[tile_index(tb, hi_tab, i)];
                 }
                 __syncthreads();
-                run_passes<K>(st, lq, a.passes, sg.pass0, sg.npass, a.subs, a.pool);
+                run_passes<K>(st, lq, passes, sg.pass0, sg.npass, subs, pool);
                 if (tl > 0) {
                     for (u32 i = tid; i < L; i += bd) gs[tile_index(tb, hi_tab, i)] = st[slot_of(i)];
-                } else if (a.keep) {
+                } else if (keep) {
                     for (u32 i = tid; i < L; i += bd) gs[i] = st[slot_of(i)];
                 }
                 __syncthreads();
@@ -355,9 +348,9 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(BatchArgs a) {
         }
 
         // sum_k c_k Re <psi| P_k |psi>
-        const double acc = tl ? energy_sum<false>(gs, st, c, a.terms) : energy_sum<true>(gs, st, c, a.terms);
+        const double acc = tl ? energy_sum<false>(gs, st, c, terms) : energy_sum<true>(gs, st, c, terms);
         const double total = block_sum(acc, red);
-        if (tid == 0) a.energies[ci] = total;
+        if (tid == 0) energies[ci] = total;
         __syncthreads();
     }
 }
@@ -446,8 +439,20 @@ struct qm_batch {
     double* d_energies = nullptr;
     int* d_counter = nullptr;
     size_t cap[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // bytes held by each device buffer above
+    int grid = 0;                       // work-groups of the launch (computed once per compile)
+    unsigned lq_cap = 13;               // qubits held in LDS per tile (qm_batch_set_local_qubits)
+    hipGraph_t graph = nullptr;         // captured launch (qm_batch_capture)
+    hipGraphExec_t graph_exec = nullptr;
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+    }
 
     void release() {
+        drop_graph();
+        grid = 0;
         if (device >= 0) {
             (void)hipSetDevice(device);
             void* ptrs[] = {d_circs, d_stages, d_passes, d_subs, d_pool, d_terms, d_order, d_scratch, d_energies, d_counter};
@@ -752,6 +757,24 @@ bool convert_gates(u32 n, const qm_batch_gate* gates, UINT gate_count, HCirc& c,
     return true;
 }
 
+// Sets the kernel's LDS limit and sizes the grid (resident work-groups, at most
+// one per circuit).  Kept out of the launch path so a graph capture only sees
+// stream operations.
+bool batch_prepare_launch(qm_batch* b, unsigned dev, const char* what) {
+    const size_t lds = sizeof(double2) << b->lq_max;
+    const void* fn = b->K == 2 ? (const void*)k_batch<2> : b->K == 3 ? (const void*)k_batch<3> : (const void*)k_batch<4>;
+    if (!hip_ok(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), what)) return false;
+    int per_cu = 0, cus = 0;
+    if (!hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBatchThreads, lds), what)) return false;
+    if (!hip_ok(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, (int)dev), what)) return false;
+    if (per_cu < 1) {
+        set_error(QM_ERR_HIP, "%s: the kernel does not fit a compute unit with %zu bytes of LDS", what, lds);
+        return false;
+    }
+    b->grid = (int)std::min<u64>(b->h_circs.size(), (u64)per_cu * cus);
+    return true;
+}
+
 bool batch_ok(qm_batch* b, const char* what) {
     if (!b) {
         set_error(QM_ERR_INVALID_ARGUMENT, "%s: null batch", what);
@@ -821,6 +844,8 @@ int qm_batch_compile(qm_batch* b, UINT block_qubits, int keep_states, unsigned d
                 return QM_ERR_INVALID_ARGUMENT;
             }
     if (b->device >= 0 && b->device != (int)dev) b->release();
+    b->drop_graph();  // device buffers and the queue length may change
+    b->grid = 0;
     b->h_circs.clear();
     if (!use_device(dev)) return QM_ERR_HIP;
     Flat f;    std::vector<DTerm> terms;
@@ -830,14 +855,14 @@ int qm_batch_compile(qm_batch* b, UINT block_qubits, int keep_states, unsigned d
     for (const HCirc& c : b->circs) {
         DCirc d{};
         d.n = c.ne;
-        d.lq = std::min(c.ne, kMaxLocal);
+        d.lq = std::min(c.ne, (u32)b->lq_cap);
         d.stage0 = (u32)f.stages.size();
-        d.nstage = schedule_circuit(f, c, K);
+        d.nstage = schedule_circuit(f, c, K, b->lq_cap);
         d.term0 = (u32)terms.size();
         d.nterm = (u32)c.terms.size();
         terms.insert(terms.end(), c.terms.begin(), c.terms.end());
         d.state_off = state_off;
-        if (c.ne > kMaxLocal || keep_states) state_off += 1ULL << c.ne;
+        if (c.ne > b->lq_cap || keep_states) state_off += 1ULL << c.ne;
         lq_max = std::max(lq_max, d.lq);
         dc.push_back(d);
     }
@@ -894,24 +919,18 @@ const double* qm_batch_run_async(qm_batch* b, void* stream, unsigned dev) {
     const u32 ncirc = (u32)b->h_circs.size();
     if (ncirc == 0) return b->d_energies;
     const size_t lds = sizeof(double2) << b->lq_max;
-    const void* fn = b->K == 2 ? (const void*)k_batch<2> : b->K == 3 ? (const void*)k_batch<3> : (const void*)k_batch<4>;
-    if (!hip_ok(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), what)) return nullptr;
-    int per_cu = 0, cus = 0;
-    if (!hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBatchThreads, lds), what)) return nullptr;
-    if (!hip_ok(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, (int)dev), what)) return nullptr;
-    if (per_cu < 1) {
-        set_error(QM_ERR_HIP, "%s: the kernel does not fit a compute unit with %zu bytes of LDS", what, lds);
-        return nullptr;
-    }
-    const u32 grid = std::min<u32>(ncirc, (u32)(per_cu * cus));
-    BatchArgs a{b->d_circs, b->d_stages, b->d_passes, b->d_subs, b->d_pool, b->d_terms, b->d_order,
-                b->d_scratch, b->d_energies, b->d_counter, ncirc, b->keep};
+    if (b->grid == 0 && !batch_prepare_launch(b, dev, what)) return nullptr;
+    const u32 grid = (u32)b->grid;
+#define QM_BATCH_ARGS                                                                                       \
+    b->d_circs, b->d_stages, b->d_passes, b->d_subs, b->d_pool, b->d_terms, b->d_order, b->d_scratch, b->d_energies, \
+        b->d_counter, ncirc, b->keep
     k_batch_reset<<<1, 64, 0, st>>>(b->d_counter);
     switch (b->K) {
-        case 2: k_batch<2><<<grid, kBatchThreads, lds, st>>>(a); break;
-        case 3: k_batch<3><<<grid, kBatchThreads, lds, st>>>(a); break;
-        default: k_batch<4><<<grid, kBatchThreads, lds, st>>>(a); break;
+        case 2: k_batch<2><<<grid, kBatchThreads, lds, st>>>(QM_BATCH_ARGS); break;
+        case 3: k_batch<3><<<grid, kBatchThreads, lds, st>>>(QM_BATCH_ARGS); break;
+        default: k_batch<4><<<grid, kBatchThreads, lds, st>>>(QM_BATCH_ARGS); break;
     }
+#undef QM_BATCH_ARGS
     if (!hip_ok(hipGetLastError(), what)) return nullptr;
     return b->d_energies;
 }
@@ -943,6 +962,61 @@ int qm_batch_get_state(qm_batch* b, UINT index, qm_complex* out, void* stream, u
     if (!hip_ok(hipMemcpyAsync(out, b->d_scratch + c.state_off, sizeof(double2) << n, hipMemcpyDeviceToHost, st), what))
         return QM_ERR_HIP;
     if (!hip_ok(hipStreamSynchronize(st), what)) return QM_ERR_HIP;
+    return QM_OK;
+}
+
+int qm_batch_capture(qm_batch* b, void* stream, unsigned dev) {
+    const char* what = "qm_batch_capture";
+    if (!batch_ok(b, what)) return QM_ERR_INVALID_ARGUMENT;
+    if (!stream || b->device != (int)dev || b->h_circs.empty()) {
+        set_error(QM_ERR_INVALID_ARGUMENT, "%s: needs a compiled, non-empty batch and a created stream", what);
+        return QM_ERR_INVALID_ARGUMENT;
+    }
+    if (!use_device(dev)) return QM_ERR_HIP;
+    if (b->grid == 0 && !batch_prepare_launch(b, dev, what)) return QM_ERR_HIP;
+    b->drop_graph();
+    hipStream_t st = stream_of(stream);
+    if (!hip_ok(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), what)) return QM_ERR_HIP;
+    const double* d = qm_batch_run_async(b, stream, dev);
+    hipGraph_t g = nullptr;
+    const bool ended = hip_ok(hipStreamEndCapture(st, &g), what);
+    if (!d || !ended || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        return QM_ERR_HIP;
+    }
+    b->graph = g;
+    if (!hip_ok(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0), what)) {
+        b->drop_graph();
+        return QM_ERR_HIP;
+    }
+    return QM_OK;
+}
+
+int qm_batch_run_graph(qm_batch* b, double* energies, void* stream, unsigned dev) {
+    const char* what = "qm_batch_run_graph";
+    if (!batch_ok(b, what)) return QM_ERR_INVALID_ARGUMENT;
+    if (!b->graph_exec || b->device != (int)dev) {
+        set_error(QM_ERR_INVALID_ARGUMENT, "%s: capture the batch first (qm_batch_capture)", what);
+        return QM_ERR_INVALID_ARGUMENT;
+    }
+    if (!use_device(dev)) return QM_ERR_HIP;
+    hipStream_t st = stream_of(stream);
+    if (!hip_ok(hipGraphLaunch(b->graph_exec, st), what)) return QM_ERR_HIP;
+    const size_t n = b->h_circs.size();
+    if (energies && n &&
+        !hip_ok(hipMemcpyAsync(energies, b->d_energies, n * sizeof(double), hipMemcpyDeviceToHost, st), what))
+        return QM_ERR_HIP;
+    if (!hip_ok(hipStreamSynchronize(st), what)) return QM_ERR_HIP;
+    return QM_OK;
+}
+
+int qm_batch_set_local_qubits(qm_batch* b, UINT local_qubits) {
+    if (!batch_ok(b, "qm_batch_set_local_qubits")) return QM_ERR_INVALID_ARGUMENT;
+    if (local_qubits < 8 || local_qubits > kMaxLocal) {
+        set_error(QM_ERR_INVALID_ARGUMENT, "qm_batch_set_local_qubits: %u outside 8..%u", local_qubits, kMaxLocal);
+        return QM_ERR_INVALID_ARGUMENT;
+    }
+    b->lq_cap = local_qubits;
     return QM_OK;
 }
 

@@ -81,3 +81,15 @@ def test_adjoint_circuit_validation_is_host_only():
     c.close()
     with pytest.raises(QulacsMI355Error):
         AdjointCircuit(3, [("rot", [0], [4], 0, 1.0)], 1)  # Pauli id 4
+
+
+def test_batch_window_setter_validation():
+    from qulacs_amd import _lib
+
+    b = Batch()
+    with pytest.raises(QulacsMI355Error):
+        _lib.call("qm_batch_set_local_qubits", b.h, 7)
+    with pytest.raises(QulacsMI355Error):
+        _lib.call("qm_batch_set_local_qubits", b.h, 14)
+    _lib.call("qm_batch_set_local_qubits", b.h, 11)
+    b.close()

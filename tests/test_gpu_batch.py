@@ -27,9 +27,9 @@ def _check(oracle, b, circuits, states=True):
     return e
 
 
-@pytest.mark.parametrize("block", [4, 3, 2])
-def test_random_circuits_all_sizes(oracle, block):
-    rng = np.random.default_rng(100 + block)
+@pytest.mark.parametrize("block,local", [(4, 13), (3, 13), (2, 13), (4, 12), (3, 10)])
+def test_random_circuits_all_sizes(oracle, block, local):
+    rng = np.random.default_rng(100 + block + local)
     b = Batch()
     circuits = []
     for n in [1, 2, 3, 4, 5, 7, 9, 12, 13, 14, 15, 16, 17]:
@@ -37,8 +37,8 @@ def test_random_circuits_all_sizes(oracle, block):
         terms = random_terms(rng, n, 6)
         circuits.append((n, gates, terms))
         b.add(n, gates, terms)
-    b.compile(block_qubits=block, keep_states=True)
-    print("block", block, b.stats(), flush=True)
+    b.compile(block_qubits=block, keep_states=True, local_qubits=local)
+    print("block", block, "window", local, b.stats(), flush=True)
     e1 = _check(oracle, b, circuits)
     e2 = b.run()
     assert np.array_equal(e1, e2)
@@ -107,3 +107,23 @@ def test_fused_circuit_matches_oracle(oracle, n, block, local):
         got = s.download()
     assert rel(got, want) < 1e-10
     fc.close()
+
+
+def test_graph_replays_match_eager_runs():
+    from qulacs_amd import _lib
+
+    rng = np.random.default_rng(55)
+    b = Batch()
+    for n in [4, 9, 13, 15]:
+        b.add(n, random_batch_circuit(rng, n, 30), random_terms(rng, n, 5))
+    b.compile(block_qubits=4)
+    eager = b.run()
+    stream = _lib.call("allocate_cuda_stream_host", 1, 0)
+    try:
+        b.capture(stream)
+        for _ in range(4):
+            assert np.array_equal(b.run_graph(stream), eager)
+            assert np.array_equal(b.run(), eager)  # eager work between replays
+    finally:
+        _lib.call("release_cuda_stream_host", stream, 1, 0)
+    b.close()
