@@ -350,8 +350,8 @@ def many_small(args, lib_mod, hip):
             t0 = time.perf_counter()
             psi = np.zeros(1 << c.n, dtype=np.complex128)
             psi[0] = 1.0
-            for kind, q, m in c.gates:
-                if kind == "CZ":
+            for kind_g, q, m in c.gates:
+                if kind_g == "CZ":
                     k.two("CZ", q[0], q[1], psi)
                 else:
                     k.dense1(q[0], m, psi)
@@ -441,6 +441,22 @@ def brickwork(args, lib_mod, hip):
     gpu_ms = (t1 - t0) * 1e3 / reps
     fused_stats = fused.stats()
     fused.close()
+    # the fused executor's register-block width and LDS window
+    variants = {}
+    lib_mod.call("qm_set_synchronous", 0)
+    for kq, lq in ((3, 12), (4, 12), (3, 13), (4, 13)):
+        fv = FusedCircuit(n, gates, block_qubits=kq, local_qubits=lq)
+        L.initialize_quantum_state_host(st, dim, stream, 0)
+        L.qm_circuit_apply(fv.h, st, dim, stream, 0)
+        lib_mod.call("qm_synchronize", stream, 0)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            L.qm_circuit_apply(fv.h, st, dim, stream, 0)
+        lib_mod.call("qm_synchronize", stream, 0)
+        variants[f"K{kq}/window{lq}"] = round((time.perf_counter() - t0) * 1e3 / reps, 3)
+        fv.close()
+    lib_mod.call("qm_set_synchronous", 1)
+    lib_mod.check()
     psi_gpu = np.zeros(dim, dtype=np.complex128)
     lib_mod.call("copy_quantum_state_from_device_to_host", psi_gpu.ctypes.data, st, dim, stream, 0)
     lib_mod.call("release_quantum_state_host", st, 0)
@@ -459,6 +475,7 @@ def brickwork(args, lib_mod, hip):
     scale = sum(abs(x[3]) for x in obs)
     out = {"workload": "20-qubit brickwork, 10 layers (200 Haar 1q + 95 CNOT), 16-term observable, 1024 samples",
            "gpu_ms": round(gpu_ms, 3), "fused_ms": round(fused_ms, 3), "fused_schedule": fused_stats,
+           "fused_circuit_only_ms": variants,
            "batched_executor_ms": round(batch_ms, 3), "energy": e_gpu,
            "fused_matches": bool(abs(e_fused - e_gpu) <= 1e-10 * scale and np.array_equal(fused_samples, samples))}
     out["batched_energy_matches"] = bool(abs(e_batch - e_gpu) <= 1e-10 * scale)
