@@ -796,7 +796,7 @@ def main() -> None:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the sweep is a single 30-qubit "
-              "state on one device; see the sharding notes in DESIGN.md)", file=sys.stderr)
+              "state on one device)", file=sys.stderr)
         sys.exit(2)
 
     from qulacs_amd import _lib

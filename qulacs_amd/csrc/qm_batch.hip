@@ -49,10 +49,23 @@ using cd = std::complex<double>;
 
 constexpr u32 kMaxLocal = 13;   // qubits resident in LDS
 constexpr u32 kLowFixed = 4;    // qubits 0..3 are local in every stage
-constexpr u32 kBatchThreads = 512;
+// Work-group size per register-block width: 8 waves at K = 4 (the register
+// block limits a SIMD to 2 waves), 16 waves below, so a CU holding one
+// 128 KiB tile still has 4 waves per SIMD to hide LDS latency.
+template <int K>
+constexpr u32 block_threads() { return K >= 4 ? 512u : 1024u; }
+constexpr u32 kMaxBlockThreads = 1024;
 constexpr u32 kMinQubits = 4;   // smaller circuits run on 4 qubits (extra qubits stay |0>)
 
-enum : u32 { SUB_D1 = 0, SUB_D2 = 1, SUB_DK = 2, SUB_DIAG = 3, SUB_CZ = 4 };
+// Sub-operation kinds (low byte of DSub::type).
+// LAYER: up to kCzSetMax CZ gates, then one 2x2 matrix on each block bit of a
+//   mask.  type bits 8..11: the block-bit mask; bits 12..14: the CZ count; CZ c
+//   acts on the local positions in bytes 2 (c & 1), 2 (c & 1) + 1 of (c < 2 ? a : m);
+//   data: the matrices (4 complex each) of the mask's bits in ascending order.
+// D2: 4x4 on block bits byte 0 / byte 1 of a; DK: 2^K x 2^K on the block;
+// DIAG: diagonal on the m local positions in the bytes of a.
+enum : u32 { SUB_LAYER = 0, SUB_D2 = 1, SUB_DK = 2, SUB_DIAG = 3 };
+constexpr u32 kCzSetMax = 4;
 
 struct DCirc {
     u32 n, lq, nstage, stage0, term0, nterm;
@@ -105,15 +118,62 @@ struct HCirc {
 __host__ __device__ __forceinline__ u32 fold3(u32 x) { return ((x >> 3) ^ (x >> 6) ^ (x >> 9) ^ (x >> 12)) & 7u; }
 __host__ __device__ __forceinline__ u32 slot_of(u32 x) { return x ^ fold3(x); }
 
+// (x, y) <- (m0 x + m1 y, m2 x + m3 y), complex, updated in place; the matrix
+// entries are wave-uniform (scalar registers).  Written as one asm block so the
+// four results land in the registers of the four inputs: the sub-operation loop
+// of run_passes then keeps the register block at fixed registers across all
+// sub-operation kinds instead of copying it at every merge point.  The last
+// reader of each input writes a result into it (x.re holds the x.re chain from
+// its 4th op on), so three temporaries suffice.
+__device__ __forceinline__ void pair_update(double2& x, double2& y, const double2 m0, const double2 m1,
+                                            const double2 m2, const double2 m3) {
+    double tb, tc, td;
+    asm("v_mul_f64 %[tb], %[m0i], %[xr]\n\t"          // B = x'.im
+        "v_mul_f64 %[tc], %[m2r], %[xr]\n\t"          // C = y'.re
+        "v_mul_f64 %[td], %[m2i], %[xr]\n\t"          // D = y'.im
+        "v_mul_f64 %[xr], %[m0r], %[xr]\n\t"          // A = x'.re
+        "v_fma_f64 %[xr], -%[m0i], %[xi], %[xr]\n\t"
+        "v_fma_f64 %[tb], %[m0r], %[xi], %[tb]\n\t"
+        "v_fma_f64 %[tc], -%[m2i], %[xi], %[tc]\n\t"
+        "v_fma_f64 %[td], %[m2r], %[xi], %[td]\n\t"
+        "v_fma_f64 %[xr], %[m1r], %[yr], %[xr]\n\t"
+        "v_fma_f64 %[tb], %[m1i], %[yr], %[tb]\n\t"
+        "v_fma_f64 %[tc], %[m3r], %[yr], %[tc]\n\t"
+        "v_fma_f64 %[td], %[m3i], %[yr], %[td]\n\t"
+        "v_fma_f64 %[xi], %[m1r], %[yi], %[tb]\n\t"
+        "v_fma_f64 %[yr], -%[m3i], %[yi], %[tc]\n\t"
+        "v_fma_f64 %[xr], -%[m1i], %[yi], %[xr]\n\t"
+        "v_fma_f64 %[yi], %[m3r], %[yi], %[td]"
+        : [xr] "+v"(x.x), [xi] "+v"(x.y), [yr] "+v"(y.x), [yi] "+v"(y.y), [tb] "=&v"(tb), [tc] "=&v"(tc),
+          [td] "=&v"(td)
+        : [m0r] "s"(m0.x), [m0i] "s"(m0.y), [m1r] "s"(m1.x), [m1i] "s"(m1.y), [m2r] "s"(m2.x), [m2i] "s"(m2.y),
+          [m3r] "s"(m3.x), [m3i] "s"(m3.y));
+}
+
+// v <- -v when flip is 1, in place (multiplied by +-1.0, exact).
+__device__ __forceinline__ void negate_if(double2& v, u32 flip) {
+    const double f = __hiloint2double((int)(0x3ff00000u | (flip << 31)), 0);
+    asm("v_mul_f64 %0, %0, %2\n\tv_mul_f64 %1, %1, %2" : "+v"(v.x), "+v"(v.y) : "v"(f));
+}
+
+// v <- v * z, complex, in place.
+__device__ __forceinline__ void cmul_inplace(double2& v, const double2 z) {
+    double t;
+    asm("v_mul_f64 %[t], %[vr], %[zi]\n\t"
+        "v_mul_f64 %[vr], %[vr], %[zr]\n\t"
+        "v_fma_f64 %[vr], -%[vi], %[zi], %[vr]\n\t"
+        "v_fma_f64 %[vi], %[vi], %[zr], %[t]"
+        : [vr] "+v"(v.x), [vi] "+v"(v.y), [t] "=&v"(t)
+        : [zr] "v"(z.x), [zi] "v"(z.y));
+}
+
 template <int K, int P>
 __device__ __forceinline__ void sub_d1(double2* v, const double2* mp) {
     const double2 m0 = mp[0], m1 = mp[1], m2 = mp[2], m3 = mp[3];
 #pragma unroll
     for (int j = 0; j < (1 << K); ++j) {
         if (j & (1 << P)) continue;
-        const double2 x = v[j], y = v[j | (1 << P)];
-        v[j] = cmac(cmul(m0, x), m1, y);
-        v[j | (1 << P)] = cmac(cmul(m2, x), m3, y);
+        pair_update(v[j], v[j | (1 << P)], m0, m1, m2, m3);
     }
 }
 
@@ -146,17 +206,6 @@ __device__ __forceinline__ void sub_dk(double2* v, const double2* mp) {
     }
 #pragma unroll
     for (int r = 0; r < D; ++r) v[r] = out[r];
-}
-
-template <int K>
-__device__ __forceinline__ void apply_d1(double2* v, u32 p, const double2* mp) {
-    switch (p) {
-        case 0: sub_d1<K, 0>(v, mp); break;
-        case 1: sub_d1<K, 1>(v, mp); break;
-        case 2: if constexpr (K > 2) sub_d1<K, 2>(v, mp); break;
-        case 3: if constexpr (K > 3) sub_d1<K, 3>(v, mp); break;
-        default: break;
-    }
 }
 
 #define QM_D2_CASE(A, B) \
@@ -233,6 +282,58 @@ __device__ __forceinline__ u64 tile_index(u64 tb, const u32* hi_tab, u32 i) {
     return tb + hi_tab[i >> kLowFixed] + (i & ((1u << kLowFixed) - 1));
 }
 
+// Bit j set where bit i of register-block index j is 1, for the block position
+// p == pos[i]; zero for a position outside the block.
+template <int K>
+__device__ __forceinline__ u32 block_pattern(const DPass& ps, u32 p) {
+    constexpr u32 kPat[4] = {0xAAAAu, 0xCCCCu, 0xF0F0u, 0xFF00u};
+    u32 r = 0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) r |= (ps.pos[i] == p) ? kPat[i] : 0u;
+    return __builtin_amdgcn_readfirstlane(r);  // wave-uniform
+}
+
+// One LAYER sub-operation on the register block of the thread whose block
+// base (local index with zeros at the block positions) is base.
+template <int K>
+__device__ __forceinline__ void apply_layer(double2* v, const DSub& sub, const DPass& ps, u32 base,
+                                            const double2* __restrict__ pool) {
+    constexpr int D = 1 << K;
+    const u32 ncz = (sub.type >> 12) & 7u, d1 = (sub.type >> 8) & 15u;
+    if (ncz) {
+        // bit j of sgn: amplitude j changes sign.  A qubit's value over the
+        // register block is the block pattern of its bit (a block position) or
+        // all ones / all zeros (any other position; base is zero at block
+        // positions).
+        u32 sgn = 0;
+        for (u32 c = 0; c < ncz; ++c) {
+            const u32 w = (c < 2 ? sub.a : sub.m) >> (16 * (c & 1));
+            const u32 p0 = w & 0xffu, p1 = (w >> 8) & 0xffu;
+            const u32 a0 = block_pattern<K>(ps, p0) | (0u - ((base >> p0) & 1u));
+            const u32 a1 = block_pattern<K>(ps, p1) | (0u - ((base >> p1) & 1u));
+            sgn ^= a0 & a1;
+        }
+#pragma unroll
+        for (int j = 0; j < D; ++j) negate_if(v[j], (sgn >> j) & 1u);
+    }
+    const double2* mp = pool + sub.data;
+    if (d1 & 1u) {
+        sub_d1<K, 0>(v, mp);
+        mp += 4;
+    }
+    if (d1 & 2u) {
+        sub_d1<K, 1>(v, mp);
+        mp += 4;
+    }
+    if constexpr (K > 2)
+        if (d1 & 4u) {
+            sub_d1<K, 2>(v, mp);
+            mp += 4;
+        }
+    if constexpr (K > 3)
+        if (d1 & 8u) sub_d1<K, 3>(v, mp);
+}
+
 // Applies passes [pass0, pass0 + npass) to the 2^lq amplitudes in LDS; ends
 // with a barrier after every pass.
 template <int K>
@@ -262,32 +363,34 @@ __device__ __forceinline__ void run_passes(double2* st, u32 lq, const DPass* __r
             double2 v[D];
 #pragma unroll
             for (int j = 0; j < D; ++j) v[j] = st[sb ^ slot_of(off[j])];
-            for (u32 k = 0; k < ps.nsub; ++k) {
-                const DSub sub = subs[ps.sub0 + k];
-                const double2* mp = pool + sub.data;
-                switch (sub.type) {
-                    case SUB_D1: apply_d1<K>(v, byte_at(sub.a, 0), mp); break;
-                    case SUB_D2: apply_d2<K>(v, byte_at(sub.a, 0), byte_at(sub.a, 1), mp); break;
-                    case SUB_DK: sub_dk<K>(v, mp); break;
-                    case SUB_DIAG:
-#pragma unroll
-                        for (int j = 0; j < D; ++j) {
-                            const u32 li = base | off[j];
-                            u32 idx = 0;
-                            for (u32 i = 0; i < sub.m; ++i) idx |= ((li >> byte_at(sub.a, i)) & 1u) << i;
-                            v[j] = cmul(v[j], mp[idx]);
-                        }
-                        break;
-                    case SUB_CZ:
-#pragma unroll
-                        for (int j = 0; j < D; ++j) {
-                            const u32 li = base | off[j];
-                            if ((li >> byte_at(sub.a, 0)) & (li >> byte_at(sub.a, 1)) & 1u)
-                                v[j] = make_double2(-v[j].x, -v[j].y);
-                        }
-                        break;
-                    default: break;
+            // LAYER sub-operations run in an inner loop of their own and update the
+            // register block in place, so the block stays at fixed registers there;
+            // the other kinds are handled between runs of LAYERs.
+            u32 k = 0;
+            while (k < ps.nsub) {
+                DSub sub = subs[ps.sub0 + k];
+                while ((sub.type & 0xffu) == SUB_LAYER) {
+                    apply_layer<K>(v, sub, ps, base, pool);
+                    if (++k == ps.nsub) break;
+                    sub = subs[ps.sub0 + k];
                 }
+                if (k == ps.nsub) break;
+                const u32 kind = sub.type & 0xffu;
+                const double2* mp = pool + sub.data;
+                if (kind == SUB_D2) {
+                    apply_d2<K>(v, byte_at(sub.a, 0), byte_at(sub.a, 1), mp);
+                } else if (kind == SUB_DK) {
+                    sub_dk<K>(v, mp);
+                } else if (kind == SUB_DIAG) {
+#pragma unroll
+                    for (int j = 0; j < D; ++j) {
+                        const u32 li = base | off[j];
+                        u32 idx = 0;
+                        for (u32 i = 0; i < sub.m; ++i) idx |= ((li >> byte_at(sub.a, i)) & 1u) << i;
+                        cmul_inplace(v[j], mp[idx]);
+                    }
+                }
+                ++k;
             }
 #pragma unroll
             for (int j = 0; j < D; ++j) st[sb ^ slot_of(off[j])] = v[j];
@@ -300,7 +403,7 @@ __device__ __forceinline__ void run_passes(double2* st, u32 lq, const DPass* __r
 // compiler reads them through the scalar cache (inside a by-value struct they
 // became per-lane vector loads).
 template <int K>
-__global__ __launch_bounds__(kBatchThreads) void k_batch(
+__global__ __launch_bounds__(block_threads<K>()) void k_batch(
     const DCirc* __restrict__ circs, const DStage* __restrict__ stages, const DPass* __restrict__ passes,
     const DSub* __restrict__ subs, const double2* __restrict__ pool, const DTerm* __restrict__ terms,
     const u32* __restrict__ order, double2* __restrict__ scratch, double* __restrict__ energies, int* counter,
@@ -360,7 +463,7 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(
 // it back in place.  Tiles of one stage are disjoint, so work-groups never
 // touch the same amplitudes.
 template <int K>
-__global__ __launch_bounds__(kBatchThreads) void k_stage(double2* __restrict__ state, u32 lq, u32 tl, DStage sg,
+__global__ __launch_bounds__(block_threads<K>()) void k_stage(double2* __restrict__ state, u32 lq, u32 tl, DStage sg,
                                                          const DPass* __restrict__ passes,
                                                          const DSub* __restrict__ subs,
                                                          const double2* __restrict__ pool) {
@@ -552,20 +655,46 @@ void schedule_passes(Flat& f, std::vector<HOp> ops, u32 lq, u32 K, DStage& sg) {
                 if (bpos[i] == p) return i;
             return 0;
         };
+        // Consecutive CZ gates, then consecutive single-qubit matrices on distinct
+        // block bits (these commute with each other), become one LAYER.
+        std::vector<const std::vector<cd>*> pend_d1(K, nullptr);
+        u32 d1mask = 0;
+        std::vector<u32> pend_cz;  // packed local positions (p0 | p1 << 8)
+        auto flush = [&]() {
+            if (d1mask == 0 && pend_cz.empty()) return;
+            DSub sb{};
+            sb.type = SUB_LAYER | (d1mask << 8) | ((u32)pend_cz.size() << 12);
+            for (u32 c = 0; c < pend_cz.size(); ++c) (c < 2 ? sb.a : sb.m) |= pend_cz[c] << (16 * (c & 1));
+            sb.data = (u32)f.pool.size();
+            for (u32 b = 0; b < K; ++b)
+                if (pend_d1[b]) pool_put(f, *pend_d1[b]);
+            f.subs.push_back(sb);
+            std::fill(pend_d1.begin(), pend_d1.end(), nullptr);
+            d1mask = 0;
+            pend_cz.clear();
+        };
         for (size_t i : sel) {
             const HOp& o = ops[i];
-            DSub sb{};
+            done[i] = true;
+            --left;
             if (o.type == 2) {
-                sb.type = SUB_CZ;
-                sb.a = o.q[0] | (o.q[1] << 8);
-            } else if (o.type == 1) {
+                if (d1mask || pend_cz.size() == kCzSetMax) flush();
+                pend_cz.push_back(o.q[0] | (o.q[1] << 8));
+                continue;
+            }
+            if (o.type == 0 && o.q.size() == 1) {
+                const u32 b = bit_of(o.q[0]);
+                if ((d1mask >> b) & 1) flush();
+                pend_d1[b] = &o.m;
+                d1mask |= 1u << b;
+                continue;
+            }
+            flush();
+            DSub sb{};
+            if (o.type == 1) {
                 sb.type = SUB_DIAG;
                 sb.m = (u32)o.q.size();
                 for (size_t k = 0; k < o.q.size(); ++k) sb.a |= o.q[k] << (8 * k);
-                sb.data = pool_put(f, o.m);
-            } else if (o.q.size() == 1) {
-                sb.type = SUB_D1;
-                sb.a = bit_of(o.q[0]);
                 sb.data = pool_put(f, o.m);
             } else if (o.q.size() == 2) {
                 sb.type = SUB_D2;
@@ -594,9 +723,8 @@ void schedule_passes(Flat& f, std::vector<HOp> ops, u32 lq, u32 K, DStage& sg) {
                 sb.data = pool_put(f, M);
             }
             f.subs.push_back(sb);
-            done[i] = true;
-            --left;
         }
+        flush();
         ps.nsub = (u32)f.subs.size() - ps.sub0;
         f.passes.push_back(ps);
         ++sg.npass;
@@ -686,6 +814,33 @@ u32 schedule_circuit(Flat& f, const HCirc& c, u32 K, u32 lq_cap = kMaxLocal) {
 
 // Gates in the C ABI form -> internal ops, with consecutive single-qubit dense /
 // diagonal gates on the same qubit multiplied together.
+// Appends o; a single-qubit dense / diagonal gate is fused into the previous
+// op on its qubit when that is one too (last[q]: index of the last op on q).
+void push_op(HCirc& c, std::vector<long>& last, HOp o) {
+    if (o.q.size() == 1 && o.type != 2) {
+        const long p = last[o.q[0]];
+        if (p >= 0 && c.ops[p].q.size() == 1 && c.ops[p].type != 2) {
+            HOp& prev = c.ops[p];
+            auto full = [](const HOp& h) {
+                if (h.type == 0) return h.m;
+                return std::vector<cd>{h.m[0], 0.0, 0.0, h.m[1]};
+            };
+            if (prev.type == 1 && o.type == 1) {
+                prev.m[0] *= o.m[0];
+                prev.m[1] *= o.m[1];
+            } else {
+                std::vector<cd> prod;
+                mat_mul(full(o), full(prev), prod, 2);  // new after old
+                prev.m = prod;
+                prev.type = 0;
+            }
+            return;
+        }
+    }
+    c.ops.push_back(std::move(o));
+    for (u32 q : c.ops.back().q) last[q] = (long)c.ops.size() - 1;
+}
+
 bool convert_gates(u32 n, const qm_batch_gate* gates, UINT gate_count, HCirc& c, const char* what) {
     c.n = n;
     c.ne = std::max<u32>(n, kMinQubits);
@@ -710,19 +865,27 @@ bool convert_gates(u32 n, const qm_batch_gate* gates, UINT gate_count, HCirc& c,
             set_error(QM_ERR_INVALID_ARGUMENT, "%s: gate %u: null matrix", what, gi);
             return false;
         }
+        if (g.kind == QM_BATCH_CNOT) {
+            // CNOT(c, t) = H_t CZ(c, t) H_t; the Hadamards fuse into the
+            // single-qubit gates next to them on t
+            const double h = 0.70710678118654752440;
+            HOp hd;
+            hd.type = 0;
+            hd.q = {g.qubits[1]};
+            hd.m = {h, h, h, -h};
+            HOp cz;
+            cz.type = 2;
+            cz.q = {g.qubits[0], g.qubits[1]};
+            push_op(c, last, hd);
+            push_op(c, last, std::move(cz));
+            push_op(c, last, std::move(hd));
+            continue;
+        }
         HOp o;
         o.q.assign(g.qubits, g.qubits + k);
         const u32 d = 1u << k;
         if (g.kind == QM_BATCH_CZ) {
             o.type = 2;
-        } else if (g.kind == QM_BATCH_CNOT) {
-            // bit 0 <-> control, bit 1 <-> target: |c t> -> |c, t xor c>
-            o.type = 0;
-            o.m.assign(16, 0.0);
-            for (u32 col = 0; col < 4; ++col) {
-                const u32 cbit = col & 1, tbit = (col >> 1) & 1;
-                o.m[(cbit | ((tbit ^ cbit) << 1)) * 4 + col] = 1.0;
-            }
         } else if (g.kind == QM_BATCH_DIAG) {
             o.type = 1;
             for (u32 i = 0; i < d; ++i) o.m.push_back(cd(g.matrix[i].re, g.matrix[i].im));
@@ -730,29 +893,7 @@ bool convert_gates(u32 n, const qm_batch_gate* gates, UINT gate_count, HCirc& c,
             o.type = 0;
             for (u32 i = 0; i < d * d; ++i) o.m.push_back(cd(g.matrix[i].re, g.matrix[i].im));
         }
-        // fuse consecutive single-qubit dense / diagonal gates on the same qubit
-        if (k == 1 && o.type != 2) {
-            const long p = last[o.q[0]];
-            if (p >= 0 && c.ops[p].q.size() == 1 && c.ops[p].type != 2) {
-                HOp& prev = c.ops[p];
-                auto full = [](const HOp& h) {
-                    if (h.type == 0) return h.m;
-                    return std::vector<cd>{h.m[0], 0.0, 0.0, h.m[1]};
-                };
-                if (prev.type == 1 && o.type == 1) {
-                    prev.m[0] *= o.m[0];
-                    prev.m[1] *= o.m[1];
-                } else {
-                    std::vector<cd> prod;
-                    mat_mul(full(o), full(prev), prod, 2);  // new after old
-                    prev.m = prod;
-                    prev.type = 0;
-                }
-                continue;
-            }
-        }
-        c.ops.push_back(std::move(o));
-        for (u32 q : c.ops.back().q) last[q] = (long)c.ops.size() - 1;
+        push_op(c, last, std::move(o));
     }
     return true;
 }
@@ -765,7 +906,8 @@ bool batch_prepare_launch(qm_batch* b, unsigned dev, const char* what) {
     const void* fn = b->K == 2 ? (const void*)k_batch<2> : b->K == 3 ? (const void*)k_batch<3> : (const void*)k_batch<4>;
     if (!hip_ok(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), what)) return false;
     int per_cu = 0, cus = 0;
-    if (!hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBatchThreads, lds), what)) return false;
+    const int threads = (int)(b->K >= 4 ? block_threads<4>() : block_threads<3>());
+    if (!hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, lds), what)) return false;
     if (!hip_ok(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, (int)dev), what)) return false;
     if (per_cu < 1) {
         set_error(QM_ERR_HIP, "%s: the kernel does not fit a compute unit with %zu bytes of LDS", what, lds);
@@ -926,9 +1068,9 @@ const double* qm_batch_run_async(qm_batch* b, void* stream, unsigned dev) {
         b->d_counter, ncirc, b->keep
     k_batch_reset<<<1, 64, 0, st>>>(b->d_counter);
     switch (b->K) {
-        case 2: k_batch<2><<<grid, kBatchThreads, lds, st>>>(QM_BATCH_ARGS); break;
-        case 3: k_batch<3><<<grid, kBatchThreads, lds, st>>>(QM_BATCH_ARGS); break;
-        default: k_batch<4><<<grid, kBatchThreads, lds, st>>>(QM_BATCH_ARGS); break;
+        case 2: k_batch<2><<<grid, block_threads<2>(), lds, st>>>(QM_BATCH_ARGS); break;
+        case 3: k_batch<3><<<grid, block_threads<3>(), lds, st>>>(QM_BATCH_ARGS); break;
+        default: k_batch<4><<<grid, block_threads<4>(), lds, st>>>(QM_BATCH_ARGS); break;
     }
 #undef QM_BATCH_ARGS
     if (!hip_ok(hipGetLastError(), what)) return nullptr;
@@ -1117,7 +1259,8 @@ bool circuit_enqueue(qm_circuit* c, void* state, u64 dim, hipStream_t st, unsign
     if (c->per_cu == 0) {
         int per_cu = 0, cus = 0;
         if (!hip_ok(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), what) ||
-            !hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBatchThreads, lds), what) ||
+            !hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &per_cu, fn, (int)(c->K >= 4 ? block_threads<4>() : block_threads<3>()), lds), what) ||
             !hip_ok(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, (int)dev), what))
             return false;
         if (per_cu < 1) {
@@ -1133,9 +1276,9 @@ bool circuit_enqueue(qm_circuit* c, void* state, u64 dim, hipStream_t st, unsign
     for (const DStage& sg : c->stages) {
         if (sg.npass == 0) continue;
         switch (c->K) {
-            case 2: k_stage<2><<<grid, kBatchThreads, lds, st>>>((double2*)state, c->lq, tl, sg, c->d_passes, c->d_subs, c->d_pool); break;
-            case 3: k_stage<3><<<grid, kBatchThreads, lds, st>>>((double2*)state, c->lq, tl, sg, c->d_passes, c->d_subs, c->d_pool); break;
-            default: k_stage<4><<<grid, kBatchThreads, lds, st>>>((double2*)state, c->lq, tl, sg, c->d_passes, c->d_subs, c->d_pool); break;
+            case 2: k_stage<2><<<grid, block_threads<2>(), lds, st>>>((double2*)state, c->lq, tl, sg, c->d_passes, c->d_subs, c->d_pool); break;
+            case 3: k_stage<3><<<grid, block_threads<3>(), lds, st>>>((double2*)state, c->lq, tl, sg, c->d_passes, c->d_subs, c->d_pool); break;
+            default: k_stage<4><<<grid, block_threads<4>(), lds, st>>>((double2*)state, c->lq, tl, sg, c->d_passes, c->d_subs, c->d_pool); break;
         }
     }
     return hip_ok(hipGetLastError(), what);
