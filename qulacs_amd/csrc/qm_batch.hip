@@ -58,14 +58,19 @@ constexpr u32 kMaxBlockThreads = 1024;
 constexpr u32 kMinQubits = 4;   // smaller circuits run on 4 qubits (extra qubits stay |0>)
 
 // Sub-operation kinds (low byte of DSub::type).
-// LAYER: up to kCzSetMax CZ gates, then one 2x2 matrix on each block bit of a
-//   mask.  type bits 8..11: the block-bit mask; bits 12..14: the CZ count; CZ c
-//   acts on the local positions in bytes 2 (c & 1), 2 (c & 1) + 1 of (c < 2 ? a : m);
+// LAYER: CZ gates, then one 2x2 matrix on each block bit of a mask.  type bits
+//   8..11: the block-bit mask; bits 16..31: the sign pattern over the register
+//   block of the CZs between two block bits; bits 12..14: the number (<= 4) of
+//   other CZs, 16 bits each in (c < 2 ? a : m) >> 16 (c & 1): bit 15 clear: a
+//   block bit (bits 8..9) and a local position outside the block (bits 0..7);
+//   set: two local positions outside the block (bits 0..7, 8..14).
 //   data: the matrices (4 complex each) of the mask's bits in ascending order.
 // D2: 4x4 on block bits byte 0 / byte 1 of a; DK: 2^K x 2^K on the block;
 // DIAG: diagonal on the m local positions in the bytes of a.
 enum : u32 { SUB_LAYER = 0, SUB_D2 = 1, SUB_DK = 2, SUB_DIAG = 3 };
-constexpr u32 kCzSetMax = 4;
+// bits 16 i .. 16 i + 15: the register-block indices j (< 16) with bit i set
+constexpr u64 kBlockPatterns = 0xFF00F0F0CCCCAAAAull;
+constexpr u32 kCzItemsMax = 4;
 
 struct DCirc {
     u32 n, lq, nstage, stage0, term0, nterm;
@@ -150,12 +155,6 @@ __device__ __forceinline__ void pair_update(double2& x, double2& y, const double
           [m3r] "s"(m3.x), [m3i] "s"(m3.y));
 }
 
-// v <- -v when flip is 1, in place (multiplied by +-1.0, exact).
-__device__ __forceinline__ void negate_if(double2& v, u32 flip) {
-    const double f = __hiloint2double((int)(0x3ff00000u | (flip << 31)), 0);
-    asm("v_mul_f64 %0, %0, %2\n\tv_mul_f64 %1, %1, %2" : "+v"(v.x), "+v"(v.y) : "v"(f));
-}
-
 // v <- v * z, complex, in place.
 __device__ __forceinline__ void cmul_inplace(double2& v, const double2 z) {
     double t;
@@ -167,14 +166,61 @@ __device__ __forceinline__ void cmul_inplace(double2& v, const double2 z) {
         : [zr] "v"(z.x), [zi] "v"(z.y));
 }
 
+// Two independent pair updates interleaved instruction by instruction, so a
+// dependent FMA follows its producer 8 instructions later (pair_update alone:
+// 4), which keeps the FP64 pipeline busy with only two waves per SIMD.
+__device__ __forceinline__ void pair_update2(double2& x0, double2& y0, double2& x1, double2& y1, const double2 m0,
+                                             const double2 m1, const double2 m2, const double2 m3) {
+    double tb0, tc0, td0, tb1, tc1, td1;
+    asm("v_mul_f64 %[tb0], %[m0i], %[xr0]\n\t"
+        "v_mul_f64 %[tb1], %[m0i], %[xr1]\n\t"
+        "v_mul_f64 %[tc0], %[m2r], %[xr0]\n\t"
+        "v_mul_f64 %[tc1], %[m2r], %[xr1]\n\t"
+        "v_mul_f64 %[td0], %[m2i], %[xr0]\n\t"
+        "v_mul_f64 %[td1], %[m2i], %[xr1]\n\t"
+        "v_mul_f64 %[xr0], %[m0r], %[xr0]\n\t"
+        "v_mul_f64 %[xr1], %[m0r], %[xr1]\n\t"
+        "v_fma_f64 %[xr0], -%[m0i], %[xi0], %[xr0]\n\t"
+        "v_fma_f64 %[xr1], -%[m0i], %[xi1], %[xr1]\n\t"
+        "v_fma_f64 %[tb0], %[m0r], %[xi0], %[tb0]\n\t"
+        "v_fma_f64 %[tb1], %[m0r], %[xi1], %[tb1]\n\t"
+        "v_fma_f64 %[tc0], -%[m2i], %[xi0], %[tc0]\n\t"
+        "v_fma_f64 %[tc1], -%[m2i], %[xi1], %[tc1]\n\t"
+        "v_fma_f64 %[td0], %[m2r], %[xi0], %[td0]\n\t"
+        "v_fma_f64 %[td1], %[m2r], %[xi1], %[td1]\n\t"
+        "v_fma_f64 %[xr0], %[m1r], %[yr0], %[xr0]\n\t"
+        "v_fma_f64 %[xr1], %[m1r], %[yr1], %[xr1]\n\t"
+        "v_fma_f64 %[tb0], %[m1i], %[yr0], %[tb0]\n\t"
+        "v_fma_f64 %[tb1], %[m1i], %[yr1], %[tb1]\n\t"
+        "v_fma_f64 %[tc0], %[m3r], %[yr0], %[tc0]\n\t"
+        "v_fma_f64 %[tc1], %[m3r], %[yr1], %[tc1]\n\t"
+        "v_fma_f64 %[td0], %[m3i], %[yr0], %[td0]\n\t"
+        "v_fma_f64 %[td1], %[m3i], %[yr1], %[td1]\n\t"
+        "v_fma_f64 %[xi0], %[m1r], %[yi0], %[tb0]\n\t"
+        "v_fma_f64 %[xi1], %[m1r], %[yi1], %[tb1]\n\t"
+        "v_fma_f64 %[yr0], -%[m3i], %[yi0], %[tc0]\n\t"
+        "v_fma_f64 %[yr1], -%[m3i], %[yi1], %[tc1]\n\t"
+        "v_fma_f64 %[xr0], -%[m1i], %[yi0], %[xr0]\n\t"
+        "v_fma_f64 %[xr1], -%[m1i], %[yi1], %[xr1]\n\t"
+        "v_fma_f64 %[yi0], %[m3r], %[yi0], %[td0]\n\t"
+        "v_fma_f64 %[yi1], %[m3r], %[yi1], %[td1]"
+        : [xr0] "+v"(x0.x), [xi0] "+v"(x0.y), [yr0] "+v"(y0.x), [yi0] "+v"(y0.y), [xr1] "+v"(x1.x),
+          [xi1] "+v"(x1.y), [yr1] "+v"(y1.x), [yi1] "+v"(y1.y), [tb0] "=&v"(tb0), [tc0] "=&v"(tc0),
+          [td0] "=&v"(td0), [tb1] "=&v"(tb1), [tc1] "=&v"(tc1), [td1] "=&v"(td1)
+        : [m0r] "s"(m0.x), [m0i] "s"(m0.y), [m1r] "s"(m1.x), [m1i] "s"(m1.y), [m2r] "s"(m2.x), [m2i] "s"(m2.y),
+          [m3r] "s"(m3.x), [m3i] "s"(m3.y));
+}
+
 template <int K, int P>
 __device__ __forceinline__ void sub_d1(double2* v, const double2* mp) {
     const double2 m0 = mp[0], m1 = mp[1], m2 = mp[2], m3 = mp[3];
+    // the 2^(K-1) pairs (j, j | 2^P), two at a time
+    constexpr int NP = 1 << (K - 1);
+    auto pair_j = [](int q) { return ((q >> P) << (P + 1)) | (q & ((1 << P) - 1)); };
 #pragma unroll
-    for (int j = 0; j < (1 << K); ++j) {
-        if (j & (1 << P)) continue;
-        pair_update(v[j], v[j | (1 << P)], m0, m1, m2, m3);
-    }
+    for (int q = 0; q < NP; q += 2)
+        pair_update2(v[pair_j(q)], v[pair_j(q) | (1 << P)], v[pair_j(q + 1)], v[pair_j(q + 1) | (1 << P)], m0, m1, m2,
+                     m3);
 }
 
 template <int K, int P0, int P1>
@@ -282,39 +328,32 @@ __device__ __forceinline__ u64 tile_index(u64 tb, const u32* hi_tab, u32 i) {
     return tb + hi_tab[i >> kLowFixed] + (i & ((1u << kLowFixed) - 1));
 }
 
-// Bit j set where bit i of register-block index j is 1, for the block position
-// p == pos[i]; zero for a position outside the block.
-template <int K>
-__device__ __forceinline__ u32 block_pattern(const DPass& ps, u32 p) {
-    constexpr u32 kPat[4] = {0xAAAAu, 0xCCCCu, 0xF0F0u, 0xFF00u};
-    u32 r = 0;
-#pragma unroll
-    for (int i = 0; i < K; ++i) r |= (ps.pos[i] == p) ? kPat[i] : 0u;
-    return __builtin_amdgcn_readfirstlane(r);  // wave-uniform
-}
-
 // One LAYER sub-operation on the register block of the thread whose block
 // base (local index with zeros at the block positions) is base.
 template <int K>
-__device__ __forceinline__ void apply_layer(double2* v, const DSub& sub, const DPass& ps, u32 base,
-                                            const double2* __restrict__ pool) {
+__device__ __forceinline__ void apply_layer(double2* v, const DSub& sub, u32 base, const double2* __restrict__ pool) {
     constexpr int D = 1 << K;
-    const u32 ncz = (sub.type >> 12) & 7u, d1 = (sub.type >> 8) & 15u;
-    if (ncz) {
-        // bit j of sgn: amplitude j changes sign.  A qubit's value over the
-        // register block is the block pattern of its bit (a block position) or
-        // all ones / all zeros (any other position; base is zero at block
-        // positions).
-        u32 sgn = 0;
-        for (u32 c = 0; c < ncz; ++c) {
+    const u32 nit = (sub.type >> 12) & 7u, d1 = (sub.type >> 8) & 15u;
+    if (nit || (sub.type >> 16)) {
+        // bit j of sgn: amplitude j of the register block changes sign; base is
+        // zero at the block positions, so a local position p outside the block
+        // is (base >> p) & 1 for the whole block
+        u32 sgn = sub.type >> 16;
+        for (u32 c = 0; c < nit; ++c) {
             const u32 w = (c < 2 ? sub.a : sub.m) >> (16 * (c & 1));
-            const u32 p0 = w & 0xffu, p1 = (w >> 8) & 0xffu;
-            const u32 a0 = block_pattern<K>(ps, p0) | (0u - ((base >> p0) & 1u));
-            const u32 a1 = block_pattern<K>(ps, p1) | (0u - ((base >> p1) & 1u));
-            sgn ^= a0 & a1;
+            const u32 b0 = 0u - ((base >> (w & 0xffu)) & 1u);
+            if (w & 0x8000u)
+                sgn ^= b0 & (0u - ((base >> ((w >> 8) & 0x7fu)) & 1u));
+            else
+                sgn ^= b0 & (u32)(kBlockPatterns >> (16 * ((w >> 8) & 3u)));
         }
 #pragma unroll
-        for (int j = 0; j < D; ++j) negate_if(v[j], (sgn >> j) & 1u);
+        for (int j = 0; j < D; ++j) {
+            // flip the sign bits of both components where bit j of sgn is set
+            const int f = (int)((sgn << (31 - j)) & 0x80000000u);
+            v[j].x = __hiloint2double(__double2hiint(v[j].x) ^ f, __double2loint(v[j].x));
+            v[j].y = __hiloint2double(__double2hiint(v[j].y) ^ f, __double2loint(v[j].y));
+        }
     }
     const double2* mp = pool + sub.data;
     if (d1 & 1u) {
@@ -370,7 +409,7 @@ __device__ __forceinline__ void run_passes(double2* st, u32 lq, const DPass* __r
             while (k < ps.nsub) {
                 DSub sub = subs[ps.sub0 + k];
                 while ((sub.type & 0xffu) == SUB_LAYER) {
-                    apply_layer<K>(v, sub, ps, base, pool);
+                    apply_layer<K>(v, sub, base, pool);
                     if (++k == ps.nsub) break;
                     sub = subs[ps.sub0 + k];
                 }
@@ -382,11 +421,24 @@ __device__ __forceinline__ void run_passes(double2* st, u32 lq, const DPass* __r
                 } else if (kind == SUB_DK) {
                     sub_dk<K>(v, mp);
                 } else if (kind == SUB_DIAG) {
+                    // bit i of the diagonal index: local position p_i, i.e. bit p_i of
+                    // base or, for a block position, bit j of the block pattern
+                    u32 tbit[4], pat[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const u32 p = byte_at(sub.a, i);
+                        tbit[i] = i < (int)sub.m ? (base >> p) & 1u : 0u;
+                        u32 pt = 0;
+#pragma unroll
+                        for (int b = 0; b < K; ++b)
+                            pt |= (ps.pos[b] == p) ? (u32)(kBlockPatterns >> (16 * b)) & 0xffffu : 0u;
+                        pat[i] = i < (int)sub.m ? pt : 0u;
+                    }
 #pragma unroll
                     for (int j = 0; j < D; ++j) {
-                        const u32 li = base | off[j];
                         u32 idx = 0;
-                        for (u32 i = 0; i < sub.m; ++i) idx |= ((li >> byte_at(sub.a, i)) & 1u) << i;
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) idx |= ((tbit[i] | (pat[i] >> j)) & 1u) << i;
                         cmul_inplace(v[j], mp[idx]);
                     }
                 }
@@ -461,9 +513,54 @@ __global__ __launch_bounds__(block_threads<K>()) void k_batch(
 // One stage of a single-state circuit: every work-group takes tiles of 2^lq
 // amplitudes (grid-stride), runs the stage's passes on each in LDS and writes
 // it back in place.  Tiles of one stage are disjoint, so work-groups never
-// touch the same amplitudes.
+// touch the same amplitudes.  The next tile is loaded into registers while the
+// passes run on the current one (a barrier does not wait for global loads), so
+// the HBM traffic of consecutive tiles overlaps the arithmetic.
+constexpr u32 kStageThreads = 512;
+
+// Tiles of exactly P * kStageThreads amplitudes, P per thread, with prefetch.
+template <int K, u32 P>
+__device__ __forceinline__ void stage_tiles(double2* __restrict__ state, double2* st, const u32* hi_tab, u32 lq, u32 tl,
+                                            const DStage& sg, const DPass* __restrict__ passes,
+                                            const DSub* __restrict__ subs, const double2* __restrict__ pool) {
+    const u32 tid = threadIdx.x;
+    const u64 ntiles = 1ULL << tl;
+    double rx[P], ry[P];  // the prefetched tile (separate scalars keep it in registers)
+    u64 t = blockIdx.x;
+    if (t >= ntiles) return;
+    {
+        const u64 tb = tile_base(t, tl, sg);
+#pragma unroll
+        for (u32 k = 0; k < P; ++k) {
+            const double2 a = state[tile_index(tb, hi_tab, tid + k * kStageThreads)];
+            rx[k] = a.x;
+            ry[k] = a.y;
+        }
+    }
+    for (; t < ntiles; t += gridDim.x) {
+#pragma unroll
+        for (u32 k = 0; k < P; ++k) st[slot_of(tid + k * kStageThreads)] = make_double2(rx[k], ry[k]);
+        __syncthreads();
+        const u64 tn = t + gridDim.x;
+        if (tn < ntiles) {
+            const u64 tb = tile_base(tn, tl, sg);
+#pragma unroll
+            for (u32 k = 0; k < P; ++k) {
+                const double2 a = state[tile_index(tb, hi_tab, tid + k * kStageThreads)];
+                rx[k] = a.x;
+                ry[k] = a.y;
+            }
+        }
+        run_passes<K>(st, lq, passes, sg.pass0, sg.npass, subs, pool);
+        const u64 tb = tile_base(t, tl, sg);
+#pragma unroll
+        for (u32 k = 0; k < P; ++k) state[tile_index(tb, hi_tab, tid + k * kStageThreads)] = st[slot_of(tid + k * kStageThreads)];
+        __syncthreads();
+    }
+}
+
 template <int K>
-__global__ __launch_bounds__(block_threads<K>()) void k_stage(double2* __restrict__ state, u32 lq, u32 tl, DStage sg,
+__global__ __launch_bounds__(kStageThreads) void k_stage(double2* __restrict__ state, u32 lq, u32 tl, DStage sg,
                                                          const DPass* __restrict__ passes,
                                                          const DSub* __restrict__ subs,
                                                          const double2* __restrict__ pool) {
@@ -472,14 +569,22 @@ __global__ __launch_bounds__(block_threads<K>()) void k_stage(double2* __restric
     const u32 L = 1u << lq;
     build_hi_tab(hi_tab, lq, sg);
     __syncthreads();
-    const u64 ntiles = 1ULL << tl;
-    for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const u64 tb = tile_base(t, tl, sg);
-        for (u32 i = threadIdx.x; i < L; i += blockDim.x) st[slot_of(i)] = state[tile_index(tb, hi_tab, i)];
-        __syncthreads();
-        run_passes<K>(st, lq, passes, sg.pass0, sg.npass, subs, pool);
-        for (u32 i = threadIdx.x; i < L; i += blockDim.x) state[tile_index(tb, hi_tab, i)] = st[slot_of(i)];
-        __syncthreads();
+    // the prefetched tile needs 4 P registers per thread on top of the register
+    // block: it fits beside K <= 3 blocks only
+    if (K <= 3 && L == 16 * kStageThreads) {
+        stage_tiles<K, 16>(state, st, hi_tab, lq, tl, sg, passes, subs, pool);
+    } else if (K <= 3 && L == 8 * kStageThreads) {
+        stage_tiles<K, 8>(state, st, hi_tab, lq, tl, sg, passes, subs, pool);
+    } else {
+        const u64 ntiles = 1ULL << tl;
+        for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+            const u64 tb = tile_base(t, tl, sg);
+            for (u32 i = threadIdx.x; i < L; i += blockDim.x) st[slot_of(i)] = state[tile_index(tb, hi_tab, i)];
+            __syncthreads();
+            run_passes<K>(st, lq, passes, sg.pass0, sg.npass, subs, pool);
+            for (u32 i = threadIdx.x; i < L; i += blockDim.x) state[tile_index(tb, hi_tab, i)] = st[slot_of(i)];
+            __syncthreads();
+        }
     }
 }
 
@@ -658,28 +763,46 @@ void schedule_passes(Flat& f, std::vector<HOp> ops, u32 lq, u32 K, DStage& sg) {
         // Consecutive CZ gates, then consecutive single-qubit matrices on distinct
         // block bits (these commute with each other), become one LAYER.
         std::vector<const std::vector<cd>*> pend_d1(K, nullptr);
-        u32 d1mask = 0;
-        std::vector<u32> pend_cz;  // packed local positions (p0 | p1 << 8)
+        u32 d1mask = 0, cz_const = 0;
+        std::vector<u32> cz_items;
+        bool cz_any = false;
         auto flush = [&]() {
-            if (d1mask == 0 && pend_cz.empty()) return;
+            if (d1mask == 0 && !cz_any) return;
             DSub sb{};
-            sb.type = SUB_LAYER | (d1mask << 8) | ((u32)pend_cz.size() << 12);
-            for (u32 c = 0; c < pend_cz.size(); ++c) (c < 2 ? sb.a : sb.m) |= pend_cz[c] << (16 * (c & 1));
+            sb.type = SUB_LAYER | (d1mask << 8) | ((u32)cz_items.size() << 12) | (cz_const << 16);
+            for (u32 c = 0; c < cz_items.size(); ++c) (c < 2 ? sb.a : sb.m) |= cz_items[c] << (16 * (c & 1));
             sb.data = (u32)f.pool.size();
             for (u32 b = 0; b < K; ++b)
                 if (pend_d1[b]) pool_put(f, *pend_d1[b]);
             f.subs.push_back(sb);
             std::fill(pend_d1.begin(), pend_d1.end(), nullptr);
             d1mask = 0;
-            pend_cz.clear();
+            cz_const = 0;
+            cz_items.clear();
+            cz_any = false;
+        };
+        const u32 jmask = (1u << (1u << K)) - 1;
+        auto add_cz = [&](u32 p0, u32 p1) {
+            const bool in0 = (block >> p0) & 1, in1 = (block >> p1) & 1;
+            if (in0 && in1) {
+                cz_const ^= (u32)(kBlockPatterns >> (16 * bit_of(p0))) & (u32)(kBlockPatterns >> (16 * bit_of(p1))) &
+                            jmask;
+            } else {
+                if (cz_items.size() == kCzItemsMax) flush();
+                if (in0 || in1)
+                    cz_items.push_back((in0 ? p1 : p0) | (bit_of(in0 ? p0 : p1) << 8));
+                else
+                    cz_items.push_back(p0 | (p1 << 8) | 0x8000u);
+            }
+            cz_any = true;
         };
         for (size_t i : sel) {
             const HOp& o = ops[i];
             done[i] = true;
             --left;
             if (o.type == 2) {
-                if (d1mask || pend_cz.size() == kCzSetMax) flush();
-                pend_cz.push_back(o.q[0] | (o.q[1] << 8));
+                if (d1mask) flush();
+                add_cz(o.q[0], o.q[1]);
                 continue;
             }
             if (o.type == 0 && o.q.size() == 1) {
@@ -1259,8 +1382,7 @@ bool circuit_enqueue(qm_circuit* c, void* state, u64 dim, hipStream_t st, unsign
     if (c->per_cu == 0) {
         int per_cu = 0, cus = 0;
         if (!hip_ok(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), what) ||
-            !hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                        &per_cu, fn, (int)(c->K >= 4 ? block_threads<4>() : block_threads<3>()), lds), what) ||
+            !hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)kStageThreads, lds), what) ||
             !hip_ok(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, (int)dev), what))
             return false;
         if (per_cu < 1) {
@@ -1276,9 +1398,9 @@ bool circuit_enqueue(qm_circuit* c, void* state, u64 dim, hipStream_t st, unsign
     for (const DStage& sg : c->stages) {
         if (sg.npass == 0) continue;
         switch (c->K) {
-            case 2: k_stage<2><<<grid, block_threads<2>(), lds, st>>>((double2*)state, c->lq, tl, sg, c->d_passes, c->d_subs, c->d_pool); break;
-            case 3: k_stage<3><<<grid, block_threads<3>(), lds, st>>>((double2*)state, c->lq, tl, sg, c->d_passes, c->d_subs, c->d_pool); break;
-            default: k_stage<4><<<grid, block_threads<4>(), lds, st>>>((double2*)state, c->lq, tl, sg, c->d_passes, c->d_subs, c->d_pool); break;
+            case 2: k_stage<2><<<grid, kStageThreads, lds, st>>>((double2*)state, c->lq, tl, sg, c->d_passes, c->d_subs, c->d_pool); break;
+            case 3: k_stage<3><<<grid, kStageThreads, lds, st>>>((double2*)state, c->lq, tl, sg, c->d_passes, c->d_subs, c->d_pool); break;
+            default: k_stage<4><<<grid, kStageThreads, lds, st>>>((double2*)state, c->lq, tl, sg, c->d_passes, c->d_subs, c->d_pool); break;
         }
     }
     return hip_ok(hipGetLastError(), what);

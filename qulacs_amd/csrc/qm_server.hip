@@ -44,7 +44,10 @@ namespace {
 
 using clk = std::chrono::steady_clock;
 constexpr double kStarveSeconds = 0.25;
-constexpr u32 kBatchBlockQubits = 4;
+// register-block widths of the batched and the fused executors (the fastest
+// measured for brickwork / layered circuits: 3)
+constexpr u32 kBatchBlockQubits = 3;
+constexpr u32 kLargeBlockQubits = 3;
 
 struct Request {
     long long id = 0;
@@ -420,7 +423,7 @@ long long qm_server_submit(qm_server* s, UINT n, const qm_batch_gate* gates, UIN
         qm_batch_destroy(probe);
         if (idx < 0) return -1;
     } else {
-        r->circuit = qm_circuit_create(n, r->gates.data(), gate_count, kBatchBlockQubits, 0);
+        r->circuit = qm_circuit_create(n, r->gates.data(), gate_count, kLargeBlockQubits, 0);
         if (!r->circuit) return -1;
     }
     std::lock_guard<std::mutex> lk(s->mu);
