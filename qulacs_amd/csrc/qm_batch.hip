@@ -516,10 +516,11 @@ __global__ __launch_bounds__(block_threads<K>()) void k_batch(
 // touch the same amplitudes.  The next tile is loaded into registers while the
 // passes run on the current one (a barrier does not wait for global loads), so
 // the HBM traffic of consecutive tiles overlaps the arithmetic.
-constexpr u32 kStageThreads = 512;
+template <int K>
+constexpr u32 stage_threads() { return K >= 4 ? 512u : 1024u; }
 
 // Tiles of exactly P * kStageThreads amplitudes, P per thread, with prefetch.
-template <int K, u32 P>
+template <int K, u32 P, u32 T = stage_threads<K>()>
 __device__ __forceinline__ void stage_tiles(double2* __restrict__ state, double2* st, const u32* hi_tab, u32 lq, u32 tl,
                                             const DStage& sg, const DPass* __restrict__ passes,
                                             const DSub* __restrict__ subs, const double2* __restrict__ pool) {
@@ -532,21 +533,21 @@ __device__ __forceinline__ void stage_tiles(double2* __restrict__ state, double2
         const u64 tb = tile_base(t, tl, sg);
 #pragma unroll
         for (u32 k = 0; k < P; ++k) {
-            const double2 a = state[tile_index(tb, hi_tab, tid + k * kStageThreads)];
+            const double2 a = state[tile_index(tb, hi_tab, tid + k * T)];
             rx[k] = a.x;
             ry[k] = a.y;
         }
     }
     for (; t < ntiles; t += gridDim.x) {
 #pragma unroll
-        for (u32 k = 0; k < P; ++k) st[slot_of(tid + k * kStageThreads)] = make_double2(rx[k], ry[k]);
+        for (u32 k = 0; k < P; ++k) st[slot_of(tid + k * T)] = make_double2(rx[k], ry[k]);
         __syncthreads();
         const u64 tn = t + gridDim.x;
         if (tn < ntiles) {
             const u64 tb = tile_base(tn, tl, sg);
 #pragma unroll
             for (u32 k = 0; k < P; ++k) {
-                const double2 a = state[tile_index(tb, hi_tab, tid + k * kStageThreads)];
+                const double2 a = state[tile_index(tb, hi_tab, tid + k * T)];
                 rx[k] = a.x;
                 ry[k] = a.y;
             }
@@ -554,13 +555,13 @@ __device__ __forceinline__ void stage_tiles(double2* __restrict__ state, double2
         run_passes<K>(st, lq, passes, sg.pass0, sg.npass, subs, pool);
         const u64 tb = tile_base(t, tl, sg);
 #pragma unroll
-        for (u32 k = 0; k < P; ++k) state[tile_index(tb, hi_tab, tid + k * kStageThreads)] = st[slot_of(tid + k * kStageThreads)];
+        for (u32 k = 0; k < P; ++k) state[tile_index(tb, hi_tab, tid + k * T)] = st[slot_of(tid + k * T)];
         __syncthreads();
     }
 }
 
 template <int K>
-__global__ __launch_bounds__(kStageThreads) void k_stage(double2* __restrict__ state, u32 lq, u32 tl, DStage sg,
+__global__ __launch_bounds__(stage_threads<K>()) void k_stage(double2* __restrict__ state, u32 lq, u32 tl, DStage sg,
                                                          const DPass* __restrict__ passes,
                                                          const DSub* __restrict__ subs,
                                                          const double2* __restrict__ pool) {
@@ -571,10 +572,10 @@ __global__ __launch_bounds__(kStageThreads) void k_stage(double2* __restrict__ s
     __syncthreads();
     // the prefetched tile needs 4 P registers per thread on top of the register
     // block: it fits beside K <= 3 blocks only
-    if (K <= 3 && L == 16 * kStageThreads) {
-        stage_tiles<K, 16>(state, st, hi_tab, lq, tl, sg, passes, subs, pool);
-    } else if (K <= 3 && L == 8 * kStageThreads) {
+    if (K <= 3 && L == 8 * stage_threads<K>()) {
         stage_tiles<K, 8>(state, st, hi_tab, lq, tl, sg, passes, subs, pool);
+    } else if (K <= 3 && L == 4 * stage_threads<K>()) {
+        stage_tiles<K, 4>(state, st, hi_tab, lq, tl, sg, passes, subs, pool);
     } else {
         const u64 ntiles = 1ULL << tl;
         for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -1382,7 +1383,7 @@ bool circuit_enqueue(qm_circuit* c, void* state, u64 dim, hipStream_t st, unsign
     if (c->per_cu == 0) {
         int per_cu = 0, cus = 0;
         if (!hip_ok(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), what) ||
-            !hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)kStageThreads, lds), what) ||
+            !hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)(c->K >= 4 ? stage_threads<4>() : stage_threads<3>()), lds), what) ||
             !hip_ok(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, (int)dev), what))
             return false;
         if (per_cu < 1) {
@@ -1398,9 +1399,9 @@ bool circuit_enqueue(qm_circuit* c, void* state, u64 dim, hipStream_t st, unsign
     for (const DStage& sg : c->stages) {
         if (sg.npass == 0) continue;
         switch (c->K) {
-            case 2: k_stage<2><<<grid, kStageThreads, lds, st>>>((double2*)state, c->lq, tl, sg, c->d_passes, c->d_subs, c->d_pool); break;
-            case 3: k_stage<3><<<grid, kStageThreads, lds, st>>>((double2*)state, c->lq, tl, sg, c->d_passes, c->d_subs, c->d_pool); break;
-            default: k_stage<4><<<grid, kStageThreads, lds, st>>>((double2*)state, c->lq, tl, sg, c->d_passes, c->d_subs, c->d_pool); break;
+            case 2: k_stage<2><<<grid, stage_threads<2>(), lds, st>>>((double2*)state, c->lq, tl, sg, c->d_passes, c->d_subs, c->d_pool); break;
+            case 3: k_stage<3><<<grid, stage_threads<3>(), lds, st>>>((double2*)state, c->lq, tl, sg, c->d_passes, c->d_subs, c->d_pool); break;
+            default: k_stage<4><<<grid, stage_threads<4>(), lds, st>>>((double2*)state, c->lq, tl, sg, c->d_passes, c->d_subs, c->d_pool); break;
         }
     }
     return hip_ok(hipGetLastError(), what);
