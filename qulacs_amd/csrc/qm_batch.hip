@@ -303,12 +303,15 @@ __global__ void k_batch_reset(int* counter) {
     if (threadIdx.x == 0) *counter = 0;
 }
 
-// hi_tab[j] = global offset of local index j << 4 (the window qubits 4.. of the stage)
-__device__ __forceinline__ void build_hi_tab(u32* hi_tab, u32 lq, const DStage& sg) {
+// hi_tab[j] = global offset of local index j << 4 (the window qubits 4.. of the
+// stage).  TI = u32 holds offsets below 2^32 (states of at most 32 qubits);
+// larger states use u64.
+template <typename TI>
+__device__ __forceinline__ void build_hi_tab(TI* hi_tab, u32 lq, const DStage& sg) {
     const u64 win0 = sg.win[0], win1 = sg.win[1];
     for (u32 j = threadIdx.x; j < (1u << (lq - kLowFixed)); j += blockDim.x) {
-        u32 o = 0;
-        for (u32 i = 0; i + kLowFixed < lq; ++i) o |= ((j >> i) & 1u) << byte_at(i < 8 ? win0 : win1, i & 7);
+        TI o = 0;
+        for (u32 i = 0; i + kLowFixed < lq; ++i) o |= (TI)((j >> i) & 1u) << byte_at(i < 8 ? win0 : win1, i & 7);
         hi_tab[j] = o;
     }
 }
@@ -324,7 +327,8 @@ __device__ __forceinline__ u64 tile_base(u64 t, u32 tl, const DStage& sg) {
     return tb;
 }
 
-__device__ __forceinline__ u64 tile_index(u64 tb, const u32* hi_tab, u32 i) {
+template <typename TI>
+__device__ __forceinline__ u64 tile_index(u64 tb, const TI* hi_tab, u32 i) {
     return tb + hi_tab[i >> kLowFixed] + (i & ((1u << kLowFixed) - 1));
 }
 
@@ -521,7 +525,7 @@ constexpr u32 stage_threads() { return K >= 4 ? 512u : 1024u; }
 
 // Tiles of exactly P * kStageThreads amplitudes, P per thread, with prefetch.
 template <int K, u32 P, u32 T = stage_threads<K>()>
-__device__ __forceinline__ void stage_tiles(double2* __restrict__ state, double2* st, const u32* hi_tab, u32 lq, u32 tl,
+__device__ __forceinline__ void stage_tiles(double2* __restrict__ state, double2* st, const u64* hi_tab, u32 lq, u32 tl,
                                             const DStage& sg, const DPass* __restrict__ passes,
                                             const DSub* __restrict__ subs, const double2* __restrict__ pool) {
     const u32 tid = threadIdx.x;
@@ -566,7 +570,7 @@ __global__ __launch_bounds__(stage_threads<K>()) void k_stage(double2* __restric
                                                          const DSub* __restrict__ subs,
                                                          const double2* __restrict__ pool) {
     extern __shared__ double2 st[];
-    __shared__ u32 hi_tab[1u << (kMaxLocal - kLowFixed)];
+    __shared__ u64 hi_tab[1u << (kMaxLocal - kLowFixed)];  // offsets of states above 32 qubits exceed 32 bits
     const u32 L = 1u << lq;
     build_hi_tab(hi_tab, lq, sg);
     __syncthreads();

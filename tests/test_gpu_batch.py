@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from qulacs_amd.batch import Batch
-from qulacs_amd.workloads import many_small_circuits
+from qulacs_amd.workloads import haar_unitary, many_small_circuits
 from tests.batch_util import oracle_run, random_batch_circuit, random_terms
 from tests.gpu_util import rel
 
@@ -127,3 +127,41 @@ def test_graph_replays_match_eager_runs():
     finally:
         _lib.call("release_cuda_stream_host", stream, 1, 0)
     b.close()
+
+
+def test_fused_circuit_above_32_qubits_round_trip():
+    """33 qubits (128 GiB): offsets of window qubits >= 32 exceed 32 bits.  U then
+    U^dagger from |0...0> must return |0...0>: P(q = 0) = 1 for every probed qubit."""
+    from qulacs_amd import _lib
+    from qulacs_amd.batch import FusedCircuit
+
+    n = 33
+    rng = np.random.default_rng(3333)
+    gates = [("dense", [32], haar_unitary(rng, 1)), ("CNOT", [32, 0], None), ("CZ", [31, 32], None),
+             ("dense", [32, 5], haar_unitary(rng, 2))]
+    gates += random_batch_circuit(rng, n, 20, max_dense=3)
+    inverse = []
+    for kind, q, m in reversed(gates):
+        if kind == "dense":
+            inverse.append((kind, q, np.conj(np.asarray(m)).T))
+        elif kind == "diag":
+            inverse.append((kind, q, np.conj(np.asarray(m))))
+        else:
+            inverse.append((kind, q, m))
+    fc = FusedCircuit(n, gates + inverse, block_qubits=3)
+    dim = 1 << n
+    print("fused 33 qubits", fc.stats(), flush=True)
+    try:
+        st = _lib.call("allocate_quantum_state_host", dim, 0)
+    except _lib.QulacsMI355Error as e:
+        pytest.skip(f"no room for a 33-qubit state: {e}")
+    try:
+        _lib.call("initialize_quantum_state_host", st, dim, None, 0)
+        fc.apply(st)
+        for q in (0, 5, 17, 31, 32):
+            p0 = _lib.call("M0_prob_host", q, st, dim, None, 0)
+            assert abs(p0 - 1.0) < 1e-10, (q, p0)
+        assert abs(_lib.call("state_norm_squared_host", st, dim, None, 0) - 1.0) < 1e-10
+    finally:
+        _lib.call("release_quantum_state_host", st, 0)
+        fc.close()
