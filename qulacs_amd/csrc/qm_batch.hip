@@ -455,6 +455,46 @@ __device__ __forceinline__ void run_passes(double2* st, u32 lq, const DPass* __r
     }
 }
 
+// Tiles of one stage of a batched circuit held in device memory (every stage
+// after the first), the next tile loaded into registers while the passes run
+// on the current one.  P amplitudes per thread, T threads.
+constexpr u32 kBatchTile = 1u << kMaxLocal;
+template <int K, u32 P, u32 T>
+__device__ __forceinline__ void batch_tiles(double2* st, double2* __restrict__ gs, const u32* hi_tab, u32 lq, u32 tl,
+                                            const DStage& sg, const DPass* __restrict__ passes,
+                                            const DSub* __restrict__ subs, const double2* __restrict__ pool) {
+    const u32 tid = threadIdx.x, ntiles = 1u << tl;
+    double rx[P], ry[P];
+    {
+        const u64 tb = tile_base(0, tl, sg);
+#pragma unroll
+        for (u32 k = 0; k < P; ++k) {
+            const double2 a = gs[tile_index(tb, hi_tab, tid + k * T)];
+            rx[k] = a.x;
+            ry[k] = a.y;
+        }
+    }
+    for (u32 t = 0; t < ntiles; ++t) {
+#pragma unroll
+        for (u32 k = 0; k < P; ++k) st[slot_of(tid + k * T)] = make_double2(rx[k], ry[k]);
+        __syncthreads();
+        if (t + 1 < ntiles) {
+            const u64 tb = tile_base(t + 1, tl, sg);
+#pragma unroll
+            for (u32 k = 0; k < P; ++k) {
+                const double2 a = gs[tile_index(tb, hi_tab, tid + k * T)];
+                rx[k] = a.x;
+                ry[k] = a.y;
+            }
+        }
+        run_passes<K>(st, lq, passes, sg.pass0, sg.npass, subs, pool);
+        const u64 tb = tile_base(t, tl, sg);
+#pragma unroll
+        for (u32 k = 0; k < P; ++k) gs[tile_index(tb, hi_tab, tid + k * T)] = st[slot_of(tid + k * T)];
+        __syncthreads();
+    }
+}
+
 // The program arrays arrive as separate read-only, non-aliasing arguments so the
 // compiler reads them through the scalar cache (inside a by-value struct they
 // became per-lane vector loads).
@@ -486,6 +526,11 @@ __global__ __launch_bounds__(block_threads<K>()) void k_batch(
             if (tl > 0) {
                 build_hi_tab(hi_tab, lq, sg);
                 __syncthreads();
+            }
+            if (K <= 3 && tl > 0 && s > 0 && L == kBatchTile * bd) {  // K = 4 has no registers to spare
+                batch_tiles<K, kBatchTile / block_threads<K>(), block_threads<K>()>(st, gs, hi_tab, lq, tl, sg,
+                                                                                     passes, subs, pool);
+                continue;
             }
             for (u32 t = 0; t < (1u << tl); ++t) {
                 const u64 tb = tile_base(t, tl, sg);
