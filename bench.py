@@ -439,6 +439,8 @@ def brickwork(args, lib_mod, hip):
     lib_mod.check()
     lib_mod.call("qm_set_synchronous", 1)
     gpu_ms = (t1 - t0) * 1e3 / reps
+    psi_gpu = np.zeros(dim, dtype=np.complex128)  # the per-gate path's final state (the variants reuse st)
+    lib_mod.call("copy_quantum_state_from_device_to_host", psi_gpu.ctypes.data, st, dim, stream, 0)
     fused_stats = fused.stats()
     fused.close()
     # the fused executor's register-block width and LDS window
@@ -457,8 +459,6 @@ def brickwork(args, lib_mod, hip):
         fv.close()
     lib_mod.call("qm_set_synchronous", 1)
     lib_mod.check()
-    psi_gpu = np.zeros(dim, dtype=np.complex128)
-    lib_mod.call("copy_quantum_state_from_device_to_host", psi_gpu.ctypes.data, st, dim, stream, 0)
     lib_mod.call("release_quantum_state_host", st, 0)
     lib_mod.call("release_cuda_stream_host", stream, 1, 0)
 
@@ -791,8 +791,9 @@ def main() -> None:
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--adj-batch", type=int, default=256)
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--only", choices=["dm", "brickwork", "small", "adjoint", "serve"],
-                    help="run one secondary leg alone and print its JSON (for profiling)")
+    ap.add_argument("--only", choices=["sweep", "dm", "brickwork", "small", "adjoint", "serve"],
+                    help="run one leg alone (for profiling): 'sweep' prints the full line with the other legs "
+                         "left out, the others print their leg's JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the sweep is a single 30-qubit "
@@ -802,7 +803,9 @@ def main() -> None:
     from qulacs_amd import _lib
 
     hip = HipEvents()
-    if args.only:
+    if args.only == "sweep":
+        args.no_dm = args.no_small = args.no_serve = args.no_adjoint = True
+    elif args.only:
         leg = {"dm": dm_sweep, "brickwork": brickwork, "small": many_small, "adjoint": adjoint,
                "serve": serving}[args.only]
         print(json.dumps({args.only: leg(args, _lib, hip)}, default=str), flush=True)
@@ -815,8 +818,10 @@ def main() -> None:
     if not args.no_dm:
         dm = dm_sweep(args, _lib, hip)
         print(f"[bench] dm sweep {dm['ms']} ms", file=sys.stderr, flush=True)
-    c20 = brickwork(args, _lib, hip)
-    print(f"[bench] 20q brickwork {c20['gpu_ms']} ms", file=sys.stderr, flush=True)
+    c20 = None
+    if args.only != "sweep":
+        c20 = brickwork(args, _lib, hip)
+        print(f"[bench] 20q brickwork {c20['gpu_ms']} ms", file=sys.stderr, flush=True)
     small = None
     if not args.no_small:
         small = many_small(args, _lib, hip)
