@@ -17,6 +17,15 @@ from __future__ import annotations
 
 import sys as _sys
 
+# One HIP runtime per process.  PyTorch ships its own HIP/HSA runtime; when torch
+# is installed it is loaded first, so this library binds to that same runtime
+# (by soname) instead of bringing a second HIP/HSA stack into the process, which
+# cannot open the device once the other stack has.
+try:
+    import torch as _torch  # noqa: F401
+except ImportError:  # pragma: no cover - torch is optional for the library itself
+    _torch = None
+
 try:
     from . import _core
 except ImportError as e:  # pragma: no cover - exercised only without a build
