@@ -56,6 +56,8 @@ from ._core import (  # noqa: F401
     circuit,
     gate,
     get_num_device,
+    observable,
+    quantum_operator,
     get_synchronous,
     set_synchronous,
     state,
@@ -72,5 +74,7 @@ DensityMatrix = DensityMatrixMI355
 _sys.modules[__name__ + ".gate"] = gate
 _sys.modules[__name__ + ".state"] = state
 _sys.modules[__name__ + ".circuit"] = circuit
+_sys.modules[__name__ + ".quantum_operator"] = quantum_operator
+_sys.modules[__name__ + ".observable"] = observable
 
 __version__ = _core.__version__

@@ -228,6 +228,7 @@ class QuantumGateDiagonalMatrix : public QuantumGateBase {
     void update_quantum_state(QuantumStateBase* state) override;
     QuantumGateBase* copy() const override { return new QuantumGateDiagonalMatrix(*this); }
     ComplexMatrix get_matrix() const override;
+    const std::vector<CPPCTYPE>& get_diagonal() const { return diag_; }
 
    private:
     std::vector<CPPCTYPE> diag_;
@@ -242,6 +243,7 @@ class ClsPauliGate : public QuantumGateBase {
     ComplexMatrix get_matrix() const override;
     QuantumGateBase* get_inverse() const override;
     const std::vector<unsigned>& get_pauli_id_list() const { return paulis_; }
+    bool is_rotation() const { return rotation_; }
     double get_angle() const { return angle_; }
     void set_angle(double a) { angle_ = a; }
 
@@ -265,6 +267,7 @@ class QuantumGate_Probabilistic : public QuantumGateBase {
     bool is_noise() const override { return true; }
     void set_seed(uint64_t seed) { random_.set_seed(seed); }
     const std::vector<double>& get_distribution() const { return dist_; }
+    const std::vector<QuantumGateBase*>& get_gate_list() const { return gates_; }
     // superoperator over targets_ (4^k x 4^k, member = row bits * 2^k + column bits)
     ComplexMatrix superoperator() const;
 
@@ -286,6 +289,7 @@ class QuantumGate_CPTP : public QuantumGateBase {
     bool is_noise() const override { return true; }
     ComplexMatrix superoperator() const;
     void set_seed(uint64_t seed) { random_.set_seed(seed); }
+    const std::vector<QuantumGateMatrix>& get_kraus_list() const { return kraus_; }
 
    private:
     std::vector<QuantumGateMatrix> kraus_;

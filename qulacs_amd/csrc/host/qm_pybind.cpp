@@ -9,6 +9,7 @@
 #include <pybind11/stl.h>
 
 #include "qm_host.hpp"
+#include "qm_json.hpp"
 
 namespace py = pybind11;
 using namespace qmh;
@@ -119,7 +120,8 @@ PYBIND11_MODULE(_core, m) {
         .def("apply_to_state",
              [](const GeneralQuantumOperator& o, QuantumStateMI355* work, const QuantumStateMI355& src,
                 QuantumStateMI355* dst) { o.apply_to_state(work, src, dst); },
-             py::arg("work_state"), py::arg("state_to_be_multiplied"), py::arg("dst_state"));
+             py::arg("work_state"), py::arg("state_to_be_multiplied"), py::arg("dst_state"))
+        .def("to_json", [](const GeneralQuantumOperator& o) { return ptree::to_json(to_ptree(o)); });
 
     py::class_<Observable, GeneralQuantumOperator>(m, "Observable")
         .def(py::init<unsigned>(), py::arg("qubit_count"))
@@ -138,6 +140,15 @@ PYBIND11_MODULE(_core, m) {
             },
             py::arg("pauli_operator"))
         .def("get_expectation_value", &Observable::get_expectation_value_real, py::arg("state"));
+
+    auto mquantum_operator = m.def_submodule("quantum_operator");
+    mquantum_operator.def(
+        "from_json", [](const std::string& json) { return operator_from_ptree(ptree::from_json(json)); },
+        py::return_value_policy::take_ownership, py::arg("json"));
+    auto mobservable = m.def_submodule("observable");
+    mobservable.def(
+        "from_json", [](const std::string& json) { return observable_from_ptree(ptree::from_json(json)); },
+        py::return_value_policy::take_ownership, py::arg("json"));
 
     // ---------------------------------------------------------------- states
     py::class_<QuantumStateBase>(m, "QuantumStateBase")
@@ -168,7 +179,8 @@ PYBIND11_MODULE(_core, m) {
         .def("sampling", py::overload_cast<unsigned, unsigned>(&QuantumStateBase::sampling), py::arg("sampling_count"),
              py::arg("random_seed"))
         .def("to_string", &QuantumStateBase::to_string)
-        .def("__str__", &QuantumStateBase::to_string);
+        .def("__str__", &QuantumStateBase::to_string)
+        .def("to_json", [](const QuantumStateBase& s) { return ptree::to_json(to_ptree(s)); });
 
     py::class_<QuantumStateMI355, QuantumStateBase>(m, "QuantumStateMI355")
         .def(py::init<unsigned, unsigned>(), py::arg("qubit_count"), py::arg("gpu_id") = 0)
@@ -185,7 +197,13 @@ PYBIND11_MODULE(_core, m) {
              py::arg("index"))
         .def("sampling_with_uniforms", &QuantumStateMI355::sampling_with_uniforms, py::arg("uniforms"))
         .def("data_ptr", [](const QuantumStateMI355& s) { return reinterpret_cast<uintptr_t>(s.data()); })
-        .def("stream_ptr", [](const QuantumStateMI355& s) { return reinterpret_cast<uintptr_t>(s.get_cuda_stream()); });
+        .def("stream_ptr", [](const QuantumStateMI355& s) { return reinterpret_cast<uintptr_t>(s.get_cuda_stream()); })
+        .def(py::pickle([](const QuantumStateMI355& s) { return ptree::to_json(to_ptree(s)); },
+                        [](const std::string& json) {
+                            std::unique_ptr<QuantumStateBase> b(state_from_ptree(ptree::from_json(json)));
+                            if (!b->is_state_vector()) throw QulacsError("pickle: not a state vector");
+                            return std::unique_ptr<QuantumStateMI355>(static_cast<QuantumStateMI355*>(b.release()));
+                        }));
 
     py::class_<DensityMatrixMI355, QuantumStateBase>(m, "DensityMatrixMI355")
         .def(py::init<unsigned, unsigned>(), py::arg("qubit_count"), py::arg("gpu_id") = 0)
@@ -202,10 +220,19 @@ PYBIND11_MODULE(_core, m) {
              }, py::arg("state"))
         .def("load", [](DensityMatrixMI355& s, const carray& a) { s.load(to_vector(a)); }, py::arg("state"))
         .def("sampling_with_uniforms", &DensityMatrixMI355::sampling_with_uniforms, py::arg("uniforms"))
-        .def("data_ptr", [](const DensityMatrixMI355& s) { return reinterpret_cast<uintptr_t>(s.data()); });
+        .def("data_ptr", [](const DensityMatrixMI355& s) { return reinterpret_cast<uintptr_t>(s.data()); })
+        .def(py::pickle([](const DensityMatrixMI355& s) { return ptree::to_json(to_ptree(s)); },
+                        [](const std::string& json) {
+                            std::unique_ptr<QuantumStateBase> b(state_from_ptree(ptree::from_json(json)));
+                            if (b->is_state_vector()) throw QulacsError("pickle: not a density matrix");
+                            return std::unique_ptr<DensityMatrixMI355>(static_cast<DensityMatrixMI355*>(b.release()));
+                        }));
 
     auto mstate = m.def_submodule("state");
     mstate.def("inner_product", &inner_product, py::arg("state_bra"), py::arg("state_ket"));
+    mstate.def(
+        "from_json", [](const std::string& json) { return state_from_ptree(ptree::from_json(json)); },
+        py::return_value_policy::take_ownership, py::arg("json"));
 
     // ---------------------------------------------------------------- gates
     py::class_<QuantumGateBase>(m, "QuantumGateBase")
@@ -220,7 +247,8 @@ PYBIND11_MODULE(_core, m) {
         .def("is_parametric", &QuantumGateBase::is_parametric)
         .def("is_noise", &QuantumGateBase::is_noise)
         .def("to_string", &QuantumGateBase::to_string)
-        .def("__str__", &QuantumGateBase::to_string);
+        .def("__str__", &QuantumGateBase::to_string)
+        .def("to_json", [](const QuantumGateBase& g) { return ptree::to_json(to_ptree(g)); });
     py::class_<QuantumGateMatrix, QuantumGateBase>(m, "QuantumGateMatrix")
         .def("add_control_qubit", &QuantumGateMatrix::add_control_qubit, py::arg("index"), py::arg("control_value"));
     py::class_<QuantumGateDiagonalMatrix, QuantumGateBase>(m, "QuantumGateDiagonalMatrix");
@@ -229,10 +257,24 @@ PYBIND11_MODULE(_core, m) {
         .def("get_angle", &ClsPauliGate::get_angle);
     py::class_<QuantumGate_Probabilistic, QuantumGateBase>(m, "QuantumGate_Probabilistic")
         .def("get_distribution", &QuantumGate_Probabilistic::get_distribution)
+        .def("get_gate_list",
+             [](const QuantumGate_Probabilistic& g) {
+                 std::vector<QuantumGateBase*> out;
+                 for (const QuantumGateBase* x : g.get_gate_list()) out.push_back(x->copy());
+                 return out;
+             },
+             py::return_value_policy::take_ownership)
         .def("set_seed", &QuantumGate_Probabilistic::set_seed, py::arg("seed"))
         .def("get_superoperator", [](const QuantumGate_Probabilistic& g) { return from_matrix(g.superoperator()); });
     py::class_<QuantumGate_CPTP, QuantumGateBase>(m, "QuantumGate_CPTP")
         .def("set_seed", &QuantumGate_CPTP::set_seed, py::arg("seed"))
+        .def("get_gate_list",
+             [](const QuantumGate_CPTP& g) {
+                 std::vector<QuantumGateBase*> out;
+                 for (const QuantumGateMatrix& k : g.get_kraus_list()) out.push_back(k.copy());
+                 return out;
+             },
+             py::return_value_policy::take_ownership)
         .def("get_superoperator", [](const QuantumGate_CPTP& g) { return from_matrix(g.superoperator()); });
     py::class_<QuantumGate_SingleParameter, QuantumGateBase>(m, "QuantumGate_SingleParameter")
         .def("get_parameter_value", &QuantumGate_SingleParameter::get_parameter_value)
@@ -327,10 +369,21 @@ PYBIND11_MODULE(_core, m) {
     mg.def("ParametricRZ", &gate::ParametricRZ, own_, py::arg("index"), py::arg("angle"));
     mg.def("ParametricPauliRotation", &gate::ParametricPauliRotation, own_, py::arg("index_list"),
            py::arg("pauli_ids"), py::arg("angle"));
+    mg.def(
+        "from_json", [](const std::string& json) { return gate_from_ptree(ptree::from_json(json)); }, own_,
+        py::arg("json"));
 
     // ---------------------------------------------------------------- circuits
     py::class_<QuantumCircuit>(m, "QuantumCircuit")
         .def(py::init<unsigned>(), py::arg("qubit_count"))
+        .def("to_json", [](const QuantumCircuit& c) { return ptree::to_json(to_ptree(c)); })
+        .def(py::pickle([](const QuantumCircuit& c) { return ptree::to_json(to_ptree(c)); },
+                        [](const std::string& json) {
+                            const PTree pt = ptree::from_json(json);
+                            if (pt.get("name") != "QuantumCircuit")
+                                throw QulacsError("pickle: not a QuantumCircuit: " + pt.get("name"));
+                            return std::unique_ptr<QuantumCircuit>(circuit_from_ptree(pt));
+                        }))
         .def("copy", &QuantumCircuit::copy, py::return_value_policy::take_ownership)
         .def("add_gate", py::overload_cast<const QuantumGateBase&>(&QuantumCircuit::add_gate), py::arg("gate"))
         .def("add_gate", py::overload_cast<const QuantumGateBase&, size_t>(&QuantumCircuit::add_gate), py::arg("gate"),
@@ -431,7 +484,12 @@ PYBIND11_MODULE(_core, m) {
         .def("get_parametric_gate_position", &ParametricQuantumCircuit::get_parametric_gate_position,
              py::arg("index"))
         .def("backprop", &ParametricQuantumCircuit::backprop, py::arg("obs"))
-        .def("backprop_inner_product", &ParametricQuantumCircuit::backprop_inner_product, py::arg("state"));
+        .def("backprop_inner_product", &ParametricQuantumCircuit::backprop_inner_product, py::arg("state"))
+        .def(py::pickle([](const ParametricQuantumCircuit& c) { return ptree::to_json(to_ptree(c)); },
+                        [](const std::string& json) {
+                            return std::unique_ptr<ParametricQuantumCircuit>(
+                                parametric_circuit_from_ptree(ptree::from_json(json)));
+                        }));
 
     py::class_<GradCalculator>(m, "GradCalculator")
         .def(py::init<>())
@@ -445,6 +503,9 @@ PYBIND11_MODULE(_core, m) {
              py::arg("parametric_circuit"), py::arg("observable"), py::arg("angles_of_gates"));
 
     auto mcircuit = m.def_submodule("circuit");
+    mcircuit.def(
+        "from_json", [](const std::string& json) { return circuit_from_ptree(ptree::from_json(json)); },
+        py::return_value_policy::take_ownership, py::arg("json"));
     py::class_<QuantumCircuitOptimizer>(mcircuit, "QuantumCircuitOptimizer")
         .def(py::init([](unsigned mpi_size) {
                  if (mpi_size > 1) throw QulacsError("QuantumCircuitOptimizer: one device only");
