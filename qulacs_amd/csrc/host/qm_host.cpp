@@ -701,6 +701,7 @@ void QuantumGate_CPTP::update_quantum_state(QuantumStateBase* state) {
     const double org = state->get_squared_norm();
     std::unique_ptr<QuantumStateBase> buf(state->allocate_buffer());
     double sum = 0.0;
+    unsigned index = 0;
     for (auto& k : kraus_) {
         buf->load(state);
         k.update_quantum_state(buf.get());
@@ -709,9 +710,11 @@ void QuantumGate_CPTP::update_quantum_state(QuantumStateBase* state) {
         if (r < sum) {
             state->load(buf.get());
             state->normalize(rel);
-            return;
+            break;
         }
+        ++index;
     }
+    if (instrument_) state->set_classical_value(register_, index);
 }
 
 QuantumGate_SingleParameter::QuantumGate_SingleParameter(std::vector<unsigned> t, std::vector<unsigned> p, double angle,
@@ -928,8 +931,73 @@ QuantumGate_Probabilistic* Probabilistic(const std::vector<double>& dist, const 
 }
 QuantumGate_CPTP* CPTP(const std::vector<QuantumGateBase*>& kraus) {
     std::vector<QuantumGateMatrix> k;
-    for (auto* g : kraus) k.emplace_back(g->get_target_index_list(), g->get_matrix());
+    for (auto* g : kraus) {
+        std::vector<unsigned> on;  // a controlled Kraus operator acts as its controlled matrix
+        const ComplexMatrix m = with_controls(g, &on);
+        k.emplace_back(on, m);
+    }
     return new QuantumGate_CPTP(k);
+}
+QuantumGate_CPTP* Instrument(const std::vector<QuantumGateBase*>& kraus, unsigned reg) {
+    QuantumGate_CPTP* g = CPTP(kraus);
+    g->set_instrument(reg);
+    return g;
+}
+// projective measurement of one qubit into a classical register (src/cppsim/gate_factory.cpp:522-537)
+QuantumGate_CPTP* Measurement(unsigned q, unsigned reg) {
+    std::unique_ptr<QuantumGateMatrix> p0(P0(q)), p1(P1(q));
+    return Instrument({p0.get(), p1.get()}, reg);
+}
+QuantumGateMatrix* TOFFOLI(unsigned c1, unsigned c2, unsigned t) {
+    if (c1 == c2) throw QulacsError("TOFFOLI: controls must differ");
+    QuantumGateMatrix* g = new QuantumGateMatrix({t}, m2(0, 1, 1, 0), {}, {}, "DenseMatrix");
+    g->add_control_qubit(c1, 1);
+    g->add_control_qubit(c2, 1);
+    return g;
+}
+QuantumGateMatrix* FREDKIN(unsigned c, unsigned t1, unsigned t2) {
+    std::unique_ptr<QuantumGateMatrix> s(SWAP(t1, t2));
+    QuantumGateMatrix* g = new QuantumGateMatrix(s->get_target_index_list(), s->get_matrix(), {}, {}, "DenseMatrix");
+    g->add_control_qubit(c, 1);
+    return g;
+}
+// Sum of two gates' matrices (src/cppsim/gate_merge.cpp:372-430): a control on the
+// same qubit with the same value in both stays a control of the sum; other
+// controls are folded into the matrices over the union of the qubits.
+QuantumGateMatrix* add(const QuantumGateBase* a, const QuantumGateBase* b) {
+    std::vector<unsigned> kq, kv;
+    const auto &ca = a->get_control_index_list(), &va = a->get_control_value_list();
+    const auto &cb = b->get_control_index_list(), &vb = b->get_control_value_list();
+    for (size_t i = 0; i < ca.size(); ++i)
+        for (size_t j = 0; j < cb.size(); ++j)
+            if (ca[i] == cb[j] && va[i] == vb[j]) {
+                kq.push_back(ca[i]);
+                kv.push_back(va[i]);
+            }
+    auto fold = [&](const QuantumGateBase* g, std::vector<unsigned>* on) {
+        std::vector<unsigned> c, v;
+        const auto &gc = g->get_control_index_list(), &gv = g->get_control_value_list();
+        for (size_t i = 0; i < gc.size(); ++i)
+            if (std::find(kq.begin(), kq.end(), gc[i]) == kq.end()) {
+                c.push_back(gc[i]);
+                v.push_back(gv[i]);
+            }
+        QuantumGateMatrix rest(g->get_target_index_list(), g->get_matrix(), c, v);
+        return with_controls(&rest, on);
+    };
+    std::vector<unsigned> on1, on2;
+    const ComplexMatrix m1 = fold(a, &on1), mb = fold(b, &on2);
+    const auto u = sorted_union({on1, on2});
+    ComplexMatrix sum = embed(m1, on1, u);
+    const ComplexMatrix e2 = embed(mb, on2, u);
+    for (size_t i = 0; i < sum.data.size(); ++i) sum.data[i] += e2.data[i];
+    return new QuantumGateMatrix(u, sum, kq, kv);
+}
+QuantumGateMatrix* add(const std::vector<QuantumGateBase*>& gates) {
+    if (gates.empty()) throw QulacsError("add: empty gate list");
+    std::unique_ptr<QuantumGateMatrix> acc(to_matrix_gate(gates[0]));
+    for (size_t i = 1; i < gates.size(); ++i) acc.reset(add(acc.get(), gates[i]));
+    return acc.release();
 }
 QuantumGateMatrix* to_matrix_gate(const QuantumGateBase* g) {
     return new QuantumGateMatrix(g->get_target_index_list(), g->get_matrix(), g->get_control_index_list(),

@@ -290,10 +290,21 @@ class QuantumGate_CPTP : public QuantumGateBase {
     ComplexMatrix superoperator() const;
     void set_seed(uint64_t seed) { random_.set_seed(seed); }
     const std::vector<QuantumGateMatrix>& get_kraus_list() const { return kraus_; }
+    // Instrument: on a state vector, the index of the chosen Kraus operator (the
+    // operator count when none was chosen) is written to this classical register
+    // address (src/cppsim/gate_general.hpp:380-470).
+    void set_instrument(unsigned classical_register_address) {
+        instrument_ = true;
+        register_ = classical_register_address;
+    }
+    bool is_instrument() const { return instrument_; }
+    unsigned get_classical_register_address() const { return register_; }
 
    private:
     std::vector<QuantumGateMatrix> kraus_;
     Random random_;
+    bool instrument_ = false;
+    unsigned register_ = 0;
 };
 
 // Parametric rotation gates (src/vqcsim/parametric_gate.hpp:31-212).
@@ -360,6 +371,12 @@ QuantumGate_Probabilistic* IndependentXZNoise(unsigned q, double p);
 QuantumGate_CPTP* AmplitudeDampingNoise(unsigned q, double p);
 QuantumGate_Probabilistic* Probabilistic(const std::vector<double>& dist, const std::vector<QuantumGateBase*>& gates);
 QuantumGate_CPTP* CPTP(const std::vector<QuantumGateBase*>& kraus);
+QuantumGate_CPTP* Instrument(const std::vector<QuantumGateBase*>& kraus, unsigned classical_register_address);
+QuantumGate_CPTP* Measurement(unsigned q, unsigned classical_register_address);
+QuantumGateMatrix* TOFFOLI(unsigned control1, unsigned control2, unsigned target);
+QuantumGateMatrix* FREDKIN(unsigned control, unsigned target1, unsigned target2);
+QuantumGateMatrix* add(const QuantumGateBase* a, const QuantumGateBase* b);
+QuantumGateMatrix* add(const std::vector<QuantumGateBase*>& gates);
 QuantumGateMatrix* to_matrix_gate(const QuantumGateBase* g);
 QuantumGateMatrix* merge(const QuantumGateBase* first, const QuantumGateBase* second);
 QuantumGate_SingleParameter* ParametricRX(unsigned q, double angle);

@@ -415,7 +415,8 @@ PTree to_ptree(const QuantumGateBase& g) {
         PTree gates;
         for (const QuantumGateMatrix& k : c->get_kraus_list()) gates.push_back(to_ptree(k));
         pt.put("gate_list", std::move(gates));
-        pt.put("is_instrument", "false");
+        pt.put("is_instrument", c->is_instrument() ? "true" : "false");
+        if (c->is_instrument()) pt.put("classical_register_address", u2s(c->get_classical_register_address()));
         return pt;
     }
     throw QulacsError("to_json: gate " + g.get_name() + " has no JSON form");
@@ -473,15 +474,17 @@ QuantumGateBase* gate_from_ptree(const PTree& pt) {
         return gate::PauliRotation(p.get_index_list(), p.get_pauli_id_list(), pt.get_double("angle"));
     }
     if (name == "ProbabilisticGate" || name == "CPTPMapGate") {
-        if (pt.get_bool("is_instrument"))
-            throw QulacsError("JSON: instruments (gates writing a classical register) are not supported");
+        const bool instrument = pt.get_bool("is_instrument");
+        if (instrument && name == "ProbabilisticGate")
+            throw QulacsError("JSON: probabilistic instruments are not supported");
         std::vector<std::unique_ptr<QuantumGateBase>> owned;
         std::vector<QuantumGateBase*> gates;
         for (const auto& kv : pt.child("gate_list").children) {
             owned.emplace_back(gate_from_ptree(kv.second));
             gates.push_back(owned.back().get());
         }
-        if (name == "CPTPMapGate") return gate::CPTP(gates);
+        if (name == "CPTPMapGate")
+            return instrument ? gate::Instrument(gates, pt.get_uint("classical_register_address")) : gate::CPTP(gates);
         std::vector<double> dist;
         for (const auto& kv : pt.child("distribution").children) {
             PTree leaf;

@@ -275,7 +275,8 @@ PYBIND11_MODULE(_core, m) {
                  return out;
              },
              py::return_value_policy::take_ownership)
-        .def("get_superoperator", [](const QuantumGate_CPTP& g) { return from_matrix(g.superoperator()); });
+        .def("get_superoperator", [](const QuantumGate_CPTP& g) { return from_matrix(g.superoperator()); })
+        .def("is_instrument", &QuantumGate_CPTP::is_instrument);
     py::class_<QuantumGate_SingleParameter, QuantumGateBase>(m, "QuantumGate_SingleParameter")
         .def("get_parameter_value", &QuantumGate_SingleParameter::get_parameter_value)
         .def("set_parameter_value", &QuantumGate_SingleParameter::set_parameter_value, py::arg("value"));
@@ -372,6 +373,29 @@ PYBIND11_MODULE(_core, m) {
     mg.def(
         "from_json", [](const std::string& json) { return gate_from_ptree(ptree::from_json(json)); }, own_,
         py::arg("json"));
+    mg.def("TOFFOLI", &gate::TOFFOLI, own_, py::arg("control1"), py::arg("control2"), py::arg("target"));
+    mg.def("FREDKIN", &gate::FREDKIN, own_, py::arg("control"), py::arg("target1"), py::arg("target2"));
+    mg.def("add", py::overload_cast<const QuantumGateBase*, const QuantumGateBase*>(&gate::add), own_,
+           py::arg("gate1"), py::arg("gate2"));
+    mg.def("add", py::overload_cast<const std::vector<QuantumGateBase*>&>(&gate::add), own_, py::arg("gate_list"));
+    mg.def("Instrument", &gate::Instrument, own_, py::arg("kraus_list"), py::arg("register"));
+    mg.def("Measurement", &gate::Measurement, own_, py::arg("index"), py::arg("register"));
+    mg.def(
+        "Measurement",
+        [](unsigned q, unsigned reg, uint64_t seed) {
+            QuantumGate_CPTP* g = gate::Measurement(q, reg);
+            g->set_seed(seed);
+            return g;
+        },
+        own_, py::arg("index"), py::arg("register"), py::arg("seed"));
+    mg.def(
+        "SparseMatrix",
+        [](const std::vector<unsigned>& targets, const py::object& sparse) {
+            // a scipy.sparse matrix, applied through its dense form
+            const carray dense = sparse.attr("toarray")().cast<carray>();
+            return gate::DenseMatrix(targets, to_matrix(dense));
+        },
+        own_, py::arg("index_list"), py::arg("matrix"));
 
     // ---------------------------------------------------------------- circuits
     py::class_<QuantumCircuit>(m, "QuantumCircuit")

@@ -70,3 +70,23 @@ def test_circuit_from_json_acts_identically():
     c.update_quantum_state(a)
     q.circuit.from_json(c.to_json()).update_quantum_state(b)
     assert np.array_equal(a.get_vector(), b.get_vector())
+
+
+def test_measurement_writes_the_classical_register():
+    """Measurement(q, reg) = Instrument([P0(q), P1(q)], reg) (src/cppsim/gate_factory.cpp:522-537):
+    the outcome index goes to the register and the state collapses."""
+    s = q.QuantumState(3)
+    s.set_computational_basis(0b010)
+    gate.Measurement(1, 0).update_quantum_state(s)
+    gate.Measurement(0, 2).update_quantum_state(s)
+    assert s.get_classical_value(0) == 1 and s.get_classical_value(2) == 0
+    s = q.QuantumState(2)
+    gate.H(0).update_quantum_state(s)
+    counts = [0, 0]
+    for seed in range(40):
+        t = s.copy()
+        gate.Measurement(0, 0, seed).update_quantum_state(t)
+        k = t.get_classical_value(0)
+        counts[k] += 1
+        assert abs(t.get_zero_probability(0) - (1.0 - k)) < 1e-12  # collapsed onto the outcome
+    assert counts[0] > 5 and counts[1] > 5

@@ -177,3 +177,47 @@ def test_operators_round_trip():
     bad.add_operator(1j, "X 0")
     with pytest.raises(RuntimeError):
         observable.from_json(bad.to_json())
+
+
+def test_toffoli_fredkin_add_sparse():
+    """TOFFOLI / FREDKIN as controlled matrix gates (python/cppsim_wrapper.cpp:910-933),
+    add as the sum of the gates' matrices over the union of their qubits with shared
+    controls kept (src/cppsim/gate_merge.cpp:372-430), SparseMatrix through its dense form."""
+    import scipy.sparse as sp
+
+    t = gate.TOFFOLI(0, 1, 2)
+    assert t.get_target_index_list() == [2] and t.get_control_index_list() == [0, 1]
+    assert t.get_control_value_list() == [1, 1] and np.array_equal(t.get_matrix(), [[0, 1], [1, 0]])
+    f = gate.FREDKIN(0, 1, 2)
+    assert f.get_target_index_list() == [1, 2] and f.get_control_index_list() == [0]
+    assert np.array_equal(f.get_matrix(), gate.SWAP(1, 2).get_matrix())
+    X = np.array([[0, 1], [1, 0]], dtype=complex)
+    Y = np.array([[0, -1j], [1j, 0]])
+    Z = np.diag([1.0 + 0j, -1.0])
+    I2 = np.eye(2)
+    assert np.array_equal(gate.add(gate.X(0), gate.Y(0)).get_matrix(), X + Y)
+    s = gate.add(gate.X(0), gate.Z(1))
+    assert s.get_target_index_list() == [0, 1]
+    assert np.array_equal(s.get_matrix(), np.kron(I2, X) + np.kron(Z, I2))  # bit 0 <-> qubit 0
+    a, b = gate.X(2), gate.Z(2)
+    a.add_control_qubit(0, 1)
+    b.add_control_qubit(0, 1)
+    c = gate.add(a, b)
+    assert c.get_control_index_list() == [0] and np.array_equal(c.get_matrix(), X + Z)
+    assert np.array_equal(gate.add([gate.X(0), gate.Y(0), gate.Z(0)]).get_matrix(), X + Y + Z)
+    m = np.array([[0.5, 0.25j], [0, 1.0]])
+    g = gate.SparseMatrix([1], sp.csc_matrix(m))
+    assert g.get_target_index_list() == [1] and np.array_equal(g.get_matrix(), m)
+    for g in (t, f, c):
+        g2 = gate.from_json(g.to_json())
+        assert g2.get_control_index_list() == g.get_control_index_list()
+        assert np.array_equal(g2.get_matrix(), g.get_matrix())
+
+
+def test_instrument_round_trip():
+    m = gate.Measurement(1, 3)
+    d = json.loads(m.to_json())
+    assert d["name"] == "CPTPMapGate" and d["is_instrument"] == "true" and d["classical_register_address"] == "3"
+    m2 = gate.from_json(m.to_json())
+    assert m2.is_instrument() and len(m2.get_gate_list()) == 2
+    assert not gate.AmplitudeDampingNoise(0, 0.1).is_instrument()
