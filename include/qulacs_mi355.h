@@ -241,6 +241,19 @@ QM_EXPORT void dm_normalize_host(double squared_norm, void* state, ITYPE dim, vo
 /* DM sampling over |rho_ii| (src/cppsim/state_dm.hpp:383-408), same contract as qm_sampling_host. */
 QM_EXPORT void dm_sampling_host(const double* uniforms, UINT count, ITYPE* out, void* state, ITYPE dim, void* stream, unsigned int device_number);
 
+/* ------------------------------------------------------------------ state utilities
+ * No GPU counterpart in the reference (its QuantumStateGpu has none of these);
+ * the csim names and arguments of src/csim/stat_ops.cpp:111-160 and
+ * stat_ops_dm.cpp:193-298 with the device / stream pair appended.  dst is a
+ * separate, already allocated device buffer of the output size. */
+QM_EXPORT void state_tensor_product_host(const void* state_left, ITYPE dim_left, const void* state_right, ITYPE dim_right, void* state_dst, void* stream, unsigned int device_number); /* stat_ops.cpp:111-121 */
+QM_EXPORT void state_permutate_qubit_host(const UINT* qubit_order, const void* state_src, void* state_dst, UINT qubit_count, ITYPE dim, void* stream, unsigned int device_number); /* stat_ops.cpp:122-135 */
+QM_EXPORT void state_drop_qubits_host(const UINT* target, const UINT* projection, UINT target_count, const void* state_src, void* state_dst, ITYPE dim, void* stream, unsigned int device_number); /* stat_ops.cpp:136-160 */
+QM_EXPORT void dm_state_tensor_product_host(const void* state_left, ITYPE dim_left, const void* state_right, ITYPE dim_right, void* state_dst, void* stream, unsigned int device_number); /* stat_ops_dm.cpp:193-212 */
+QM_EXPORT void dm_state_permutate_qubit_host(const UINT* qubit_order, const void* state_src, void* state_dst, UINT qubit_count, ITYPE dim, void* stream, unsigned int device_number); /* stat_ops_dm.cpp:213-232 */
+QM_EXPORT void dm_state_partial_trace_from_density_matrix_host(const UINT* target, UINT target_count, const void* state_src, void* state_dst, ITYPE dim, void* stream, unsigned int device_number); /* stat_ops_dm.cpp:233-265 */
+QM_EXPORT void dm_state_partial_trace_from_state_vector_host(const UINT* target, UINT target_count, const void* state_src, void* state_dst, ITYPE dim, void* stream, unsigned int device_number); /* stat_ops_dm.cpp:266-298 */
+
 /* ------------------------------------------------------------------ batched circuits
  * Many small circuits in one launch.  Replaces, for each circuit of a batch,
  * QuantumCircuit::update_quantum_state on |0...0> (src/cppsim/circuit.cpp:46-57)

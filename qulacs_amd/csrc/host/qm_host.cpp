@@ -303,6 +303,99 @@ CPPCTYPE inner_product(const QuantumStateMI355* bra, const QuantumStateMI355* ke
     return cc(z);
 }
 
+namespace stateops {
+namespace {
+void wait_for(const QuantumStateBase* s) { qm_synchronize(s->get_cuda_stream(), s->device_number); }
+}  // namespace
+QuantumStateMI355* tensor_product(const QuantumStateMI355* l, const QuantumStateMI355* r) {
+    std::unique_ptr<QuantumStateMI355> out(new QuantumStateMI355(l->qubit_count + r->qubit_count, l->device_number));
+    wait_for(l);
+    wait_for(r);
+    state_tensor_product_host(l->data(), l->dim, r->data(), r->dim, out->data(), out->get_cuda_stream(),
+                              out->device_number);
+    check_abi("tensor_product");
+    return out.release();
+}
+DensityMatrixMI355* tensor_product(const DensityMatrixMI355* l, const DensityMatrixMI355* r) {
+    std::unique_ptr<DensityMatrixMI355> out(new DensityMatrixMI355(l->qubit_count + r->qubit_count, l->device_number));
+    wait_for(l);
+    wait_for(r);
+    dm_state_tensor_product_host(l->data(), l->dim, r->data(), r->dim, out->data(), out->get_cuda_stream(),
+                                 out->device_number);
+    check_abi("tensor_product");
+    return out.release();
+}
+QuantumStateMI355* permutate_qubit(const QuantumStateMI355* s, const std::vector<unsigned>& order) {
+    if (order.size() != s->qubit_count) throw QulacsError("permutate_qubit: invalid qubit count");
+    std::unique_ptr<QuantumStateMI355> out(new QuantumStateMI355(s->qubit_count, s->device_number));
+    wait_for(s);
+    state_permutate_qubit_host(order.data(), s->data(), out->data(), s->qubit_count, s->dim, out->get_cuda_stream(),
+                               out->device_number);
+    check_abi("permutate_qubit");
+    return out.release();
+}
+DensityMatrixMI355* permutate_qubit(const DensityMatrixMI355* s, const std::vector<unsigned>& order) {
+    if (order.size() != s->qubit_count) throw QulacsError("permutate_qubit: invalid qubit count");
+    std::unique_ptr<DensityMatrixMI355> out(new DensityMatrixMI355(s->qubit_count, s->device_number));
+    wait_for(s);
+    dm_state_permutate_qubit_host(order.data(), s->data(), out->data(), s->qubit_count, s->dim,
+                                  out->get_cuda_stream(), out->device_number);
+    check_abi("permutate_qubit");
+    return out.release();
+}
+QuantumStateMI355* drop_qubit(const QuantumStateMI355* s, const std::vector<unsigned>& target,
+                              const std::vector<unsigned>& projection) {
+    if (s->qubit_count <= target.size() || target.size() != projection.size())
+        throw QulacsError("drop_qubit: invalid qubit count");
+    std::unique_ptr<QuantumStateMI355> out(
+        new QuantumStateMI355(s->qubit_count - (unsigned)target.size(), s->device_number));
+    wait_for(s);
+    state_drop_qubits_host(target.data(), projection.data(), (UINT)target.size(), s->data(), out->data(), s->dim,
+                           out->get_cuda_stream(), out->device_number);
+    check_abi("drop_qubit");
+    return out.release();
+}
+DensityMatrixMI355* partial_trace(const QuantumStateBase* s, const std::vector<unsigned>& target) {
+    if (s->qubit_count <= target.size()) throw QulacsError("partial_trace: invalid qubit count");
+    std::unique_ptr<DensityMatrixMI355> out(
+        new DensityMatrixMI355(s->qubit_count - (unsigned)target.size(), s->device_number));
+    wait_for(s);
+    if (s->is_state_vector())
+        dm_state_partial_trace_from_state_vector_host(target.data(), (UINT)target.size(), s->data(), out->data(),
+                                                      s->dim, out->get_cuda_stream(), out->device_number);
+    else
+        dm_state_partial_trace_from_density_matrix_host(target.data(), (UINT)target.size(), s->data(), out->data(),
+                                                        s->dim, out->get_cuda_stream(), out->device_number);
+    check_abi("partial_trace");
+    return out.release();
+}
+QuantumStateMI355* make_superposition(CPPCTYPE c1, const QuantumStateMI355* s1, CPPCTYPE c2,
+                                      const QuantumStateMI355* s2) {
+    if (s1->qubit_count != s2->qubit_count) throw QulacsError("make_superposition: invalid qubit count");
+    std::unique_ptr<QuantumStateMI355> out(new QuantumStateMI355(s1->qubit_count, s1->device_number));
+    out->multiply_coef(0.0);  // the zero vector
+    out->add_state_with_coef(c1, s1);
+    out->add_state_with_coef(c2, s2);
+    return out.release();
+}
+DensityMatrixMI355* make_mixture(CPPCTYPE p1, const QuantumStateBase* s1, CPPCTYPE p2, const QuantumStateBase* s2) {
+    if (s1->qubit_count != s2->qubit_count) throw QulacsError("make_mixture: invalid qubit count");
+    const unsigned n = s1->qubit_count;
+    auto as_density = [&](const QuantumStateBase* s) {
+        std::unique_ptr<DensityMatrixMI355> d(new DensityMatrixMI355(n, s->device_number));
+        if (s->is_state_vector()) d->load_pure(static_cast<const QuantumStateMI355*>(s));
+        else d->load(s);
+        return d;
+    };
+    std::unique_ptr<DensityMatrixMI355> d1 = as_density(s1), d2 = as_density(s2);
+    std::unique_ptr<DensityMatrixMI355> out(new DensityMatrixMI355(n, s1->device_number));
+    out->multiply_coef(0.0);
+    out->add_state_with_coef(p1, d1.get());
+    out->add_state_with_coef(p2, d2.get());
+    return out.release();
+}
+}  // namespace stateops
+
 // ------------------------------------------------------------------ density matrix
 DensityMatrixMI355::DensityMatrixMI355(unsigned n, unsigned device) : QuantumStateBase(n, 1ULL << n, device) {
     if (n > 20) throw QulacsError("DensityMatrix: at most 20 qubits");

@@ -176,6 +176,21 @@ class DensityMatrixMI355 : public QuantumStateBase {
 
 CPPCTYPE inner_product(const QuantumStateMI355* bra, const QuantumStateMI355* ket);
 
+// New states built from existing ones on the device (src/cppsim/state.cpp:32-100,
+// src/cppsim/state_dm.cpp:9-90).
+namespace stateops {
+QuantumStateMI355* tensor_product(const QuantumStateMI355* left, const QuantumStateMI355* right);
+DensityMatrixMI355* tensor_product(const DensityMatrixMI355* left, const DensityMatrixMI355* right);
+QuantumStateMI355* permutate_qubit(const QuantumStateMI355* s, const std::vector<unsigned>& order);
+DensityMatrixMI355* permutate_qubit(const DensityMatrixMI355* s, const std::vector<unsigned>& order);
+QuantumStateMI355* drop_qubit(const QuantumStateMI355* s, const std::vector<unsigned>& target,
+                              const std::vector<unsigned>& projection);
+DensityMatrixMI355* partial_trace(const QuantumStateBase* s, const std::vector<unsigned>& target);
+QuantumStateMI355* make_superposition(CPPCTYPE c1, const QuantumStateMI355* s1, CPPCTYPE c2,
+                                      const QuantumStateMI355* s2);
+DensityMatrixMI355* make_mixture(CPPCTYPE p1, const QuantumStateBase* s1, CPPCTYPE p2, const QuantumStateBase* s2);
+}  // namespace stateops
+
 // ------------------------------------------------------------------ gates
 class QuantumGateBase {
    public:

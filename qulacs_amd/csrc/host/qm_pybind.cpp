@@ -233,6 +233,23 @@ PYBIND11_MODULE(_core, m) {
     mstate.def(
         "from_json", [](const std::string& json) { return state_from_ptree(ptree::from_json(json)); },
         py::return_value_policy::take_ownership, py::arg("json"));
+    const auto own_s = py::return_value_policy::take_ownership;
+    mstate.def("tensor_product", py::overload_cast<const QuantumStateMI355*, const QuantumStateMI355*>(
+                                     &stateops::tensor_product), own_s, py::arg("state_left"), py::arg("state_right"));
+    mstate.def("tensor_product", py::overload_cast<const DensityMatrixMI355*, const DensityMatrixMI355*>(
+                                     &stateops::tensor_product), own_s, py::arg("state_left"), py::arg("state_right"));
+    mstate.def("permutate_qubit",
+               py::overload_cast<const QuantumStateMI355*, const std::vector<unsigned>&>(&stateops::permutate_qubit),
+               own_s, py::arg("state"), py::arg("qubit_order"));
+    mstate.def("permutate_qubit",
+               py::overload_cast<const DensityMatrixMI355*, const std::vector<unsigned>&>(&stateops::permutate_qubit),
+               own_s, py::arg("state"), py::arg("qubit_order"));
+    mstate.def("drop_qubit", &stateops::drop_qubit, own_s, py::arg("state"), py::arg("target"), py::arg("projection"));
+    mstate.def("partial_trace", &stateops::partial_trace, own_s, py::arg("state"), py::arg("target_traceout"));
+    mstate.def("make_superposition", &stateops::make_superposition, own_s, py::arg("coef1"), py::arg("state1"),
+               py::arg("coef2"), py::arg("state2"));
+    mstate.def("make_mixture", &stateops::make_mixture, own_s, py::arg("prob1"), py::arg("state1"), py::arg("prob2"),
+               py::arg("state2"));
 
     // ---------------------------------------------------------------- gates
     py::class_<QuantumGateBase>(m, "QuantumGateBase")
