@@ -3,6 +3,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <limits>
+#include <memory>
 #include <numeric>
 #include <sstream>
 
@@ -1196,10 +1201,10 @@ PauliOperator::PauliOperator(const std::string& s, CPPCTYPE coef) : coef_(coef) 
         if (id != 0) add_single_Pauli(idx, id);
     }
 }
+// src/cppsim/pauli_operator.cpp:87-96: identity entries stay in the index list
 PauliOperator::PauliOperator(const std::vector<unsigned>& t, const std::vector<unsigned>& p, CPPCTYPE coef) : coef_(coef) {
     if (t.size() != p.size()) throw QulacsError("one Pauli id per target");
-    for (size_t i = 0; i < t.size(); ++i)
-        if (p[i] != 0) add_single_Pauli(t[i], p[i]);
+    for (size_t i = 0; i < t.size(); ++i) add_single_Pauli(t[i], p[i]);
 }
 void PauliOperator::add_single_Pauli(unsigned q, unsigned p) {
     if (p > 3) throw QulacsError("Pauli id must be 0..3");
@@ -1209,8 +1214,53 @@ void PauliOperator::add_single_Pauli(unsigned q, unsigned p) {
 std::string PauliOperator::get_pauli_string() const {
     std::ostringstream os;
     const char* names = "IXYZ";
-    for (size_t i = 0; i < targets_.size(); ++i) os << (i ? " " : "") << names[paulis_[i]] << " " << targets_[i];
+    bool first = true;
+    for (size_t i = 0; i < targets_.size(); ++i) {
+        if (paulis_[i] == 0) continue;  // identities are not printed (src/cppsim/pauli_operator.cpp:263-285)
+        os << (first ? "" : " ") << names[paulis_[i]] << " " << targets_[i];
+        first = false;
+    }
     return os.str();
+}
+void PauliOperator::xz_bits(std::vector<uint8_t>* x, std::vector<uint8_t>* z) const {
+    unsigned n = 0;
+    for (unsigned q : targets_) n = std::max(n, q + 1);
+    x->assign(n, 0);
+    z->assign(n, 0);
+    for (size_t i = 0; i < targets_.size(); ++i) {
+        if (paulis_[i] == 1 || paulis_[i] == 2) (*x)[targets_[i]] = 1;
+        if (paulis_[i] == 2 || paulis_[i] == 3) (*z)[targets_[i]] = 1;
+    }
+}
+bool PauliOperator::same_pauli_string(const PauliOperator& o) const {
+    std::vector<uint8_t> x0, z0, x1, z1;
+    xz_bits(&x0, &z0);
+    o.xz_bits(&x1, &z1);
+    const size_t n = std::max(x0.size(), x1.size());
+    x0.resize(n), z0.resize(n), x1.resize(n), z1.resize(n);
+    return x0 == x1 && z0 == z1;
+}
+// Single-qubit products: XY = iZ, YZ = iX, ZX = iY and the reversed orders with -i.
+PauliOperator PauliOperator::operator*(const PauliOperator& t) const {
+    std::vector<uint8_t> x0, z0, x1, z1;
+    xz_bits(&x0, &z0);
+    t.xz_bits(&x1, &z1);
+    const size_t n = std::max(x0.size(), x1.size());
+    x0.resize(n), z0.resize(n), x1.resize(n), z1.resize(n);
+    const CPPCTYPE I(0.0, 1.0);
+    CPPCTYPE phase = 1.0;
+    auto id = [](uint8_t x, uint8_t z) -> unsigned { return x ? (z ? 2u : 1u) : (z ? 3u : 0u); };
+    std::vector<unsigned> tl, pl;
+    for (size_t q = 0; q < n; ++q) {
+        const unsigned a = id(x0[q], z0[q]), b = id(x1[q], z1[q]);
+        if (a != 0 && b != 0 && a != b) phase *= ((b == a % 3 + 1) ? I : -I);  // X->Y->Z->X is +i
+        const unsigned c = id(x0[q] ^ x1[q], z0[q] ^ z1[q]);
+        if (c != 0) {
+            tl.push_back((unsigned)q);
+            pl.push_back(c);
+        }
+    }
+    return PauliOperator(tl, pl, coef_ * t.get_coef() * phase);
 }
 // src/csim/utility.cpp:12-43
 void PauliOperator::masks(ITYPE* bf, ITYPE* pf, unsigned* rot) const {
@@ -1254,6 +1304,10 @@ void GeneralQuantumOperator::add_operator(const PauliOperator& p) {
     for (unsigned q : p.get_index_list())
         if (q >= qubit_count_) throw QulacsError("add_operator: Pauli operator acts beyond qubit_count");
     terms_.push_back(p);
+}
+void Observable::add_operator(const PauliOperator& p) {
+    if (p.get_coef().imag() != 0.0) throw QulacsError("Observable: PauliOperator must be Hermitian (real coefficient)");
+    GeneralQuantumOperator::add_operator(p);
 }
 void GeneralQuantumOperator::add_operator(CPPCTYPE coef, const std::string& s) { add_operator(PauliOperator(s, coef)); }
 void GeneralQuantumOperator::add_operator(const std::vector<unsigned>& t, const std::vector<unsigned>& p, CPPCTYPE coef) {
@@ -1307,6 +1361,515 @@ void GeneralQuantumOperator::apply_to_state(QuantumStateMI355* work, const Quant
         dst->add_state_with_coef(t.get_coef(), work);
     }
 }
+void GeneralQuantumOperator::apply_to_state(QuantumStateMI355* state, QuantumStateMI355* dst) const {
+    if (state->qubit_count != dst->qubit_count) throw QulacsError("apply_to_state: qubit counts differ");
+    dst->multiply_coef(0.0);
+    for (const auto& t : terms_) {
+        const auto& tl = t.get_index_list();
+        const auto& pl = t.get_pauli_id_list();
+        if (!tl.empty()) {
+            multi_qubit_Pauli_gate_partial_list_host(tl.data(), pl.data(), (UINT)tl.size(), state->data(), state->dim,
+                                                     state->get_cuda_stream(), state->device_number);
+            check_abi("apply_to_state");
+        }
+        dst->add_state_with_coef(t.get_coef(), state);
+        if (!tl.empty()) {  // every Pauli string is its own inverse
+            multi_qubit_Pauli_gate_partial_list_host(tl.data(), pl.data(), (UINT)tl.size(), state->data(), state->dim,
+                                                     state->get_cuda_stream(), state->device_number);
+            check_abi("apply_to_state");
+        }
+    }
+}
+
+GeneralQuantumOperator GeneralQuantumOperator::copy_base() const {
+    GeneralQuantumOperator c(qubit_count_);
+    c.terms_ = terms_;
+    return c;
+}
+GeneralQuantumOperator GeneralQuantumOperator::get_dagger() const {
+    GeneralQuantumOperator d(qubit_count_);
+    for (const auto& t : terms_) d.add_operator(std::conj(t.get_coef()), t.get_pauli_string());
+    return d;
+}
+std::string GeneralQuantumOperator::to_string() const {
+    std::ostringstream os;
+    for (size_t i = 0; i < terms_.size(); ++i)
+        os << terms_[i].get_coef() << " " << terms_[i].get_pauli_string() << (i + 1 < terms_.size() ? " + " : "");
+    return os.str();
+}
+std::string Observable::to_string() const {
+    std::ostringstream os;
+    for (size_t i = 0; i < terms_.size(); ++i)
+        os << terms_[i].get_coef().real() << " " << terms_[i].get_pauli_string()
+           << (i + 1 < terms_.size() ? " + " : "");
+    return os.str();
+}
+// Draw order per term: one int32 per qubit (Pauli id = value % 4), then one uniform.
+void GeneralQuantumOperator::add_random_operator(unsigned operator_count) {
+    for (unsigned k = 0; k < operator_count; ++k) {
+        std::vector<unsigned> tl(qubit_count_), pl(qubit_count_);
+        for (unsigned q = 0; q < qubit_count_; ++q) {
+            tl[q] = q;
+            pl[q] = random_.int32() % 4;
+        }
+        const CPPCTYPE coef = random_.uniform() * 2 - 1.0;
+        add_operator(PauliOperator(tl, pl, coef));
+    }
+}
+void GeneralQuantumOperator::add_random_operator(unsigned operator_count, unsigned seed) {
+    random_.set_seed(seed);
+    add_random_operator(operator_count);
+}
+CPPCTYPE GeneralQuantumOperator::calculate_default_mu() const {
+    double mu = 0.0;
+    for (const auto& t : terms_) mu += std::abs(t.get_coef().real());
+    return mu;
+}
+
+// terms_[i] += sign * (every term of `terms` with the same Pauli string); then each term
+// of `terms` that matches nothing in the list (as the list grows) is appended with
+// coefficient sign * coef (src/cppsim/general_quantum_operator.cpp:557-608 and 650-703).
+void GeneralQuantumOperator::merge_terms(const std::vector<PauliOperator>& in, double sign) {
+    const std::vector<PauliOperator> terms(in);  // `in` may be this operator's own list
+    for (auto& t : terms_)
+        for (const auto& a : terms)
+            if (t.same_pauli_string(a)) t.change_coef(t.get_coef() + sign * a.get_coef());
+    for (const auto& a : terms) {
+        bool found = false;
+        for (const auto& t : terms_)
+            if (t.same_pauli_string(a)) {
+                found = true;
+                break;
+            }
+        if (!found) {
+            PauliOperator c(a);
+            c.change_coef(sign * a.get_coef());
+            add_operator(c);
+        }
+    }
+}
+GeneralQuantumOperator& GeneralQuantumOperator::operator+=(const GeneralQuantumOperator& t) {
+    merge_terms(t.terms_, 1.0);
+    return *this;
+}
+GeneralQuantumOperator& GeneralQuantumOperator::operator+=(const PauliOperator& t) {
+    merge_terms({t}, 1.0);
+    return *this;
+}
+GeneralQuantumOperator& GeneralQuantumOperator::operator-=(const GeneralQuantumOperator& t) {
+    merge_terms(t.terms_, -1.0);
+    return *this;
+}
+GeneralQuantumOperator& GeneralQuantumOperator::operator-=(const PauliOperator& t) {
+    merge_terms({t}, -1.0);
+    return *this;
+}
+// Products term by term, each folded in with += (src/cppsim/general_quantum_operator.cpp:750-805).
+GeneralQuantumOperator& GeneralQuantumOperator::operator*=(const GeneralQuantumOperator& target) {
+    const std::vector<PauliOperator> left(terms_), right(target.terms_);
+    terms_.clear();
+    for (const auto& a : left)
+        for (const auto& b : right) *this += a * b;
+    return *this;
+}
+GeneralQuantumOperator& GeneralQuantumOperator::operator*=(const PauliOperator& target) {
+    const std::vector<PauliOperator> left(terms_);
+    terms_.clear();
+    for (const auto& a : left) *this += a * target;
+    return *this;
+}
+GeneralQuantumOperator& GeneralQuantumOperator::operator*=(CPPCTYPE c) {
+    for (auto& t : terms_) t *= c;
+    return *this;
+}
+
+// ------------------------------------------------------------------ eigenvalue solvers
+namespace {
+
+using CMat = std::vector<std::vector<CPPCTYPE>>;
+
+// Eigenvalues of a complex upper Hessenberg matrix by the shifted QR iteration
+// (Givens rotations, Wilkinson shift, deflation on negligible subdiagonals).
+std::vector<CPPCTYPE> hessenberg_eigenvalues(CMat h) {
+    const int n = (int)h.size();
+    std::vector<CPPCTYPE> ev(n);
+    const double eps = std::numeric_limits<double>::epsilon();
+    double hnorm = 0.0;
+    for (const auto& row : h)
+        for (const auto& v : row) hnorm = std::max(hnorm, std::abs(v));
+    int hi = n - 1, iter = 0;
+    while (hi >= 0) {
+        if (hi == 0) {
+            ev[0] = h[0][0];
+            break;
+        }
+        int l = hi;
+        for (; l > 0; --l)
+            if (std::abs(h[l][l - 1]) <=
+                eps * std::max(std::abs(h[l - 1][l - 1]) + std::abs(h[l][l]), eps * hnorm)) {
+                h[l][l - 1] = 0.0;
+                break;
+            }
+        if (l == hi) {
+            ev[hi] = h[hi][hi];
+            --hi;
+            iter = 0;
+            continue;
+        }
+        if (++iter > 1000) throw QulacsError("eigenvalue iteration did not converge");
+        // Wilkinson shift: the eigenvalue of the trailing 2x2 block nearer h[hi][hi]
+        const CPPCTYPE a = h[hi - 1][hi - 1], b = h[hi - 1][hi], c = h[hi][hi - 1], d = h[hi][hi];
+        const CPPCTYPE half = 0.5 * (a - d), disc = std::sqrt(half * half + b * c);
+        CPPCTYPE mu = d - b * c / (half + (std::abs(half + disc) >= std::abs(half - disc) ? disc : -disc));
+        if (!std::isfinite(mu.real()) || !std::isfinite(mu.imag())) mu = d;
+        if (iter % 11 == 10) mu = d + std::abs(c);  // exceptional shift
+        for (int k = l; k <= hi; ++k) h[k][k] -= mu;
+        std::vector<CPPCTYPE> gx(hi + 1), gy(hi + 1);
+        std::vector<double> gr(hi + 1);
+        for (int k = l; k < hi; ++k) {  // QR: rotate rows k, k+1 so that h[k+1][k] = 0
+            const CPPCTYPE x = h[k][k], y = h[k + 1][k];
+            const double r = std::sqrt(std::norm(x) + std::norm(y));
+            gx[k] = x, gy[k] = y, gr[k] = r;
+            if (r == 0.0) continue;
+            for (int j = k; j <= hi; ++j) {
+                const CPPCTYPE u = h[k][j], v = h[k + 1][j];
+                h[k][j] = (std::conj(x) * u + std::conj(y) * v) / r;
+                h[k + 1][j] = (-y * u + x * v) / r;
+            }
+        }
+        for (int k = l; k < hi; ++k) {  // RQ: the same rotations from the right
+            const double r = gr[k];
+            if (r == 0.0) continue;
+            const CPPCTYPE x = gx[k], y = gy[k];
+            for (int i = l; i <= std::min(k + 2, hi); ++i) {
+                const CPPCTYPE u = h[i][k], v = h[i][k + 1];
+                h[i][k] = (u * x + v * y) / r;
+                h[i][k + 1] = (-u * std::conj(y) + v * std::conj(x)) / r;
+            }
+        }
+        for (int k = l; k <= hi; ++k) h[k][k] += mu;
+    }
+    return ev;
+}
+
+// Unit eigenvector of `a` for the eigenvalue `lambda` by inverse iteration with a
+// partially pivoted LU of (a - lambda I).
+std::vector<CPPCTYPE> eigenvector_for(const CMat& a, CPPCTYPE lambda) {
+    const int n = (int)a.size();
+    double scale = 0.0;
+    for (const auto& row : a)
+        for (const auto& v : row) scale = std::max(scale, std::abs(v));
+    const double tiny = std::max(scale, 1.0) * 1e-14;
+    CMat m = a;
+    for (int i = 0; i < n; ++i) m[i][i] -= lambda;
+    std::vector<int> piv(n);
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        for (int i = k + 1; i < n; ++i)
+            if (std::abs(m[i][k]) > std::abs(m[p][k])) p = i;
+        std::swap(m[k], m[p]);
+        piv[k] = p;
+        if (std::abs(m[k][k]) < tiny) m[k][k] = tiny;
+        for (int i = k + 1; i < n; ++i) {
+            const CPPCTYPE f = m[i][k] / m[k][k];
+            m[i][k] = f;
+            for (int j = k + 1; j < n; ++j) m[i][j] -= f * m[k][j];
+        }
+    }
+    std::vector<CPPCTYPE> v(n, 1.0);
+    for (int it = 0; it < 3; ++it) {
+        for (int k = 0; k < n; ++k) std::swap(v[k], v[piv[k]]);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < i; ++j) v[i] -= m[i][j] * v[j];
+        for (int i = n - 1; i >= 0; --i) {
+            for (int j = i + 1; j < n; ++j) v[i] -= m[i][j] * v[j];
+            v[i] /= m[i][i];
+        }
+        double nrm = 0.0;
+        for (const auto& x : v) nrm += std::norm(x);
+        nrm = std::sqrt(nrm);
+        for (auto& x : v) x /= nrm;
+    }
+    return v;
+}
+
+// Eigen-decomposition of the symmetric tridiagonal matrix (alpha on the diagonal, beta
+// beside it) by cyclic Jacobi rotations; column k of `vec` belongs to val[k].
+void tridiagonal_eigen(const std::vector<double>& alpha, const std::vector<double>& beta, std::vector<double>* val,
+                       std::vector<std::vector<double>>* vec) {
+    const int n = (int)alpha.size();
+    std::vector<std::vector<double>> a(n, std::vector<double>(n, 0.0)), v(n, std::vector<double>(n, 0.0));
+    for (int i = 0; i < n; ++i) {
+        a[i][i] = alpha[i];
+        v[i][i] = 1.0;
+        if (i + 1 < n) a[i][i + 1] = a[i + 1][i] = beta[i];
+    }
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, total = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) (i == j ? total : off) += a[i][j] * a[i][j];
+        if (off <= 1e-30 * (total + off)) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                if (a[p][q] == 0.0) continue;
+                const double theta = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::abs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < n; ++k) {  // columns p, q
+                    const double akp = a[k][p], akq = a[k][q];
+                    a[k][p] = c * akp - s * akq;
+                    a[k][q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {  // rows p, q
+                    const double apk = a[p][k], aqk = a[q][k];
+                    a[p][k] = c * apk - s * aqk;
+                    a[q][k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double vkp = v[k][p], vkq = v[k][q];
+                    v[k][p] = c * vkp - s * vkq;
+                    v[k][q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    val->resize(n);
+    for (int i = 0; i < n; ++i) (*val)[i] = a[i][i];
+    *vec = v;
+}
+
+void require_terms(const GeneralQuantumOperator& o, const char* who) {
+    if (o.get_term_count() == 0) throw QulacsError(std::string(who) + ": at least one PauliOperator is required");
+}
+
+}  // namespace
+
+// src/cppsim/general_quantum_operator.cpp:290-328. The value returned adds the `mu`
+// argument (not the default shift) to <A>, as the reference does.
+CPPCTYPE GeneralQuantumOperator::solve_ground_state_eigenvalue_by_power_method(QuantumStateMI355* state,
+                                                                               unsigned iter_count,
+                                                                               CPPCTYPE mu) const {
+    require_terms(*this, "solve_ground_state_eigenvalue_by_power_method");
+    const CPPCTYPE shift = (mu == 0.0) ? calculate_default_mu() : mu;
+    QuantumStateMI355 multiplied(state->qubit_count, state->device_number);
+    QuantumStateMI355 work(state->qubit_count, state->device_number);
+    for (unsigned i = 0; i < iter_count; ++i) {
+        apply_to_state(&work, *state, &multiplied);  // A q
+        multiplied.add_state_with_coef(-shift, state);  // (A - mu) q
+        state->load(&multiplied);
+        state->normalize(state->get_squared_norm());
+    }
+    return get_expectation_value(state) + mu;
+}
+
+// Arnoldi: orthonormal Krylov basis q_0..q_m of (A - mu), Hessenberg H = Q^* (A - mu) Q,
+// ground eigenvalue = the eigenvalue of H with the smallest real part, plus mu
+// (src/cppsim/general_quantum_operator.cpp:204-288).
+CPPCTYPE GeneralQuantumOperator::solve_ground_state_eigenvalue_by_arnoldi_method(QuantumStateMI355* state,
+                                                                                 unsigned iter_count,
+                                                                                 CPPCTYPE mu) const {
+    require_terms(*this, "solve_ground_state_eigenvalue_by_arnoldi_method");
+    if (iter_count == 0) throw QulacsError("solve_ground_state_eigenvalue_by_arnoldi_method: iter_count must be > 0");
+    const unsigned n = state->qubit_count, dev = state->device_number;
+    const CPPCTYPE shift = (mu == 0.0) ? calculate_default_mu() : mu;
+    std::vector<std::unique_ptr<QuantumStateMI355>> basis;
+    state->normalize(state->get_squared_norm());
+    basis.emplace_back(new QuantumStateMI355(n, dev));
+    basis.back()->load(state);
+    QuantumStateMI355 work(n, dev);
+    CMat h(iter_count, std::vector<CPPCTYPE>(iter_count, 0.0));
+    for (unsigned i = 0; i < iter_count; ++i) {
+        auto v = std::make_unique<QuantumStateMI355>(n, dev);
+        apply_to_state(&work, *basis[i], v.get());
+        v->add_state_with_coef(-shift, basis[i].get());
+        for (unsigned j = 0; j <= i; ++j) {
+            const CPPCTYPE c = inner_product(basis[j].get(), v.get());
+            h[j][i] = c;
+            v->add_state_with_coef(-c, basis[j].get());
+        }
+        const double norm = v->get_squared_norm();
+        if (i + 1 != iter_count) h[i + 1][i] = std::sqrt(norm);
+        v->normalize(norm);
+        basis.push_back(std::move(v));
+    }
+    const std::vector<CPPCTYPE> ev = hessenberg_eigenvalues(h);
+    size_t best = 0;
+    for (size_t i = 0; i < ev.size(); ++i)
+        if (ev[i].real() < ev[best].real()) best = i;
+    const std::vector<CPPCTYPE> y = eigenvector_for(h, ev[best]);
+    QuantumStateMI355 acc(n, dev);
+    acc.multiply_coef(0.0);
+    for (unsigned i = 0; i < iter_count; ++i) acc.add_state_with_coef(y[i], basis[i].get());
+    state->load(&acc);
+    return ev[best] + shift;
+}
+
+// Lanczos three-term recurrence on (A - mu); the tridiagonal matrix's lowest eigenpair
+// gives the eigenvalue and, by re-running the recurrence, the eigenvector
+// (src/cppsim/observable.cpp:61-190).
+CPPCTYPE Observable::solve_ground_state_eigenvalue_by_lanczos_method(QuantumStateMI355* state, unsigned iter_count,
+                                                                     CPPCTYPE mu) const {
+    require_terms(*this, "solve_ground_state_eigenvalue_by_lanczos_method");
+    if (iter_count == 0) throw QulacsError("solve_ground_state_eigenvalue_by_lanczos_method: iter_count must be > 0");
+    const unsigned n = state->qubit_count, dev = state->device_number;
+    const CPPCTYPE shift = (mu == 0.0) ? calculate_default_mu() : mu;
+    QuantumStateMI355 work(n, dev), q_prev(n, dev), q(n, dev), v(n, dev);
+    state->normalize(state->get_squared_norm());
+    std::vector<double> alpha(iter_count), beta(iter_count > 1 ? iter_count - 1 : 0);
+    auto recurrence = [&](bool measure, unsigned i) {
+        // v = (A - mu) q_i - alpha_i q_i - beta_{i-1} q_{i-1}; q_{i+1} = v / beta_i
+        apply_to_state(&work, q, &v);
+        v.add_state_with_coef(-shift, &q);
+        if (measure) alpha[i] = inner_product(&q, &v).real();
+        if (i + 1 == iter_count) return false;
+        v.add_state_with_coef(-alpha[i], &q);
+        if (i != 0) v.add_state_with_coef(-beta[i - 1], &q_prev);
+        if (measure) beta[i] = std::sqrt(v.get_squared_norm());
+        v.multiply_coef(1.0 / beta[i]);
+        q_prev.load(&q);
+        q.load(&v);
+        return true;
+    };
+    q.load(state);
+    for (unsigned i = 0; i < iter_count && recurrence(true, i); ++i) {
+    }
+    std::vector<double> val;
+    std::vector<std::vector<double>> vec;
+    tridiagonal_eigen(alpha, beta, &val, &vec);
+    size_t best = 0;
+    for (size_t i = 0; i < val.size(); ++i)
+        if (val[i] < val[best]) best = i;
+    // eigenvector = sum_i vec[i][best] q_i, with the q_i regenerated from the start vector
+    q.load(state);
+    state->multiply_coef(0.0);
+    for (unsigned i = 0; i < iter_count; ++i) {
+        state->add_state_with_coef(vec[i][best], &q);
+        if (!recurrence(false, i)) break;
+    }
+    return val[best] + shift;
+}
+
+// ------------------------------------------------------------------ OpenFermion text
+namespace {
+
+// One line "coef [P0 P1 ...]": coef is "(re+imj)", "(re-imj)", "re" or "imj"; a
+// missing or empty bracket is the identity "I0" (src/cppsim/utility.cpp:63-106).
+// Returns false for a blank line.
+bool parse_openfermion_line(const std::string& line, CPPCTYPE* coef, std::string* ops) {
+    size_t p = line.find_first_not_of(" \t\r");
+    if (p == std::string::npos) return false;
+    const char* s = line.c_str() + p;
+    char* end = nullptr;
+    double re = 0.0, im = 0.0;
+    auto bad = [&]() { return QulacsError("invalid OpenFermion term: " + line); };
+    if (*s == '(') {
+        re = std::strtod(s + 1, &end);
+        if (end == s + 1) throw bad();
+        const char* t = end;
+        im = std::strtod(t, &end);
+        if (end == t || *end != 'j' || end[1] != ')') throw bad();
+        s = end + 2;
+    } else {
+        const double v = std::strtod(s, &end);
+        if (end == s) throw bad();
+        if (*end == 'j') {
+            im = v;
+            s = end + 1;
+        } else {
+            re = v;
+            s = end;
+        }
+    }
+    *coef = CPPCTYPE(re, im);
+    std::string body;
+    const char* lb = std::strchr(s, '[');
+    if (lb) {
+        const char* rb = std::strchr(lb, ']');
+        if (!rb) throw bad();
+        body.assign(lb + 1, rb);
+    }
+    if (body.find_first_not_of(" ") == std::string::npos) body = "I0";
+    // "X0 Y1" -> "X 0 Y 1" (src/cppsim/utility.cpp:56-62)
+    ops->clear();
+    for (char ch : body) {
+        ops->push_back(ch);
+        if (ch == 'X' || ch == 'Y' || ch == 'Z' || ch == 'I') ops->push_back(' ');
+    }
+    return true;
+}
+
+struct OpenFermionTerms {
+    unsigned qubit_count = 0;
+    std::vector<CPPCTYPE> coefs;
+    std::vector<std::string> ops;
+};
+
+OpenFermionTerms parse_openfermion_lines(std::istream& in) {
+    OpenFermionTerms r;
+    std::string line, ops;
+    CPPCTYPE coef;
+    while (std::getline(in, line)) {
+        if (!parse_openfermion_line(line, &coef, &ops)) continue;
+        std::string idx;
+        auto flush = [&]() {
+            if (!idx.empty()) r.qubit_count = std::max(r.qubit_count, (unsigned)std::stoul(idx) + 1);
+            idx.clear();
+        };
+        for (char ch : ops) {
+            if (std::strchr("IXYZ ", ch)) flush();
+            else idx.push_back(ch);
+        }
+        flush();
+        r.coefs.push_back(coef);
+        r.ops.push_back(ops);
+    }
+    return r;
+}
+
+OpenFermionTerms read_openfermion_file(const std::string& path) {
+    std::ifstream f(path);
+    if (!f) throw QulacsError("cannot open OpenFermion file: " + path);
+    return parse_openfermion_lines(f);
+}
+
+template <typename Op>
+Op* build_operator(const OpenFermionTerms& t, int select) {  // select: -1 all, 0 Z/I only, 1 with X or Y
+    auto op = std::make_unique<Op>(t.qubit_count);
+    for (size_t i = 0; i < t.ops.size(); ++i) {
+        const bool offdiag = t.ops[i].find_first_of("XY") != std::string::npos;
+        if (select < 0 || (select == 1) == offdiag) op->add_operator(t.coefs[i], t.ops[i]);
+    }
+    return op.release();
+}
+
+}  // namespace
+
+namespace quantum_operator {
+GeneralQuantumOperator* create_general_quantum_operator_from_openfermion_text(const std::string& text) {
+    std::istringstream in(text);
+    return build_operator<GeneralQuantumOperator>(parse_openfermion_lines(in), -1);
+}
+GeneralQuantumOperator* create_general_quantum_operator_from_openfermion_file(const std::string& path) {
+    return build_operator<GeneralQuantumOperator>(read_openfermion_file(path), -1);
+}
+std::pair<GeneralQuantumOperator*, GeneralQuantumOperator*> create_split_general_quantum_operator(
+    const std::string& path) {
+    const OpenFermionTerms t = read_openfermion_file(path);
+    return {build_operator<GeneralQuantumOperator>(t, 0), build_operator<GeneralQuantumOperator>(t, 1)};
+}
+}  // namespace quantum_operator
+namespace observable {
+Observable* create_observable_from_openfermion_text(const std::string& text) {
+    std::istringstream in(text);
+    return build_operator<Observable>(parse_openfermion_lines(in), -1);
+}
+Observable* create_observable_from_openfermion_file(const std::string& path) {
+    return build_operator<Observable>(read_openfermion_file(path), -1);
+}
+std::pair<Observable*, Observable*> create_split_observable(const std::string& path) {
+    const OpenFermionTerms t = read_openfermion_file(path);
+    return {build_operator<Observable>(t, 0), build_operator<Observable>(t, 1)};
+}
+}  // namespace observable
 
 }  // namespace qmh
 

@@ -576,6 +576,19 @@ class PauliOperator {
     CPPCTYPE get_transition_amplitude(const QuantumStateBase* bra, const QuantumStateBase* ket) const;
     void masks(ITYPE* bit_flip, ITYPE* phase_flip, unsigned* rot90) const;
     PauliOperator* copy() const { return new PauliOperator(*this); }
+    // x and z bit of every qubit up to the highest target; adding a Pauli sets bits
+    // (X: x, Y: x and z, Z: z), never clears them (src/cppsim/pauli_operator.cpp:116-131).
+    void xz_bits(std::vector<uint8_t>* x, std::vector<uint8_t>* z) const;
+    bool same_pauli_string(const PauliOperator& other) const;
+    // Product with the i^k phase of the single-qubit products; the result lists its
+    // Paulis in ascending qubit order (src/cppsim/pauli_operator.cpp:300-413).
+    PauliOperator operator*(const PauliOperator& target) const;
+    PauliOperator operator*(CPPCTYPE c) const { return PauliOperator(targets_, paulis_, coef_ * c); }
+    PauliOperator& operator*=(const PauliOperator& target) { return *this = *this * target; }
+    PauliOperator& operator*=(CPPCTYPE c) {
+        coef_ *= c;
+        return *this;
+    }
 
    private:
     std::vector<unsigned> targets_, paulis_;
@@ -586,7 +599,8 @@ class GeneralQuantumOperator {
    public:
     explicit GeneralQuantumOperator(unsigned qubit_count) : qubit_count_(qubit_count) {}
     virtual ~GeneralQuantumOperator() = default;
-    void add_operator(const PauliOperator& p);
+    // Every term enters through this one (an Observable rejects complex coefficients).
+    virtual void add_operator(const PauliOperator& p);
     void add_operator(CPPCTYPE coef, const std::string& pauli_string);
     void add_operator(const std::vector<unsigned>& targets, const std::vector<unsigned>& paulis, CPPCTYPE coef);
     size_t get_term_count() const { return terms_.size(); }
@@ -599,19 +613,85 @@ class GeneralQuantumOperator {
     CPPCTYPE get_transition_amplitude(const QuantumStateBase* bra, const QuantumStateBase* ket) const;
     // dst = (sum_k coef_k P_k) src, using `work` as scratch (same size)
     void apply_to_state(QuantumStateMI355* work, const QuantumStateMI355& src, QuantumStateMI355* dst) const;
+    // Same, with `state` itself as scratch: each Pauli is applied and undone
+    // (src/cppsim/general_quantum_operator.cpp:350-368).
+    void apply_to_state(QuantumStateMI355* state, QuantumStateMI355* dst) const;
+    // Terms with conjugated coefficients (src/cppsim/general_quantum_operator.cpp:526-533).
+    GeneralQuantumOperator get_dagger() const;
+    // "coef pauli_string + ..." with coef printed as a std::complex
+    // (src/cppsim/general_quantum_operator.cpp:1011-1022).
+    virtual std::string to_string() const;
+    // operator_count terms on every qubit, uniform Pauli ids and coefficients in [-1, 1)
+    // (src/cppsim/general_quantum_operator.cpp:179-202).
+    void add_random_operator(unsigned operator_count);
+    void add_random_operator(unsigned operator_count, unsigned seed);
+    // sum of |Re coef| (src/cppsim/general_quantum_operator.cpp:442-450)
+    CPPCTYPE calculate_default_mu() const;
+    // Ground-state eigenvalue of the operator; `state` is the start vector and is left
+    // holding the solver's eigenvector estimate. mu == 0 selects calculate_default_mu()
+    // (src/cppsim/general_quantum_operator.cpp:204-328).
+    CPPCTYPE solve_ground_state_eigenvalue_by_power_method(QuantumStateMI355* state, unsigned iter_count,
+                                                           CPPCTYPE mu = 0.0) const;
+    CPPCTYPE solve_ground_state_eigenvalue_by_arnoldi_method(QuantumStateMI355* state, unsigned iter_count,
+                                                             CPPCTYPE mu = 0.0) const;
+
+    // Sums merge terms with the same Pauli string; products multiply every pair of
+    // terms (src/cppsim/general_quantum_operator.cpp:547-805).
+    GeneralQuantumOperator& operator+=(const GeneralQuantumOperator& target);
+    GeneralQuantumOperator& operator+=(const PauliOperator& target);
+    GeneralQuantumOperator& operator-=(const GeneralQuantumOperator& target);
+    GeneralQuantumOperator& operator-=(const PauliOperator& target);
+    GeneralQuantumOperator& operator*=(const GeneralQuantumOperator& target);
+    GeneralQuantumOperator& operator*=(const PauliOperator& target);
+    GeneralQuantumOperator& operator*=(CPPCTYPE target);
+    GeneralQuantumOperator operator+(const GeneralQuantumOperator& t) const { return copy_base() += t; }
+    GeneralQuantumOperator operator+(const PauliOperator& t) const { return copy_base() += t; }
+    GeneralQuantumOperator operator-(const GeneralQuantumOperator& t) const { return copy_base() -= t; }
+    GeneralQuantumOperator operator-(const PauliOperator& t) const { return copy_base() -= t; }
+    GeneralQuantumOperator operator*(const GeneralQuantumOperator& t) const { return copy_base() *= t; }
+    GeneralQuantumOperator operator*(const PauliOperator& t) const { return copy_base() *= t; }
+    GeneralQuantumOperator operator*(CPPCTYPE t) const { return copy_base() *= t; }
 
    protected:
+    GeneralQuantumOperator copy_base() const;
+    void merge_terms(const std::vector<PauliOperator>& terms, double sign);
     unsigned qubit_count_;
     std::vector<PauliOperator> terms_;
+    Random random_;
 };
 
-// HermitianQuantumOperator: expectation values are real (src/cppsim/observable.cpp:56-58).
+// HermitianQuantumOperator: expectation values are real and every coefficient is real
+// (src/cppsim/observable.cpp:19-58).
 class Observable : public GeneralQuantumOperator {
    public:
     using GeneralQuantumOperator::GeneralQuantumOperator;
+    using GeneralQuantumOperator::add_operator;
+    void add_operator(const PauliOperator& p) override;
     double get_expectation_value_real(const QuantumStateBase* state) const {
         return GeneralQuantumOperator::get_expectation_value(state).real();
     }
+    // "coef pauli_string + ..." with the real part of coef (src/cppsim/observable.cpp:192-203)
+    std::string to_string() const override;
+    // Lanczos on the operator shifted by -mu (src/cppsim/observable.cpp:61-190).
+    CPPCTYPE solve_ground_state_eigenvalue_by_lanczos_method(QuantumStateMI355* state, unsigned iter_count,
+                                                             CPPCTYPE mu = 0.0) const;
 };
+
+// Operators from OpenFermion's text form, one "coef [P0 P1 ...]" term per line
+// (src/cppsim/general_quantum_operator.cpp:806-952, src/cppsim/observable.cpp:205-350,
+// src/cppsim/utility.cpp:63-106). A blank line is skipped; a term without Paulis is
+// the identity.
+namespace quantum_operator {
+GeneralQuantumOperator* create_general_quantum_operator_from_openfermion_text(const std::string& text);
+GeneralQuantumOperator* create_general_quantum_operator_from_openfermion_file(const std::string& path);
+// (terms with only Z/I, terms with an X or Y)
+std::pair<GeneralQuantumOperator*, GeneralQuantumOperator*> create_split_general_quantum_operator(
+    const std::string& path);
+}  // namespace quantum_operator
+namespace observable {
+Observable* create_observable_from_openfermion_text(const std::string& text);
+Observable* create_observable_from_openfermion_file(const std::string& path);
+std::pair<Observable*, Observable*> create_split_observable(const std::string& path);
+}  // namespace observable
 
 }  // namespace qmh

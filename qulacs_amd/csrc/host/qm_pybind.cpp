@@ -94,7 +94,14 @@ PYBIND11_MODULE(_core, m) {
         .def("get_expectation_value", &PauliOperator::get_expectation_value, py::arg("state"))
         .def("get_transition_amplitude", &PauliOperator::get_transition_amplitude, py::arg("state_bra"),
              py::arg("state_ket"))
-        .def("copy", [](const PauliOperator& p) { return PauliOperator(p); });
+        .def("copy", [](const PauliOperator& p) { return PauliOperator(p); })
+        .def("__mul__", [](const PauliOperator& a, const PauliOperator& b) { return a * b; }, py::is_operator())
+        .def("__mul__", [](const PauliOperator& a, CPPCTYPE c) { return a * c; }, py::is_operator())
+        .def("__imul__", [](PauliOperator& a, const PauliOperator& b) -> PauliOperator& { return a *= b; },
+             py::is_operator())
+        .def("__imul__", [](PauliOperator& a, CPPCTYPE c) -> PauliOperator& { return a *= c; }, py::is_operator())
+        .def(py::pickle([](const PauliOperator& p) { return ptree::to_json(to_ptree(p)); },
+                        [](const std::string& j) { return pauli_operator_from_ptree(ptree::from_json(j)); }));
 
     py::class_<GeneralQuantumOperator>(m, "GeneralQuantumOperator")
         .def(py::init<unsigned>(), py::arg("qubit_count"))
@@ -121,7 +128,67 @@ PYBIND11_MODULE(_core, m) {
              [](const GeneralQuantumOperator& o, QuantumStateMI355* work, const QuantumStateMI355& src,
                 QuantumStateMI355* dst) { o.apply_to_state(work, src, dst); },
              py::arg("work_state"), py::arg("state_to_be_multiplied"), py::arg("dst_state"))
-        .def("to_json", [](const GeneralQuantumOperator& o) { return ptree::to_json(to_ptree(o)); });
+        .def("apply_to_state",
+             [](const GeneralQuantumOperator& o, const QuantumStateMI355& src, QuantumStateMI355* dst) {
+                 QuantumStateMI355 work(src.qubit_count, src.device_number);
+                 o.apply_to_state(&work, src, dst);
+             },
+             py::arg("state_to_be_multiplied"), py::arg("dst_state"))
+        .def("get_dagger", &GeneralQuantumOperator::get_dagger)
+        .def("copy", [](const GeneralQuantumOperator& o) { return GeneralQuantumOperator(o); })
+        .def("__str__", &GeneralQuantumOperator::to_string)
+        .def("add_random_operator", py::overload_cast<unsigned>(&GeneralQuantumOperator::add_random_operator),
+             py::arg("operator_count"))
+        .def("add_random_operator", py::overload_cast<unsigned, unsigned>(&GeneralQuantumOperator::add_random_operator),
+             py::arg("operator_count"), py::arg("seed"))
+        .def("solve_ground_state_eigenvalue_by_arnoldi_method",
+             &GeneralQuantumOperator::solve_ground_state_eigenvalue_by_arnoldi_method, py::arg("state"),
+             py::arg("iter_count"), py::arg("mu") = CPPCTYPE(0.0))
+        .def("solve_ground_state_eigenvalue_by_power_method",
+             &GeneralQuantumOperator::solve_ground_state_eigenvalue_by_power_method, py::arg("state"),
+             py::arg("iter_count"), py::arg("mu") = CPPCTYPE(0.0))
+        .def("to_json", [](const GeneralQuantumOperator& o) { return ptree::to_json(to_ptree(o)); })
+        .def("__add__", [](const GeneralQuantumOperator& a, const GeneralQuantumOperator& b) { return a + b; },
+             py::is_operator())
+        .def("__add__", [](const GeneralQuantumOperator& a, const PauliOperator& b) { return a + b; },
+             py::is_operator())
+        .def("__sub__", [](const GeneralQuantumOperator& a, const GeneralQuantumOperator& b) { return a - b; },
+             py::is_operator())
+        .def("__sub__", [](const GeneralQuantumOperator& a, const PauliOperator& b) { return a - b; },
+             py::is_operator())
+        .def("__mul__", [](const GeneralQuantumOperator& a, const GeneralQuantumOperator& b) { return a * b; },
+             py::is_operator())
+        .def("__mul__", [](const GeneralQuantumOperator& a, const PauliOperator& b) { return a * b; },
+             py::is_operator())
+        .def("__mul__", [](const GeneralQuantumOperator& a, CPPCTYPE c) { return a * c; }, py::is_operator())
+        .def("__iadd__",
+             [](GeneralQuantumOperator& a, const GeneralQuantumOperator& b) -> GeneralQuantumOperator& {
+                 return a += b;
+             },
+             py::is_operator())
+        .def("__iadd__", [](GeneralQuantumOperator& a, const PauliOperator& b) -> GeneralQuantumOperator& { return a += b; },
+             py::is_operator())
+        .def("__isub__",
+             [](GeneralQuantumOperator& a, const GeneralQuantumOperator& b) -> GeneralQuantumOperator& {
+                 return a -= b;
+             },
+             py::is_operator())
+        .def("__isub__", [](GeneralQuantumOperator& a, const PauliOperator& b) -> GeneralQuantumOperator& { return a -= b; },
+             py::is_operator())
+        .def("__imul__",
+             [](GeneralQuantumOperator& a, const GeneralQuantumOperator& b) -> GeneralQuantumOperator& {
+                 return a *= b;
+             },
+             py::is_operator())
+        .def("__imul__", [](GeneralQuantumOperator& a, const PauliOperator& b) -> GeneralQuantumOperator& { return a *= b; },
+             py::is_operator())
+        .def("__imul__", [](GeneralQuantumOperator& a, CPPCTYPE c) -> GeneralQuantumOperator& { return a *= c; },
+             py::is_operator())
+        .def(py::pickle([](const GeneralQuantumOperator& o) { return ptree::to_json(to_ptree(o)); },
+                        [](const std::string& j) {
+                            std::unique_ptr<GeneralQuantumOperator> p(operator_from_ptree(ptree::from_json(j)));
+                            return GeneralQuantumOperator(*p);
+                        }));
 
     py::class_<Observable, GeneralQuantumOperator>(m, "Observable")
         .def(py::init<unsigned>(), py::arg("qubit_count"))
@@ -139,16 +206,59 @@ PYBIND11_MODULE(_core, m) {
                 o.add_operator(p);
             },
             py::arg("pauli_operator"))
-        .def("get_expectation_value", &Observable::get_expectation_value_real, py::arg("state"));
+        .def("get_expectation_value", &Observable::get_expectation_value_real, py::arg("state"))
+        .def("solve_ground_state_eigenvalue_by_lanczos_method",
+             &Observable::solve_ground_state_eigenvalue_by_lanczos_method, py::arg("state"), py::arg("iter_count"),
+             py::arg("mu") = CPPCTYPE(0.0))
+        .def("__str__", &Observable::to_string)
+        .def("copy",
+             [](const Observable& o) {
+                 Observable c(o.get_qubit_count());
+                 for (size_t i = 0; i < o.get_term_count(); ++i) c.add_operator(o.get_term(i));
+                 return c;
+             })
+        .def(py::pickle([](const Observable& o) { return ptree::to_json(to_ptree(o)); },
+                        [](const std::string& j) {
+                            std::unique_ptr<Observable> p(observable_from_ptree(ptree::from_json(j)));
+                            Observable c(p->get_qubit_count());
+                            for (size_t i = 0; i < p->get_term_count(); ++i) c.add_operator(p->get_term(i));
+                            return c;
+                        }));
 
     auto mquantum_operator = m.def_submodule("quantum_operator");
     mquantum_operator.def(
         "from_json", [](const std::string& json) { return operator_from_ptree(ptree::from_json(json)); },
         py::return_value_policy::take_ownership, py::arg("json"));
+    mquantum_operator.def("create_quantum_operator_from_openfermion_file",
+                          &quantum_operator::create_general_quantum_operator_from_openfermion_file,
+                          py::return_value_policy::take_ownership, py::arg("file_path"));
+    mquantum_operator.def("create_quantum_operator_from_openfermion_text",
+                          &quantum_operator::create_general_quantum_operator_from_openfermion_text,
+                          py::return_value_policy::take_ownership, py::arg("text"));
+    mquantum_operator.def(
+        "create_split_quantum_operator",
+        [](const std::string& path) {
+            auto pr = quantum_operator::create_split_general_quantum_operator(path);
+            std::unique_ptr<GeneralQuantumOperator> a(pr.first), b(pr.second);
+            return py::make_tuple(GeneralQuantumOperator(*a), GeneralQuantumOperator(*b));
+        },
+        py::arg("file_path"));
     auto mobservable = m.def_submodule("observable");
     mobservable.def(
         "from_json", [](const std::string& json) { return observable_from_ptree(ptree::from_json(json)); },
         py::return_value_policy::take_ownership, py::arg("json"));
+    mobservable.def("create_observable_from_openfermion_file", &observable::create_observable_from_openfermion_file,
+                    py::return_value_policy::take_ownership, py::arg("file_path"));
+    mobservable.def("create_observable_from_openfermion_text", &observable::create_observable_from_openfermion_text,
+                    py::return_value_policy::take_ownership, py::arg("text"));
+    mobservable.def(
+        "create_split_observable",
+        [](const std::string& path) {
+            auto pr = observable::create_split_observable(path);
+            return py::make_tuple(py::cast(pr.first, py::return_value_policy::take_ownership),
+                                  py::cast(pr.second, py::return_value_policy::take_ownership));
+        },
+        py::arg("file_path"));
 
     // ---------------------------------------------------------------- states
     py::class_<QuantumStateBase>(m, "QuantumStateBase")
