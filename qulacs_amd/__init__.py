@@ -53,6 +53,7 @@ from ._core import (  # noqa: F401
     QuantumStateBase,
     QuantumStateMI355,
     QulacsError,
+    check_build_for_mpi,
     circuit,
     gate,
     get_num_device,
@@ -62,6 +63,7 @@ from ._core import (  # noqa: F401
     set_synchronous,
     state,
     synchronize,
+    to_general_quantum_operator,
 )
 
 QuantumState = QuantumStateMI355
@@ -69,6 +71,17 @@ QuantumStateGpu = QuantumStateMI355
 StateVector = QuantumStateMI355
 StateVectorGpu = QuantumStateMI355
 DensityMatrix = DensityMatrixMI355
+
+# The reference's per-family gate classes.  Here every named, rotation, CNOT/CZ/SWAP
+# and sparse gate is a QuantumGateMatrix that dispatches on its kind, and a Pauli
+# rotation is a ClsPauliGate, so the family names refer to those classes.
+ClsOneQubitGate = QuantumGateMatrix
+ClsOneQubitRotationGate = QuantumGateMatrix
+ClsOneControlOneTargetGate = QuantumGateMatrix
+ClsTwoQubitGate = QuantumGateMatrix
+QuantumGateSparseMatrix = QuantumGateMatrix
+ClsPauliRotationGate = ClsPauliGate
+QuantumGate_Instrument = QuantumGate_CPTP
 
 # `import qulacs_amd.gate` / `from qulacs_amd.state import inner_product`
 _sys.modules[__name__ + ".gate"] = gate

@@ -89,6 +89,27 @@ std::vector<unsigned> sorted_union(const std::vector<std::vector<unsigned>>& lis
 
 // Matrix of g over its targets followed by its controls (control qubits become
 // targets; the matrix is the identity where a control value does not match).
+// X -> commutes with X, Y -> Y, Z -> Z, I -> none (src/cppsim/gate_named_pauli.hpp:27-43)
+std::vector<unsigned> pauli_commutation(const std::vector<unsigned>& paulis) {
+    std::vector<unsigned> f(paulis.size(), 0);
+    for (size_t i = 0; i < paulis.size(); ++i)
+        f[i] = paulis[i] == 1 ? QuantumGateBase::COMMUTE_X
+             : paulis[i] == 2 ? QuantumGateBase::COMMUTE_Y
+             : paulis[i] == 3 ? QuantumGateBase::COMMUTE_Z
+                              : 0u;
+    return f;
+}
+// Flags of a gate built from `gates` over `targets`: on each qubit the AND of every
+// gate's flags there (a control counts as Z, an unused qubit as all three), which is
+// the merge rule of src/cppsim/gate_merge.cpp:19-150.
+std::vector<unsigned> merged_commutation(const std::vector<const QuantumGateBase*>& gates,
+                                         const std::vector<unsigned>& targets) {
+    std::vector<unsigned> f(targets.size(), 7u);
+    for (size_t i = 0; i < targets.size(); ++i)
+        for (const auto* g : gates) f[i] &= g->commutation_of(targets[i]);
+    return f;
+}
+
 ComplexMatrix with_controls(const QuantumGateBase* g, std::vector<unsigned>* on) {
     const auto& tl = g->get_target_index_list();
     const auto& cl = g->get_control_index_list();
@@ -506,17 +527,57 @@ std::string DensityMatrixMI355::to_string() const {
 
 // ------------------------------------------------------------------ gates
 QuantumGateBase* QuantumGateBase::get_inverse() const {
-    return new QuantumGateMatrix(targets_, get_matrix().adjoint(), controls_, cvals_, name_ + "^dagger");
+    auto* g = new QuantumGateMatrix(targets_, get_matrix().adjoint(), controls_, cvals_, name_ + "^dagger");
+    g->set_target_commutation(tcomm_);
+    return g;
 }
 
+std::vector<std::pair<unsigned, unsigned>> QuantumGateBase::get_control_index_value_list() const {
+    std::vector<std::pair<unsigned, unsigned>> r;
+    for (size_t i = 0; i < controls_.size(); ++i) r.emplace_back(controls_[i], cvals_[i]);
+    return r;
+}
+unsigned QuantumGateBase::commutation_of(unsigned q) const {
+    for (size_t i = 0; i < targets_.size(); ++i)
+        if (targets_[i] == q) return get_target_commutation(i);
+    for (unsigned c : controls_)
+        if (c == q) return COMMUTE_Z;
+    return COMMUTE_X | COMMUTE_Y | COMMUTE_Z;
+}
+bool QuantumGateBase::is_diagonal() const {
+    for (size_t i = 0; i < targets_.size(); ++i)
+        if (!(get_target_commutation(i) & COMMUTE_Z)) return false;
+    return true;
+}
+bool QuantumGateBase::is_commute(const QuantumGateBase* o) const {
+    for (size_t i = 0; i < targets_.size(); ++i) {
+        const unsigned f = get_target_commutation(i);
+        for (size_t j = 0; j < o->targets_.size(); ++j)
+            if (o->targets_[j] == targets_[i] && !(f & o->get_target_commutation(j))) return false;
+        for (unsigned c : o->controls_)
+            if (c == targets_[i] && !(f & COMMUTE_Z)) return false;
+    }
+    for (unsigned c : controls_)
+        for (size_t j = 0; j < o->targets_.size(); ++j)
+            if (o->targets_[j] == c && !(o->get_target_commutation(j) & COMMUTE_Z)) return false;
+    return true;
+}
+
+// src/cppsim/gate.cpp:100-126
 std::string QuantumGateBase::to_string() const {
     std::ostringstream os;
     os << " *** gate info *** \n * gate name : " << get_name() << "\n * target    : \n";
-    for (unsigned t : targets_) os << " " << t << " : commute   \n";
+    for (size_t i = 0; i < targets_.size(); ++i) {
+        const unsigned f = get_target_commutation(i);
+        os << " " << targets_[i] << " : commute " << (f & COMMUTE_X ? "X" : " ") << " " << (f & COMMUTE_Y ? "Y" : " ")
+           << " " << (f & COMMUTE_Z ? "Z" : " ") << " \n";
+    }
     os << " * control   : \n";
     for (size_t i = 0; i < controls_.size(); ++i) os << " " << controls_[i] << " : value " << cvals_[i] << "\n";
-    os << " * Pauli     : no \n * Clifford  : no \n * Gaussian  : no \n * Parametric: " << (is_parametric() ? "yes" : "no")
-       << " \n * Diagonal  : no \n";
+    auto yn = [](bool b) { return b ? "yes" : "no"; };
+    os << " * Pauli     : " << yn(is_Pauli()) << "\n * Clifford  : " << yn(is_Clifford())
+       << "\n * Gaussian  : " << yn(is_Gaussian()) << "\n * Parametric: " << yn(is_parametric())
+       << "\n * Diagonal  : " << yn(is_diagonal()) << "\n";
     return os.str();
 }
 
@@ -539,10 +600,17 @@ void QuantumGateMatrix::add_control_qubit(unsigned q, unsigned v) {
     controls_.push_back(q);
     cvals_.push_back(v);
     kind = GENERIC;
+    property_ &= ~(PROPERTY_PAULI | PROPERTY_GAUSSIAN);
+}
+void QuantumGateMatrix::multiply_scalar(CPPCTYPE value) {
+    for (auto& z : matrix_.data) z *= value;
+    kind = GENERIC;
 }
 
 QuantumGateBase* QuantumGateMatrix::get_inverse() const {
     auto* g = new QuantumGateMatrix(targets_, matrix_.adjoint(), controls_, cvals_, name_);
+    g->set_target_commutation(tcomm_);
+    if (kind != GENERIC) g->set_gate_property(property_);  // S <-> Sdag etc. share their class
     switch (kind) {
         case RX: case RY: case RZ: g->kind = kind; g->angle = -angle; break;
         case X: case Y: case Z: case H: case CNOT: case CZ: case SWAP: case IDENTITY: g->kind = kind; break;
@@ -672,6 +740,7 @@ ClsPauliGate::ClsPauliGate(std::vector<unsigned> t, std::vector<unsigned> p, boo
     if (targets_.size() != paulis_.size()) throw QulacsError("one Pauli id per target");
     for (unsigned x : paulis_)
         if (x > 3) throw QulacsError("Pauli id must be 0..3");
+    tcomm_ = pauli_commutation(paulis_);
 }
 ComplexMatrix ClsPauliGate::get_matrix() const {
     ComplexMatrix P = pauli_product(paulis_);
@@ -714,6 +783,21 @@ QuantumGate_Probabilistic::QuantumGate_Probabilistic(std::vector<double> dist, s
     }
     targets_ = sorted_union(lists);
     name_ = "Probabilistic";
+    std::vector<const QuantumGateBase*> cg(gates_.begin(), gates_.end());
+    tcomm_ = merged_commutation(cg, targets_);
+}
+void QuantumGate_Probabilistic::optimize_ProbablisticGate() {
+    std::vector<std::pair<double, size_t>> order;
+    for (size_t i = 0; i < dist_.size(); ++i) order.emplace_back(dist_[i], i);
+    std::sort(order.rbegin(), order.rend());  // descending probability, ties by descending index
+    std::vector<QuantumGateBase*> gates;
+    for (size_t i = 0; i < order.size(); ++i) {
+        dist_[i] = order[i].first;
+        gates.push_back(gates_[order[i].second]);
+    }
+    gates_ = gates;
+    cumulative_.assign(1, 0.0);
+    for (double p : dist_) cumulative_.push_back(cumulative_.back() + p);
 }
 QuantumGate_Probabilistic::QuantumGate_Probabilistic(const QuantumGate_Probabilistic& o)
     : QuantumGateBase(o), dist_(o.dist_), cumulative_(o.cumulative_), random_(o.random_) {
@@ -774,6 +858,9 @@ QuantumGate_CPTP::QuantumGate_CPTP(std::vector<QuantumGateMatrix> kraus) : kraus
     }
     targets_ = sorted_union(lists);
     name_ = "CPTP";
+    std::vector<const QuantumGateBase*> cg;
+    for (const auto& k : kraus_) cg.push_back(&k);
+    tcomm_ = merged_commutation(cg, targets_);
 }
 ComplexMatrix QuantumGate_CPTP::get_matrix() const { throw QulacsError("CPTP map has no single matrix"); }
 ComplexMatrix QuantumGate_CPTP::superoperator() const {
@@ -821,6 +908,7 @@ QuantumGate_SingleParameter::QuantumGate_SingleParameter(std::vector<unsigned> t
     targets_ = std::move(t);
     name_ = std::move(name);
     if (targets_.size() != paulis_.size()) throw QulacsError("one Pauli id per target");
+    tcomm_ = pauli_commutation(paulis_);
 }
 ComplexMatrix QuantumGate_SingleParameter::get_matrix() const {
     return ClsPauliGate(targets_, paulis_, true, angle_).get_matrix();
@@ -853,9 +941,17 @@ void QuantumGate_SingleParameter::update_quantum_state(QuantumStateBase* state) 
 // ------------------------------------------------------------------ gate factory
 namespace gate {
 namespace {
-QuantumGateMatrix* named1(unsigned q, QuantumGateMatrix::Kind k, ComplexMatrix m, const char* name) {
+constexpr unsigned CX = QuantumGateBase::COMMUTE_X, CY = QuantumGateBase::COMMUTE_Y, CZf = QuantumGateBase::COMMUTE_Z;
+constexpr unsigned PP = QuantumGateBase::PROPERTY_PAULI, PC = QuantumGateBase::PROPERTY_CLIFFORD,
+                   PG = QuantumGateBase::PROPERTY_GAUSSIAN;
+// commutation and class flags as the reference's named gates set them
+// (src/cppsim/gate_named_one.hpp:80-330)
+QuantumGateMatrix* named1(unsigned q, QuantumGateMatrix::Kind k, ComplexMatrix m, const char* name, unsigned comm,
+                          unsigned prop) {
     auto* g = new QuantumGateMatrix({q}, std::move(m), {}, {}, name);
     g->kind = k;
+    g->set_target_commutation({comm});
+    g->set_gate_property(prop);
     return g;
 }
 ComplexMatrix m2(CPPCTYPE a, CPPCTYPE b, CPPCTYPE c, CPPCTYPE d) {
@@ -870,6 +966,7 @@ QuantumGateMatrix* rotation(unsigned q, unsigned pauli, double angle, const char
     auto* g = new QuantumGateMatrix({q}, m, {}, {}, name);
     g->kind = pauli == 1 ? QuantumGateMatrix::RX : pauli == 2 ? QuantumGateMatrix::RY : QuantumGateMatrix::RZ;
     g->angle = angle;
+    g->set_target_commutation({pauli == 1 ? CX : pauli == 2 ? CY : CZf});
     return g;
 }
 // get_IBMQ_matrix (src/cppsim/gate_factory.cpp:169-181)
@@ -880,36 +977,42 @@ ComplexMatrix ibmq(double theta, double phi, double lambda) {
 }
 }  // namespace
 
-QuantumGateMatrix* Identity(unsigned q) { return named1(q, QuantumGateMatrix::IDENTITY, ComplexMatrix::identity(2), "I"); }
-QuantumGateMatrix* X(unsigned q) { return named1(q, QuantumGateMatrix::X, m2(0, 1, 1, 0), "X"); }
-QuantumGateMatrix* Y(unsigned q) { return named1(q, QuantumGateMatrix::Y, m2(0, -I_, I_, 0), "Y"); }
-QuantumGateMatrix* Z(unsigned q) { return named1(q, QuantumGateMatrix::Z, m2(1, 0, 0, -1), "Z"); }
+QuantumGateMatrix* Identity(unsigned q) {
+    return named1(q, QuantumGateMatrix::IDENTITY, ComplexMatrix::identity(2), "I", CX | CY | CZf, PP | PC | PG);
+}
+QuantumGateMatrix* X(unsigned q) { return named1(q, QuantumGateMatrix::X, m2(0, 1, 1, 0), "X", CX, PP | PC); }
+QuantumGateMatrix* Y(unsigned q) { return named1(q, QuantumGateMatrix::Y, m2(0, -I_, I_, 0), "Y", CY, PP | PC); }
+QuantumGateMatrix* Z(unsigned q) { return named1(q, QuantumGateMatrix::Z, m2(1, 0, 0, -1), "Z", CZf, PP | PC | PG); }
 QuantumGateMatrix* H(unsigned q) {
     const double r = 1.0 / std::sqrt(2.0);
-    return named1(q, QuantumGateMatrix::H, m2(r, r, r, -r), "H");
+    return named1(q, QuantumGateMatrix::H, m2(r, r, r, -r), "H", 0, PC);
 }
-QuantumGateMatrix* S(unsigned q) { return named1(q, QuantumGateMatrix::S, m2(1, 0, 0, I_), "S"); }
-QuantumGateMatrix* Sdag(unsigned q) { return named1(q, QuantumGateMatrix::SDAG, m2(1, 0, 0, -I_), "Sdag"); }
+QuantumGateMatrix* S(unsigned q) { return named1(q, QuantumGateMatrix::S, m2(1, 0, 0, I_), "S", CZf, PC | PG); }
+QuantumGateMatrix* Sdag(unsigned q) { return named1(q, QuantumGateMatrix::SDAG, m2(1, 0, 0, -I_), "Sdag", CZf, PC | PG); }
 QuantumGateMatrix* T(unsigned q) {
-    return named1(q, QuantumGateMatrix::T, m2(1, 0, 0, (1.0 + I_) / std::sqrt(2.0)), "T");
+    return named1(q, QuantumGateMatrix::T, m2(1, 0, 0, (1.0 + I_) / std::sqrt(2.0)), "T", CZf, PG);
 }
 QuantumGateMatrix* Tdag(unsigned q) {
-    return named1(q, QuantumGateMatrix::TDAG, m2(1, 0, 0, (1.0 - I_) / std::sqrt(2.0)), "Tdag");
+    return named1(q, QuantumGateMatrix::TDAG, m2(1, 0, 0, (1.0 - I_) / std::sqrt(2.0)), "Tdag", CZf, PG);
 }
 QuantumGateMatrix* sqrtX(unsigned q) {
-    return named1(q, QuantumGateMatrix::SQRTX, m2(0.5 + 0.5 * I_, 0.5 - 0.5 * I_, 0.5 - 0.5 * I_, 0.5 + 0.5 * I_), "sqrtX");
+    return named1(q, QuantumGateMatrix::SQRTX, m2(0.5 + 0.5 * I_, 0.5 - 0.5 * I_, 0.5 - 0.5 * I_, 0.5 + 0.5 * I_), "sqrtX",
+                  CX, PC);
 }
 QuantumGateMatrix* sqrtXdag(unsigned q) {
-    return named1(q, QuantumGateMatrix::SQRTXDAG, m2(0.5 - 0.5 * I_, 0.5 + 0.5 * I_, 0.5 + 0.5 * I_, 0.5 - 0.5 * I_), "sqrtXdag");
+    return named1(q, QuantumGateMatrix::SQRTXDAG, m2(0.5 - 0.5 * I_, 0.5 + 0.5 * I_, 0.5 + 0.5 * I_, 0.5 - 0.5 * I_),
+                  "sqrtXdag", CX, PC);
 }
 QuantumGateMatrix* sqrtY(unsigned q) {
-    return named1(q, QuantumGateMatrix::SQRTY, m2(0.5 + 0.5 * I_, -0.5 - 0.5 * I_, 0.5 + 0.5 * I_, 0.5 + 0.5 * I_), "sqrtY");
+    return named1(q, QuantumGateMatrix::SQRTY, m2(0.5 + 0.5 * I_, -0.5 - 0.5 * I_, 0.5 + 0.5 * I_, 0.5 + 0.5 * I_), "sqrtY",
+                  CY, PC);
 }
 QuantumGateMatrix* sqrtYdag(unsigned q) {
-    return named1(q, QuantumGateMatrix::SQRTYDAG, m2(0.5 - 0.5 * I_, 0.5 - 0.5 * I_, -0.5 + 0.5 * I_, 0.5 - 0.5 * I_), "sqrtYdag");
+    return named1(q, QuantumGateMatrix::SQRTYDAG, m2(0.5 - 0.5 * I_, 0.5 - 0.5 * I_, -0.5 + 0.5 * I_, 0.5 - 0.5 * I_),
+                  "sqrtYdag", CY, PC);
 }
-QuantumGateMatrix* P0(unsigned q) { return named1(q, QuantumGateMatrix::P0, m2(1, 0, 0, 0), "Projection-0"); }
-QuantumGateMatrix* P1(unsigned q) { return named1(q, QuantumGateMatrix::P1, m2(0, 0, 0, 1), "Projection-1"); }
+QuantumGateMatrix* P0(unsigned q) { return named1(q, QuantumGateMatrix::P0, m2(1, 0, 0, 0), "Projection-0", 0, PC | PG); }
+QuantumGateMatrix* P1(unsigned q) { return named1(q, QuantumGateMatrix::P1, m2(0, 0, 0, 1), "Projection-1", 0, PC | PG); }
 QuantumGateMatrix* RX(unsigned q, double a) { return rotation(q, 1, a, "X-rotation"); }
 QuantumGateMatrix* RY(unsigned q, double a) { return rotation(q, 2, a, "Y-rotation"); }
 QuantumGateMatrix* RZ(unsigned q, double a) { return rotation(q, 3, a, "Z-rotation"); }
@@ -931,12 +1034,16 @@ QuantumGateMatrix* CNOT(unsigned c, unsigned t) {
     if (c == t) throw QulacsError("CNOT: control and target must differ");
     auto* g = new QuantumGateMatrix({t}, m2(0, 1, 1, 0), {c}, {1}, "CNOT");
     g->kind = QuantumGateMatrix::CNOT;
+    g->set_target_commutation({CX});
+    g->set_gate_property(PC);
     return g;
 }
 QuantumGateMatrix* CZ(unsigned c, unsigned t) {
     if (c == t) throw QulacsError("CZ: control and target must differ");
     auto* g = new QuantumGateMatrix({t}, m2(1, 0, 0, -1), {c}, {1}, "CZ");
     g->kind = QuantumGateMatrix::CZ;
+    g->set_target_commutation({CZf});
+    g->set_gate_property(PC);
     return g;
 }
 QuantumGateMatrix* SWAP(unsigned a, unsigned b) {
@@ -945,6 +1052,7 @@ QuantumGateMatrix* SWAP(unsigned a, unsigned b) {
     m(0, 0) = 1; m(1, 2) = 1; m(2, 1) = 1; m(3, 3) = 1;
     auto* g = new QuantumGateMatrix({a, b}, m, {}, {}, "SWAP");
     g->kind = QuantumGateMatrix::SWAP;
+    g->set_gate_property(PC);
     return g;
 }
 QuantumGateMatrix* DenseMatrix(const std::vector<unsigned>& t, const ComplexMatrix& m) {
@@ -1049,6 +1157,7 @@ QuantumGate_CPTP* Measurement(unsigned q, unsigned reg) {
 QuantumGateMatrix* TOFFOLI(unsigned c1, unsigned c2, unsigned t) {
     if (c1 == c2) throw QulacsError("TOFFOLI: controls must differ");
     QuantumGateMatrix* g = new QuantumGateMatrix({t}, m2(0, 1, 1, 0), {}, {}, "DenseMatrix");
+    g->set_target_commutation({CX});
     g->add_control_qubit(c1, 1);
     g->add_control_qubit(c2, 1);
     return g;
@@ -1089,7 +1198,9 @@ QuantumGateMatrix* add(const QuantumGateBase* a, const QuantumGateBase* b) {
     ComplexMatrix sum = embed(m1, on1, u);
     const ComplexMatrix e2 = embed(mb, on2, u);
     for (size_t i = 0; i < sum.data.size(); ++i) sum.data[i] += e2.data[i];
-    return new QuantumGateMatrix(u, sum, kq, kv);
+    auto* g = new QuantumGateMatrix(u, sum, kq, kv);
+    g->set_target_commutation(merged_commutation({a, b}, u));
+    return g;
 }
 QuantumGateMatrix* add(const std::vector<QuantumGateBase*>& gates) {
     if (gates.empty()) throw QulacsError("add: empty gate list");
@@ -1098,16 +1209,45 @@ QuantumGateMatrix* add(const std::vector<QuantumGateBase*>& gates) {
     return acc.release();
 }
 QuantumGateMatrix* to_matrix_gate(const QuantumGateBase* g) {
-    return new QuantumGateMatrix(g->get_target_index_list(), g->get_matrix(), g->get_control_index_list(),
-                                 g->get_control_value_list());
+    auto* m = new QuantumGateMatrix(g->get_target_index_list(), g->get_matrix(), g->get_control_index_list(),
+                                    g->get_control_value_list());
+    std::vector<unsigned> f(g->get_target_index_list().size());
+    for (size_t i = 0; i < f.size(); ++i) f[i] = g->get_target_commutation(i);
+    m->set_target_commutation(f);
+    return m;
 }
-// second * first over the ascending union of target and control qubits
-// (src/cppsim/gate_merge.cpp:153-357); controls are folded into the matrix.
+// second * first (src/cppsim/gate_merge.cpp:153-357): a control on the same qubit with
+// the same value in both gates stays a control; every other qubit becomes a target,
+// in ascending order, with the other controls folded into the matrix. The class
+// flags are the AND of both gates'.
 QuantumGateMatrix* merge(const QuantumGateBase* first, const QuantumGateBase* second) {
+    std::vector<unsigned> kq, kv;
+    const auto &ca = first->get_control_index_list(), &va = first->get_control_value_list();
+    const auto &cb = second->get_control_index_list(), &vb = second->get_control_value_list();
+    for (size_t i = 0; i < ca.size(); ++i)
+        for (size_t j = 0; j < cb.size(); ++j)
+            if (ca[i] == cb[j] && va[i] == vb[j]) {
+                kq.push_back(ca[i]);
+                kv.push_back(va[i]);
+            }
+    auto fold = [&](const QuantumGateBase* g, std::vector<unsigned>* on) {
+        std::vector<unsigned> c, v;
+        const auto &gc = g->get_control_index_list(), &gv = g->get_control_value_list();
+        for (size_t i = 0; i < gc.size(); ++i)
+            if (std::find(kq.begin(), kq.end(), gc[i]) == kq.end()) {
+                c.push_back(gc[i]);
+                v.push_back(gv[i]);
+            }
+        QuantumGateMatrix rest(g->get_target_index_list(), g->get_matrix(), c, v);
+        return with_controls(&rest, on);
+    };
     std::vector<unsigned> on1, on2;
-    const ComplexMatrix m1 = with_controls(first, &on1), m2_ = with_controls(second, &on2);
+    const ComplexMatrix m1 = fold(first, &on1), m2_ = fold(second, &on2);
     const auto u = sorted_union({on1, on2});
-    return new QuantumGateMatrix(u, embed(m2_, on2, u) * embed(m1, on1, u));
+    auto* g = new QuantumGateMatrix(u, embed(m2_, on2, u) * embed(m1, on1, u), kq, kv);
+    g->set_target_commutation(merged_commutation({first, second}, u));
+    g->set_gate_property(first->get_property_value() & second->get_property_value() & ~QuantumGateBase::PROPERTY_PARAMETRIC);
+    return g;
 }
 QuantumGate_SingleParameter* ParametricRX(unsigned q, double a) { return new QuantumGate_SingleParameter({q}, {1}, a, "ParametricRX"); }
 QuantumGate_SingleParameter* ParametricRY(unsigned q, double a) { return new QuantumGate_SingleParameter({q}, {2}, a, "ParametricRY"); }
@@ -1151,6 +1291,63 @@ void QuantumCircuit::remove_gate(size_t i) {
     delete gates_[i];
     gates_.erase(gates_.begin() + i);
 }
+void QuantumCircuit::merge_circuit(const QuantumCircuit& c) {
+    const size_t n = c.gates_.size();  // `c` may be this circuit
+    for (size_t i = 0; i < n; ++i) add_gate(*c.gates_[i]);
+}
+void QuantumCircuit::add_noise_gate(const QuantumGateBase& g, const std::string& type, double p) {
+    std::vector<unsigned> qs = g.get_target_index_list();
+    for (unsigned c : g.get_control_index_list()) qs.push_back(c);
+    auto need = [&](size_t k) {
+        if (qs.size() > k || (k == 1 && qs.size() != 1))
+            throw QulacsError("add_noise_gate: " + type + " noise takes " + std::to_string(k) + " qubit(s), the gate has " +
+                              std::to_string(qs.size()));
+    };
+    QuantumGateBase* noise = nullptr;
+    if (type == "Depolarizing") {
+        need(2);
+        if (qs.size() == 1) noise = gate::DepolarizingNoise(qs[0], p);
+        else if (qs.size() == 2) noise = gate::TwoQubitDepolarizingNoise(qs[0], qs[1], p);
+        else throw QulacsError("add_noise_gate: Depolarizing noise takes 1 or 2 qubits");
+    } else if (type == "BitFlip") {
+        need(1);
+        noise = gate::BitFlipNoise(qs[0], p);
+    } else if (type == "Dephasing") {
+        need(1);
+        noise = gate::DephasingNoise(qs[0], p);
+    } else if (type == "IndependentXZ") {
+        need(1);
+        noise = gate::IndependentXZNoise(qs[0], p);
+    } else if (type == "AmplitudeDamping") {
+        need(1);
+        noise = gate::AmplitudeDampingNoise(qs[0], p);
+    } else {
+        throw QulacsError("add_noise_gate: unknown noise type '" + type + "'");
+    }
+    add_gate(g);
+    add_gate_consume(noise);
+}
+void QuantumCircuit::add_observable_rotation_gate(const GeneralQuantumOperator& o, double angle, unsigned repeats) {
+    if (!o.is_hermitian()) throw QulacsError("add_observable_rotation_gate: the observable must be Hermitian");
+    if (repeats == 0) repeats = (unsigned)std::ceil(angle * (double)o.get_qubit_count() * 100.0);
+    for (unsigned r = 0; r < repeats; ++r)
+        for (size_t k = 0; k < o.get_term_count(); ++k) {
+            const PauliOperator& t = o.get_term(k);
+            add_gate_consume(
+                gate::PauliRotation(t.get_index_list(), t.get_pauli_id_list(), t.get_coef().real() * angle / repeats));
+        }
+}
+void QuantumCircuit::add_diagonal_observable_rotation_gate(const GeneralQuantumOperator& o, double angle) {
+    if (!o.is_hermitian()) throw QulacsError("add_diagonal_observable_rotation_gate: the observable must be Hermitian");
+    for (size_t k = 0; k < o.get_term_count(); ++k) {
+        const PauliOperator& t = o.get_term(k);
+        std::unique_ptr<ClsPauliGate> g(
+            gate::PauliRotation(t.get_index_list(), t.get_pauli_id_list(), t.get_coef().real() * angle));
+        if (!g->is_diagonal()) throw QulacsError("add_diagonal_observable_rotation_gate: the observable is not diagonal");
+        add_gate_consume(g.release());
+    }
+}
+
 // src/cppsim/circuit.cpp:46-57
 void QuantumCircuit::update_quantum_state(QuantumStateBase* s) { update_quantum_state(s, 0, gates_.size()); }
 void QuantumCircuit::update_quantum_state(QuantumStateBase* s, size_t start, size_t end) {
@@ -1748,6 +1945,43 @@ CPPCTYPE Observable::solve_ground_state_eigenvalue_by_lanczos_method(QuantumStat
     return val[best] + shift;
 }
 
+GeneralQuantumOperator* to_general_quantum_operator(const QuantumGateBase* g, unsigned qubits, double tol) {
+    if (!g->get_control_index_list().empty()) throw QulacsError("to_general_quantum_operator: gate must not have controls");
+    const ComplexMatrix u = g->get_matrix();
+    const auto& tl = g->get_target_index_list();
+    const unsigned k = (unsigned)tl.size();
+    const size_t d = size_t(1) << k;
+    auto op = std::make_unique<GeneralQuantumOperator>(qubits);
+    const CPPCTYPE ipow[4] = {1.0, CPPCTYPE(0, 1), -1.0, CPPCTYPE(0, -1)};
+    for (size_t i = 0; i < d; ++i)
+        for (size_t j = 0; j < d; ++j) {
+            // string: bit h of (i, j) = (0,0) I, (0,1) X, (1,0) Y, (1,1) Z
+            size_t flip = 0, phase = 0;
+            unsigned ny = 0;
+            std::vector<unsigned> idx, pid;
+            for (unsigned h = 0; h < k; ++h) {
+                const bool a = (i >> h) & 1, b = (j >> h) & 1;
+                if (!a && !b) continue;
+                const unsigned p = a ? (b ? 3u : 2u) : 1u;
+                idx.push_back(tl[h]);
+                pid.push_back(p);
+                if (p != 3) flip |= size_t(1) << h;
+                if (p != 1) phase |= size_t(1) << h;
+                ny += p == 2;
+            }
+            // tr(P U) = sum_y <y ^ flip| P |y> U[y][y ^ flip], P|y> = i^#Y (-1)^|y & phase| |y ^ flip>
+            CPPCTYPE tr = 0.0;
+            for (size_t y = 0; y < d; ++y) {
+                const double sgn = (__builtin_popcountll(y & phase) & 1) ? -1.0 : 1.0;
+                tr += sgn * u(y, y ^ flip);
+            }
+            tr *= ipow[ny % 4];
+            if (std::abs(tr) <= tol) continue;
+            op->add_operator(PauliOperator(idx, pid, tr / (double)d));
+        }
+    return op.release();
+}
+
 // ------------------------------------------------------------------ OpenFermion text
 namespace {
 
@@ -1893,6 +2127,12 @@ void ParametricQuantumCircuit::add_gate(const QuantumGateBase& g, size_t positio
     for (auto& p : param_pos_)
         if (p >= position) ++p;
 }
+void ParametricQuantumCircuit::merge_circuit(const ParametricQuantumCircuit& c) {
+    const size_t base = gates_.size();
+    const std::vector<size_t> pos(c.param_pos_);  // `c` may be this circuit
+    QuantumCircuit::merge_circuit(c);
+    for (size_t p : pos) param_pos_.push_back(base + p);
+}
 void ParametricQuantumCircuit::remove_gate(size_t i) {
     QuantumCircuit::remove_gate(i);
     std::vector<size_t> kept;
@@ -2000,23 +2240,10 @@ uint64_t qubit_mask(const QuantumGateBase* g) {
     for (unsigned q : g->get_control_index_list()) m |= 1ULL << q;
     return m;
 }
-
-bool is_diagonal(const QuantumGateBase* g) {
-    if (g->is_noise()) return false;
-    const ComplexMatrix m = g->get_matrix();
-    for (size_t y = 0; y < m.rows; ++y)
-        for (size_t x = 0; x < m.cols; ++x)
-            if (x != y && m(y, x) != CPPCTYPE(0)) return false;
-    return true;  // a controlled diagonal is diagonal too
-}
 }  // namespace
 
-bool QuantumCircuitOptimizer::commute(size_t a, size_t b) const {
-    const QuantumGateBase* ga = c_->gates_[a];
-    const QuantumGateBase* gb = c_->gates_[b];
-    if ((qubit_mask(ga) & qubit_mask(gb)) == 0) return true;
-    return is_diagonal(ga) && is_diagonal(gb);
-}
+// the gates' own commutation flags decide (src/cppsim/circuit_optimizer.cpp:35-54)
+bool QuantumCircuitOptimizer::commute(size_t a, size_t b) const { return c_->gates_[a]->is_commute(c_->gates_[b]); }
 // src/cppsim/circuit_optimizer.cpp:35-54
 size_t QuantumCircuitOptimizer::rightmost_commute(size_t i) const {
     size_t cur = i + 1;

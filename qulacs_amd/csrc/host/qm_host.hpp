@@ -21,6 +21,7 @@
 #include <random>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../../include/qulacs_mi355.h"
@@ -207,10 +208,33 @@ class QuantumGateBase {
     const std::vector<unsigned>& get_target_index_list() const { return targets_; }
     const std::vector<unsigned>& get_control_index_list() const { return controls_; }
     const std::vector<unsigned>& get_control_value_list() const { return cvals_; }
+    std::vector<std::pair<unsigned, unsigned>> get_control_index_value_list() const;
+
+    // Which single-qubit Paulis the gate commutes with on each target: bit 0 X,
+    // bit 1 Y, bit 2 Z (src/cppsim/qubit_info.hpp:7-11); a control commutes with Z only.
+    enum : unsigned { COMMUTE_X = 1, COMMUTE_Y = 2, COMMUTE_Z = 4 };
+    // Gate class flags (src/cppsim/gate.hpp:69-75).
+    enum : unsigned { PROPERTY_PAULI = 1, PROPERTY_CLIFFORD = 2, PROPERTY_GAUSSIAN = 4, PROPERTY_PARAMETRIC = 8 };
+    unsigned get_target_commutation(size_t i) const { return i < tcomm_.size() ? tcomm_[i] : 0u; }
+    // flags of qubit q: its target flags, COMMUTE_Z for a control, all three when unused
+    unsigned commutation_of(unsigned q) const;
+    void set_target_commutation(std::vector<unsigned> flags) { tcomm_ = std::move(flags); }
+    unsigned get_property_value() const { return property_ | (is_parametric() ? PROPERTY_PARAMETRIC : 0u); }
+    void set_gate_property(unsigned p) { property_ = p; }
+    bool is_Pauli() const { return property_ & PROPERTY_PAULI; }
+    bool is_Clifford() const { return property_ & PROPERTY_CLIFFORD; }
+    bool is_Gaussian() const { return property_ & PROPERTY_GAUSSIAN; }
+    // every target commutes with Z (src/cppsim/gate.cpp:46-60)
+    bool is_diagonal() const;
+    // target-target pairs on one qubit share a Pauli they commute with; a target on
+    // another gate's control commutes with Z (src/cppsim/gate.cpp:12-31)
+    bool is_commute(const QuantumGateBase* other) const;
 
    protected:
     std::string name_;
     std::vector<unsigned> targets_, controls_, cvals_;
+    std::vector<unsigned> tcomm_;  // parallel to targets_ (missing entries are 0)
+    unsigned property_ = 0;
 };
 
 // Dense (optionally controlled) matrix gate; also the base of every unitary
@@ -223,7 +247,10 @@ class QuantumGateMatrix : public QuantumGateBase {
     QuantumGateBase* copy() const override { return new QuantumGateMatrix(*this); }
     ComplexMatrix get_matrix() const override { return matrix_; }
     QuantumGateBase* get_inverse() const override;
+    // clears the Pauli and Gaussian flags (src/cppsim/gate_matrix.cpp:296-302)
     void add_control_qubit(unsigned qubit, unsigned value);
+    // matrix *= value; the gate becomes a generic dense gate
+    void multiply_scalar(CPPCTYPE value);
     // kind selects a dedicated entry point (0 = generic dense)
     enum Kind { GENERIC, X, Y, Z, H, S, SDAG, T, TDAG, SQRTX, SQRTXDAG, SQRTY, SQRTYDAG, P0, P1, RX, RY, RZ,
                 CNOT, CZ, SWAP, IDENTITY };
@@ -282,7 +309,11 @@ class QuantumGate_Probabilistic : public QuantumGateBase {
     bool is_noise() const override { return true; }
     void set_seed(uint64_t seed) { random_.set_seed(seed); }
     const std::vector<double>& get_distribution() const { return dist_; }
+    const std::vector<double>& get_cumulative_distribution() const { return cumulative_; }
     const std::vector<QuantumGateBase*>& get_gate_list() const { return gates_; }
+    // gates in descending order of probability, so the branch search ends early
+    // (src/cppsim/gate_general.hpp:350-372)
+    void optimize_ProbablisticGate();
     // superoperator over targets_ (4^k x 4^k, member = row bits * 2^k + column bits)
     ComplexMatrix superoperator() const;
 
@@ -402,6 +433,8 @@ QuantumGate_SingleParameter* ParametricPauliRotation(const std::vector<unsigned>
 }  // namespace gate
 
 // ------------------------------------------------------------------ circuits
+class GeneralQuantumOperator;
+
 class QuantumCircuit {
    public:
     explicit QuantumCircuit(unsigned qubit_count) : qubit_count_(qubit_count) {}
@@ -466,6 +499,17 @@ class QuantumCircuit {
     void add_random_unitary_gate(const std::vector<unsigned>& t, uint64_t seed) {
         add_gate_consume(gate::RandomUnitary(t, seed));
     }
+    // copies of every gate of `circuit`, appended (src/cppsim/circuit.hpp:150-155)
+    void merge_circuit(const QuantumCircuit& circuit);
+    // the gate, then the named noise channel ("Depolarizing", "BitFlip", "Dephasing",
+    // "IndependentXZ", "AmplitudeDamping") on its targets and controls
+    // (src/cppsim/circuit.cpp:167-248)
+    void add_noise_gate(const QuantumGateBase& g, const std::string& noise_type, double noise_prob);
+    // num_repeats rounds of one Pauli rotation by Re(coef_k) * angle / num_repeats per
+    // term; num_repeats == 0 picks ceil(angle * qubit_count * 100) (src/cppsim/circuit.cpp:500-522)
+    void add_observable_rotation_gate(const GeneralQuantumOperator& observable, double angle, unsigned num_repeats);
+    // one Pauli rotation per term; every term must be diagonal (src/cppsim/circuit.cpp:481-499)
+    void add_diagonal_observable_rotation_gate(const GeneralQuantumOperator& observable, double angle);
 
    protected:
     friend class QuantumCircuitOptimizer;
@@ -474,8 +518,8 @@ class QuantumCircuit {
 };
 
 // Host-side gate fusion (src/cppsim/circuit_optimizer.cpp:113-235).  Two gates
-// commute when they act on disjoint qubits or are both diagonal; parametric and
-// noise gates are never merged.  Only swap_level 0 (no distributed-qubit swap
+// commute as QuantumGateBase::is_commute says (their per-qubit Pauli commutation
+// flags); parametric and noise gates are never merged.  Only swap_level 0 (no distributed-qubit swap
 // insertion) exists on one device.
 class QuantumCircuitOptimizer {
    public:
@@ -537,6 +581,10 @@ class ParametricQuantumCircuit : public QuantumCircuit {
                                                   double a);
     void add_gate(const QuantumGateBase& g) { QuantumCircuit::add_gate(g); }
     void add_gate(const QuantumGateBase& g, size_t position);
+    // gates appended with their parameters registered after this circuit's
+    // (src/vqcsim/parametric_circuit.cpp:167-181)
+    using QuantumCircuit::merge_circuit;
+    void merge_circuit(const ParametricQuantumCircuit& circuit);
     void remove_gate(size_t i);
     size_t get_parameter_count() const { return param_pos_.size(); }
     double get_parameter(size_t i) const;
@@ -676,6 +724,14 @@ class Observable : public GeneralQuantumOperator {
     CPPCTYPE solve_ground_state_eigenvalue_by_lanczos_method(QuantumStateMI355* state, unsigned iter_count,
                                                              CPPCTYPE mu = 0.0) const;
 };
+
+// Pauli decomposition of a gate's matrix over GQO_qubits qubits: coefficient of the
+// string P on the gate's targets is tr(P U) / 2^k, terms with |tr(P U)| <= tol are
+// dropped, strings in the order (row bits, column bits) of the transform in
+// src/cppsim/gate_to_gqo.hpp:9-65 (row and column bit h of the string index:
+// 00 I, 01 X, 10 Y, 11 Z on target h). Controlled gates are rejected.
+GeneralQuantumOperator* to_general_quantum_operator(const QuantumGateBase* gate, unsigned GQO_qubits,
+                                                    double tol = 1e-6);
 
 // Operators from OpenFermion's text form, one "coef [P0 P1 ...]" term per line
 // (src/cppsim/general_quantum_operator.cpp:806-952, src/cppsim/observable.cpp:205-350,

@@ -78,6 +78,8 @@ PYBIND11_MODULE(_core, m) {
     m.doc() = "qulacs state-vector / density-matrix API on MI355X (HIP, complex128)";
     py::register_exception<QulacsError>(m, "QulacsError", PyExc_RuntimeError);
     m.attr("__version__") = qm_version();
+    // one process, one GPU: no MPI-distributed states
+    m.def("check_build_for_mpi", []() { return false; });
 
     // ---------------------------------------------------------------- operators
     py::class_<PauliOperator>(m, "PauliOperator")
@@ -92,6 +94,8 @@ PYBIND11_MODULE(_core, m) {
         .def("add_single_Pauli", &PauliOperator::add_single_Pauli, py::arg("index"), py::arg("pauli_type"))
         .def("get_pauli_string", &PauliOperator::get_pauli_string)
         .def("get_expectation_value", &PauliOperator::get_expectation_value, py::arg("state"))
+        // one device reduction either way
+        .def("get_expectation_value_single_thread", &PauliOperator::get_expectation_value, py::arg("state"))
         .def("get_transition_amplitude", &PauliOperator::get_transition_amplitude, py::arg("state_bra"),
              py::arg("state_ket"))
         .def("copy", [](const PauliOperator& p) { return PauliOperator(p); })
@@ -113,6 +117,8 @@ PYBIND11_MODULE(_core, m) {
              py::overload_cast<const std::vector<unsigned>&, const std::vector<unsigned>&, CPPCTYPE>(
                  &GeneralQuantumOperator::add_operator),
              py::arg("target_qubit_index_list"), py::arg("target_qubit_pauli_list"), py::arg("coef"))
+        .def("add_operator_move", py::overload_cast<const PauliOperator&>(&GeneralQuantumOperator::add_operator),
+             py::arg("pauli_operator"))
         .def("add_operator_copy", py::overload_cast<const PauliOperator&>(&GeneralQuantumOperator::add_operator),
              py::arg("pauli_operator"))
         .def("is_hermitian", &GeneralQuantumOperator::is_hermitian)
@@ -121,6 +127,7 @@ PYBIND11_MODULE(_core, m) {
         .def("get_term_count", &GeneralQuantumOperator::get_term_count)
         .def("get_term", [](const GeneralQuantumOperator& o, size_t i) { return o.get_term(i); }, py::arg("index"))
         .def("get_expectation_value", &GeneralQuantumOperator::get_expectation_value, py::arg("state"))
+        .def("get_expectation_value_single_thread", &GeneralQuantumOperator::get_expectation_value, py::arg("state"))
         .def("get_matrix", &operator_matrix)
         .def("get_transition_amplitude", &GeneralQuantumOperator::get_transition_amplitude, py::arg("state_bra"),
              py::arg("state_ket"))
@@ -207,6 +214,11 @@ PYBIND11_MODULE(_core, m) {
             },
             py::arg("pauli_operator"))
         .def("get_expectation_value", &Observable::get_expectation_value_real, py::arg("state"))
+        .def("get_expectation_value_single_thread", &Observable::get_expectation_value_real, py::arg("state"))
+        .def("add_operator_move", [](Observable& o, const PauliOperator& p) { o.add_operator(p); },
+             py::arg("pauli_operator"))
+        .def("add_operator_copy", [](Observable& o, const PauliOperator& p) { o.add_operator(p); },
+             py::arg("pauli_operator"))
         .def("solve_ground_state_eigenvalue_by_lanczos_method",
              &Observable::solve_ground_state_eigenvalue_by_lanczos_method, py::arg("state"), py::arg("iter_count"),
              py::arg("mu") = CPPCTYPE(0.0))
@@ -224,6 +236,9 @@ PYBIND11_MODULE(_core, m) {
                             for (size_t i = 0; i < p->get_term_count(); ++i) c.add_operator(p->get_term(i));
                             return c;
                         }));
+
+    m.def("to_general_quantum_operator", &to_general_quantum_operator, py::return_value_policy::take_ownership,
+          py::arg("gate"), py::arg("qubits"), py::arg("tol"));
 
     auto mquantum_operator = m.def_submodule("quantum_operator");
     mquantum_operator.def(
@@ -371,19 +386,29 @@ PYBIND11_MODULE(_core, m) {
         .def("get_target_index_list", &QuantumGateBase::get_target_index_list)
         .def("get_control_index_list", &QuantumGateBase::get_control_index_list)
         .def("get_control_value_list", &QuantumGateBase::get_control_value_list)
+        .def("get_control_index_value_list", &QuantumGateBase::get_control_index_value_list)
         .def("is_parametric", &QuantumGateBase::is_parametric)
         .def("is_noise", &QuantumGateBase::is_noise)
+        .def("is_Pauli", &QuantumGateBase::is_Pauli)
+        .def("is_Clifford", &QuantumGateBase::is_Clifford)
+        .def("is_Gaussian", &QuantumGateBase::is_Gaussian)
+        .def("is_diagonal", &QuantumGateBase::is_diagonal)
+        .def("is_commute", &QuantumGateBase::is_commute, py::arg("gate"))
+        .def("get_property_value", &QuantumGateBase::get_property_value)
         .def("to_string", &QuantumGateBase::to_string)
         .def("__str__", &QuantumGateBase::to_string)
         .def("to_json", [](const QuantumGateBase& g) { return ptree::to_json(to_ptree(g)); });
     py::class_<QuantumGateMatrix, QuantumGateBase>(m, "QuantumGateMatrix")
-        .def("add_control_qubit", &QuantumGateMatrix::add_control_qubit, py::arg("index"), py::arg("control_value"));
+        .def("add_control_qubit", &QuantumGateMatrix::add_control_qubit, py::arg("index"), py::arg("control_value"))
+        .def("multiply_scalar", &QuantumGateMatrix::multiply_scalar, py::arg("value"));
     py::class_<QuantumGateDiagonalMatrix, QuantumGateBase>(m, "QuantumGateDiagonalMatrix");
     py::class_<ClsPauliGate, QuantumGateBase>(m, "ClsPauliGate")
         .def("get_pauli_id_list", &ClsPauliGate::get_pauli_id_list)
         .def("get_angle", &ClsPauliGate::get_angle);
     py::class_<QuantumGate_Probabilistic, QuantumGateBase>(m, "QuantumGate_Probabilistic")
         .def("get_distribution", &QuantumGate_Probabilistic::get_distribution)
+        .def("get_cumulative_distribution", &QuantumGate_Probabilistic::get_cumulative_distribution)
+        .def("optimize_ProbablisticGate", &QuantumGate_Probabilistic::optimize_ProbablisticGate)
         .def("get_gate_list",
              [](const QuantumGate_Probabilistic& g) {
                  std::vector<QuantumGateBase*> out;
@@ -527,6 +552,13 @@ PYBIND11_MODULE(_core, m) {
     // ---------------------------------------------------------------- circuits
     py::class_<QuantumCircuit>(m, "QuantumCircuit")
         .def(py::init<unsigned>(), py::arg("qubit_count"))
+        .def("merge_circuit", &QuantumCircuit::merge_circuit, py::arg("circuit"))
+        .def("add_noise_gate", &QuantumCircuit::add_noise_gate, py::arg("gate"), py::arg("NoiseType"),
+             py::arg("NoiseProbability"))
+        .def("add_observable_rotation_gate", &QuantumCircuit::add_observable_rotation_gate, py::arg("observable"),
+             py::arg("angle"), py::arg("repeat"))
+        .def("add_diagonal_observable_rotation_gate", &QuantumCircuit::add_diagonal_observable_rotation_gate,
+             py::arg("observable"), py::arg("angle"))
         .def("to_json", [](const QuantumCircuit& c) { return ptree::to_json(to_ptree(c)); })
         .def(py::pickle([](const QuantumCircuit& c) { return ptree::to_json(to_ptree(c)); },
                         [](const std::string& json) {
@@ -607,6 +639,8 @@ PYBIND11_MODULE(_core, m) {
 
     py::class_<ParametricQuantumCircuit, QuantumCircuit>(m, "ParametricQuantumCircuit")
         .def(py::init<unsigned>(), py::arg("qubit_count"))
+        .def("merge_circuit", py::overload_cast<const ParametricQuantumCircuit&>(&ParametricQuantumCircuit::merge_circuit),
+             py::arg("circuit"))
         .def("copy", &ParametricQuantumCircuit::copy, py::return_value_policy::take_ownership)
         .def("add_parametric_gate",
              py::overload_cast<const QuantumGate_SingleParameter&>(&ParametricQuantumCircuit::add_parametric_gate),
