@@ -288,6 +288,12 @@ void QuantumStateMI355::normalize(double sq) {
     normalize_host(sq, data_, dim, stream_, device_number);
     check_abi("normalize");
 }
+void QuantumStateMI355::multiply_elementwise_function(const std::function<CPPCTYPE(ITYPE)>& f) {
+    std::vector<CPPCTYPE> d(dim);
+    for (ITYPE i = 0; i < dim; ++i) d[i] = f(i);
+    multi_qubit_diagonal_matrix_gate_host(qc(d.data()), data_, dim, stream_, device_number);
+    check_abi("multiply_elementwise_function");
+}
 QuantumStateBase* QuantumStateMI355::allocate_buffer() const { return new QuantumStateMI355(qubit_count, device_number); }
 QuantumStateBase* QuantumStateMI355::copy() const {
     auto* s = new QuantumStateMI355(qubit_count, device_number);
@@ -800,7 +806,12 @@ void QuantumGate_Probabilistic::optimize_ProbablisticGate() {
     for (double p : dist_) cumulative_.push_back(cumulative_.back() + p);
 }
 QuantumGate_Probabilistic::QuantumGate_Probabilistic(const QuantumGate_Probabilistic& o)
-    : QuantumGateBase(o), dist_(o.dist_), cumulative_(o.cumulative_), random_(o.random_) {
+    : QuantumGateBase(o),
+      dist_(o.dist_),
+      cumulative_(o.cumulative_),
+      random_(o.random_),
+      instrument_(o.instrument_),
+      register_(o.register_) {
     gates_.clear();
     for (auto* g : o.gates_) gates_.push_back(g->copy());
 }
@@ -829,6 +840,7 @@ void QuantumGate_Probabilistic::update_quantum_state(QuantumStateBase* state) {
         const double r = random_.uniform();
         const size_t idx = (size_t)(std::upper_bound(cumulative_.begin(), cumulative_.end(), r) - cumulative_.begin()) - 1;
         if (idx < gates_.size()) gates_[idx]->update_quantum_state(state);
+        if (instrument_) state->set_classical_value(register_, (unsigned)idx);
         return;
     }
     auto* d = as_dm(state);
@@ -850,56 +862,222 @@ void QuantumGate_Probabilistic::update_quantum_state(QuantumStateBase* state) {
     }
 }
 
-QuantumGate_CPTP::QuantumGate_CPTP(std::vector<QuantumGateMatrix> kraus) : kraus_(std::move(kraus)) {
+QuantumGate_CPTP::QuantumGate_CPTP(const std::vector<const QuantumGateBase*>& kraus) {
     std::vector<std::vector<unsigned>> lists;
-    for (auto& k : kraus_) {
-        if (!k.get_control_index_list().empty()) throw QulacsError("CPTP: Kraus operators without controls expected");
-        lists.push_back(k.get_target_index_list());
+    for (const auto* k : kraus) {
+        gates_.push_back(k->copy());
+        lists.push_back(k->get_target_index_list());
+        lists.push_back(k->get_control_index_list());
     }
     targets_ = sorted_union(lists);
     name_ = "CPTP";
-    std::vector<const QuantumGateBase*> cg;
-    for (const auto& k : kraus_) cg.push_back(&k);
-    tcomm_ = merged_commutation(cg, targets_);
+    tcomm_ = merged_commutation(std::vector<const QuantumGateBase*>(gates_.begin(), gates_.end()), targets_);
 }
-ComplexMatrix QuantumGate_CPTP::get_matrix() const { throw QulacsError("CPTP map has no single matrix"); }
+QuantumGate_CPTP::QuantumGate_CPTP(const QuantumGate_CPTP& o)
+    : QuantumGateBase(o),
+      random_(o.random_),
+      instrument_(o.instrument_),
+      register_(o.register_),
+      state_normalize_(o.state_normalize_),
+      probability_normalize_(o.probability_normalize_),
+      assign_zero_(o.assign_zero_) {
+    gates_.clear();
+    for (auto* g : o.gates_) gates_.push_back(g->copy());
+}
+QuantumGate_CPTP::~QuantumGate_CPTP() {
+    for (auto* g : gates_) delete g;
+}
+ComplexMatrix QuantumGate_CPTP::get_matrix() const { throw QulacsError(name_ + " map has no single matrix"); }
 ComplexMatrix QuantumGate_CPTP::superoperator() const {
     const size_t d = size_t(1) << targets_.size();
     ComplexMatrix S(d * d, d * d);
-    for (const auto& k : kraus_) accumulate_superop(S, embed(k.get_matrix(), k.get_target_index_list(), targets_), 1.0);
+    for (const auto* k : gates_) {
+        if (!k->has_matrix()) throw QulacsError(name_ + ": a Kraus operator has no matrix");
+        std::vector<unsigned> on;
+        const ComplexMatrix m = with_controls(k, &on);
+        accumulate_superop(S, embed(m, on, targets_), 1.0);
+    }
     return S;
 }
 void QuantumGate_CPTP::update_quantum_state(QuantumStateBase* state) {
-    check_targets(state, targets_, "CPTP");
+    check_targets(state, targets_, name_.c_str());
     if (auto* d = as_dm(state)) {
-        if (targets_.size() > 3) throw QulacsError("CPTP on a density matrix: at most 3 target qubits");
-        const ComplexMatrix S = superoperator();
-        dm_superoperator_host(targets_.data(), (UINT)targets_.size(), qc(S.data.data()), d->data(), d->dim,
-                              d->get_cuda_stream(), d->device_number);
-        check_abi("CPTP");
+        bool matrices = targets_.size() <= 3;
+        for (const auto* k : gates_) matrices = matrices && k->has_matrix();
+        if (matrices) {
+            const ComplexMatrix S = superoperator();
+            dm_superoperator_host(targets_.data(), (UINT)targets_.size(), qc(S.data.data()), d->data(), d->dim,
+                                  d->get_cuda_stream(), d->device_number);
+            check_abi(name_.c_str());
+            return;
+        }
+        // rho <- sum_i K_i rho K_i^dagger on copies (src/cppsim/gate_general.hpp:462-478)
+        std::unique_ptr<QuantumStateBase> org(d->copy()), tmp(d->allocate_buffer());
+        d->multiply_coef(0.0);
+        for (auto* k : gates_) {
+            tmp->load(org.get());
+            k->update_quantum_state(tmp.get());
+            d->add_state(tmp.get());
+        }
         return;
     }
-    // state vector: K_i is kept with probability ||K_i psi||^2 / ||psi||^2 and the
-    // state rescaled to its old norm; identity if the map is not trace preserving
-    // (src/cppsim/gate_general.hpp:433-460)
+    // state vector (src/cppsim/gate_general.hpp:433-461 and 600-640): branch i is
+    // taken when r * (sum of branch probabilities, or 1) first falls below the running
+    // sum of ||K_i psi||^2 / ||psi||^2
     const double r = random_.uniform();
     const double org = state->get_squared_norm();
     std::unique_ptr<QuantumStateBase> buf(state->allocate_buffer());
+    double total = 1.0;
+    if (probability_normalize_) {
+        total = 0.0;
+        for (auto* k : gates_) {
+            buf->load(state);
+            k->update_quantum_state(buf.get());
+            total += buf->get_squared_norm() / org;
+        }
+    }
     double sum = 0.0;
     unsigned index = 0;
-    for (auto& k : kraus_) {
+    bool chosen = false;
+    for (auto* k : gates_) {
         buf->load(state);
-        k.update_quantum_state(buf.get());
+        k->update_quantum_state(buf.get());
         const double rel = buf->get_squared_norm() / org;
         sum += rel;
-        if (r < sum) {
+        if (r * total < sum) {
             state->load(buf.get());
-            state->normalize(rel);
+            if (state_normalize_) state->normalize(rel);
+            chosen = true;
             break;
         }
         ++index;
     }
+    if (!chosen && assign_zero_) state->multiply_coef(0.0);
     if (instrument_) state->set_classical_value(register_, index);
+}
+
+QuantumGate_CP::QuantumGate_CP(const std::vector<const QuantumGateBase*>& kraus, bool state_normalize,
+                               bool probability_normalize, bool assign_zero_if_not_matched)
+    : QuantumGate_CPTP(kraus) {
+    name_ = "CP";
+    state_normalize_ = state_normalize;
+    probability_normalize_ = probability_normalize;
+    assign_zero_ = assign_zero_if_not_matched;
+}
+
+QuantumGate_LinearCombination::QuantumGate_LinearCombination(const std::vector<CPPCTYPE>& coefs,
+                                                             const std::vector<const QuantumGateBase*>& gates)
+    : coefs_(coefs) {
+    if (coefs.size() != gates.size()) throw QulacsError("LinearCombination: one coefficient per gate");
+    std::vector<std::vector<unsigned>> lists;
+    for (const auto* g : gates) {
+        gates_.push_back(g->copy());
+        lists.push_back(g->get_target_index_list());
+        lists.push_back(g->get_control_index_list());
+    }
+    targets_ = sorted_union(lists);
+    name_ = "LinearCombination";
+    tcomm_ = merged_commutation(std::vector<const QuantumGateBase*>(gates_.begin(), gates_.end()), targets_);
+}
+QuantumGate_LinearCombination::QuantumGate_LinearCombination(const QuantumGate_LinearCombination& o)
+    : QuantumGateBase(o), coefs_(o.coefs_) {
+    gates_.clear();
+    for (auto* g : o.gates_) gates_.push_back(g->copy());
+}
+QuantumGate_LinearCombination::~QuantumGate_LinearCombination() {
+    for (auto* g : gates_) delete g;
+}
+ComplexMatrix QuantumGate_LinearCombination::get_matrix() const {
+    const size_t d = size_t(1) << targets_.size();
+    ComplexMatrix m(d, d);
+    for (size_t i = 0; i < gates_.size(); ++i) {
+        std::vector<unsigned> on;
+        const ComplexMatrix e = embed(with_controls(gates_[i], &on), on, targets_);
+        for (size_t k = 0; k < m.data.size(); ++k) m.data[k] += coefs_[i] * e.data[k];
+    }
+    return m;
+}
+void QuantumGate_LinearCombination::update_quantum_state(QuantumStateBase* state) {
+    if (!state->is_state_vector()) throw QulacsError("LinearCombination: density matrices are not supported");
+    check_targets(state, targets_, "LinearCombination");
+    std::unique_ptr<QuantumStateBase> acc(state->allocate_buffer()), tmp(state->allocate_buffer());
+    acc->multiply_coef(0.0);
+    for (size_t i = 0; i < gates_.size(); ++i) {
+        tmp->load(state);
+        gates_[i]->update_quantum_state(tmp.get());
+        acc->add_state_with_coef(coefs_[i], tmp.get());
+    }
+    state->load(acc.get());
+}
+
+ClsStateReflectionGate::ClsStateReflectionGate(const QuantumStateMI355* r)
+    : r_(new QuantumStateMI355(r->qubit_count, r->device_number)) {
+    r_->load(r);
+    for (unsigned q = 0; q < r->qubit_count; ++q) targets_.push_back(q);
+    name_ = "Reflection";
+}
+ClsStateReflectionGate::ClsStateReflectionGate(const ClsStateReflectionGate& o)
+    : QuantumGateBase(o), r_(new QuantumStateMI355(o.r_->qubit_count, o.r_->device_number)) {
+    r_->load(o.r_.get());
+}
+ComplexMatrix ClsStateReflectionGate::get_matrix() const { throw QulacsError("Reflection gate has no matrix"); }
+void ClsStateReflectionGate::update_quantum_state(QuantumStateBase* state) {
+    auto* s = as_sv(state);
+    if (!s) throw QulacsError("Reflection: density matrices are not supported");
+    if (s->qubit_count != r_->qubit_count) throw QulacsError("Reflection: qubit count must equal the reflection state's");
+    if (s->device_number != r_->device_number) throw QulacsError("Reflection: states on different devices");
+    const CPPCTYPE c = inner_product(r_.get(), s);
+    s->multiply_coef(-1.0);
+    s->add_state_with_coef(2.0 * c, r_.get());
+}
+
+namespace {
+void adaptive_targets(const QuantumGateBase* g, std::vector<unsigned>* t) {
+    *t = sorted_union({g->get_target_index_list(), g->get_control_index_list()});
+}
+}  // namespace
+QuantumGate_Adaptive::QuantumGate_Adaptive(const QuantumGateBase* g, Condition c) : gate_(g->copy()), cond_(std::move(c)) {
+    adaptive_targets(g, &targets_);
+    tcomm_ = merged_commutation({g}, targets_);
+    name_ = "Adaptive";
+}
+QuantumGate_Adaptive::QuantumGate_Adaptive(const QuantumGateBase* g, ConditionWithId c, unsigned id)
+    : gate_(g->copy()), cond_id_(std::move(c)), id_((int)id) {
+    adaptive_targets(g, &targets_);
+    tcomm_ = merged_commutation({g}, targets_);
+    name_ = "Adaptive";
+}
+QuantumGate_Adaptive::QuantumGate_Adaptive(const QuantumGate_Adaptive& o)
+    : QuantumGateBase(o), gate_(o.gate_->copy()), cond_(o.cond_), cond_id_(o.cond_id_), id_(o.id_) {}
+ComplexMatrix QuantumGate_Adaptive::get_matrix() const { throw QulacsError("Adaptive gate has no matrix"); }
+void QuantumGate_Adaptive::update_quantum_state(QuantumStateBase* state) {
+    const bool go = id_ < 0 ? cond_(state->get_classical_register()) : cond_id_(state->get_classical_register(), (unsigned)id_);
+    if (go) gate_->update_quantum_state(state);
+}
+
+ClsNpairQubitGate::ClsNpairQubitGate(unsigned t1, unsigned t2, unsigned b) : block_(b) {
+    if (b == 0) throw QulacsError("FusedSWAP: block size must be positive");
+    if (std::min(t1, t2) + b > std::max(t1, t2)) throw QulacsError("FusedSWAP: the two blocks overlap");
+    for (unsigned i = 0; i < b; ++i) targets_.push_back(t1 + i);
+    for (unsigned i = 0; i < b; ++i) targets_.push_back(t2 + i);
+    tcomm_.assign(targets_.size(), 0u);
+    property_ = PROPERTY_CLIFFORD;
+    name_ = "FusedSWAP";
+}
+ComplexMatrix ClsNpairQubitGate::get_matrix() const {
+    if (2 * block_ > 12) throw QulacsError("FusedSWAP: matrix above 12 qubits");
+    const size_t p = size_t(1) << block_, d = p * p;
+    ComplexMatrix m(d, d);
+    for (size_t i = 0; i < p; ++i)
+        for (size_t j = 0; j < p; ++j) m(i * p + j, i + j * p) = 1.0;
+    return m;
+}
+void ClsNpairQubitGate::update_quantum_state(QuantumStateBase* state) {
+    check_targets(state, targets_, "FusedSWAP");
+    for (unsigned i = 0; i < block_; ++i) {
+        std::unique_ptr<QuantumGateMatrix> sw(gate::SWAP(targets_[i], targets_[block_ + i]));
+        sw->update_quantum_state(state);
+    }
 }
 
 QuantumGate_SingleParameter::QuantumGate_SingleParameter(std::vector<unsigned> t, std::vector<unsigned> p, double angle,
@@ -1127,23 +1305,62 @@ QuantumGate_Probabilistic* IndependentXZNoise(unsigned q, double p) {
     return new QuantumGate_Probabilistic({p1, p1, p2, 1 - 2 * p1 - p2}, {x.get(), z.get(), y.get(), i.get()});
 }
 QuantumGate_CPTP* AmplitudeDampingNoise(unsigned q, double p) {
-    std::vector<QuantumGateMatrix> k;
-    k.emplace_back(std::vector<unsigned>{q}, m2(1, 0, 0, std::sqrt(1 - p)));
-    k.emplace_back(std::vector<unsigned>{q}, m2(0, std::sqrt(p), 0, 0));
-    return new QuantumGate_CPTP(k);
+    const QuantumGateMatrix k0({q}, m2(1, 0, 0, std::sqrt(1 - p))), k1({q}, m2(0, std::sqrt(p), 0, 0));
+    return new QuantumGate_CPTP({&k0, &k1});
 }
 QuantumGate_Probabilistic* Probabilistic(const std::vector<double>& dist, const std::vector<QuantumGateBase*>& gates) {
     return new QuantumGate_Probabilistic(dist, gates);
 }
 QuantumGate_CPTP* CPTP(const std::vector<QuantumGateBase*>& kraus) {
-    std::vector<QuantumGateMatrix> k;
-    for (auto* g : kraus) {
-        std::vector<unsigned> on;  // a controlled Kraus operator acts as its controlled matrix
-        const ComplexMatrix m = with_controls(g, &on);
-        k.emplace_back(on, m);
-    }
-    return new QuantumGate_CPTP(k);
+    return new QuantumGate_CPTP(std::vector<const QuantumGateBase*>(kraus.begin(), kraus.end()));
 }
+QuantumGate_CP* CP(const std::vector<QuantumGateBase*>& kraus, bool state_normalize, bool probability_normalize,
+                   bool assign_zero_if_not_matched) {
+    return new QuantumGate_CP(std::vector<const QuantumGateBase*>(kraus.begin(), kraus.end()), state_normalize,
+                              probability_normalize, assign_zero_if_not_matched);
+}
+QuantumGate_Probabilistic* ProbabilisticInstrument(const std::vector<double>& dist,
+                                                   const std::vector<QuantumGateBase*>& gates, unsigned reg) {
+    auto* g = new QuantumGate_Probabilistic(dist, gates);
+    g->set_instrument(reg);
+    return g;
+}
+QuantumGate_LinearCombination* LinearCombination(const std::vector<CPPCTYPE>& coefs,
+                                                 const std::vector<QuantumGateBase*>& gates) {
+    return new QuantumGate_LinearCombination(coefs, std::vector<const QuantumGateBase*>(gates.begin(), gates.end()));
+}
+QuantumGate_CPTP* MultiQubitPauliMeasurement(const std::vector<unsigned>& t, const std::vector<unsigned>& p,
+                                             unsigned reg) {
+    std::unique_ptr<QuantumGateBase> id(Identity(0)), pg(Pauli(t, p));
+    const QuantumGate_LinearCombination plus({0.5, 0.5}, {id.get(), pg.get()}), minus({0.5, -0.5}, {id.get(), pg.get()});
+    auto* g = new QuantumGate_CPTP({&plus, &minus});
+    g->set_instrument(reg);
+    return g;
+}
+ClsStateReflectionGate* StateReflection(const QuantumStateMI355* r) { return new ClsStateReflectionGate(r); }
+QuantumGate_Adaptive* Adaptive(const QuantumGateBase* g, QuantumGate_Adaptive::Condition c) {
+    return new QuantumGate_Adaptive(g, std::move(c));
+}
+QuantumGate_Adaptive* Adaptive(const QuantumGateBase* g, QuantumGate_Adaptive::ConditionWithId c, unsigned id) {
+    return new QuantumGate_Adaptive(g, std::move(c), id);
+}
+QuantumGateMatrix* ReversibleBoolean(const std::vector<unsigned>& t, const std::function<ITYPE(ITYPE, ITYPE)>& f) {
+    std::vector<unsigned> s(t);
+    std::sort(s.begin(), s.end());
+    if (std::adjacent_find(s.begin(), s.end()) != s.end()) throw QulacsError("ReversibleBoolean: duplicated target");
+    if (t.size() > 12) throw QulacsError("ReversibleBoolean: at most 12 targets");
+    const ITYPE d = ITYPE(1) << t.size();
+    ComplexMatrix m(d, d);
+    std::vector<char> hit(d, 0);
+    for (ITYPE x = 0; x < d; ++x) {
+        const ITYPE y = f(x, d);
+        if (y >= d || hit[y]) throw QulacsError("ReversibleBoolean: the function is not a permutation of 0..2^k-1");
+        hit[y] = 1;
+        m(y, x) = 1.0;
+    }
+    return new QuantumGateMatrix(t, m, {}, {}, "ReversibleBoolean");
+}
+ClsNpairQubitGate* FusedSWAP(unsigned t1, unsigned t2, unsigned b) { return new ClsNpairQubitGate(t1, t2, b); }
 QuantumGate_CPTP* Instrument(const std::vector<QuantumGateBase*>& kraus, unsigned reg) {
     QuantumGate_CPTP* g = CPTP(kraus);
     g->set_instrument(reg);

@@ -17,6 +17,7 @@
 
 #include <complex>
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <random>
 #include <stdexcept>
@@ -147,6 +148,9 @@ class QuantumStateMI355 : public QuantumStateBase {
     // Sampling with caller-supplied uniforms (same indices as the sequential CPU loop).
     std::vector<ITYPE> sampling_with_uniforms(const std::vector<double>& uniforms) const;
     std::string to_string() const override;
+    // psi[i] *= f(i): f is evaluated on the host for every index, then one device
+    // pass multiplies (src/cppsim/state.hpp:885-891, state_gpu.hpp:353).
+    void multiply_elementwise_function(const std::function<CPPCTYPE(ITYPE)>& f);
 };
 
 // Density matrix of `qubit_count` qubits (dim x dim, row-major) in GPU memory.
@@ -204,6 +208,8 @@ class QuantumGateBase {
     virtual std::string get_name() const { return name_; }
     virtual bool is_parametric() const { return false; }
     virtual bool is_noise() const { return false; }
+    // false when get_matrix() has no meaning for the gate (channels, reflections, ...)
+    virtual bool has_matrix() const { return true; }
     std::string to_string() const;
     const std::vector<unsigned>& get_target_index_list() const { return targets_; }
     const std::vector<unsigned>& get_control_index_list() const { return controls_; }
@@ -314,6 +320,16 @@ class QuantumGate_Probabilistic : public QuantumGateBase {
     // gates in descending order of probability, so the branch search ends early
     // (src/cppsim/gate_general.hpp:350-372)
     void optimize_ProbablisticGate();
+    bool has_matrix() const override { return false; }
+    // ProbabilisticInstrument: on a state vector the drawn branch index (the gate count
+    // for the implicit identity) goes to this classical register address
+    // (src/cppsim/gate_general.hpp:230-262)
+    void set_instrument(unsigned classical_register_address) {
+        instrument_ = true;
+        register_ = classical_register_address;
+    }
+    bool is_instrument() const { return instrument_; }
+    unsigned get_classical_register_address() const { return register_; }
     // superoperator over targets_ (4^k x 4^k, member = row bits * 2^k + column bits)
     ComplexMatrix superoperator() const;
 
@@ -321,24 +337,34 @@ class QuantumGate_Probabilistic : public QuantumGateBase {
     std::vector<double> dist_, cumulative_;
     std::vector<QuantumGateBase*> gates_;
     Random random_;
+    bool instrument_ = false;
+    unsigned register_ = 0;
 };
 
-// Kraus channel sum_i K_i rho K_i^dagger (src/cppsim/gate_general.hpp QuantumGate_CPTP).
-// On a state vector one K_i is chosen with probability ||K_i psi||^2 and the
-// state renormalised; on a density matrix it is one superoperator pass.
+// Kraus channel sum_i K_i rho K_i^dagger (src/cppsim/gate_general.hpp:380-470); the K_i
+// may be any gates. On a state vector one K_i is chosen with probability
+// ||K_i psi||^2 / ||psi||^2 and the state rescaled to its old norm (identity when the
+// map is not trace preserving and no branch is hit); on a density matrix it is one
+// superoperator pass when every K_i has a matrix and at most 3 qubits are involved,
+// else the sum of the K_i applied to copies.
 class QuantumGate_CPTP : public QuantumGateBase {
    public:
-    QuantumGate_CPTP(std::vector<QuantumGateMatrix> kraus);
+    explicit QuantumGate_CPTP(const std::vector<const QuantumGateBase*>& kraus);
+    QuantumGate_CPTP(const QuantumGate_CPTP& o);
+    QuantumGate_CPTP& operator=(const QuantumGate_CPTP&) = delete;
+    ~QuantumGate_CPTP() override;
     void update_quantum_state(QuantumStateBase* state) override;
     QuantumGateBase* copy() const override { return new QuantumGate_CPTP(*this); }
     ComplexMatrix get_matrix() const override;
+    bool has_matrix() const override { return false; }
     bool is_noise() const override { return true; }
+    // superoperator over targets_ (4^k x 4^k); throws when a Kraus operator has no matrix
     ComplexMatrix superoperator() const;
     void set_seed(uint64_t seed) { random_.set_seed(seed); }
-    const std::vector<QuantumGateMatrix>& get_kraus_list() const { return kraus_; }
+    const std::vector<QuantumGateBase*>& get_gate_list() const { return gates_; }
     // Instrument: on a state vector, the index of the chosen Kraus operator (the
     // operator count when none was chosen) is written to this classical register
-    // address (src/cppsim/gate_general.hpp:380-470).
+    // address.
     void set_instrument(unsigned classical_register_address) {
         instrument_ = true;
         register_ = classical_register_address;
@@ -346,11 +372,103 @@ class QuantumGate_CPTP : public QuantumGateBase {
     bool is_instrument() const { return instrument_; }
     unsigned get_classical_register_address() const { return register_; }
 
-   private:
-    std::vector<QuantumGateMatrix> kraus_;
+   protected:
+    std::vector<QuantumGateBase*> gates_;
     Random random_;
     bool instrument_ = false;
     unsigned register_ = 0;
+    // Branch rules (a CPTP map: rescale the chosen branch, raw probabilities, leave the
+    // state alone when no branch is chosen).
+    bool state_normalize_ = true, probability_normalize_ = false, assign_zero_ = false;
+};
+
+// Completely positive map (src/cppsim/gate_general.hpp:560-690): a CPTP map whose
+// branch rules are chosen: state_normalize rescales the chosen branch,
+// probability_normalize divides the branch probabilities by their sum, and
+// assign_zero_if_not_matched zeroes the state when no branch is chosen.
+class QuantumGate_CP : public QuantumGate_CPTP {
+   public:
+    QuantumGate_CP(const std::vector<const QuantumGateBase*>& kraus, bool state_normalize, bool probability_normalize,
+                   bool assign_zero_if_not_matched);
+    QuantumGateBase* copy() const override { return new QuantumGate_CP(*this); }
+    bool get_state_normalize() const { return state_normalize_; }
+    bool get_probability_normalize() const { return probability_normalize_; }
+    bool get_assign_zero_if_not_matched() const { return assign_zero_; }
+};
+
+// sum_i c_i G_i on a state vector (src/cppsim/gate_general.hpp:17-150): each G_i acts
+// on a copy and the copies are summed with their coefficients.
+class QuantumGate_LinearCombination : public QuantumGateBase {
+   public:
+    QuantumGate_LinearCombination(const std::vector<CPPCTYPE>& coefs, const std::vector<const QuantumGateBase*>& gates);
+    QuantumGate_LinearCombination(const QuantumGate_LinearCombination& o);
+    QuantumGate_LinearCombination& operator=(const QuantumGate_LinearCombination&) = delete;
+    ~QuantumGate_LinearCombination() override;
+    void update_quantum_state(QuantumStateBase* state) override;
+    QuantumGateBase* copy() const override { return new QuantumGate_LinearCombination(*this); }
+    // sum_i c_i (matrix of G_i) over targets_
+    ComplexMatrix get_matrix() const override;
+    const std::vector<CPPCTYPE>& get_coef_list() const { return coefs_; }
+    const std::vector<QuantumGateBase*>& get_gate_list() const { return gates_; }
+
+   private:
+    std::vector<CPPCTYPE> coefs_;
+    std::vector<QuantumGateBase*> gates_;
+};
+
+// psi <- 2 <r|psi> r - psi for a fixed state r (src/cppsim/gate_reflect.hpp,
+// src/csim/update_ops_reflection.cpp:14-27); state vectors only.
+class ClsStateReflectionGate : public QuantumGateBase {
+   public:
+    explicit ClsStateReflectionGate(const QuantumStateMI355* reflection_state);
+    ClsStateReflectionGate(const ClsStateReflectionGate& o);
+    ClsStateReflectionGate& operator=(const ClsStateReflectionGate&) = delete;
+    void update_quantum_state(QuantumStateBase* state) override;
+    QuantumGateBase* copy() const override { return new ClsStateReflectionGate(*this); }
+    ComplexMatrix get_matrix() const override;
+    bool has_matrix() const override { return false; }
+    const QuantumStateMI355* get_reflection_state() const { return r_.get(); }
+
+   private:
+    std::unique_ptr<QuantumStateMI355> r_;
+};
+
+// Applies the gate when a condition on the classical register holds
+// (src/cppsim/gate_general.hpp:693-790).
+class QuantumGate_Adaptive : public QuantumGateBase {
+   public:
+    using Condition = std::function<bool(const std::vector<unsigned>&)>;
+    using ConditionWithId = std::function<bool(const std::vector<unsigned>&, unsigned)>;
+    QuantumGate_Adaptive(const QuantumGateBase* g, Condition c);
+    QuantumGate_Adaptive(const QuantumGateBase* g, ConditionWithId c, unsigned id);
+    QuantumGate_Adaptive(const QuantumGate_Adaptive& o);
+    QuantumGate_Adaptive& operator=(const QuantumGate_Adaptive&) = delete;
+    void update_quantum_state(QuantumStateBase* state) override;
+    QuantumGateBase* copy() const override { return new QuantumGate_Adaptive(*this); }
+    ComplexMatrix get_matrix() const override;
+    bool has_matrix() const override { return false; }
+
+   private:
+    std::unique_ptr<QuantumGateBase> gate_;
+    Condition cond_;
+    ConditionWithId cond_id_;
+    int id_ = -1;
+};
+
+// FusedSWAP: swaps the qubit block [t1, t1 + b) with [t2, t2 + b)
+// (src/cppsim/gate_named_npair.hpp:12-140); its matrix index takes the first block in
+// the low bits.
+class ClsNpairQubitGate : public QuantumGateBase {
+   public:
+    ClsNpairQubitGate(unsigned t1, unsigned t2, unsigned block_size);
+    void update_quantum_state(QuantumStateBase* state) override;
+    QuantumGateBase* copy() const override { return new ClsNpairQubitGate(*this); }
+    ComplexMatrix get_matrix() const override;
+    QuantumGateBase* get_inverse() const override { return copy(); }
+    unsigned get_block_size() const { return block_; }
+
+   private:
+    unsigned block_;
 };
 
 // Parametric rotation gates (src/vqcsim/parametric_gate.hpp:31-212).
@@ -419,6 +537,25 @@ QuantumGate_Probabilistic* Probabilistic(const std::vector<double>& dist, const 
 QuantumGate_CPTP* CPTP(const std::vector<QuantumGateBase*>& kraus);
 QuantumGate_CPTP* Instrument(const std::vector<QuantumGateBase*>& kraus, unsigned classical_register_address);
 QuantumGate_CPTP* Measurement(unsigned q, unsigned classical_register_address);
+QuantumGate_CP* CP(const std::vector<QuantumGateBase*>& kraus, bool state_normalize, bool probability_normalize,
+                   bool assign_zero_if_not_matched);
+QuantumGate_Probabilistic* ProbabilisticInstrument(const std::vector<double>& dist,
+                                                   const std::vector<QuantumGateBase*>& gates,
+                                                   unsigned classical_register_address);
+QuantumGate_LinearCombination* LinearCombination(const std::vector<CPPCTYPE>& coefs,
+                                                 const std::vector<QuantumGateBase*>& gates);
+// Instrument with the projectors (I + P) / 2 and (I - P) / 2 of a Pauli string
+// (src/cppsim/gate_factory.cpp:539-570): 0 is written for the +1 outcome, 1 for -1.
+QuantumGate_CPTP* MultiQubitPauliMeasurement(const std::vector<unsigned>& targets, const std::vector<unsigned>& paulis,
+                                             unsigned classical_register_address);
+ClsStateReflectionGate* StateReflection(const QuantumStateMI355* reflection_state);
+QuantumGate_Adaptive* Adaptive(const QuantumGateBase* g, QuantumGate_Adaptive::Condition c);
+QuantumGate_Adaptive* Adaptive(const QuantumGateBase* g, QuantumGate_Adaptive::ConditionWithId c, unsigned id);
+// Basis permutation |x> -> |f(x, 2^k)> on the targets (src/cppsim/gate_reversible.hpp,
+// src/csim/update_ops_reversible_boolean.cpp:5-46), evaluated once into a permutation
+// matrix; f must be a bijection of 0..2^k-1.
+QuantumGateMatrix* ReversibleBoolean(const std::vector<unsigned>& targets, const std::function<ITYPE(ITYPE, ITYPE)>& f);
+ClsNpairQubitGate* FusedSWAP(unsigned t1, unsigned t2, unsigned block_size);
 QuantumGateMatrix* TOFFOLI(unsigned control1, unsigned control2, unsigned target);
 QuantumGateMatrix* FREDKIN(unsigned control, unsigned target1, unsigned target2);
 QuantumGateMatrix* add(const QuantumGateBase* a, const QuantumGateBase* b);

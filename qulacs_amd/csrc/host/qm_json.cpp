@@ -407,16 +407,51 @@ PTree to_ptree(const QuantumGateBase& g) {
         for (const QuantumGateBase* x : p->get_gate_list()) gates.push_back(to_ptree(*x));
         pt.put("distribution", std::move(dist));
         pt.put("gate_list", std::move(gates));
-        pt.put("is_instrument", "false");
+        pt.put("is_instrument", p->is_instrument() ? "true" : "false");
+        if (p->is_instrument()) pt.put("classical_register_address", u2s(p->get_classical_register_address()));
+        return pt;
+    }
+    // src/cppsim/gate_general.hpp:676-690
+    if (const auto* c = dynamic_cast<const QuantumGate_CP*>(&g)) {
+        pt.put("name", "CPMapGate");
+        PTree gates;
+        for (const QuantumGateBase* k : c->get_gate_list()) gates.push_back(to_ptree(*k));
+        pt.put("gate_list", std::move(gates));
+        pt.put("state_normalize", c->get_state_normalize() ? "true" : "false");
+        pt.put("probability_normalize", c->get_probability_normalize() ? "true" : "false");
+        pt.put("assign_zero_if_not_matched", c->get_assign_zero_if_not_matched() ? "true" : "false");
         return pt;
     }
     if (const auto* c = dynamic_cast<const QuantumGate_CPTP*>(&g)) {
         pt.put("name", "CPTPMapGate");
         PTree gates;
-        for (const QuantumGateMatrix& k : c->get_kraus_list()) gates.push_back(to_ptree(k));
+        for (const QuantumGateBase* k : c->get_gate_list()) gates.push_back(to_ptree(*k));
         pt.put("gate_list", std::move(gates));
         pt.put("is_instrument", c->is_instrument() ? "true" : "false");
         if (c->is_instrument()) pt.put("classical_register_address", u2s(c->get_classical_register_address()));
+        return pt;
+    }
+    // src/cppsim/gate_general.hpp:131-146: each coefficient as the text "(re,im)"
+    if (const auto* c = dynamic_cast<const QuantumGate_LinearCombination*>(&g)) {
+        pt.put("name", "LinearCombinationGate");
+        PTree coefs, gates;
+        for (CPPCTYPE z : c->get_coef_list())
+            coefs.push_back(PTree("(" + ptree::format_double(z.real()) + "," + ptree::format_double(z.imag()) + ")"));
+        for (const QuantumGateBase* k : c->get_gate_list()) gates.push_back(to_ptree(*k));
+        pt.put("coefs", std::move(coefs));
+        pt.put("gate_list", std::move(gates));
+        return pt;
+    }
+    // src/cppsim/gate_reflect.hpp:98-103
+    if (const auto* r = dynamic_cast<const ClsStateReflectionGate*>(&g)) {
+        pt.put("name", "StateReflectionGate");
+        pt.put("reflection_state", to_ptree(*r->get_reflection_state()));
+        return pt;
+    }
+    // src/cppsim/gate_named_npair.hpp:117-128
+    if (dynamic_cast<const ClsNpairQubitGate*>(&g)) {
+        pt.put("name", "FusedSWAPGate");
+        pt.put("target_qubit_list", uint_array_pt(g.get_target_index_list()));
         return pt;
     }
     throw QulacsError("to_json: gate " + g.get_name() + " has no JSON form");
@@ -473,16 +508,27 @@ QuantumGateBase* gate_from_ptree(const PTree& pt) {
         if (name == "PauliGate") return gate::Pauli(p.get_index_list(), p.get_pauli_id_list());
         return gate::PauliRotation(p.get_index_list(), p.get_pauli_id_list(), pt.get_double("angle"));
     }
-    if (name == "ProbabilisticGate" || name == "CPTPMapGate") {
-        const bool instrument = pt.get_bool("is_instrument");
-        if (instrument && name == "ProbabilisticGate")
-            throw QulacsError("JSON: probabilistic instruments are not supported");
+    if (name == "ProbabilisticGate" || name == "CPTPMapGate" || name == "CPMapGate" || name == "LinearCombinationGate") {
         std::vector<std::unique_ptr<QuantumGateBase>> owned;
         std::vector<QuantumGateBase*> gates;
         for (const auto& kv : pt.child("gate_list").children) {
             owned.emplace_back(gate_from_ptree(kv.second));
             gates.push_back(owned.back().get());
         }
+        if (name == "CPMapGate")
+            return gate::CP(gates, pt.get_bool("state_normalize"), pt.get_bool("probability_normalize"),
+                            pt.get_bool("assign_zero_if_not_matched"));
+        if (name == "LinearCombinationGate") {
+            std::vector<CPPCTYPE> coefs;
+            for (const auto& kv : pt.child("coefs").children) {
+                double re = 0, im = 0;
+                if (std::sscanf(kv.second.value.c_str(), "(%lf,%lf)", &re, &im) != 2)
+                    throw QulacsError("JSON: bad LinearCombination coefficient " + kv.second.value);
+                coefs.emplace_back(re, im);
+            }
+            return gate::LinearCombination(coefs, gates);
+        }
+        const bool instrument = pt.get_bool("is_instrument");
         if (name == "CPTPMapGate")
             return instrument ? gate::Instrument(gates, pt.get_uint("classical_register_address")) : gate::CPTP(gates);
         std::vector<double> dist;
@@ -491,7 +537,19 @@ QuantumGateBase* gate_from_ptree(const PTree& pt) {
             leaf.put("p", kv.second.value);
             dist.push_back(leaf.get_double("p"));
         }
+        if (instrument) return gate::ProbabilisticInstrument(dist, gates, pt.get_uint("classical_register_address"));
         return gate::Probabilistic(dist, gates);
+    }
+    if (name == "StateReflectionGate") {
+        std::unique_ptr<QuantumStateBase> r(state_from_ptree(pt.child("reflection_state")));
+        auto* sv = dynamic_cast<QuantumStateMI355*>(r.get());
+        if (!sv) throw QulacsError("JSON: a reflection state must be a state vector");
+        return gate::StateReflection(sv);
+    }
+    if (name == "FusedSWAPGate") {
+        const std::vector<unsigned> t = uint_array_from(pt.child("target_qubit_list"));
+        if (t.empty() || t.size() % 2) throw QulacsError("JSON: FusedSWAP needs two equal blocks");
+        return gate::FusedSWAP(t[0], t[t.size() / 2], (unsigned)(t.size() / 2));
     }
     throw QulacsError("unknown value for property \"name\":" + name);
 }

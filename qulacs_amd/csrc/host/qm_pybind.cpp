@@ -4,6 +4,7 @@
 // bindings so that `from qulacs_amd import QuantumState, gate, ...` replaces
 // `from qulacs import ...` for this path.
 #include <pybind11/complex.h>
+#include <pybind11/functional.h>
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
@@ -310,6 +311,7 @@ PYBIND11_MODULE(_core, m) {
     py::class_<QuantumStateMI355, QuantumStateBase>(m, "QuantumStateMI355")
         .def(py::init<unsigned, unsigned>(), py::arg("qubit_count"), py::arg("gpu_id") = 0)
         .def("get_vector", [](const QuantumStateMI355& s) { return from_vector(s.duplicate_data_cpp()); })
+        .def("multiply_elementwise_function", &QuantumStateMI355::multiply_elementwise_function, py::arg("func"))
         .def("get_amplitude",
              [](const QuantumStateMI355& s, ITYPE i) {
                  if (i >= s.dim) throw QulacsError("get_amplitude: index out of range");
@@ -408,6 +410,7 @@ PYBIND11_MODULE(_core, m) {
     py::class_<QuantumGate_Probabilistic, QuantumGateBase>(m, "QuantumGate_Probabilistic")
         .def("get_distribution", &QuantumGate_Probabilistic::get_distribution)
         .def("get_cumulative_distribution", &QuantumGate_Probabilistic::get_cumulative_distribution)
+        .def("is_instrument", &QuantumGate_Probabilistic::is_instrument)
         .def("optimize_ProbablisticGate", &QuantumGate_Probabilistic::optimize_ProbablisticGate)
         .def("get_gate_list",
              [](const QuantumGate_Probabilistic& g) {
@@ -418,17 +421,30 @@ PYBIND11_MODULE(_core, m) {
              py::return_value_policy::take_ownership)
         .def("set_seed", &QuantumGate_Probabilistic::set_seed, py::arg("seed"))
         .def("get_superoperator", [](const QuantumGate_Probabilistic& g) { return from_matrix(g.superoperator()); });
+    py::class_<QuantumGate_LinearCombination, QuantumGateBase>(m, "QuantumGate_LinearCombination")
+        .def("get_coef_list", &QuantumGate_LinearCombination::get_coef_list)
+        .def("get_gate_list",
+             [](const QuantumGate_LinearCombination& g) {
+                 std::vector<QuantumGateBase*> out;
+                 for (const QuantumGateBase* x : g.get_gate_list()) out.push_back(x->copy());
+                 return out;
+             },
+             py::return_value_policy::take_ownership);
+    py::class_<ClsStateReflectionGate, QuantumGateBase>(m, "ClsStateReflectionGate");
+    py::class_<QuantumGate_Adaptive, QuantumGateBase>(m, "QuantumGate_Adaptive");
+    py::class_<ClsNpairQubitGate, QuantumGateBase>(m, "ClsNpairQubitGate");
     py::class_<QuantumGate_CPTP, QuantumGateBase>(m, "QuantumGate_CPTP")
         .def("set_seed", &QuantumGate_CPTP::set_seed, py::arg("seed"))
         .def("get_gate_list",
              [](const QuantumGate_CPTP& g) {
                  std::vector<QuantumGateBase*> out;
-                 for (const QuantumGateMatrix& k : g.get_kraus_list()) out.push_back(k.copy());
+                 for (const QuantumGateBase* k : g.get_gate_list()) out.push_back(k->copy());
                  return out;
              },
              py::return_value_policy::take_ownership)
         .def("get_superoperator", [](const QuantumGate_CPTP& g) { return from_matrix(g.superoperator()); })
         .def("is_instrument", &QuantumGate_CPTP::is_instrument);
+    py::class_<QuantumGate_CP, QuantumGate_CPTP>(m, "QuantumGate_CP");
     py::class_<QuantumGate_SingleParameter, QuantumGateBase>(m, "QuantumGate_SingleParameter")
         .def("get_parameter_value", &QuantumGate_SingleParameter::get_parameter_value)
         .def("set_parameter_value", &QuantumGate_SingleParameter::set_parameter_value, py::arg("value"));
@@ -531,6 +547,30 @@ PYBIND11_MODULE(_core, m) {
            py::arg("gate1"), py::arg("gate2"));
     mg.def("add", py::overload_cast<const std::vector<QuantumGateBase*>&>(&gate::add), own_, py::arg("gate_list"));
     mg.def("Instrument", &gate::Instrument, own_, py::arg("kraus_list"), py::arg("register"));
+    mg.def("CP", &gate::CP, own_, py::arg("kraus_list"), py::arg("state_normalize"), py::arg("probability_normalize"),
+           py::arg("assign_zero_if_not_matched"));
+    mg.def("ProbabilisticInstrument", &gate::ProbabilisticInstrument, own_, py::arg("prob_list"), py::arg("gate_list"),
+           py::arg("register"));
+    mg.def("LinearCombination", &gate::LinearCombination, own_, py::arg("coefs"), py::arg("gate_list"));
+    mg.def("MultiQubitPauliMeasurement", &gate::MultiQubitPauliMeasurement, own_, py::arg("target_index_list"),
+           py::arg("pauli_id_list"), py::arg("classical_register_address"));
+    mg.def(
+        "MultiQubitPauliMeasurement",
+        [](const std::vector<unsigned>& t, const std::vector<unsigned>& p, unsigned reg, uint64_t seed) {
+            QuantumGate_CPTP* g = gate::MultiQubitPauliMeasurement(t, p, reg);
+            g->set_seed(seed);
+            return g;
+        },
+        own_, py::arg("target_index_list"), py::arg("pauli_id_list"), py::arg("classical_register_address"),
+        py::arg("seed"));
+    mg.def("StateReflection", &gate::StateReflection, own_, py::arg("state"));
+    mg.def("Adaptive", py::overload_cast<const QuantumGateBase*, QuantumGate_Adaptive::Condition>(&gate::Adaptive), own_,
+           py::arg("gate"), py::arg("condition"));
+    mg.def("Adaptive",
+           py::overload_cast<const QuantumGateBase*, QuantumGate_Adaptive::ConditionWithId, unsigned>(&gate::Adaptive),
+           own_, py::arg("gate"), py::arg("condition"), py::arg("id"));
+    mg.def("ReversibleBoolean", &gate::ReversibleBoolean, own_, py::arg("index_list"), py::arg("func"));
+    mg.def("FusedSWAP", &gate::FusedSWAP, own_, py::arg("target1"), py::arg("target2"), py::arg("block_size"));
     mg.def("Measurement", &gate::Measurement, own_, py::arg("index"), py::arg("register"));
     mg.def(
         "Measurement",
@@ -553,6 +593,9 @@ PYBIND11_MODULE(_core, m) {
     py::class_<QuantumCircuit>(m, "QuantumCircuit")
         .def(py::init<unsigned>(), py::arg("qubit_count"))
         .def("merge_circuit", &QuantumCircuit::merge_circuit, py::arg("circuit"))
+        .def("add_FusedSWAP_gate",
+             [](QuantumCircuit& c, unsigned t1, unsigned t2, unsigned b) { c.add_gate_consume(gate::FusedSWAP(t1, t2, b)); },
+             py::arg("target1"), py::arg("target2"), py::arg("block_size"))
         .def("add_noise_gate", &QuantumCircuit::add_noise_gate, py::arg("gate"), py::arg("NoiseType"),
              py::arg("NoiseProbability"))
         .def("add_observable_rotation_gate", &QuantumCircuit::add_observable_rotation_gate, py::arg("observable"),
