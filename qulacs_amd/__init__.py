@@ -35,16 +35,23 @@ except ImportError as e:  # pragma: no cover - exercised only without a build
     ) from e
 
 from ._core import (  # noqa: F401
+    CausalConeSimulator,
+    ClsNpairQubitGate,
     ClsPauliGate,
+    ClsStateReflectionGate,
     DensityMatrixMI355,
     GeneralQuantumOperator,
     GradCalculator,
+    NoiseSimulator,
     Observable,
     ParametricQuantumCircuit,
     PauliOperator,
     QuantumCircuit,
     QuantumCircuitSimulator,
+    QuantumGate_Adaptive,
+    QuantumGate_CP,
     QuantumGate_CPTP,
+    QuantumGate_LinearCombination,
     QuantumGate_Probabilistic,
     QuantumGate_SingleParameter,
     QuantumGateBase,
@@ -53,6 +60,7 @@ from ._core import (  # noqa: F401
     QuantumStateBase,
     QuantumStateMI355,
     QulacsError,
+    SimulationResult,
     check_build_for_mpi,
     circuit,
     gate,

@@ -569,6 +569,17 @@ bool QuantumGateBase::is_commute(const QuantumGateBase* o) const {
     return true;
 }
 
+QuantumGateBase* QuantumGateBase::create_gate_whose_qubit_indices_are_replaced(const std::vector<unsigned>& t,
+                                                                              const std::vector<unsigned>& c) const {
+    if (t.size() != targets_.size()) throw QulacsError("create_gate_whose_qubit_indices_are_replaced: target count differs");
+    if (c.size() != controls_.size())
+        throw QulacsError("create_gate_whose_qubit_indices_are_replaced: control count differs");
+    QuantumGateBase* g = copy();
+    g->targets_ = t;
+    g->controls_ = c;
+    return g;
+}
+
 // src/cppsim/gate.cpp:100-126
 std::string QuantumGateBase::to_string() const {
     std::ostringstream os;
@@ -1078,6 +1089,82 @@ void ClsNpairQubitGate::update_quantum_state(QuantumStateBase* state) {
         std::unique_ptr<QuantumGateMatrix> sw(gate::SWAP(targets_[i], targets_[block_ + i]));
         sw->update_quantum_state(state);
     }
+}
+
+namespace {
+// old qubit -> new qubit from a gate's (targets, controls) and their replacements
+std::vector<std::pair<unsigned, unsigned>> qubit_map(const QuantumGateBase& g, const std::vector<unsigned>& t,
+                                                     const std::vector<unsigned>& c) {
+    const auto &gt = g.get_target_index_list(), &gc = g.get_control_index_list();
+    if (t.size() != gt.size() || c.size() != gc.size())
+        throw QulacsError("create_gate_whose_qubit_indices_are_replaced: qubit count differs");
+    std::vector<std::pair<unsigned, unsigned>> m;
+    for (size_t i = 0; i < gt.size(); ++i) m.emplace_back(gt[i], t[i]);
+    for (size_t i = 0; i < gc.size(); ++i) m.emplace_back(gc[i], c[i]);
+    return m;
+}
+std::vector<unsigned> map_qubits(const std::vector<unsigned>& l, const std::vector<std::pair<unsigned, unsigned>>& m) {
+    std::vector<unsigned> out;
+    for (unsigned q : l) {
+        auto it = std::find_if(m.begin(), m.end(), [q](const std::pair<unsigned, unsigned>& e) { return e.first == q; });
+        if (it == m.end()) throw QulacsError("create_gate_whose_qubit_indices_are_replaced: unmapped qubit");
+        out.push_back(it->second);
+    }
+    return out;
+}
+QuantumGateBase* remap_gate(const QuantumGateBase* g, const std::vector<std::pair<unsigned, unsigned>>& m) {
+    return g->create_gate_whose_qubit_indices_are_replaced(map_qubits(g->get_target_index_list(), m),
+                                                           map_qubits(g->get_control_index_list(), m));
+}
+}  // namespace
+
+QuantumGateBase* QuantumGate_Probabilistic::create_gate_whose_qubit_indices_are_replaced(
+    const std::vector<unsigned>& t, const std::vector<unsigned>& c) const {
+    const auto m = qubit_map(*this, t, c);
+    auto* r = new QuantumGate_Probabilistic(*this);
+    for (auto*& g : r->gates_) {
+        QuantumGateBase* n = remap_gate(g, m);
+        delete g;
+        g = n;
+    }
+    r->targets_ = t;
+    r->controls_ = c;
+    return r;
+}
+QuantumGateBase* QuantumGate_CPTP::create_gate_whose_qubit_indices_are_replaced(const std::vector<unsigned>& t,
+                                                                               const std::vector<unsigned>& c) const {
+    const auto m = qubit_map(*this, t, c);
+    auto* r = static_cast<QuantumGate_CPTP*>(copy());
+    for (auto*& g : r->gates_) {
+        QuantumGateBase* n = remap_gate(g, m);
+        delete g;
+        g = n;
+    }
+    r->targets_ = t;
+    r->controls_ = c;
+    return r;
+}
+QuantumGateBase* QuantumGate_LinearCombination::create_gate_whose_qubit_indices_are_replaced(
+    const std::vector<unsigned>& t, const std::vector<unsigned>& c) const {
+    const auto m = qubit_map(*this, t, c);
+    auto* r = new QuantumGate_LinearCombination(*this);
+    for (auto*& g : r->gates_) {
+        QuantumGateBase* n = remap_gate(g, m);
+        delete g;
+        g = n;
+    }
+    r->targets_ = t;
+    r->controls_ = c;
+    return r;
+}
+QuantumGateBase* QuantumGate_Adaptive::create_gate_whose_qubit_indices_are_replaced(const std::vector<unsigned>& t,
+                                                                                   const std::vector<unsigned>& c) const {
+    const auto m = qubit_map(*this, t, c);
+    auto* r = new QuantumGate_Adaptive(*this);
+    r->gate_.reset(remap_gate(gate_.get(), m));
+    r->targets_ = t;
+    r->controls_ = c;
+    return r;
 }
 
 QuantumGate_SingleParameter::QuantumGate_SingleParameter(std::vector<unsigned> t, std::vector<unsigned> p, double angle,
@@ -2197,6 +2284,193 @@ GeneralQuantumOperator* to_general_quantum_operator(const QuantumGateBase* g, un
             op->add_operator(PauliOperator(idx, pid, tr / (double)d));
         }
     return op.release();
+}
+
+// ------------------------------------------------------------------ noise simulator
+NoiseSimulator::Result::~Result() {
+    for (auto& p : result) delete p.first;
+}
+std::vector<ITYPE> NoiseSimulator::Result::sampling() const {
+    std::vector<ITYPE> out;
+    for (const auto& p : result) {
+        const std::vector<ITYPE> s = p.first->sampling(p.second);
+        out.insert(out.end(), s.begin(), s.end());
+    }
+    return out;
+}
+NoiseSimulator::NoiseSimulator(const QuantumCircuit* c, const QuantumStateMI355* init)
+    : circuit_(c->copy()),
+      initial_(new QuantumStateMI355(c->get_qubit_count(), init ? init->device_number : 0)) {
+    if (init) {
+        if (init->qubit_count != c->get_qubit_count()) throw QulacsError("NoiseSimulator: qubit counts differ");
+        initial_->load(init);
+    } else {
+        initial_->set_zero_state();
+    }
+    for (auto* g : circuit_->gates_) {
+        if (auto* p = dynamic_cast<QuantumGate_Probabilistic*>(g)) p->optimize_ProbablisticGate();
+        else if (g->is_noise())
+            throw QulacsError("NoiseSimulator: only probabilistic noise gates can be sampled (got " + g->get_name() + ")");
+    }
+}
+// branch by lower_bound on the cumulative distribution (src/cppsim/noisesimulator.cpp:164-182);
+// the index past the last gate is the implicit identity
+unsigned NoiseSimulator::choose_branch(const QuantumGateBase* g) {
+    const auto* p = dynamic_cast<const QuantumGate_Probabilistic*>(g);
+    if (!p) return 0;
+    const auto& cum = p->get_cumulative_distribution();
+    const double r = random_.uniform();
+    const long i = (long)(std::lower_bound(cum.begin(), cum.end(), r) - cum.begin()) - 1;
+    return (unsigned)std::max(0L, i);
+}
+void NoiseSimulator::apply_from(const std::vector<unsigned>& branch, QuantumStateMI355* s, size_t start) const {
+    for (size_t k = start; k < circuit_->gates_.size(); ++k) {
+        const QuantumGateBase* g = circuit_->gates_[k];
+        if (const auto* p = dynamic_cast<const QuantumGate_Probabilistic*>(g)) {
+            if (branch[k] < p->get_gate_list().size()) p->get_gate_list()[branch[k]]->update_quantum_state(s);
+        } else {
+            circuit_->gates_[k]->update_quantum_state(s);
+        }
+    }
+}
+std::vector<ITYPE> NoiseSimulator::execute(unsigned count) {
+    std::unique_ptr<Result> r(execute_and_get_result(count));
+    return r->sampling();
+}
+NoiseSimulator::Result* NoiseSimulator::execute_and_get_result(unsigned count) {
+    const size_t ng = circuit_->gates_.size();
+    std::vector<std::vector<unsigned>> pick(count, std::vector<unsigned>(ng, 0));
+    for (auto& row : pick)
+        for (size_t k = 0; k < ng; ++k) row[k] = choose_branch(circuit_->gates_[k]);
+    // shots with the same branches share one trajectory; trajectories in descending
+    // lexicographic order, so those starting with branch 0 share the longest prefix
+    std::sort(pick.begin(), pick.end());
+    std::reverse(pick.begin(), pick.end());
+    std::vector<std::pair<std::vector<unsigned>, unsigned>> req;
+    for (size_t i = 0; i < pick.size(); ++i) {
+        if (!req.empty() && req.back().first == pick[i]) ++req.back().second;
+        else req.emplace_back(pick[i], 1u);
+    }
+    auto out = std::make_unique<Result>();
+    QuantumStateMI355 common(initial_->qubit_count, initial_->device_number);
+    common.load(initial_.get());
+    size_t done = 0;  // gates [0, done) are applied to `common`
+    for (const auto& rq : req) {
+        const auto& br = rq.first;
+        while (done < ng && br[done] == 0) {
+            const QuantumGateBase* g = circuit_->gates_[done];
+            if (const auto* p = dynamic_cast<const QuantumGate_Probabilistic*>(g)) {
+                if (!p->get_gate_list().empty()) p->get_gate_list()[0]->update_quantum_state(&common);
+            } else {
+                circuit_->gates_[done]->update_quantum_state(&common);
+            }
+            ++done;
+        }
+        auto* st = new QuantumStateMI355(common.qubit_count, common.device_number);
+        st->load(&common);
+        out->result.emplace_back(st, rq.second);
+        apply_from(br, st, done);
+    }
+    return out.release();
+}
+
+// ------------------------------------------------------------------ causal cone simulator
+namespace {
+struct DisjointSets {
+    std::vector<int> up;  // negative size at a root
+    explicit DisjointSets(size_t n) : up(n, -1) {}
+    int root(int a) { return up[a] < 0 ? a : (up[a] = root(up[a])); }
+    int size(int a) { return -up[root(a)]; }
+    void join(int a, int b) {
+        a = root(a), b = root(b);
+        if (a == b) return;
+        if (size(a) < size(b)) std::swap(a, b);
+        up[a] += up[b];
+        up[b] = a;
+    }
+};
+}  // namespace
+
+CausalConeSimulator::CausalConeSimulator(const ParametricQuantumCircuit& c, const Observable& o)
+    : circuit_(static_cast<ParametricQuantumCircuit*>(c.copy())), observable_(new Observable(o.get_qubit_count())) {
+    for (size_t i = 0; i < o.get_term_count(); ++i) observable_->add_operator(o.get_term(i));
+}
+CausalConeSimulator::~CausalConeSimulator() {
+    for (auto& row : circuits_)
+        for (auto* c : row) delete c;
+}
+void CausalConeSimulator::build() {
+    built_ = true;
+    const unsigned n = circuit_->get_qubit_count();
+    const size_t ng = circuit_->get_gate_count();
+    for (size_t t = 0; t < observable_->get_term_count(); ++t) {
+        const PauliOperator& term = observable_->get_term(t);
+        DisjointSets ds(n);
+        std::vector<char> qubit_used(n, 0), gate_used(ng, 0);
+        for (unsigned q : term.get_index_list()) qubit_used.at(q) = 1;
+        for (size_t k = ng; k-- > 0;) {  // the light cone, last gate first
+            const QuantumGateBase* g = circuit_->get_gate(k);
+            const auto &tl = g->get_target_index_list(), &cl = g->get_control_index_list();
+            bool hit = false;
+            for (unsigned q : tl) hit = hit || qubit_used[q];
+            for (unsigned q : cl) hit = hit || qubit_used[q];
+            if (!hit) continue;
+            gate_used[k] = 1;
+            for (unsigned q : tl) qubit_used[q] = 1;
+            for (unsigned q : cl) qubit_used[q] = 1;
+            for (size_t j = 1; j < tl.size(); ++j) ds.join(tl[j - 1], tl[j]);
+            for (size_t j = 1; j < cl.size(); ++j) ds.join(cl[j - 1], cl[j]);
+            if (!tl.empty() && !cl.empty()) ds.join(tl[0], cl[0]);
+        }
+        std::vector<int> roots;
+        for (unsigned q = 0; q < n; ++q)
+            if (qubit_used[q] && ds.root(q) == (int)q) roots.push_back((int)q);
+        std::vector<ParametricQuantumCircuit*> row;
+        std::vector<PauliOperator> prow;
+        for (int root : roots) {
+            std::vector<int> code(n, -1);
+            unsigned m = 0;
+            for (unsigned q = 0; q < n; ++q)
+                if (ds.root(q) == root) code[q] = (int)m++;
+            auto* sub = new ParametricQuantumCircuit(m);
+            auto enc = [&](const std::vector<unsigned>& l) {
+                std::vector<unsigned> r;
+                for (unsigned q : l) r.push_back((unsigned)code[q]);
+                return r;
+            };
+            for (size_t k = 0; k < ng; ++k) {
+                if (!gate_used[k]) continue;
+                const QuantumGateBase* g = circuit_->get_gate(k);
+                if (ds.root(g->get_target_index_list()[0]) != root) continue;
+                sub->add_gate_consume(g->create_gate_whose_qubit_indices_are_replaced(enc(g->get_target_index_list()),
+                                                                                      enc(g->get_control_index_list())));
+            }
+            PauliOperator p(1.0);
+            const auto &il = term.get_index_list(), &pl = term.get_pauli_id_list();
+            for (size_t j = 0; j < il.size(); ++j)
+                if (ds.root(il[j]) == root) p.add_single_Pauli((unsigned)code[il[j]], pl[j]);
+            row.push_back(sub);
+            prow.push_back(p);
+        }
+        circuits_.push_back(row);
+        paulis_.push_back(prow);
+        coefs_.push_back(term.get_coef());
+    }
+}
+CPPCTYPE CausalConeSimulator::get_expectation_value() {
+    if (!built_) build();
+    CPPCTYPE sum = 0.0;
+    for (size_t t = 0; t < circuits_.size(); ++t) {
+        CPPCTYPE e = 1.0;
+        for (size_t j = 0; j < circuits_[t].size(); ++j) {
+            QuantumStateMI355 st(circuits_[t][j]->get_qubit_count());
+            st.set_zero_state();
+            circuits_[t][j]->update_quantum_state(&st);
+            e *= paulis_[t][j].get_expectation_value(&st);
+        }
+        sum += e * coefs_[t];
+    }
+    return sum;
 }
 
 // ------------------------------------------------------------------ OpenFermion text

@@ -210,6 +210,10 @@ class QuantumGateBase {
     virtual bool is_noise() const { return false; }
     // false when get_matrix() has no meaning for the gate (channels, reflections, ...)
     virtual bool has_matrix() const { return true; }
+    // A copy acting on other qubits: targets[i] replaces target i and controls[i]
+    // control i (src/cppsim/gate.hpp:283-310); gates made of gates remap those too.
+    virtual QuantumGateBase* create_gate_whose_qubit_indices_are_replaced(const std::vector<unsigned>& targets,
+                                                                          const std::vector<unsigned>& controls) const;
     std::string to_string() const;
     const std::vector<unsigned>& get_target_index_list() const { return targets_; }
     const std::vector<unsigned>& get_control_index_list() const { return controls_; }
@@ -314,6 +318,8 @@ class QuantumGate_Probabilistic : public QuantumGateBase {
     ComplexMatrix get_matrix() const override;
     bool is_noise() const override { return true; }
     void set_seed(uint64_t seed) { random_.set_seed(seed); }
+    QuantumGateBase* create_gate_whose_qubit_indices_are_replaced(const std::vector<unsigned>& targets,
+                                                                  const std::vector<unsigned>& controls) const override;
     const std::vector<double>& get_distribution() const { return dist_; }
     const std::vector<double>& get_cumulative_distribution() const { return cumulative_; }
     const std::vector<QuantumGateBase*>& get_gate_list() const { return gates_; }
@@ -361,6 +367,8 @@ class QuantumGate_CPTP : public QuantumGateBase {
     // superoperator over targets_ (4^k x 4^k); throws when a Kraus operator has no matrix
     ComplexMatrix superoperator() const;
     void set_seed(uint64_t seed) { random_.set_seed(seed); }
+    QuantumGateBase* create_gate_whose_qubit_indices_are_replaced(const std::vector<unsigned>& targets,
+                                                                  const std::vector<unsigned>& controls) const override;
     const std::vector<QuantumGateBase*>& get_gate_list() const { return gates_; }
     // Instrument: on a state vector, the index of the chosen Kraus operator (the
     // operator count when none was chosen) is written to this classical register
@@ -410,6 +418,8 @@ class QuantumGate_LinearCombination : public QuantumGateBase {
     ComplexMatrix get_matrix() const override;
     const std::vector<CPPCTYPE>& get_coef_list() const { return coefs_; }
     const std::vector<QuantumGateBase*>& get_gate_list() const { return gates_; }
+    QuantumGateBase* create_gate_whose_qubit_indices_are_replaced(const std::vector<unsigned>& targets,
+                                                                  const std::vector<unsigned>& controls) const override;
 
    private:
     std::vector<CPPCTYPE> coefs_;
@@ -447,6 +457,8 @@ class QuantumGate_Adaptive : public QuantumGateBase {
     QuantumGateBase* copy() const override { return new QuantumGate_Adaptive(*this); }
     ComplexMatrix get_matrix() const override;
     bool has_matrix() const override { return false; }
+    QuantumGateBase* create_gate_whose_qubit_indices_are_replaced(const std::vector<unsigned>& targets,
+                                                                  const std::vector<unsigned>& controls) const override;
 
    private:
     std::unique_ptr<QuantumGateBase> gate_;
@@ -571,6 +583,7 @@ QuantumGate_SingleParameter* ParametricPauliRotation(const std::vector<unsigned>
 
 // ------------------------------------------------------------------ circuits
 class GeneralQuantumOperator;
+class NoiseSimulator;
 
 class QuantumCircuit {
    public:
@@ -650,6 +663,7 @@ class QuantumCircuit {
 
    protected:
     friend class QuantumCircuitOptimizer;
+    friend class NoiseSimulator;
     unsigned qubit_count_;
     std::vector<QuantumGateBase*> gates_;
 };
@@ -869,6 +883,60 @@ class Observable : public GeneralQuantumOperator {
 // 00 I, 01 X, 10 Y, 11 Z on target h). Controlled gates are rejected.
 GeneralQuantumOperator* to_general_quantum_operator(const QuantumGateBase* gate, unsigned GQO_qubits,
                                                     double tol = 1e-6);
+
+// Samples a circuit whose noise gates are probabilistic mixtures by drawing one branch
+// per noise gate per shot, grouping shots with the same branches, and simulating each
+// distinct trajectory once from the longest prefix they share
+// (src/cppsim/noisesimulator.cpp:18-198). Other noise gates (Kraus maps, whose branch
+// depends on the state) are rejected.
+class NoiseSimulator {
+   public:
+    struct Result {
+        std::vector<std::pair<QuantumStateMI355*, unsigned>> result;  // (final state, shot count)
+        Result() = default;
+        Result(const Result&) = delete;
+        Result& operator=(const Result&) = delete;
+        ~Result();
+        // each state sampled its shot count times, in order
+        std::vector<ITYPE> sampling() const;
+    };
+    NoiseSimulator(const QuantumCircuit* circuit, const QuantumStateMI355* initial_state = nullptr);
+    std::vector<ITYPE> execute(unsigned sample_count);
+    Result* execute_and_get_result(unsigned sample_count);
+    void set_seed(uint64_t seed) { random_.set_seed(seed); }
+
+   private:
+    std::unique_ptr<QuantumCircuit> circuit_;
+    std::unique_ptr<QuantumStateMI355> initial_;
+    Random random_;
+    unsigned choose_branch(const QuantumGateBase* g);
+    void apply_from(const std::vector<unsigned>& branches, QuantumStateMI355* s, size_t start) const;
+};
+
+// Expectation values from light cones: for each term, the gates that can reach its
+// qubits (scanning the circuit backwards), split into independent qubit groups by
+// union-find, each group simulated on its own smaller state
+// (src/vqcsim/causalcone_simulator.hpp:58-212).
+class CausalConeSimulator {
+   public:
+    CausalConeSimulator(const ParametricQuantumCircuit& circuit, const Observable& observable);
+    ~CausalConeSimulator();
+    CausalConeSimulator(const CausalConeSimulator&) = delete;
+    CausalConeSimulator& operator=(const CausalConeSimulator&) = delete;
+    void build();
+    CPPCTYPE get_expectation_value();
+    const std::vector<std::vector<ParametricQuantumCircuit*>>& get_circuit_list() const { return circuits_; }
+    const std::vector<std::vector<PauliOperator>>& get_pauli_operator_list() const { return paulis_; }
+    const std::vector<CPPCTYPE>& get_coef_list() const { return coefs_; }
+
+   private:
+    std::unique_ptr<ParametricQuantumCircuit> circuit_;
+    std::unique_ptr<Observable> observable_;
+    std::vector<std::vector<ParametricQuantumCircuit*>> circuits_;
+    std::vector<std::vector<PauliOperator>> paulis_;
+    std::vector<CPPCTYPE> coefs_;
+    bool built_ = false;
+};
 
 // Operators from OpenFermion's text form, one "coef [P0 P1 ...]" term per line
 // (src/cppsim/general_quantum_operator.cpp:806-952, src/cppsim/observable.cpp:205-350,

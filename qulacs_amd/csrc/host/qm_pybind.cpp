@@ -746,6 +746,45 @@ PYBIND11_MODULE(_core, m) {
         .def("merge_all", &QuantumCircuitOptimizer::merge_all, py::return_value_policy::take_ownership,
              py::arg("circuit"));
 
+    py::class_<NoiseSimulator::Result>(m, "SimulationResult")
+        .def("get_count", [](const NoiseSimulator::Result& r) { return (unsigned)r.result.size(); })
+        .def(
+            "get_state",
+            [](const NoiseSimulator::Result& r, unsigned i) {
+                const QuantumStateMI355* s = r.result.at(i).first;
+                auto* c = new QuantumStateMI355(s->qubit_count, s->device_number);
+                c->load(s);
+                return c;
+            },
+            py::return_value_policy::take_ownership, py::arg("i"))
+        .def("get_frequency", [](const NoiseSimulator::Result& r, unsigned i) { return r.result.at(i).second; },
+             py::arg("i"))
+        .def("sampling", &NoiseSimulator::Result::sampling);
+    py::class_<NoiseSimulator>(m, "NoiseSimulator")
+        .def(py::init<const QuantumCircuit*, const QuantumStateMI355*>(), py::arg("circuit"),
+             py::arg("init_state") = nullptr)
+        .def("execute", &NoiseSimulator::execute, py::arg("sample_count"))
+        .def("execute_and_get_result", &NoiseSimulator::execute_and_get_result, py::return_value_policy::take_ownership,
+             py::arg("execution_count"))
+        .def("set_seed", &NoiseSimulator::set_seed, py::arg("seed"));
+    py::class_<CausalConeSimulator>(m, "CausalConeSimulator")
+        .def(py::init<const ParametricQuantumCircuit&, const Observable&>(), py::arg("circuit"), py::arg("observable"))
+        .def("build", &CausalConeSimulator::build)
+        .def("get_expectation_value", &CausalConeSimulator::get_expectation_value)
+        .def("get_circuit_list",
+             [](const CausalConeSimulator& c) {
+                 py::list out;
+                 for (const auto& row : c.get_circuit_list()) {
+                     py::list r;
+                     for (const auto* x : row)
+                         r.append(py::cast(static_cast<ParametricQuantumCircuit*>(x->copy()),
+                                           py::return_value_policy::take_ownership));
+                     out.append(r);
+                 }
+                 return out;
+             })
+        .def("get_pauli_operator_list", &CausalConeSimulator::get_pauli_operator_list)
+        .def("get_coef_list", &CausalConeSimulator::get_coef_list);
     py::class_<QuantumCircuitSimulator>(m, "QuantumCircuitSimulator")
         .def(py::init<QuantumCircuit*, QuantumStateBase*>(), py::arg("circuit"), py::arg("state"),
              py::keep_alive<1, 2>(), py::keep_alive<1, 3>())

@@ -248,3 +248,66 @@ def test_multiply_elementwise_function():
     s.multiply_elementwise_function(lambda i: complex(np.cos(i), 0.5 * i))
     i = np.arange(1 << n)
     np.testing.assert_allclose(s.get_vector(), v * (np.cos(i) + 0.5j * i), atol=1e-12)
+
+
+def test_noise_simulator():
+    n = 3
+    c = q.QuantumCircuit(n)
+    c.add_H_gate(0)
+    c.add_CNOT_gate(0, 1)
+    sim = q.NoiseSimulator(c, q.QuantumState(n))
+    r = sim.execute_and_get_result(50)
+    assert r.get_count() == 1 and r.get_frequency(0) == 50  # no noise: one trajectory
+    v = r.get_state(0).get_vector()
+    np.testing.assert_allclose(np.abs(v[[0, 3]]) ** 2, [0.5, 0.5], atol=1e-12)
+    samples = sim.execute(200)
+    assert len(samples) == 200 and set(samples) <= {0, 3}
+
+    c = q.QuantumCircuit(n)
+    c.add_gate(gate.BitFlipNoise(0, 0.3))
+    c.add_X_gate(2)
+    c.add_gate(gate.DepolarizingNoise(1, 0.0))
+    sim = q.NoiseSimulator(c)
+    sim.set_seed(11)
+    r = sim.execute_and_get_result(4000)
+    freq = {}
+    for i in range(r.get_count()):
+        vec = r.get_state(i).get_vector()
+        k = int(np.argmax(np.abs(vec)))
+        assert abs(vec[k]) == pytest.approx(1.0)
+        freq[k] = freq.get(k, 0) + r.get_frequency(i)
+    assert set(freq) <= {4, 5} and sum(freq.values()) == 4000
+    assert abs(freq.get(5, 0) / 4000 - 0.3) < 0.04
+    with pytest.raises(RuntimeError):
+        bad = q.QuantumCircuit(1)
+        bad.add_gate(gate.AmplitudeDampingNoise(0, 0.1))
+        q.NoiseSimulator(bad)
+
+
+def test_causal_cone_simulator_matches_full_simulation():
+    n = 8
+    rng = np.random.default_rng(4)
+    c = q.ParametricQuantumCircuit(n)
+    for layer in range(3):
+        for i in range(n):
+            c.add_parametric_RY_gate(i, float(rng.uniform(0, 6.28)))
+        for i in range(layer % 2, n - 1, 2):
+            c.add_CNOT_gate(i, i + 1)
+    c.add_gate(gate.PauliRotation([2, 5], [1, 3], 0.3))
+    o = q.Observable(n)
+    o.add_operator(0.7, "Z 0")
+    o.add_operator(-1.2, "X 3 Z 4")
+    o.add_operator(0.4, "Y 7")
+    o.add_operator(0.25, "")
+    s = q.QuantumState(n)
+    s.set_zero_state()
+    c.update_quantum_state(s)
+    want = o.get_expectation_value(s)
+    cc = q.CausalConeSimulator(c, o)
+    got = cc.get_expectation_value()
+    assert abs(got - want) < 1e-10
+    circuits = cc.get_circuit_list()
+    assert len(circuits) == 4 and circuits[3] == []  # the identity term needs no circuit
+    # Z 0 and Y 7 see four qubits each through the three brickwork layers
+    assert [x.get_qubit_count() for x in circuits[0]] == [4] and [x.get_qubit_count() for x in circuits[2]] == [4]
+    assert cc.get_coef_list() == [0.7, -1.2, 0.4, 0.25]
