@@ -36,6 +36,8 @@ except ImportError as e:  # pragma: no cover - exercised only without a build
 
 from ._core import (  # noqa: F401
     CausalConeSimulator,
+    ClsNoisyEvolution,
+    ClsNoisyEvolution_fast,
     ClsNpairQubitGate,
     ClsPauliGate,
     ClsStateReflectionGate,
@@ -90,6 +92,7 @@ ClsTwoQubitGate = QuantumGateMatrix
 QuantumGateSparseMatrix = QuantumGateMatrix
 ClsPauliRotationGate = ClsPauliGate
 QuantumGate_Instrument = QuantumGate_CPTP
+ClsReversibleBooleanGate = QuantumGateMatrix
 
 # `import qulacs_amd.gate` / `from qulacs_amd.state import inner_product`
 _sys.modules[__name__ + ".gate"] = gate

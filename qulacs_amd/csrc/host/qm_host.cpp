@@ -2094,6 +2094,161 @@ std::vector<CPPCTYPE> eigenvector_for(const CMat& a, CPPCTYPE lambda) {
     return v;
 }
 
+// A = V diag(w) V^-1 for a general complex matrix: Householder reduction to Hessenberg
+// form, shifted QR to the complex Schur form T = Q^H A Q with Q accumulated, the
+// eigenvectors of the triangular T by back substitution (a near-zero pivot from a
+// repeated eigenvalue is replaced by a small one), V = Q X with unit columns.
+void complex_eigen(const CMat& a, std::vector<CPPCTYPE>* w, CMat* vout) {
+    const int n = (int)a.size();
+    CMat h = a, q(n, std::vector<CPPCTYPE>(n, 0.0));
+    for (int i = 0; i < n; ++i) q[i][i] = 1.0;
+    const double eps = std::numeric_limits<double>::epsilon();
+    double hnorm = 0.0;
+    for (const auto& row : a)
+        for (const auto& z : row) hnorm = std::max(hnorm, std::abs(z));
+    // Householder: zero h[k+2..][k]
+    for (int k = 0; k + 2 < n; ++k) {
+        const int m = n - k - 1;
+        std::vector<CPPCTYPE> v(m);
+        double xn = 0.0;
+        for (int i = 0; i < m; ++i) {
+            v[i] = h[k + 1 + i][k];
+            xn += std::norm(v[i]);
+        }
+        xn = std::sqrt(xn);
+        if (xn == 0.0) continue;
+        const CPPCTYPE ph = std::abs(v[0]) > 0 ? v[0] / std::abs(v[0]) : CPPCTYPE(1.0);
+        v[0] += ph * xn;  // v = x + e^{i arg x0} |x| e1
+        double vn = 0.0;
+        for (auto& z : v) vn += std::norm(z);
+        vn = std::sqrt(vn);
+        for (auto& z : v) z /= vn;
+        for (int j = 0; j < n; ++j) {  // h <- (I - 2 v v^H) h on rows k+1..
+            CPPCTYPE d = 0.0;
+            for (int i = 0; i < m; ++i) d += std::conj(v[i]) * h[k + 1 + i][j];
+            for (int i = 0; i < m; ++i) h[k + 1 + i][j] -= 2.0 * v[i] * d;
+        }
+        for (int i = 0; i < n; ++i) {  // h <- h (I - 2 v v^H), q <- q (I - 2 v v^H) on columns k+1..
+            CPPCTYPE d = 0.0, e = 0.0;
+            for (int j = 0; j < m; ++j) {
+                d += h[i][k + 1 + j] * v[j];
+                e += q[i][k + 1 + j] * v[j];
+            }
+            for (int j = 0; j < m; ++j) {
+                h[i][k + 1 + j] -= 2.0 * d * std::conj(v[j]);
+                q[i][k + 1 + j] -= 2.0 * e * std::conj(v[j]);
+            }
+        }
+        for (int i = k + 2; i < n; ++i) h[i][k] = 0.0;
+    }
+    // shifted QR with the rotations applied to whole rows / columns and to q
+    int hi = n - 1, iter = 0;
+    while (hi > 0) {
+        int l = hi;
+        for (; l > 0; --l)
+            if (std::abs(h[l][l - 1]) <= eps * std::max(std::abs(h[l - 1][l - 1]) + std::abs(h[l][l]), eps * hnorm)) {
+                h[l][l - 1] = 0.0;
+                break;
+            }
+        if (l == hi) {
+            --hi;
+            iter = 0;
+            continue;
+        }
+        if (++iter > 1000) throw QulacsError("eigen decomposition did not converge");
+        const CPPCTYPE A = h[hi - 1][hi - 1], B = h[hi - 1][hi], C = h[hi][hi - 1], D = h[hi][hi];
+        const CPPCTYPE half = 0.5 * (A - D), disc = std::sqrt(half * half + B * C);
+        CPPCTYPE mu = D - B * C / (half + (std::abs(half + disc) >= std::abs(half - disc) ? disc : -disc));
+        if (!std::isfinite(mu.real()) || !std::isfinite(mu.imag())) mu = D;
+        if (iter % 11 == 10) mu = D + std::abs(C);
+        for (int k = l; k <= hi; ++k) h[k][k] -= mu;
+        std::vector<CPPCTYPE> gx(n), gy(n);
+        std::vector<double> gr(n, 0.0);
+        for (int k = l; k < hi; ++k) {
+            const CPPCTYPE x = h[k][k], y = h[k + 1][k];
+            const double r = std::sqrt(std::norm(x) + std::norm(y));
+            gx[k] = x, gy[k] = y, gr[k] = r;
+            if (r == 0.0) continue;
+            for (int j = k; j < n; ++j) {
+                const CPPCTYPE u = h[k][j], v = h[k + 1][j];
+                h[k][j] = (std::conj(x) * u + std::conj(y) * v) / r;
+                h[k + 1][j] = (-y * u + x * v) / r;
+            }
+        }
+        for (int k = l; k < hi; ++k) {
+            const double r = gr[k];
+            if (r == 0.0) continue;
+            const CPPCTYPE x = gx[k], y = gy[k];
+            for (int i = 0; i <= std::min(k + 2, hi); ++i) {
+                const CPPCTYPE u = h[i][k], v = h[i][k + 1];
+                h[i][k] = (u * x + v * y) / r;
+                h[i][k + 1] = (-u * std::conj(y) + v * std::conj(x)) / r;
+            }
+            for (int i = 0; i < n; ++i) {
+                const CPPCTYPE u = q[i][k], v = q[i][k + 1];
+                q[i][k] = (u * x + v * y) / r;
+                q[i][k + 1] = (-u * std::conj(y) + v * std::conj(x)) / r;
+            }
+        }
+        for (int k = l; k <= hi; ++k) h[k][k] += mu;
+    }
+    w->resize(n);
+    for (int i = 0; i < n; ++i) (*w)[i] = h[i][i];
+    // eigenvectors of the upper triangular h, then V = q x
+    const double smin = std::max(eps * hnorm, std::numeric_limits<double>::min());
+    vout->assign(n, std::vector<CPPCTYPE>(n, 0.0));
+    std::vector<CPPCTYPE> x(n);
+    for (int k = 0; k < n; ++k) {
+        std::fill(x.begin(), x.end(), CPPCTYPE(0.0));
+        x[k] = 1.0;
+        for (int i = k - 1; i >= 0; --i) {
+            CPPCTYPE sum = 0.0;
+            for (int j = i + 1; j <= k; ++j) sum += h[i][j] * x[j];
+            CPPCTYPE den = h[i][i] - h[k][k];
+            if (std::abs(den) < smin) den = smin;
+            x[i] = -sum / den;
+        }
+        double nrm = 0.0;
+        std::vector<CPPCTYPE> col(n, 0.0);
+        for (int i = 0; i < n; ++i) {
+            for (int j = 0; j <= k; ++j) col[i] += q[i][j] * x[j];
+            nrm += std::norm(col[i]);
+        }
+        nrm = std::sqrt(nrm);
+        for (int i = 0; i < n; ++i) (*vout)[i][k] = col[i] / nrm;
+    }
+}
+
+// inverse by Gauss-Jordan elimination with partial pivoting
+CMat invert(CMat a) {
+    const int n = (int)a.size();
+    CMat inv(n, std::vector<CPPCTYPE>(n, 0.0));
+    for (int i = 0; i < n; ++i) inv[i][i] = 1.0;
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        for (int i = k + 1; i < n; ++i)
+            if (std::abs(a[i][k]) > std::abs(a[p][k])) p = i;
+        if (std::abs(a[p][k]) == 0.0) throw QulacsError("matrix inverse: singular matrix");
+        std::swap(a[k], a[p]);
+        std::swap(inv[k], inv[p]);
+        const CPPCTYPE d = 1.0 / a[k][k];
+        for (int j = 0; j < n; ++j) {
+            a[k][j] *= d;
+            inv[k][j] *= d;
+        }
+        for (int i = 0; i < n; ++i) {
+            if (i == k) continue;
+            const CPPCTYPE f = a[i][k];
+            if (f == 0.0) continue;
+            for (int j = 0; j < n; ++j) {
+                a[i][j] -= f * a[k][j];
+                inv[i][j] -= f * inv[k][j];
+            }
+        }
+    }
+    return inv;
+}
+
 // Eigen-decomposition of the symmetric tridiagonal matrix (alpha on the diagonal, beta
 // beside it) by cyclic Jacobi rotations; column k of `vec` belongs to val[k].
 void tridiagonal_eigen(const std::vector<double>& alpha, const std::vector<double>& beta, std::vector<double>* val,
@@ -2285,6 +2440,294 @@ GeneralQuantumOperator* to_general_quantum_operator(const QuantumGateBase* g, un
         }
     return op.release();
 }
+
+// ------------------------------------------------------------------ noisy evolution
+ClsNoisyEvolution::ClsNoisyEvolution(const Observable& h, const std::vector<const GeneralQuantumOperator*>& c_ops,
+                                     double time, double dt)
+    : h_(new Observable(h.get_qubit_count())), time_(time), dt_(dt) {
+    if (!(dt > 0) || !(time >= 0)) throw QulacsError("NoisyEvolution: time must be >= 0 and dt > 0");
+    for (size_t i = 0; i < h.get_term_count(); ++i) h_->add_operator(h.get_term(i));
+    // H_eff = H - (i/2) sum_k c_k^dagger c_k, as a general operator
+    heff_.reset(new GeneralQuantumOperator(h.get_qubit_count()));
+    for (size_t i = 0; i < h.get_term_count(); ++i) heff_->add_operator(h.get_term(i));
+    for (const auto* c : c_ops) {
+        c_ops_.emplace_back(new GeneralQuantumOperator(*c));
+        *heff_ += c->get_dagger() * *c * CPPCTYPE(0.0, -0.5);
+    }
+    name_ = "NoisyEvolution";
+}
+ClsNoisyEvolution::ClsNoisyEvolution(const ClsNoisyEvolution& o)
+    : QuantumGateBase(o),
+      h_(new Observable(o.h_->get_qubit_count())),
+      heff_(new GeneralQuantumOperator(*o.heff_)),
+      time_(o.time_),
+      dt_(o.dt_),
+      norm_tol_(o.norm_tol_),
+      max_steps_(o.max_steps_),
+      random_(o.random_) {
+    for (size_t i = 0; i < o.h_->get_term_count(); ++i) h_->add_operator(o.h_->get_term(i));
+    for (const auto& c : o.c_ops_) c_ops_.emplace_back(new GeneralQuantumOperator(*c));
+}
+ComplexMatrix ClsNoisyEvolution::get_matrix() const { throw QulacsError("NoisyEvolution has no matrix"); }
+// one RK4 step of d psi / dt = -i H_eff psi; `buffer` ends holding the state before the step
+void ClsNoisyEvolution::evolve(QuantumStateMI355* k1, QuantumStateMI355* k2, QuantumStateMI355* k3,
+                               QuantumStateMI355* k4, QuantumStateMI355* buffer, QuantumStateMI355* state,
+                               double dt) const {
+    const CPPCTYPE mi(0.0, -1.0);
+    heff_->apply_to_state(state, k1);
+    buffer->load(state);
+    buffer->add_state_with_coef(mi * (dt / 2), k1);
+    heff_->apply_to_state(buffer, k2);
+    buffer->load(state);
+    buffer->add_state_with_coef(mi * (dt / 2), k2);
+    heff_->apply_to_state(buffer, k3);
+    buffer->load(state);
+    buffer->add_state_with_coef(mi * dt, k3);
+    heff_->apply_to_state(buffer, k4);
+    buffer->load(state);
+    state->add_state_with_coef(mi * (dt / 6), k1);
+    state->add_state_with_coef(mi * (dt / 3), k2);
+    state->add_state_with_coef(mi * (dt / 3), k3);
+    state->add_state_with_coef(mi * (dt / 6), k4);
+}
+// time in [0, dt] at which the squared norm reaches `target`, bracketed between `prev`
+// (norm >= target) and `now` (norm <= target): secant steps on log(norm) for the first
+// 21 tries, bisection after; `now` ends holding the state at that time
+// (src/cppsim/gate_noisy_evolution.cpp:23-104)
+double ClsNoisyEvolution::find_collapse(QuantumStateMI355* k1, QuantumStateMI355* k2, QuantumStateMI355* k3,
+                                        QuantumStateMI355* k4, QuantumStateMI355* prev, QuantumStateMI355* now,
+                                        double target, double dt) const {
+    double lo_norm = prev->get_squared_norm(), hi_norm = now->get_squared_norm();
+    if (std::abs(lo_norm - target) < norm_tol_) {
+        now->load(prev);
+        return 0.0;
+    }
+    if (std::abs(hi_norm - target) < norm_tol_) return dt;
+    if (lo_norm < target) throw QulacsError("NoisyEvolution: the previous norm is below the jump threshold");
+    if (hi_norm > target) throw QulacsError("NoisyEvolution: the current norm is above the jump threshold");
+    double t_lo = 0.0, t_hi = dt;
+    const double lt = std::log(target);
+    double llo = std::log(lo_norm), lhi = std::log(hi_norm);
+    QuantumStateMI355 buf(prev->qubit_count, prev->device_number), scratch(prev->qubit_count, prev->device_number);
+    for (int step = 0; step < max_steps_; ++step) {
+        const double t = step <= 20 ? t_lo + (t_hi - t_lo) * (llo - lt) / (llo - lhi) : 0.5 * (t_lo + t_hi);
+        buf.load(prev);
+        evolve(k1, k2, k3, k4, &scratch, &buf, t);
+        const double nb = buf.get_squared_norm();
+        if (std::abs(nb - target) < norm_tol_) {
+            now->load(&buf);
+            return t;
+        }
+        if (nb < target) {
+            now->load(&buf);
+            t_hi = t;
+            lhi = std::log(nb);
+        } else {
+            t_lo = t;
+            llo = std::log(nb);
+        }
+    }
+    throw QulacsError("NoisyEvolution: failed to find the jump time; try a smaller dt");
+}
+void ClsNoisyEvolution::update_quantum_state(QuantumStateBase* state_base) {
+    auto* state = as_sv(state_base);
+    if (!state) throw QulacsError("NoisyEvolution: density matrices are not supported");
+    if (h_->get_qubit_count() > state->qubit_count) throw QulacsError("NoisyEvolution: operator beyond the state");
+    const unsigned n = state->qubit_count, dev = state->device_number;
+    const double norm0 = state->get_squared_norm();
+    double r = random_.uniform();
+    QuantumStateMI355 k1(n, dev), k2(n, dev), k3(n, dev), k4(n, dev), buffer(n, dev);
+    std::vector<double> cum(c_ops_.size());
+    double t = 0.0;
+    while (std::abs(t - time_) > 1e-10 * time_) {
+        const double dt = (t + dt_ > time_) ? time_ - t : dt_;
+        evolve(&k1, &k2, &k3, &k4, &buffer, state, dt);
+        if (state->get_squared_norm() <= r) {  // a jump inside this step
+            const double tj = find_collapse(&k1, &k2, &k3, &k4, &buffer, state, r, dt);
+            double sum = 0.0;
+            for (size_t k = 0; k < c_ops_.size(); ++k) {
+                c_ops_[k]->apply_to_state(state, &buffer);
+                sum += buffer.get_squared_norm();
+                cum[k] = sum;
+            }
+            const double jr = random_.uniform() * sum;
+            const size_t idx = (size_t)(std::lower_bound(cum.begin(), cum.end(), jr) - cum.begin());
+            if (idx < c_ops_.size()) {
+                c_ops_[idx]->apply_to_state(state, &buffer);
+                buffer.normalize(buffer.get_squared_norm());
+                state->load(&buffer);
+            }
+            t += tj;
+            r = random_.uniform();
+        } else {
+            t += dt;
+        }
+    }
+    state->normalize(state->get_squared_norm() / norm0);
+}
+ClsNoisyEvolution_fast::ClsNoisyEvolution_fast(const Observable& h,
+                                               const std::vector<const GeneralQuantumOperator*>& c_ops, double time)
+    : h_(new Observable(h.get_qubit_count())), time_(time) {
+    if (!(time >= 0)) throw QulacsError("NoisyEvolution_fast: time must be >= 0");
+    for (size_t i = 0; i < h.get_term_count(); ++i) h_->add_operator(h.get_term(i));
+    heff_.reset(new GeneralQuantumOperator(h.get_qubit_count()));
+    for (size_t i = 0; i < h.get_term_count(); ++i) heff_->add_operator(h.get_term(i));
+    for (const auto* c : c_ops) {
+        c_ops_.emplace_back(new GeneralQuantumOperator(*c));
+        *heff_ += c->get_dagger() * *c * CPPCTYPE(0.0, -0.5);
+    }
+    name_ = "NoisyEvolution_fast";
+    if (heff_->get_term_count() == 0) {
+        none_ = true;
+        return;
+    }
+    // -i H_eff as one matrix over the qubits its terms touch
+    std::vector<std::unique_ptr<QuantumGateBase>> owned;
+    std::vector<QuantumGateBase*> parts;
+    for (size_t i = 0; i < heff_->get_term_count(); ++i) {
+        const PauliOperator& t = heff_->get_term(i);
+        std::unique_ptr<ClsPauliGate> pg(gate::Pauli(t.get_index_list(), t.get_pauli_id_list()));
+        QuantumGateMatrix* m = gate::to_matrix_gate(pg.get());
+        m->multiply_scalar(-t.get_coef() * CPPCTYPE(0.0, 1.0));
+        owned.emplace_back(m);
+        parts.push_back(m);
+    }
+    std::vector<std::vector<unsigned>> lists;
+    for (auto* g : parts) lists.push_back(g->get_target_index_list());
+    if (sorted_union(lists).size() > 8)
+        throw QulacsError("NoisyEvolution_fast: the operators touch more than 8 qubits; split independent parts");
+    std::unique_ptr<QuantumGateMatrix> a(gate::add(parts));
+    targets_ = a->get_target_index_list();
+    const ComplexMatrix am = a->get_matrix();
+    const size_t d = am.rows;
+    CMat m(d, std::vector<CPPCTYPE>(d));
+    for (size_t i = 0; i < d; ++i)
+        for (size_t j = 0; j < d; ++j) m[i][j] = am(i, j);
+    CMat v;
+    complex_eigen(m, &eig_, &v);
+    const CMat vi = invert(v);
+    ComplexMatrix V(d, d), Vi(d, d);
+    for (size_t i = 0; i < d; ++i)
+        for (size_t j = 0; j < d; ++j) V(i, j) = v[i][j], Vi(i, j) = vi[i][j];
+    v_.reset(new QuantumGateMatrix(targets_, V));
+    vinv_.reset(new QuantumGateMatrix(targets_, Vi));
+}
+ClsNoisyEvolution_fast::ClsNoisyEvolution_fast(const ClsNoisyEvolution_fast& o)
+    : QuantumGateBase(o),
+      h_(new Observable(o.h_->get_qubit_count())),
+      heff_(new GeneralQuantumOperator(*o.heff_)),
+      time_(o.time_),
+      norm_tol_(o.norm_tol_),
+      max_steps_(o.max_steps_),
+      random_(o.random_),
+      none_(o.none_),
+      eig_(o.eig_),
+      v_(o.v_ ? new QuantumGateMatrix(*o.v_) : nullptr),
+      vinv_(o.vinv_ ? new QuantumGateMatrix(*o.vinv_) : nullptr) {
+    for (size_t i = 0; i < o.h_->get_term_count(); ++i) h_->add_operator(o.h_->get_term(i));
+    for (const auto& c : o.c_ops_) c_ops_.emplace_back(new GeneralQuantumOperator(*c));
+}
+ComplexMatrix ClsNoisyEvolution_fast::get_matrix() const { throw QulacsError("NoisyEvolution_fast has no matrix"); }
+void ClsNoisyEvolution_fast::evolve(QuantumStateMI355* s, double t) const {
+    vinv_->update_quantum_state(s);
+    std::vector<CPPCTYPE> e(eig_.size());
+    for (size_t i = 0; i < e.size(); ++i) e[i] = std::exp(eig_[i] * t);
+    QuantumGateDiagonalMatrix(targets_, e).update_quantum_state(s);
+    v_->update_quantum_state(s);
+}
+// bracketed search for the jump time as in the RK4 version; the secant estimate uses
+// weights from log-norm distances, every third try from their square roots
+// (src/cppsim/gate_noisy_evolution.cpp:332-420)
+double ClsNoisyEvolution_fast::find_collapse(QuantumStateMI355* prev, QuantumStateMI355* now, double target,
+                                             double t_step) const {
+    double lo_norm = prev->get_squared_norm(), hi_norm = now->get_squared_norm();
+    if (std::abs(lo_norm - target) < norm_tol_) {
+        now->load(prev);
+        return 0.0;
+    }
+    if (std::abs(hi_norm - target) < norm_tol_) return t_step;
+    if (lo_norm < target) throw QulacsError("NoisyEvolution_fast: the previous norm is below the jump threshold");
+    if (hi_norm > target) throw QulacsError("NoisyEvolution_fast: the current norm is above the jump threshold");
+    double t_lo = 0.0, t_hi = t_step;
+    const double lt = std::log(target);
+    double llo = std::log(lo_norm), lhi = std::log(hi_norm);
+    QuantumStateMI355 buf(prev->qubit_count, prev->device_number);
+    for (int step = 0; step < max_steps_; ++step) {
+        double t;
+        if (step <= 20) {
+            const double a = (llo - lt) / (llo - lhi), b = (lt - lhi) / (llo - lhi);
+            t = ((step - 2) % 3 != 2) ? t_lo + (t_hi - t_lo) * a / (a + b)
+                                      : t_lo + (t_hi - t_lo) * std::sqrt(a) / (std::sqrt(a) + std::sqrt(b));
+        } else {
+            t = 0.5 * (t_lo + t_hi);
+        }
+        buf.load(prev);
+        evolve(&buf, t);
+        const double nb = buf.get_squared_norm();
+        if (std::abs(nb - target) < norm_tol_) {
+            now->load(&buf);
+            return t;
+        }
+        if (nb < target) {
+            now->load(&buf);
+            t_hi = t;
+            lhi = std::log(nb);
+        } else {
+            t_lo = t;
+            llo = std::log(nb);
+        }
+    }
+    throw QulacsError("NoisyEvolution_fast: failed to find the jump time");
+}
+void ClsNoisyEvolution_fast::update_quantum_state(QuantumStateBase* state_base) {
+    if (none_) return;
+    auto* state = as_sv(state_base);
+    if (!state) throw QulacsError("NoisyEvolution_fast: density matrices are not supported");
+    check_targets(state, targets_, "NoisyEvolution_fast");
+    const unsigned n = state->qubit_count, dev = state->device_number;
+    const double norm0 = state->get_squared_norm();
+    double r = random_.uniform();
+    QuantumStateMI355 buffer(n, dev);
+    std::vector<double> cum(c_ops_.size());
+    double t = 0.0;
+    while (std::abs(t - time_) > 1e-10 * time_) {
+        const double step = time_ - t;
+        buffer.load(state);
+        evolve(state, step);
+        if (state->get_squared_norm() <= r) {
+            const double tj = find_collapse(&buffer, state, r, step);
+            double sum = 0.0;
+            for (size_t k = 0; k < c_ops_.size(); ++k) {
+                c_ops_[k]->apply_to_state(state, &buffer);
+                sum += buffer.get_squared_norm();
+                cum[k] = sum;
+            }
+            const double jr = random_.uniform() * sum;
+            const size_t idx = (size_t)(std::lower_bound(cum.begin(), cum.end(), jr) - cum.begin());
+            if (idx < c_ops_.size()) {
+                c_ops_[idx]->apply_to_state(state, &buffer);
+                buffer.normalize(buffer.get_squared_norm());
+                state->load(&buffer);
+            }
+            t += tj;
+            r = random_.uniform();
+        } else {
+            t += step;
+        }
+    }
+    state->normalize(state->get_squared_norm() / norm0);
+}
+
+namespace gate {
+ClsNoisyEvolution* NoisyEvolution(const Observable& h, const std::vector<const GeneralQuantumOperator*>& c_ops,
+                                  double time, double dt) {
+    return new ClsNoisyEvolution(h, c_ops, time, dt);
+}
+ClsNoisyEvolution_fast* NoisyEvolution_fast(const Observable& h, const std::vector<const GeneralQuantumOperator*>& c_ops,
+                                            double time) {
+    return new ClsNoisyEvolution_fast(h, c_ops, time);
+}
+}  // namespace gate
 
 // ------------------------------------------------------------------ noise simulator
 NoiseSimulator::Result::~Result() {

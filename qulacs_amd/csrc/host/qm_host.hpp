@@ -884,6 +884,87 @@ class Observable : public GeneralQuantumOperator {
 GeneralQuantumOperator* to_general_quantum_operator(const QuantumGateBase* gate, unsigned GQO_qubits,
                                                     double tol = 1e-6);
 
+// Quantum-trajectory evolution under a Lindblad master equation with Hamiltonian H and
+// collapse operators c_k for a given time (src/cppsim/gate_noisy_evolution.cpp:23-310):
+// RK4 steps of dt under H_eff = H - (i/2) sum_k c_k^dagger c_k until the squared norm
+// drops to a uniform r, the jump time found by a secant search on log(norm), then
+// c_k applied with probability ||c_k psi||^2 / sum and a new r drawn. State vectors
+// only; every step runs on the device.
+class ClsNoisyEvolution : public QuantumGateBase {
+   public:
+    ClsNoisyEvolution(const Observable& hamiltonian, const std::vector<const GeneralQuantumOperator*>& c_ops,
+                      double time, double dt = 1e-6);
+    ClsNoisyEvolution(const ClsNoisyEvolution& o);
+    ClsNoisyEvolution& operator=(const ClsNoisyEvolution&) = delete;
+    void update_quantum_state(QuantumStateBase* state) override;
+    QuantumGateBase* copy() const override { return new ClsNoisyEvolution(*this); }
+    ComplexMatrix get_matrix() const override;
+    bool has_matrix() const override { return false; }
+    bool is_noise() const override { return true; }
+    void set_seed(uint64_t seed) { random_.set_seed(seed); }
+    GeneralQuantumOperator get_effective_hamiltonian() const { return *heff_; }
+    void set_find_collapse_max_steps(int n) { max_steps_ = n; }
+    const Observable& get_hamiltonian() const { return *h_; }
+    const std::vector<std::unique_ptr<GeneralQuantumOperator>>& get_c_ops() const { return c_ops_; }
+    double get_time() const { return time_; }
+    double get_dt() const { return dt_; }
+
+   private:
+    void evolve(QuantumStateMI355* k1, QuantumStateMI355* k2, QuantumStateMI355* k3, QuantumStateMI355* k4,
+                QuantumStateMI355* buffer, QuantumStateMI355* state, double dt) const;
+    double find_collapse(QuantumStateMI355* k1, QuantumStateMI355* k2, QuantumStateMI355* k3, QuantumStateMI355* k4,
+                         QuantumStateMI355* prev, QuantumStateMI355* now, double target, double dt) const;
+    std::unique_ptr<Observable> h_;
+    std::vector<std::unique_ptr<GeneralQuantumOperator>> c_ops_;
+    std::unique_ptr<GeneralQuantumOperator> heff_;
+    double time_, dt_;
+    double norm_tol_ = 1e-6;
+    int max_steps_ = 200;
+    Random random_;
+};
+// The same trajectories with exact propagation: -i H_eff on the qubits its terms touch
+// (at most 8) is diagonalised once, V diag(w) V^-1, and exp(-i H_eff t) is applied as
+// V^-1, diag(exp(w t)), V on the device; each step runs to the end time or the next
+// jump (src/cppsim/gate_noisy_evolution.cpp:330-616).
+class ClsNoisyEvolution_fast : public QuantumGateBase {
+   public:
+    ClsNoisyEvolution_fast(const Observable& hamiltonian, const std::vector<const GeneralQuantumOperator*>& c_ops,
+                           double time);
+    ClsNoisyEvolution_fast(const ClsNoisyEvolution_fast& o);
+    ClsNoisyEvolution_fast& operator=(const ClsNoisyEvolution_fast&) = delete;
+    void update_quantum_state(QuantumStateBase* state) override;
+    QuantumGateBase* copy() const override { return new ClsNoisyEvolution_fast(*this); }
+    ComplexMatrix get_matrix() const override;
+    bool has_matrix() const override { return false; }
+    bool is_noise() const override { return true; }
+    void set_seed(uint64_t seed) { random_.set_seed(seed); }
+    GeneralQuantumOperator get_effective_hamiltonian() const { return *heff_; }
+    void set_find_collapse_max_steps(int n) { max_steps_ = n; }
+    const Observable& get_hamiltonian() const { return *h_; }
+    const std::vector<std::unique_ptr<GeneralQuantumOperator>>& get_c_ops() const { return c_ops_; }
+    double get_time() const { return time_; }
+
+   private:
+    void evolve(QuantumStateMI355* state, double t) const;
+    double find_collapse(QuantumStateMI355* prev, QuantumStateMI355* now, double target, double t_step) const;
+    std::unique_ptr<Observable> h_;
+    std::vector<std::unique_ptr<GeneralQuantumOperator>> c_ops_;
+    std::unique_ptr<GeneralQuantumOperator> heff_;
+    double time_;
+    double norm_tol_ = 1e-6;
+    int max_steps_ = 200;
+    Random random_;
+    bool none_ = false;  // no Hamiltonian terms: the gate does nothing
+    std::vector<CPPCTYPE> eig_;
+    std::unique_ptr<QuantumGateMatrix> v_, vinv_;
+};
+namespace gate {
+ClsNoisyEvolution* NoisyEvolution(const Observable& hamiltonian, const std::vector<const GeneralQuantumOperator*>& c_ops,
+                                  double time, double dt);
+ClsNoisyEvolution_fast* NoisyEvolution_fast(const Observable& hamiltonian,
+                                            const std::vector<const GeneralQuantumOperator*>& c_ops, double time);
+}  // namespace gate
+
 // Samples a circuit whose noise gates are probabilistic mixtures by drawing one branch
 // per noise gate per shot, grouping shots with the same branches, and simulating each
 // distinct trajectory once from the longest prefix they share

@@ -448,6 +448,27 @@ PTree to_ptree(const QuantumGateBase& g) {
         pt.put("reflection_state", to_ptree(*r->get_reflection_state()));
         return pt;
     }
+    // src/cppsim/gate_noisy_evolution.cpp:297-310
+    if (const auto* e = dynamic_cast<const ClsNoisyEvolution*>(&g)) {
+        pt.put("name", "NoisyEvolutionGate");
+        pt.put("hamiltonian", to_ptree(static_cast<const GeneralQuantumOperator&>(e->get_hamiltonian())));
+        PTree cops;
+        for (const auto& c : e->get_c_ops()) cops.push_back(to_ptree(*c));
+        pt.put("c_ops", std::move(cops));
+        pt.put("time", ptree::format_double(e->get_time()));
+        pt.put("dt", ptree::format_double(e->get_dt()));
+        return pt;
+    }
+    // src/cppsim/gate_noisy_evolution.cpp:605-616
+    if (const auto* e = dynamic_cast<const ClsNoisyEvolution_fast*>(&g)) {
+        pt.put("name", "NoisyEvolutionFastGate");
+        pt.put("hamiltonian", to_ptree(static_cast<const GeneralQuantumOperator&>(e->get_hamiltonian())));
+        PTree cops;
+        for (const auto& c : e->get_c_ops()) cops.push_back(to_ptree(*c));
+        pt.put("c_ops", std::move(cops));
+        pt.put("time", ptree::format_double(e->get_time()));
+        return pt;
+    }
     // src/cppsim/gate_named_npair.hpp:117-128
     if (dynamic_cast<const ClsNpairQubitGate*>(&g)) {
         pt.put("name", "FusedSWAPGate");
@@ -545,6 +566,17 @@ QuantumGateBase* gate_from_ptree(const PTree& pt) {
         auto* sv = dynamic_cast<QuantumStateMI355*>(r.get());
         if (!sv) throw QulacsError("JSON: a reflection state must be a state vector");
         return gate::StateReflection(sv);
+    }
+    if (name == "NoisyEvolutionGate" || name == "NoisyEvolutionFastGate") {
+        std::unique_ptr<Observable> h(observable_from_ptree(pt.child("hamiltonian")));
+        std::vector<std::unique_ptr<GeneralQuantumOperator>> owned;
+        std::vector<const GeneralQuantumOperator*> cops;
+        for (const auto& kv : pt.child("c_ops").children) {
+            owned.emplace_back(operator_from_ptree(kv.second));
+            cops.push_back(owned.back().get());
+        }
+        if (name == "NoisyEvolutionFastGate") return gate::NoisyEvolution_fast(*h, cops, pt.get_double("time"));
+        return gate::NoisyEvolution(*h, cops, pt.get_double("time"), pt.get_double("dt"));
     }
     if (name == "FusedSWAPGate") {
         const std::vector<unsigned> t = uint_array_from(pt.child("target_qubit_list"));

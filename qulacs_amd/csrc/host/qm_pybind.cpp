@@ -105,6 +105,7 @@ PYBIND11_MODULE(_core, m) {
         .def("__imul__", [](PauliOperator& a, const PauliOperator& b) -> PauliOperator& { return a *= b; },
              py::is_operator())
         .def("__imul__", [](PauliOperator& a, CPPCTYPE c) -> PauliOperator& { return a *= c; }, py::is_operator())
+        .def("__IMUL__", [](PauliOperator& a, CPPCTYPE c) -> PauliOperator& { return a *= c; })
         .def(py::pickle([](const PauliOperator& p) { return ptree::to_json(to_ptree(p)); },
                         [](const std::string& j) { return pauli_operator_from_ptree(ptree::from_json(j)); }));
 
@@ -192,6 +193,11 @@ PYBIND11_MODULE(_core, m) {
              py::is_operator())
         .def("__imul__", [](GeneralQuantumOperator& a, CPPCTYPE c) -> GeneralQuantumOperator& { return a *= c; },
              py::is_operator())
+        // spellings the reference also binds (python/cppsim_wrapper.cpp:170-215)
+        .def("__IADD__", [](GeneralQuantumOperator& a, const PauliOperator& b) -> GeneralQuantumOperator& { return a += b; })
+        .def("__ISUB__", [](GeneralQuantumOperator& a, const PauliOperator& b) -> GeneralQuantumOperator& { return a -= b; })
+        .def("__IMUL__", [](GeneralQuantumOperator& a, const PauliOperator& b) -> GeneralQuantumOperator& { return a *= b; })
+        .def("__IMUL__", [](GeneralQuantumOperator& a, CPPCTYPE c) -> GeneralQuantumOperator& { return a *= c; })
         .def(py::pickle([](const GeneralQuantumOperator& o) { return ptree::to_json(to_ptree(o)); },
                         [](const std::string& j) {
                             std::unique_ptr<GeneralQuantumOperator> p(operator_from_ptree(ptree::from_json(j)));
@@ -433,6 +439,14 @@ PYBIND11_MODULE(_core, m) {
     py::class_<ClsStateReflectionGate, QuantumGateBase>(m, "ClsStateReflectionGate");
     py::class_<QuantumGate_Adaptive, QuantumGateBase>(m, "QuantumGate_Adaptive");
     py::class_<ClsNpairQubitGate, QuantumGateBase>(m, "ClsNpairQubitGate");
+    py::class_<ClsNoisyEvolution, QuantumGateBase>(m, "ClsNoisyEvolution")
+        .def("set_seed", &ClsNoisyEvolution::set_seed, py::arg("seed"))
+        .def("get_effective_hamiltonian", &ClsNoisyEvolution::get_effective_hamiltonian)
+        .def("set_find_collapse_max_steps", &ClsNoisyEvolution::set_find_collapse_max_steps, py::arg("n"));
+    py::class_<ClsNoisyEvolution_fast, QuantumGateBase>(m, "ClsNoisyEvolution_fast")
+        .def("set_seed", &ClsNoisyEvolution_fast::set_seed, py::arg("seed"))
+        .def("get_effective_hamiltonian", &ClsNoisyEvolution_fast::get_effective_hamiltonian)
+        .def("set_find_collapse_max_steps", &ClsNoisyEvolution_fast::set_find_collapse_max_steps, py::arg("n"));
     py::class_<QuantumGate_CPTP, QuantumGateBase>(m, "QuantumGate_CPTP")
         .def("set_seed", &QuantumGate_CPTP::set_seed, py::arg("seed"))
         .def("get_gate_list",
@@ -571,6 +585,18 @@ PYBIND11_MODULE(_core, m) {
            own_, py::arg("gate"), py::arg("condition"), py::arg("id"));
     mg.def("ReversibleBoolean", &gate::ReversibleBoolean, own_, py::arg("index_list"), py::arg("func"));
     mg.def("FusedSWAP", &gate::FusedSWAP, own_, py::arg("target1"), py::arg("target2"), py::arg("block_size"));
+    mg.def(
+        "NoisyEvolution",
+        [](const Observable& h, const std::vector<GeneralQuantumOperator*>& c, double time, double dt) {
+            return gate::NoisyEvolution(h, std::vector<const GeneralQuantumOperator*>(c.begin(), c.end()), time, dt);
+        },
+        own_, py::arg("hamiltonian"), py::arg("c_ops"), py::arg("time"), py::arg("dt"));
+    mg.def(
+        "NoisyEvolution_fast",
+        [](const Observable& h, const std::vector<GeneralQuantumOperator*>& c, double time) {
+            return gate::NoisyEvolution_fast(h, std::vector<const GeneralQuantumOperator*>(c.begin(), c.end()), time);
+        },
+        own_, py::arg("hamiltonian"), py::arg("c_ops"), py::arg("time"));
     mg.def("Measurement", &gate::Measurement, own_, py::arg("index"), py::arg("register"));
     mg.def(
         "Measurement",
