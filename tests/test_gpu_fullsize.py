@@ -147,3 +147,69 @@ def test_14q_density_matrix_trace_hermiticity_and_pure_consistency():
         assert abs(tr - 1.0) < 1e-10
         rho = dm.download().reshape(dim, dim)
         assert np.max(np.abs(rho - rho.conj().T)) < 1e-12
+
+
+def _apply_sparse(amp, targets, m, controls=()):
+    """Apply matrix m (bit i of its index <-> targets[i]) to a sparse state {index: amplitude}."""
+    out = {}
+    for idx, a in amp.items():
+        if any(not (idx >> c) & 1 for c in controls):
+            out[idx] = out.get(idx, 0) + a
+            continue
+        col = sum(((idx >> t) & 1) << i for i, t in enumerate(targets))
+        base = idx & ~sum(1 << t for t in targets)
+        for row in range(1 << len(targets)):
+            v = m[row, col]
+            if v != 0:
+                j = base | sum(((row >> i) & 1) << t for i, t in enumerate(targets))
+                out[j] = out.get(j, 0) + v * a
+    return {k: v for k, v in out.items() if abs(v) > 0}
+
+
+def test_33q_index_math_above_32_bits():
+    """Every kernel family on a 33-qubit state (128 GiB, indices above 2^32) started from
+    a basis state with high bits set, against a sparse host simulation of the same gates,
+    read back amplitude by amplitude."""
+    import qulacs_amd as q
+    from qulacs_amd import gate
+
+    n = 33
+    rng = np.random.default_rng(33)
+    b0 = (1 << 32) | (1 << 31) | 3
+    s = q.QuantumState(n)
+    s.set_computational_basis(b0)
+    amp = {b0: 1.0 + 0j}
+    X = np.array([[0, 1], [1, 0]], dtype=complex)
+    H = np.array([[1, 1], [1, -1]], dtype=complex) / np.sqrt(2)
+    c, sn = np.cos(0.3), np.sin(0.3)
+    steps = [
+        (gate.X(32), [32], X, ()),
+        (gate.CNOT(31, 32), [32], X, (31,)),
+        (gate.H(0), [0], H, ()),
+        (gate.RZ(32, 0.6), [32], np.diag([c + 1j * sn, c - 1j * sn]), ()),
+    ]
+    u2 = haar_unitary(rng, 2)
+    steps.append((gate.DenseMatrix([32, 5], u2), [32, 5], u2, ()))
+    u5 = haar_unitary(rng, 5)
+    steps.append((gate.DenseMatrix([28, 32, 1, 30, 17], u5), [28, 32, 1, 30, 17], u5, ()))
+    u3 = haar_unitary(rng, 3)
+    steps.append((gate.DenseMatrix([32, 2, 20], u3), [32, 2, 20], u3, ()))
+    pz = np.kron(np.diag([1.0, -1.0]), np.array([[0, 1], [1, 0]]))  # X on qubit 4, Z on qubit 32
+    th = 0.45
+    steps.append((gate.PauliRotation([4, 32], [1, 3], th), [4, 32],
+                  np.cos(th / 2) * np.eye(4) + 1j * np.sin(th / 2) * pz, ()))
+    steps.append((gate.CZ(32, 31), [31], np.diag([1.0, -1.0]).astype(complex), (32,)))
+    for g, t, m, ctl in steps:
+        print("gate", g.get_name(), t, flush=True)
+        g.update_quantum_state(s)
+        amp = _apply_sparse(amp, t, m, ctl)
+    assert len(amp) <= 1024
+    for idx, a in amp.items():
+        assert abs(s.get_amplitude(idx) - a) < 1e-12, (idx, s.get_amplitude(idx), a)
+    assert abs(s.get_squared_norm() - 1.0) < 1e-12
+    p32 = sum(abs(a) ** 2 for k, a in amp.items() if not (k >> 32) & 1)
+    assert abs(s.get_zero_probability(32) - p32) < 1e-12
+    obs = q.Observable(n)
+    obs.add_operator(0.5, "Z 32 Z 0")
+    want = 0.5 * sum(abs(a) ** 2 * (1 - 2 * ((k >> 32) & 1)) * (1 - 2 * (k & 1)) for k, a in amp.items())
+    assert abs(obs.get_expectation_value(s) - want) < 1e-12
