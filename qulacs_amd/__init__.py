@@ -66,10 +66,12 @@ from ._core import (  # noqa: F401
     check_build_for_mpi,
     circuit,
     gate,
+    get_circuit_fusion,
     get_num_device,
     observable,
     quantum_operator,
     get_synchronous,
+    set_circuit_fusion,
     set_synchronous,
     state,
     synchronize,

@@ -2,8 +2,10 @@
 #include "qm_host.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
+#include <deque>
 #include <cstring>
 #include <fstream>
 #include <limits>
@@ -1566,6 +1568,7 @@ QuantumCircuit::QuantumCircuit(const QuantumCircuit& o) : qubit_count_(o.qubit_c
     for (auto* g : o.gates_) gates_.push_back(g->copy());
 }
 QuantumCircuit::~QuantumCircuit() {
+    release_plan();
     for (auto* g : gates_) delete g;
 }
 const QuantumGateBase* QuantumCircuit::get_gate(size_t i) const {
@@ -1576,6 +1579,7 @@ void QuantumCircuit::add_gate(const QuantumGateBase& g) { add_gate_consume(g.cop
 void QuantumCircuit::add_gate(const QuantumGateBase& g, size_t pos) {
     if (pos > gates_.size()) throw QulacsError("add_gate: position out of range");
     gates_.insert(gates_.begin() + pos, g.copy());
+    touched();
 }
 void QuantumCircuit::add_gate_consume(QuantumGateBase* g) {
     for (unsigned q : g->get_target_index_list())
@@ -1589,11 +1593,13 @@ void QuantumCircuit::add_gate_consume(QuantumGateBase* g) {
             throw QulacsError("add_gate: control index exceeds the circuit's qubit count");
         }
     gates_.push_back(g);
+    touched();
 }
 void QuantumCircuit::remove_gate(size_t i) {
     if (i >= gates_.size()) throw QulacsError("remove_gate: index out of range");
     delete gates_[i];
     gates_.erase(gates_.begin() + i);
+    touched();
 }
 void QuantumCircuit::merge_circuit(const QuantumCircuit& c) {
     const size_t n = c.gates_.size();  // `c` may be this circuit
@@ -1652,8 +1658,116 @@ void QuantumCircuit::add_diagonal_observable_rotation_gate(const GeneralQuantumO
     }
 }
 
+namespace {
+std::atomic<bool> g_fusion{true};
+constexpr size_t kMinFusedRun = 8;  // shorter runs stay per gate: one launch each is cheaper than a program
+
+// One gate as a stage-executor descriptor; matrices go to `store` (stable addresses).
+bool lower_gate(const QuantumGateBase* g, qm_batch_gate* out, std::deque<std::vector<qm_complex>>* store,
+                bool* skip) {
+    *skip = false;
+    if (g->is_noise() || !g->has_matrix()) return false;
+    if (const auto* m = dynamic_cast<const QuantumGateMatrix*>(g)) {
+        if (m->kind == QuantumGateMatrix::IDENTITY) {
+            *skip = true;
+            return true;
+        }
+        if (m->kind == QuantumGateMatrix::CNOT || m->kind == QuantumGateMatrix::CZ) {
+            out->kind = m->kind == QuantumGateMatrix::CNOT ? QM_BATCH_CNOT : QM_BATCH_CZ;
+            out->qubit_count = 2;
+            out->qubits[0] = m->get_control_index_list()[0];
+            out->qubits[1] = m->get_target_index_list()[0];
+            out->matrix = nullptr;
+            return true;
+        }
+    }
+    if (g->get_target_index_list().size() + g->get_control_index_list().size() > 4) return false;
+    std::vector<unsigned> on;
+    const ComplexMatrix m = with_controls(g, &on);
+    const size_t d = m.rows;
+    bool diagonal = true;
+    for (size_t y = 0; y < d && diagonal; ++y)
+        for (size_t x = 0; x < d; ++x)
+            if (x != y && m(y, x) != CPPCTYPE(0.0)) {
+                diagonal = false;
+                break;
+            }
+    store->emplace_back();
+    auto& v = store->back();
+    if (diagonal) {
+        for (size_t i = 0; i < d; ++i) v.push_back(qc(m(i, i)));
+    } else {
+        for (const auto& z : m.data) v.push_back(qc(z));
+    }
+    out->kind = diagonal ? QM_BATCH_DIAG : QM_BATCH_DENSE;
+    out->qubit_count = (UINT)on.size();
+    for (size_t i = 0; i < on.size(); ++i) out->qubits[i] = on[i];
+    out->matrix = v.data();
+    return true;
+}
+}  // namespace
+
+void set_circuit_fusion(bool enabled) { g_fusion = enabled; }
+bool get_circuit_fusion() { return g_fusion; }
+
+void QuantumCircuit::release_plan() {
+    for (auto& r : plan_) qm_circuit_destroy(r.program);
+    plan_.clear();
+    plan_version_ = ~0ULL;
+}
+void QuantumCircuit::build_plan() {
+    release_plan();
+    const size_t n = gates_.size();
+    size_t i = 0;
+    while (i < n) {
+        std::deque<std::vector<qm_complex>> store;
+        std::vector<qm_batch_gate> lowered;
+        size_t j = i, members = 0;
+        bool wide = false;
+        for (; j < n; ++j) {
+            qm_batch_gate bg{};
+            bool skip = false;
+            if (!lower_gate(gates_[j], &bg, &store, &skip)) break;
+            ++members;
+            if (skip) continue;
+            wide = wide || (bg.kind == QM_BATCH_DENSE && bg.qubit_count == 4);
+            lowered.push_back(bg);
+        }
+        if (members >= kMinFusedRun) {
+            qm_circuit* prog = qm_circuit_create(qubit_count_, lowered.data(), (UINT)lowered.size(), wide ? 4 : 3, 0);
+            if (prog) {
+                plan_.push_back({i, j, prog});
+                i = j;
+                continue;
+            }
+            qm_clear_last_error();  // fall back to per-gate execution
+        }
+        const size_t stop = (j == i) ? i + 1 : j;  // an unfusable gate, or a short run
+        if (!plan_.empty() && plan_.back().program == nullptr) plan_.back().end = stop;
+        else plan_.push_back({i, stop, nullptr});
+        i = stop;
+    }
+    plan_version_ = version_;
+}
+
 // src/cppsim/circuit.cpp:46-57
-void QuantumCircuit::update_quantum_state(QuantumStateBase* s) { update_quantum_state(s, 0, gates_.size()); }
+void QuantumCircuit::update_quantum_state(QuantumStateBase* s) {
+    auto* sv = as_sv(s);
+    if (!g_fusion || !sv || qubit_count_ < 4 || qubit_count_ > QM_CIRCUIT_MAX_QUBITS || gates_.size() < kMinFusedRun) {
+        update_quantum_state(s, 0, gates_.size());
+        return;
+    }
+    if (s->qubit_count != qubit_count_) throw QulacsError("update_quantum_state: qubit count mismatch");
+    if (plan_version_ != version_) build_plan();
+    for (const auto& r : plan_) {
+        if (!r.program) {
+            for (size_t k = r.begin; k < r.end; ++k) gates_[k]->update_quantum_state(s);
+            continue;
+        }
+        qm_circuit_apply(r.program, sv->data(), sv->dim, sv->get_cuda_stream(), sv->device_number);
+        check_abi("update_quantum_state");
+    }
+}
 void QuantumCircuit::update_quantum_state(QuantumStateBase* s, size_t start, size_t end) {
     if (s->qubit_count != qubit_count_) throw QulacsError("update_quantum_state: qubit count mismatch");
     if (start > end || end > gates_.size()) throw QulacsError("update_quantum_state: bad gate range");
@@ -3094,7 +3208,10 @@ void ParametricQuantumCircuit::add_parametric_multi_Pauli_rotation_gate(const st
     add_parametric_gate(*g);
 }
 double ParametricQuantumCircuit::get_parameter(size_t i) const { return param(i)->get_parameter_value(); }
-void ParametricQuantumCircuit::set_parameter(size_t i, double v) { param(i)->set_parameter_value(v); }
+void ParametricQuantumCircuit::set_parameter(size_t i, double v) {
+    param(i)->set_parameter_value(v);
+    touched();
+}
 size_t ParametricQuantumCircuit::get_parametric_gate_position(size_t i) const {
     if (i >= param_pos_.size()) throw QulacsError("parameter index out of range");
     return param_pos_[i];
@@ -3218,6 +3335,7 @@ void QuantumCircuitOptimizer::optimize(QuantumCircuit* circuit, unsigned max_blo
             }
         }
     }
+    circuit->touched();
     c_ = nullptr;
 }
 
@@ -3257,6 +3375,7 @@ void QuantumCircuitOptimizer::optimize_light(QuantumCircuit* circuit, unsigned s
             for (unsigned qb : qs) step[qb] = {(long)i, qs};
         }
     }
+    circuit->touched();
 }
 
 // src/cppsim/circuit_optimizer.cpp:218-235

@@ -585,10 +585,17 @@ QuantumGate_SingleParameter* ParametricPauliRotation(const std::vector<unsigned>
 class GeneralQuantumOperator;
 class NoiseSimulator;
 
+// Whether QuantumCircuit::update_quantum_state may run runs of fusable gates through
+// the stage executor (default on; off gives one device call per gate, as the
+// reference's loop).
+void set_circuit_fusion(bool enabled);
+bool get_circuit_fusion();
+
 class QuantumCircuit {
    public:
     explicit QuantumCircuit(unsigned qubit_count) : qubit_count_(qubit_count) {}
     QuantumCircuit(const QuantumCircuit& o);
+    QuantumCircuit& operator=(const QuantumCircuit&) = delete;
     virtual ~QuantumCircuit();
     unsigned get_qubit_count() const { return qubit_count_; }
     size_t get_gate_count() const { return gates_.size(); }
@@ -664,8 +671,23 @@ class QuantumCircuit {
    protected:
     friend class QuantumCircuitOptimizer;
     friend class NoiseSimulator;
+    void touched() { ++version_; }
     unsigned qubit_count_;
     std::vector<QuantumGateBase*> gates_;
+
+   private:
+    // Fused execution of a whole-circuit update on a state vector: maximal runs of
+    // gates that lower to the stage executor's kinds (dense / diagonal on <= 4 qubits
+    // counting controls, CZ, CNOT) become one compiled program each; the rest run
+    // gate by gate. The plan is rebuilt when the gate list or a parameter changes.
+    struct FusedRun {
+        size_t begin = 0, end = 0;  // gates [begin, end)
+        qm_circuit* program = nullptr;  // null: run the gates one by one
+    };
+    void release_plan();
+    void build_plan();
+    std::vector<FusedRun> plan_;
+    uint64_t version_ = 0, plan_version_ = ~0ULL;
 };
 
 // Host-side gate fusion (src/cppsim/circuit_optimizer.cpp:113-235).  Two gates

@@ -81,6 +81,8 @@ PYBIND11_MODULE(_core, m) {
     m.attr("__version__") = qm_version();
     // one process, one GPU: no MPI-distributed states
     m.def("check_build_for_mpi", []() { return false; });
+    m.def("set_circuit_fusion", &set_circuit_fusion, py::arg("enabled"));
+    m.def("get_circuit_fusion", &get_circuit_fusion);
 
     // ---------------------------------------------------------------- operators
     py::class_<PauliOperator>(m, "PauliOperator")
