@@ -1660,7 +1660,10 @@ void QuantumCircuit::add_diagonal_observable_rotation_gate(const GeneralQuantumO
 
 namespace {
 std::atomic<bool> g_fusion{true};
-constexpr size_t kMinFusedRun = 8;  // shorter runs stay per gate: one launch each is cheaper than a program
+// Shortest run worth a program: below 22 qubits a gate's launch costs less than building
+// and uploading a program, so short runs stay per gate; from 22 qubits every gate is a
+// pass over at least 64 MiB and two gates in one stage already halve the traffic.
+size_t min_fused_run(unsigned n) { return n >= 22 ? 2 : 8; }
 
 // One gate as a stage-executor descriptor; matrices go to `store` (stable addresses).
 bool lower_gate(const QuantumGateBase* g, qm_batch_gate* out, std::deque<std::vector<qm_complex>>* store,
@@ -1733,7 +1736,7 @@ void QuantumCircuit::build_plan() {
             wide = wide || (bg.kind == QM_BATCH_DENSE && bg.qubit_count == 4);
             lowered.push_back(bg);
         }
-        if (members >= kMinFusedRun) {
+        if (members >= min_fused_run(qubit_count_)) {
             qm_circuit* prog = qm_circuit_create(qubit_count_, lowered.data(), (UINT)lowered.size(), wide ? 4 : 3, 0);
             if (prog) {
                 plan_.push_back({i, j, prog});
@@ -1753,7 +1756,8 @@ void QuantumCircuit::build_plan() {
 // src/cppsim/circuit.cpp:46-57
 void QuantumCircuit::update_quantum_state(QuantumStateBase* s) {
     auto* sv = as_sv(s);
-    if (!g_fusion || !sv || qubit_count_ < 4 || qubit_count_ > QM_CIRCUIT_MAX_QUBITS || gates_.size() < kMinFusedRun) {
+    if (!g_fusion || !sv || qubit_count_ < 4 || qubit_count_ > QM_CIRCUIT_MAX_QUBITS ||
+        gates_.size() < min_fused_run(qubit_count_)) {
         update_quantum_state(s, 0, gates_.size());
         return;
     }

@@ -52,9 +52,9 @@ def _run(c, n, fused):
         q.set_circuit_fusion(True)
 
 
-@pytest.mark.parametrize("n", [6, 12, 21])
+@pytest.mark.parametrize("n", [6, 12, 21, 22])
 def test_fused_equals_per_gate(n):
-    c = _build(n, n)
+    c = _build(n, n, depth=6 if n < 22 else 3)
     ref = _run(c, n, False)
     got = _run(c, n, True)
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-12
@@ -103,3 +103,15 @@ def test_density_matrix_and_short_circuits_stay_per_gate():
     c.update_quantum_state(d)
     v = _run(c, n, False)
     np.testing.assert_allclose(d.get_matrix(), np.outer(v, v.conj()), atol=1e-12)
+
+
+def test_short_runs_from_22_qubits():
+    # from 22 qubits a run of two fusable gates between unfusable ones is a program
+    n = 22
+    rng = np.random.default_rng(5)
+    c = q.QuantumCircuit(n)
+    for k in range(3):
+        c.add_gate(gate.DenseMatrix(rng.choice(n, 5, replace=False).tolist(), haar_unitary(rng, 5)))
+        c.add_H_gate(k)
+        c.add_RZ_gate(20 - k, 0.3 + k)
+    np.testing.assert_allclose(_run(c, n, True), _run(c, n, False), atol=1e-12)
