@@ -374,8 +374,9 @@ def many_small(args, lib_mod, hip):
 
 def brickwork(args, lib_mod, hip):
     """20-qubit brickwork (10 layers of Haar 1q on every qubit + a CNOT brick),
-    16-term observable and 1024 samples: the per-gate path end to end, the
-    batched executor on the same circuit, and the reference csim on the host."""
+    16-term observable and 1024 samples: the per-gate path end to end, the fused
+    executor, the drop-in QuantumCircuit API, the batched executor on the same
+    circuit, and the reference csim on the host."""
     from oracle import oracle as O
     from qulacs_amd.batch import Batch
     from qulacs_amd.workloads import brickwork_circuit
@@ -462,6 +463,35 @@ def brickwork(args, lib_mod, hip):
     lib_mod.call("release_quantum_state_host", st, 0)
     lib_mod.call("release_cuda_stream_host", stream, 1, 0)
 
+    # the drop-in API: QuantumCircuit.update_quantum_state (runs of fusable gates go
+    # through the stage executor), Observable.get_expectation_value, sampling
+    import qulacs_amd as qa
+
+    api_c = qa.QuantumCircuit(n)
+    for kind_g, q, m in gates:
+        if kind_g == "CNOT":
+            api_c.add_CNOT_gate(q[0], q[1])
+        else:
+            api_c.add_gate(qa.gate.DenseMatrix(q[0], m))
+    api_o = qa.Observable(n)
+    for x in obs:
+        tl, pl = _mask_lists(x[0], x[1], n)
+        api_o.add_operator(qa.PauliOperator(list(tl), list(pl), float(x[3])))
+    api_s = qa.QuantumState(n)
+    uu_list = uu.tolist()
+
+    def run_api():
+        api_s.set_zero_state()
+        api_c.update_quantum_state(api_s)
+        e = api_o.get_expectation_value(api_s)
+        return e, api_s.sampling_with_uniforms(uu_list)
+
+    run_api()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        e_api, s_api = run_api()
+    api_ms = (time.perf_counter() - t0) * 1e3 / reps
+
     b = Batch()
     b.add(n, gates, obs)
     b.compile(4)
@@ -476,9 +506,10 @@ def brickwork(args, lib_mod, hip):
     out = {"workload": "20-qubit brickwork, 10 layers (200 Haar 1q + 95 CNOT), 16-term observable, 1024 samples",
            "gpu_ms": round(gpu_ms, 3), "fused_ms": round(fused_ms, 3), "fused_schedule": fused_stats,
            "fused_circuit_only_ms": variants,
-           "batched_executor_ms": round(batch_ms, 3), "energy": e_gpu,
+           "batched_executor_ms": round(batch_ms, 3), "api_circuit_ms": round(api_ms, 3), "energy": e_gpu,
            "fused_matches": bool(abs(e_fused - e_gpu) <= 1e-10 * scale and np.array_equal(fused_samples, samples))}
     out["batched_energy_matches"] = bool(abs(e_batch - e_gpu) <= 1e-10 * scale)
+    out["api_matches"] = bool(abs(e_api - e_gpu) <= 1e-10 * scale and np.array_equal(np.asarray(s_api, dtype=np.uint64), samples))
     if not args.no_cpu_baseline:
         k, kind = (O.Reference(), "reference") if O.reference_available() else (O.Oracle(), "port")
         o = O.Oracle()
