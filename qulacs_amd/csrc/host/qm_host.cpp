@@ -1762,6 +1762,7 @@ void QuantumCircuit::update_quantum_state(QuantumStateBase* s) {
         return;
     }
     if (s->qubit_count != qubit_count_) throw QulacsError("update_quantum_state: qubit count mismatch");
+    std::lock_guard<std::mutex> lock(plan_mutex_);
     if (plan_version_ != version_) build_plan();
     for (const auto& r : plan_) {
         if (!r.program) {

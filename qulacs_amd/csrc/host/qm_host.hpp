@@ -19,6 +19,7 @@
 #include <cstdint>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -688,6 +689,7 @@ class QuantumCircuit {
     void build_plan();
     std::vector<FusedRun> plan_;
     uint64_t version_ = 0, plan_version_ = ~0ULL;
+    std::mutex plan_mutex_;  // one fused update at a time per circuit (the plan uploads lazily per device)
 };
 
 // Host-side gate fusion (src/cppsim/circuit_optimizer.cpp:113-235).  Two gates
