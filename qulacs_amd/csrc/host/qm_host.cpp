@@ -89,8 +89,6 @@ std::vector<unsigned> sorted_union(const std::vector<std::vector<unsigned>>& lis
     return u;
 }
 
-// Matrix of g over its targets followed by its controls (control qubits become
-// targets; the matrix is the identity where a control value does not match).
 // X -> commutes with X, Y -> Y, Z -> Z, I -> none (src/cppsim/gate_named_pauli.hpp:27-43)
 std::vector<unsigned> pauli_commutation(const std::vector<unsigned>& paulis) {
     std::vector<unsigned> f(paulis.size(), 0);
@@ -112,6 +110,8 @@ std::vector<unsigned> merged_commutation(const std::vector<const QuantumGateBase
     return f;
 }
 
+// Matrix of g over its targets followed by its controls (control qubits become
+// targets; the matrix is the identity where a control value does not match).
 ComplexMatrix with_controls(const QuantumGateBase* g, std::vector<unsigned>* on) {
     const auto& tl = g->get_target_index_list();
     const auto& cl = g->get_control_index_list();
@@ -1708,7 +1708,89 @@ bool lower_gate(const QuantumGateBase* g, qm_batch_gate* out, std::deque<std::ve
     out->matrix = v.data();
     return true;
 }
+
+// Superoperator of g on the ascending qubit list q (4^|q| x 4^|q|, member = row bits *
+// 2^|q| + column bits): sum_i w_i K_i (x) conj(K_i) over g's Kraus form. False when a part
+// of g has no matrix.
+bool superop_on(const QuantumGateBase* g, const std::vector<unsigned>& q, ComplexMatrix* S) {
+    const size_t d = size_t(1) << q.size();
+    *S = ComplexMatrix(d * d, d * d);
+    auto add = [&](const QuantumGateBase* k, double w) {
+        if (k->is_noise() || !k->has_matrix()) return false;
+        std::vector<unsigned> on;
+        const ComplexMatrix m = with_controls(k, &on);
+        accumulate_superop(*S, embed(m, on, q), w);
+        return true;
+    };
+    if (const auto* p = dynamic_cast<const QuantumGate_Probabilistic*>(g)) {
+        const double rest = 1.0 - p->get_cumulative_distribution().back();
+        if (rest != 0.0) accumulate_superop(*S, ComplexMatrix::identity(d), rest);
+        for (size_t i = 0; i < p->get_gate_list().size(); ++i)
+            if (!add(p->get_gate_list()[i], p->get_distribution()[i])) return false;
+        return true;
+    }
+    if (const auto* c = dynamic_cast<const QuantumGate_CPTP*>(g)) {
+        for (const auto* k : c->get_gate_list())
+            if (!add(k, 1.0)) return false;
+        return true;
+    }
+    return add(g, 1.0);
+}
+bool dm_fusable(const QuantumGateBase* g) {
+    const std::vector<QuantumGateBase*>* parts = nullptr;
+    if (const auto* p = dynamic_cast<const QuantumGate_Probabilistic*>(g)) parts = &p->get_gate_list();
+    else if (const auto* c = dynamic_cast<const QuantumGate_CPTP*>(g)) parts = &c->get_gate_list();
+    if (!parts) return !g->is_noise() && g->has_matrix();
+    for (const auto* k : *parts)
+        if (k->is_noise() || !k->has_matrix()) return false;
+    return true;
+}
 }  // namespace
+
+// Density matrices: consecutive gates and channels whose qubits fit in 3 together are
+// composed on the host into one superoperator and applied in one pass
+// (dm_superoperator_host); a lone gate keeps its own entry point.
+namespace {
+constexpr size_t kDmFuseQubits = 2;
+}  // namespace
+void QuantumCircuit::update_dm_fused(DensityMatrixMI355* d) {
+    std::vector<unsigned> q;
+    size_t first = 0, count = 0;
+    auto flush = [&]() {
+        if (count == 1) gates_[first]->update_quantum_state(d);
+        if (count >= 2) {
+            const size_t dd = size_t(1) << (2 * q.size());
+            ComplexMatrix S = ComplexMatrix::identity(dd), Sg;
+            for (size_t k = first; k < first + count; ++k) {
+                if (!superop_on(gates_[k], q, &Sg)) throw QulacsError("update_quantum_state: no superoperator");
+                S = Sg * S;
+            }
+            dm_superoperator_host(q.data(), (UINT)q.size(), qc(S.data.data()), d->data(), d->dim, d->get_cuda_stream(),
+                                  d->device_number);
+            check_abi("update_quantum_state");
+        }
+        q.clear();
+        count = 0;
+    };
+    for (size_t k = 0; k < gates_.size(); ++k) {
+        const QuantumGateBase* g = gates_[k];
+        const auto qs = sorted_union({g->get_target_index_list(), g->get_control_index_list()});
+        if (qs.size() > kDmFuseQubits || !dm_fusable(g)) {
+            flush();
+            gates_[k]->update_quantum_state(d);
+            continue;
+        }
+        auto nq = sorted_union({q, qs});
+        if (nq.size() > kDmFuseQubits) {
+            flush();
+            nq = qs;
+        }
+        if (count == 0) first = k;
+        q = nq;
+        ++count;
+    }
+    flush();
+}
 
 void set_circuit_fusion(bool enabled) { g_fusion = enabled; }
 bool get_circuit_fusion() { return g_fusion; }
@@ -1755,6 +1837,15 @@ void QuantumCircuit::build_plan() {
 
 // src/cppsim/circuit.cpp:46-57
 void QuantumCircuit::update_quantum_state(QuantumStateBase* s) {
+    if (auto* dm = as_dm(s); dm && g_fusion) {
+        if (s->qubit_count != qubit_count_) throw QulacsError("update_quantum_state: qubit count mismatch");
+        for (const auto* g : gates_) {
+            check_targets(s, g->get_target_index_list(), "update_quantum_state");
+            check_targets(s, g->get_control_index_list(), "update_quantum_state");
+        }
+        update_dm_fused(dm);
+        return;
+    }
     auto* sv = as_sv(s);
     if (!g_fusion || !sv || qubit_count_ < 4 || qubit_count_ > QM_CIRCUIT_MAX_QUBITS ||
         gates_.size() < min_fused_run(qubit_count_)) {

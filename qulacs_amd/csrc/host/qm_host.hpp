@@ -687,6 +687,7 @@ class QuantumCircuit {
     };
     void release_plan();
     void build_plan();
+    void update_dm_fused(DensityMatrixMI355* d);
     std::vector<FusedRun> plan_;
     uint64_t version_ = 0, plan_version_ = ~0ULL;
     std::mutex plan_mutex_;  // one fused update at a time per circuit (the plan uploads lazily per device)

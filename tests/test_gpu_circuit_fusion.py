@@ -95,7 +95,7 @@ def test_noise_gates_split_the_runs():
     np.testing.assert_allclose(out[1], out[0], atol=1e-12)
 
 
-def test_density_matrix_and_short_circuits_stay_per_gate():
+def test_density_matrix_circuit_matches_the_pure_state():
     n = 5
     c = _build(n, 9, depth=2)
     d = q.DensityMatrix(n)
@@ -115,3 +115,40 @@ def test_short_runs_from_22_qubits():
         c.add_H_gate(k)
         c.add_RZ_gate(20 - k, 0.3 + k)
     np.testing.assert_allclose(_run(c, n, True), _run(c, n, False), atol=1e-12)
+
+
+def _dm_run(c, n, fused, v0):
+    q.set_circuit_fusion(fused)
+    try:
+        d = q.DensityMatrix(n)
+        d.load(v0)
+        c.update_quantum_state(d)
+        return d.get_matrix()
+    finally:
+        q.set_circuit_fusion(True)
+
+
+def test_density_matrix_superoperator_fusion():
+    # gates and channels on <= 3 qubits together compose into one superoperator pass
+    n = 7
+    rng = np.random.default_rng(21)
+    c = q.QuantumCircuit(n)
+    for layer in range(4):
+        for i in range(n):
+            c.add_gate(gate.DenseMatrix(i, haar_unitary(rng, 1)))
+            c.add_gate(gate.DepolarizingNoise(i, 0.05))
+        for i in range(layer % 2, n - 1, 2):
+            c.add_CNOT_gate(i, i + 1)
+            c.add_gate(gate.TwoQubitDepolarizingNoise(i, i + 1, 0.02))
+        c.add_gate(gate.AmplitudeDampingNoise(3, 0.1))
+        c.add_gate(gate.TOFFOLI(0, 1, 2))
+        c.add_gate(gate.DenseMatrix([1, 4, 6, 2], haar_unitary(rng, 4)))  # 4 qubits: its own pass
+        c.add_gate(gate.Pauli([0, 5], [2, 3]))
+        c.add_gate(gate.DiagonalMatrix([2, 5], np.exp(1j * rng.uniform(0, 6, 4))))
+    v0 = q.QuantumState(n)
+    v0.set_Haar_random_state(4)
+    v = v0.get_vector()
+    ref = _dm_run(c, n, False, v)
+    got = _dm_run(c, n, True, v)
+    np.testing.assert_allclose(got, ref, atol=1e-12)
+    assert abs(np.trace(got) - 1) < 1e-12
