@@ -221,3 +221,36 @@ def test_instrument_round_trip():
     m2 = gate.from_json(m.to_json())
     assert m2.is_instrument() and len(m2.get_gate_list()) == 2
     assert not gate.AmplitudeDampingNoise(0, 0.1).is_instrument()
+
+
+def test_newer_gate_kinds_round_trip():
+    # CPMapGate (gate_general.hpp:676-690), ProbabilisticGate with a register,
+    # LinearCombinationGate, FusedSWAPGate, NoisyEvolution(Fast)Gate
+    cp = gate.CP([gate.P0(1), gate.P1(1)], False, True, True)
+    d = json.loads(cp.to_json())
+    assert d["name"] == "CPMapGate" and d["state_normalize"] == "false" and d["probability_normalize"] == "true"
+    cp2 = gate.from_json(cp.to_json())
+    assert json.loads(cp2.to_json()) == d
+    pi = gate.ProbabilisticInstrument([0.25, 0.75], [gate.X(0), gate.Z(0)], 4)
+    d = json.loads(pi.to_json())
+    assert d["is_instrument"] == "true" and d["classical_register_address"] == "4"
+    pi2 = gate.from_json(pi.to_json())
+    assert pi2.is_instrument() and pi2.get_distribution() == [0.25, 0.75]
+    lc = gate.LinearCombination([0.5 + 0.25j, -1.0], [gate.X(0), gate.CNOT(0, 2)])
+    d = json.loads(lc.to_json())
+    assert d["name"] == "LinearCombinationGate" and d["coefs"] == ["(0.5,0.25)", "(-1,0)"]
+    lc2 = gate.from_json(lc.to_json())
+    assert lc2.get_coef_list() == [0.5 + 0.25j, -1.0]
+    assert np.array_equal(lc2.get_matrix(), lc.get_matrix())
+    fs = gate.FusedSWAP(1, 4, 2)
+    fs2 = gate.from_json(fs.to_json())
+    assert fs2.get_target_index_list() == [1, 2, 4, 5] and np.array_equal(fs2.get_matrix(), fs.get_matrix())
+    h = Observable(2)
+    h.add_operator(0.5, "Z 0 Z 1")
+    c = GeneralQuantumOperator(2)
+    c.add_operator(0.5, "X 1")
+    c.add_operator(0.5j, "Y 1")
+    for g in (gate.NoisyEvolution(h, [c], 0.7, 0.01), gate.NoisyEvolution_fast(h, [c], 0.7)):
+        g2 = gate.from_json(g.to_json())
+        assert g2.get_name() == g.get_name()
+        assert json.loads(g2.to_json()) == json.loads(g.to_json())
