@@ -2976,11 +2976,15 @@ unsigned NoiseSimulator::choose_branch(const QuantumGateBase* g) {
     const long i = (long)(std::lower_bound(cum.begin(), cum.end(), r) - cum.begin()) - 1;
     return (unsigned)std::max(0L, i);
 }
-void NoiseSimulator::apply_from(const std::vector<unsigned>& branch, QuantumStateMI355* s, size_t start) const {
+// Gates [start, end) on s; branch[c] is the drawn branch of the c-th probabilistic gate,
+// `col` the column of the first probabilistic gate at or after `start`.
+void NoiseSimulator::apply_from(const std::vector<unsigned>& branch, QuantumStateMI355* s, size_t start,
+                                size_t col) const {
     for (size_t k = start; k < circuit_->gates_.size(); ++k) {
         const QuantumGateBase* g = circuit_->gates_[k];
         if (const auto* p = dynamic_cast<const QuantumGate_Probabilistic*>(g)) {
-            if (branch[k] < p->get_gate_list().size()) p->get_gate_list()[branch[k]]->update_quantum_state(s);
+            const unsigned b = branch[col++];
+            if (b < p->get_gate_list().size()) p->get_gate_list()[b]->update_quantum_state(s);
         } else {
             circuit_->gates_[k]->update_quantum_state(s);
         }
@@ -2992,9 +2996,12 @@ std::vector<ITYPE> NoiseSimulator::execute(unsigned count) {
 }
 NoiseSimulator::Result* NoiseSimulator::execute_and_get_result(unsigned count) {
     const size_t ng = circuit_->gates_.size();
-    std::vector<std::vector<unsigned>> pick(count, std::vector<unsigned>(ng, 0));
+    std::vector<size_t> noisy;  // positions of the probabilistic gates: one column each
+    for (size_t k = 0; k < ng; ++k)
+        if (dynamic_cast<const QuantumGate_Probabilistic*>(circuit_->gates_[k])) noisy.push_back(k);
+    std::vector<std::vector<unsigned>> pick(count, std::vector<unsigned>(noisy.size(), 0));
     for (auto& row : pick)
-        for (size_t k = 0; k < ng; ++k) row[k] = choose_branch(circuit_->gates_[k]);
+        for (size_t c = 0; c < noisy.size(); ++c) row[c] = choose_branch(circuit_->gates_[noisy[c]]);
     // shots with the same branches share one trajectory; trajectories in descending
     // lexicographic order, so those starting with branch 0 share the longest prefix
     std::sort(pick.begin(), pick.end());
@@ -3004,16 +3011,19 @@ NoiseSimulator::Result* NoiseSimulator::execute_and_get_result(unsigned count) {
         if (!req.empty() && req.back().first == pick[i]) ++req.back().second;
         else req.emplace_back(pick[i], 1u);
     }
+    pick.clear();
     auto out = std::make_unique<Result>();
     QuantumStateMI355 common(initial_->qubit_count, initial_->device_number);
     common.load(initial_.get());
-    size_t done = 0;  // gates [0, done) are applied to `common`
+    size_t done = 0, col = 0;  // gates [0, done) are applied to `common`; col = probabilistic gates among them
     for (const auto& rq : req) {
         const auto& br = rq.first;
-        while (done < ng && br[done] == 0) {
+        while (done < ng) {
             const QuantumGateBase* g = circuit_->gates_[done];
             if (const auto* p = dynamic_cast<const QuantumGate_Probabilistic*>(g)) {
+                if (br[col] != 0) break;
                 if (!p->get_gate_list().empty()) p->get_gate_list()[0]->update_quantum_state(&common);
+                ++col;
             } else {
                 circuit_->gates_[done]->update_quantum_state(&common);
             }
@@ -3022,7 +3032,7 @@ NoiseSimulator::Result* NoiseSimulator::execute_and_get_result(unsigned count) {
         auto* st = new QuantumStateMI355(common.qubit_count, common.device_number);
         st->load(&common);
         out->result.emplace_back(st, rq.second);
-        apply_from(br, st, done);
+        apply_from(br, st, done, col);
     }
     return out.release();
 }

@@ -1016,7 +1016,7 @@ class NoiseSimulator {
     std::unique_ptr<QuantumStateMI355> initial_;
     Random random_;
     unsigned choose_branch(const QuantumGateBase* g);
-    void apply_from(const std::vector<unsigned>& branches, QuantumStateMI355* s, size_t start) const;
+    void apply_from(const std::vector<unsigned>& branches, QuantumStateMI355* s, size_t start, size_t col) const;
 };
 
 // Expectation values from light cones: for each term, the gates that can reach its
