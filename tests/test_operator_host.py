@@ -258,3 +258,35 @@ def test_openfermion_coefficient_forms():
         observable.create_observable_from_openfermion_text("(0.25-1.5j) [Z2]")  # not Hermitian
     with pytest.raises(RuntimeError):
         quantum_operator.create_quantum_operator_from_openfermion_text("abc [X0]")
+
+
+class _QubitOperatorLike:
+    """OpenFermion's QubitOperator layout: `terms` maps tuples of (qubit, axis)
+    pairs to coefficients."""
+
+    def __init__(self, terms):
+        self.terms = terms
+
+
+def test_convert_openfermion_op():
+    # python/tests/single_cpu/test_utils.py:15-27: three terms, any order
+    from qulacs_amd.utils import convert_openfermion_op
+
+    op = convert_openfermion_op(_QubitOperatorLike({
+        ((0, "X"),): 1.0,
+        ((0, "Z"), (1, "Y")): 2.0,
+        ((0, "Z"), (3, "Y")): 1.0,
+    }))
+    text = str(op)
+    for part in ("(1,0) X 0", "(2,0) Z 0 Y 1", "(1,0) Z 0 Y 3"):
+        text = text.replace(part, "")
+    assert text == " +  + "
+    assert op.get_qubit_count() == 4
+    # only the real part of a coefficient is kept; an explicit size wins; the empty
+    # product is the identity
+    op = convert_openfermion_op(_QubitOperatorLike({((1, "X"),): 0.5 + 2j, (): -0.25}), n_qubits=3)
+    assert op.get_qubit_count() == 3 and op.get_term_count() == 2
+    assert op.get_term(0).get_coef() == 0.5
+    m = op.get_matrix().toarray()
+    x1 = np.kron(np.eye(2), np.kron(np.array([[0, 1], [1, 0]]), np.eye(2)))
+    np.testing.assert_allclose(m, 0.5 * x1 - 0.25 * np.eye(8), atol=0)
