@@ -303,3 +303,86 @@ def test_optimizer_preserves_the_circuit_unitary(block):
                 assert len(g.get_target_index_list()) + len(g.get_control_index_list()) <= max(block, 2)
         got = QuantumCircuitOptimizer().merge_all(c2)
         assert np.allclose(got.get_matrix(), want.get_matrix(), atol=1e-12), (block, light)
+
+
+def _optimized_light(c):
+    from qulacs_amd.circuit import QuantumCircuitOptimizer
+
+    c2 = c.copy()
+    QuantumCircuitOptimizer().optimize_light(c2)
+    merged = [QuantumCircuitOptimizer().merge_all(x).get_matrix() for x in (c2, c)]
+    return c2, np.allclose(merged[0], merged[1], atol=1e-12)
+
+
+def test_optimize_light_depth_and_gate_counts():
+    """The fixed cases of test/cppsim/test_circuit_optimize_light.cpp:20-175 with
+    their depth and gate-count expectations; the circuit unitary is unchanged."""
+    n = 4
+    cases = []
+    c = q.QuantumCircuit(n)  # successive gates on one qubit merge
+    c.add_X_gate(0)
+    c.add_Y_gate(0)
+    cases.append((c, 1, 1))
+    c = q.QuantumCircuit(n)  # no tensor products of disjoint gates
+    c.add_X_gate(0)
+    c.add_Y_gate(1)
+    cases.append((c, 1, 2))
+    c = q.QuantumCircuit(n)  # Z on a CNOT's target does not commute through it
+    c.add_Z_gate(1)
+    c.add_CNOT_gate(0, 1)
+    c.add_Z_gate(1)
+    cases.append((c, 2, 2))
+    c = q.QuantumCircuit(n)  # merged controlled gates
+    c.add_Z_gate(0)
+    c.add_gate(gate.merge(gate.CNOT(0, 1), gate.Y(2)))
+    c.add_gate(gate.merge(gate.CNOT(1, 2), gate.Y(2)))
+    c.add_Z_gate(1)
+    cases.append((c, 2, 2))
+    for i, (c, depth, count) in enumerate(cases):
+        c2, same = _optimized_light(c)
+        assert same, i
+        assert (c2.calculate_depth(), c2.get_gate_count()) == (depth, count), i
+
+    # parametric gates are left as they are (test_circuit_optimize_light.cpp:150-175)
+    p = q.ParametricQuantumCircuit(2)
+    p.add_X_gate(0)
+    p.add_parametric_RX_gate(0, 0.1)
+    p.add_parametric_RY_gate(0, 0.1)
+    p2, same = _optimized_light(p)
+    assert same
+    assert (p2.calculate_depth(), p2.get_gate_count(), p2.get_parameter_count()) == (3, 3, 2)
+
+
+@pytest.mark.parametrize("kind", ["cnot_cz", "cnot_cz_swap", "random_pairs"])
+def test_optimize_light_random_circuits(kind):
+    """The random families of test_circuit_optimize_light.cpp:178-315 (nearest-
+    neighbour CNOT/CZ; sqrtX/sqrtY/T with any-pair CNOT/CZ/SWAP; two-qubit random
+    unitaries on shuffled pairs): the optimised circuit has the same unitary."""
+    rng = np.random.default_rng(len(kind))
+    n = 5
+    for _ in range(3):
+        c = q.QuantumCircuit(n)
+        if kind == "random_pairs":
+            for _ in range(10 * n):
+                pair = [int(x) for x in rng.permutation(n)[:2]]
+                c.add_gate(gate.RandomUnitary(pair, int(rng.integers(1 << 30))))
+        else:
+            for _ in range(5 if kind == "cnot_cz" else 10):
+                for i in range(n):
+                    r = int(rng.integers(2)) + 3 if kind == "cnot_cz" else int(rng.integers(6))
+                    other = int(rng.integers(n))
+                    if other == i:
+                        other = (other + 1) % n
+                    if kind == "cnot_cz":
+                        other = i + 1
+                    if r == 0:
+                        c.add_sqrtX_gate(i)
+                    elif r == 1:
+                        c.add_sqrtY_gate(i)
+                    elif r == 2:
+                        c.add_T_gate(i)
+                    elif i + 1 < n:
+                        (c.add_CNOT_gate, c.add_CZ_gate, c.add_SWAP_gate)[r - 3](i, other)
+        c2, same = _optimized_light(c)
+        assert same
+        assert c2.get_gate_count() <= c.get_gate_count()
