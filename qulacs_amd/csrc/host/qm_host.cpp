@@ -1322,23 +1322,35 @@ QuantumGateMatrix* SWAP(unsigned a, unsigned b) {
     g->set_gate_property(PC);
     return g;
 }
-QuantumGateMatrix* DenseMatrix(const std::vector<unsigned>& t, const ComplexMatrix& m) {
+// Factories given a qubit list reject repeated qubits, as the reference's do
+// (DuplicatedQubitIndexException, test/cppsim/test_gate.cpp:1540-1625).
+static void require_distinct(const std::vector<unsigned>& t, const char* what) {
     std::vector<unsigned> s(t);
     std::sort(s.begin(), s.end());
-    if (std::adjacent_find(s.begin(), s.end()) != s.end()) throw QulacsError("DenseMatrix: duplicated target");
+    if (std::adjacent_find(s.begin(), s.end()) != s.end())
+        throw QulacsError(std::string(what) + ": duplicated target");
+}
+QuantumGateMatrix* DenseMatrix(const std::vector<unsigned>& t, const ComplexMatrix& m) {
+    require_distinct(t, "DenseMatrix");
     return new QuantumGateMatrix(t, m);
 }
 QuantumGateDiagonalMatrix* DiagonalMatrix(const std::vector<unsigned>& t, const std::vector<CPPCTYPE>& d) {
+    require_distinct(t, "DiagonalMatrix");
     return new QuantumGateDiagonalMatrix(t, d);
 }
-ClsPauliGate* Pauli(const std::vector<unsigned>& t, const std::vector<unsigned>& p) { return new ClsPauliGate(t, p, false, 0.0); }
+ClsPauliGate* Pauli(const std::vector<unsigned>& t, const std::vector<unsigned>& p) {
+    require_distinct(t, "Pauli");
+    return new ClsPauliGate(t, p, false, 0.0);
+}
 ClsPauliGate* PauliRotation(const std::vector<unsigned>& t, const std::vector<unsigned>& p, double a) {
+    require_distinct(t, "PauliRotation");
     return new ClsPauliGate(t, p, true, a);
 }
 // Haar-random unitary: QR of a complex Gaussian matrix (modified Gram-Schmidt)
 // with the phases of R's diagonal divided out, as gate::RandomUnitary
 // (src/cppsim/gate_factory.cpp:325-380); the Gaussian stream is this library's.
 QuantumGateMatrix* RandomUnitary(const std::vector<unsigned>& t, uint64_t seed) {
+    require_distinct(t, "RandomUnitary");
     Random rnd;
     rnd.set_seed(seed);
     const size_t d = size_t(1) << t.size();
@@ -1559,6 +1571,7 @@ QuantumGate_SingleParameter* ParametricRX(unsigned q, double a) { return new Qua
 QuantumGate_SingleParameter* ParametricRY(unsigned q, double a) { return new QuantumGate_SingleParameter({q}, {2}, a, "ParametricRY"); }
 QuantumGate_SingleParameter* ParametricRZ(unsigned q, double a) { return new QuantumGate_SingleParameter({q}, {3}, a, "ParametricRZ"); }
 QuantumGate_SingleParameter* ParametricPauliRotation(const std::vector<unsigned>& t, const std::vector<unsigned>& p, double a) {
+    require_distinct(t, "ParametricPauliRotation");
     return new QuantumGate_SingleParameter(t, p, a, "ParametricPauliRotation");
 }
 }  // namespace gate

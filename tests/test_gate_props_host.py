@@ -261,3 +261,54 @@ def test_aliases_exist():
     o = Observable(1)
     o.add_operator_move(q.PauliOperator("Z 0", 1.0))
     assert o.get_term_count() == 1
+
+
+@pytest.mark.parametrize("first,second,controls,targets", [
+    (lambda: gate.X(0), lambda: gate.CNOT(0, 1), [], [0, 1]),
+    (lambda: gate.CNOT(0, 1), lambda: gate.X(0), [], [0, 1]),
+    (lambda: gate.X(1), lambda: gate.CNOT(0, 1), [], [0, 1]),
+    (lambda: gate.CNOT(0, 1), lambda: gate.X(1), [], [0, 1]),
+    (lambda: gate.CNOT(0, 1), lambda: gate.CZ(0, 1), [0], [1]),
+    (lambda: gate.CNOT(0, 1), lambda: gate.CZ(1, 0), [], [0, 1]),
+    (lambda: gate.X(2), lambda: gate.CNOT(0, 1), [], [0, 1, 2]),
+])
+def test_control_merge_cases(first, second, controls, targets):
+    """test/cppsim/test_gate.cpp:979-1140: the merged gate's own matrix, in
+    ascending target order, is the product of the two full matrices; CNOT then CZ
+    on the same pair keeps the shared control and leaves iY on the target."""
+    a, b = first(), second()
+    m = gate.merge(a, b)
+    assert m.get_control_index_list() == controls and m.get_target_index_list() == targets
+    n = len(targets) + len(controls)
+    want = full(b, n) @ full(a, n)
+    if controls:
+        np.testing.assert_allclose(m.get_matrix(), 1j * np.array([[0, -1j], [1j, 0]]), atol=1e-14)
+    else:
+        np.testing.assert_allclose(m.get_matrix(), want, atol=1e-14)
+    np.testing.assert_allclose(full(m, n), want, atol=1e-14)
+
+
+def test_duplicate_qubits_are_rejected():
+    """test/cppsim/test_gate.cpp:1540-1625: a gate given the same qubit twice is an
+    error; the same factories with distinct qubits build."""
+    from scipy.sparse import identity
+
+    good = [lambda: gate.CNOT(10, 13), lambda: gate.CZ(10, 13), lambda: gate.SWAP(10, 13),
+            lambda: gate.Pauli([2, 1, 0, 3, 7, 9, 4], [0] * 7),
+            lambda: gate.PauliRotation([2, 1, 0, 3, 7, 9, 4], [0] * 7, 0.2),
+            lambda: gate.DenseMatrix([10, 13], np.eye(4)), lambda: gate.SparseMatrix([10, 13], identity(4)),
+            lambda: gate.DiagonalMatrix([10, 13], np.ones(4)), lambda: gate.RandomUnitary([10, 13]),
+            lambda: gate.ParametricPauliRotation([1, 2], [1, 3], 0.1), lambda: gate.TOFFOLI(1, 2, 3),
+            lambda: gate.FREDKIN(1, 2, 3)]
+    bad = [lambda: gate.CNOT(21, 21), lambda: gate.CZ(21, 21), lambda: gate.SWAP(21, 21),
+           lambda: gate.Pauli([0, 1, 3, 1, 5, 6, 2], [0] * 7),
+           lambda: gate.PauliRotation([0, 1, 3, 1, 5, 6, 2], [0] * 7, 0.2),
+           lambda: gate.DenseMatrix([21, 21], np.eye(4)), lambda: gate.SparseMatrix([21, 21], identity(4)),
+           lambda: gate.DiagonalMatrix([21, 21], np.ones(4)), lambda: gate.RandomUnitary([3, 3]),
+           lambda: gate.ParametricPauliRotation([2, 2], [1, 3], 0.1), lambda: gate.TOFFOLI(1, 1, 3),
+           lambda: gate.FREDKIN(1, 3, 3)]
+    for make in good:
+        assert make() is not None
+    for i, make in enumerate(bad):
+        with pytest.raises(Exception, match="must differ|duplicated"):
+            make()
