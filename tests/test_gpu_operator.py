@@ -127,3 +127,12 @@ def test_h2_expectation_from_openfermion_text():
     st.set_Haar_random_state(4)
     e = o.solve_ground_state_eigenvalue_by_lanczos_method(st, 16).real
     assert abs(e - np.linalg.eigvalsh(m)[0]) < 1e-8
+
+
+def test_pauli_beyond_the_state_is_an_error():
+    # test/cppsim/test_pauli_operator.cpp:20-30
+    st = q.QuantumState(2)
+    st.set_Haar_random_state(1)
+    with pytest.raises(Exception):
+        q.PauliOperator("X 0 X 1 X 3", 2.0).get_expectation_value(st)
+    assert abs(q.PauliOperator("X 0 X 1", 2.0).get_expectation_value(st)) <= 2.0 + 1e-12

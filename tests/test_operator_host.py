@@ -290,3 +290,33 @@ def test_convert_openfermion_op():
     m = op.get_matrix().toarray()
     x1 = np.kron(np.eye(2), np.kron(np.array([[0, 1], [1, 0]]), np.eye(2)))
     np.testing.assert_allclose(m, 0.5 * x1 - 0.25 * np.eye(8), atol=0)
+
+
+def test_pauli_string_parsing_edge_cases():
+    """test/cppsim/test_pauli_operator.cpp:5-75: trailing whitespace is ignored,
+    the empty string is the identity, and a string whose axes and indices do not
+    alternate is an error."""
+    p = PauliOperator("X 0 ", 1.0)
+    assert p.get_index_list() == [0] and p.get_pauli_id_list() == [1]
+    assert p.get_pauli_string() == PauliOperator("X 0", 1.0).get_pauli_string()
+    e = PauliOperator("", 1.0)
+    assert e.get_index_list() == [] and e.get_pauli_id_list() == [] and e.get_pauli_string() == ""
+    assert len(PauliOperator("X 0 Y 1 ", 2.0).get_index_list()) == 2
+    for broken in ("X 0 X Z 1 Y 2", "X {i}", "X 4x", "4 X", "X 0 Y "):
+        with pytest.raises(Exception):
+            PauliOperator(broken, 2.0)
+
+
+@pytest.mark.parametrize("a,b,string,coef", [
+    ("X 0", "X 0", "", 4), ("X 0", "Y 0", "Z 0", 4j), ("X 0", "Z 0", "Y 0", -4j),
+    ("Y 0", "X 0", "Z 0", -4j), ("Y 0", "Y 0", "", 4), ("Y 0", "Z 0", "X 0", 4j),
+    ("Z 0", "X 0", "Y 0", 4j), ("Z 0", "Y 0", "X 0", -4j), ("Z 0", "Z 0", "", 4),
+    ("X 0", "Y 1", "X 0 Y 1", 4), ("X 0 Y 1", "Y 0 X 1", "Z 0 Z 1", 4),
+])
+def test_pauli_products_of_the_reference_table(a, b, string, coef):
+    """test/cppsim/test_pauli_operator.cpp:80-140, both operator* and operator*=."""
+    r = PauliOperator(a, 2.0) * PauliOperator(b, 2.0)
+    assert r.get_pauli_string() == string and r.get_coef() == coef
+    r = PauliOperator(a, 2.0)
+    r *= PauliOperator(b, 2.0)
+    assert r.get_pauli_string() == string and r.get_coef() == coef
