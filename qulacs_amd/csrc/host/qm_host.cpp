@@ -244,6 +244,10 @@ QuantumStateMI355::QuantumStateMI355(unsigned n, unsigned device) : QuantumState
     check_abi("allocate_quantum_state_host");
     set_zero_state();
 }
+void QuantumStateBase::set_zero_norm_state() {
+    set_zero_state();  // every entry finite, so the product below is exactly zero
+    multiply_coef(0.0);
+}
 void QuantumStateMI355::set_zero_state() {
     initialize_quantum_state_host(data_, dim, stream_, device_number);
     check_abi("set_zero_state");

@@ -89,6 +89,8 @@ class QuantumStateBase {
     virtual ITYPE element_count() const = 0;
 
     virtual void set_zero_state() = 0;
+    // Every entry 0 (src/cppsim/state.hpp:136; the reference's GPU state throws here).
+    void set_zero_norm_state();
     virtual void set_computational_basis(ITYPE comp_basis) = 0;
     virtual void set_Haar_random_state() = 0;
     virtual void set_Haar_random_state(unsigned seed) = 0;

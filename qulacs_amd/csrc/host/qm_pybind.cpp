@@ -289,6 +289,7 @@ PYBIND11_MODULE(_core, m) {
         .def("get_qubit_count", [](const QuantumStateBase& s) { return s.qubit_count; })
         .def("get_device_name", &QuantumStateBase::get_device_name)
         .def("set_zero_state", &QuantumStateBase::set_zero_state)
+        .def("set_zero_norm_state", &QuantumStateBase::set_zero_norm_state)
         .def("set_computational_basis", &QuantumStateBase::set_computational_basis, py::arg("comp_basis"))
         .def("set_Haar_random_state", py::overload_cast<>(&QuantumStateBase::set_Haar_random_state))
         .def("set_Haar_random_state", py::overload_cast<unsigned>(&QuantumStateBase::set_Haar_random_state),
