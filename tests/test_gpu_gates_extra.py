@@ -311,3 +311,29 @@ def test_causal_cone_simulator_matches_full_simulation():
     # Z 0 and Y 7 see four qubits each through the three brickwork layers
     assert [x.get_qubit_count() for x in circuits[0]] == [4] and [x.get_qubit_count() for x in circuits[2]] == [4]
     assert cc.get_coef_list() == [0.7, -1.2, 0.4, 0.25]
+
+
+def test_fused_swap_every_placement_on_10_qubits():
+    # test/cppsim/test_gate.cpp:461-473: every (t0, t1, block) whose two blocks fit
+    # and do not overlap, against the bit-block permutation of the indices
+    n = 10
+    idx = np.arange(1 << n)
+    st = q.QuantumState(n)
+    st.set_Haar_random_state(17)
+    v = st.get_vector()
+    cases = 0
+    for t0 in range(n):
+        for t1 in range(n):
+            if t0 == t1:
+                continue
+            for bs in range(1, min(abs(t1 - t0), n - t0, n - t1) + 1):
+                mask = (1 << bs) - 1
+                lo, hi = (idx >> t0) & mask, (idx >> t1) & mask
+                moved = idx & ~((mask << t0) | (mask << t1)) | (lo << t1) | (hi << t0)
+                want = np.empty_like(v)
+                want[moved] = v
+                st.load(v)
+                gate.FusedSWAP(t0, t1, bs).update_quantum_state(st)
+                np.testing.assert_allclose(st.get_vector(), want, atol=0, err_msg=str((t0, t1, bs)))
+                cases += 1
+    assert cases == 190

@@ -136,3 +136,26 @@ def test_pauli_beyond_the_state_is_an_error():
     with pytest.raises(Exception):
         q.PauliOperator("X 0 X 1 X 3", 2.0).get_expectation_value(st)
     assert abs(q.PauliOperator("X 0 X 1", 2.0).get_expectation_value(st)) <= 2.0 + 1e-12
+
+
+def test_observable_on_a_larger_state_and_identity_term():
+    # test/cppsim/test_hamiltonian.cpp:566-639: an observable read from OpenFermion
+    # text acts on a state with two more qubits as the sum of its terms; an "I 0"
+    # term applied to a state scales it by its coefficient
+    o = observable.create_observable_from_openfermion_text((GOLDEN / "openfermion_H2.txt").read_text())
+    st = q.QuantumState(o.get_qubit_count() + 2)
+    for seed in (None, 5):
+        if seed is None:
+            st.set_computational_basis(0)
+        else:
+            st.set_Haar_random_state(seed)
+        want = sum(o.get_term(i).get_expectation_value(st) for i in range(o.get_term_count()))
+        got = o.get_expectation_value(st)
+        assert abs(got - want) < 1e-12 and abs(complex(want).imag) < 1e-12
+    ident = q.Observable(3)
+    ident.add_operator(0.5, "I 0")
+    src, dst = q.QuantumState(3), q.QuantumState(3)
+    src.set_Haar_random_state(2)
+    ident.apply_to_state(src, dst)
+    src.add_state_with_coef(-1 / 0.5, dst)
+    assert src.get_squared_norm() < 1e-24
