@@ -119,6 +119,15 @@ void oracle_set_num_threads(int n) {
 }
 
 // ---------------------------------------------------------------- memory / init
+// Fills a freshly allocated buffer with one value using the same static
+// schedule as the kernels' parallel loops, so each page is first touched (and
+// placed) by the thread that later works on it.
+void oracle_parallel_fill(double* state, ITYPE dim, double re, double im) {
+    CTYPE* s = C(state);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)dim; ++i) s[i] = CTYPE(re, im);
+}
+
 // src/csim/init_ops_fill.cpp:10-23
 void oracle_initialize_quantum_state(double* state, ITYPE dim) {
     CTYPE* s = C(state);
@@ -558,18 +567,20 @@ void oracle_mt64_raw(unsigned long long seed, UINT count, unsigned long long* ou
 }
 
 // src/cppsim/state_dm.hpp:383-408: same loop over |rho_ii| (std::abs of the
-// complex diagonal element).
-void oracle_dm_sampling(const double* rho, ITYPE dim, const double* uniforms, UINT count,
-                        unsigned long long* out) {
+// complex diagonal element, which libstdc++ evaluates with libm's cabs/hypot).
+void oracle_dm_sampling_cdf(const double* rho, ITYPE dim, double* cdf) {
     const CTYPE* r = C(rho);
-    std::vector<double> cdf;
-    cdf.reserve(dim + 1);
     double sum = 0.;
-    cdf.push_back(0.);
+    cdf[0] = 0.;
     for (ITYPE i = 0; i < dim; ++i) {
         sum += std::abs(r[i * dim + i]);
-        cdf.push_back(sum);
+        cdf[i + 1] = sum;
     }
+}
+void oracle_dm_sampling(const double* rho, ITYPE dim, const double* uniforms, UINT count,
+                        unsigned long long* out) {
+    std::vector<double> cdf(dim + 1);
+    oracle_dm_sampling_cdf(rho, dim, cdf.data());
     for (UINT c = 0; c < count; ++c) {
         auto it = std::lower_bound(cdf.begin(), cdf.end(), uniforms[c]);
         out[c] = (unsigned long long)(std::distance(cdf.begin(), it) - 1);

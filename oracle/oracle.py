@@ -19,6 +19,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 ORACLE_SO = HERE / "liboracle.so"
 REF_SO = HERE / "_ref" / "libcsim_ref.so"
+NORM_PROBE_SO = HERE / "_ref" / "libnorm_probe.so"
 
 _u = C.c_uint
 _ull = C.c_ulonglong
@@ -28,7 +29,7 @@ _p = C.c_void_p
 
 def build(ref: bool = True) -> None:
     """make -C oracle (and the reference build when its sources are present)."""
-    targets = ["all"]
+    targets = ["all", "norm-probe"]
     if ref and Path("/root/reference/src/csim").is_dir():
         targets.append("ref")
     subprocess.run(["make", "-s", "-C", str(HERE), "-j8", *targets], check=True)
@@ -215,6 +216,11 @@ class Oracle(_Kernels):
     def __init__(self):
         super().__init__(ORACLE_SO)
 
+    def parallel_fill(self, psi: np.ndarray, value: complex) -> None:
+        """First touch of a new buffer by the OpenMP threads (static schedule);
+        the compiled reference in the same process shares the thread pool."""
+        self.f("parallel_fill")(_ptr(psi), _ull(psi.size), _d(complex(value).real), _d(complex(value).imag))
+
     def sampling(self, psi: np.ndarray, uniforms) -> np.ndarray:
         u = np.ascontiguousarray(np.asarray(uniforms, dtype=np.float64))
         out = np.zeros(u.size, dtype=np.uint64)
@@ -231,6 +237,11 @@ class Oracle(_Kernels):
         out = np.zeros(u.size, dtype=np.uint64)
         self.f("dm_sampling")(_ptr(rho), _ull(dim), _ptr(u), _u(u.size), _ptr(out))
         return out
+
+    def dm_sampling_cdf(self, rho: np.ndarray, dim: int) -> np.ndarray:
+        cdf = np.zeros(dim + 1)
+        self.f("dm_sampling_cdf")(_ptr(rho), _ull(dim), _ptr(cdf))
+        return cdf
 
     def dm_depolarizing(self, t: int, p: float, rho: np.ndarray, dim: int) -> None:
         self.f("dm_depolarizing")(_u(t), _d(p), _ptr(rho), _ull(dim))
@@ -270,3 +281,34 @@ class Reference(_Kernels):
 
 def reference_available() -> bool:
     return REF_SO.exists()
+
+
+def norm_probe_stacked_prob(psi: np.ndarray) -> np.ndarray:
+    """The reference's sampling prefix sum (sum += std::norm(psi_i)) compiled from
+    oracle/norm_probe.cpp with the reference's release flags (built here, used on
+    this machine only: -march=native)."""
+    lib = C.CDLL(str(NORM_PROBE_SO))
+    psi = np.ascontiguousarray(psi, dtype=np.complex128)
+    out = np.zeros(psi.size + 1)
+    lib.norm_probe_stacked_prob.argtypes = [_p, C.c_size_t, _p]
+    lib.norm_probe_stacked_prob(_ptr(psi), psi.size, _ptr(out))
+    return out
+
+
+def parameter_shift_grad(energy, theta) -> np.ndarray:
+    """GradCalculator::calculate_grad (src/vqcsim/GradCalculator.cpp:8-44): for
+    each parameter k, the energy with theta_k + pi/2 and with theta_k - pi/2
+    (every other parameter at theta), and grad_k = (E+ - E-) / 2.  `energy` maps
+    a parameter vector (one angle per parametric gate, in circuit order) to
+    <H>; the reference evaluates it with CausalConeSimulator, whose value is the
+    full simulation's, so any exact evaluation (here the oracle's kernels) gives
+    the same gradient up to rounding."""
+    theta = np.asarray(theta, dtype=np.float64)
+    grad = np.zeros(theta.size)
+    for k in range(theta.size):
+        tp = theta.copy()
+        tm = theta.copy()
+        tp[k] = theta[k] + np.pi / 2
+        tm[k] = theta[k] - np.pi / 2
+        grad[k] = (energy(tp) - energy(tm)) / 2.0
+    return grad

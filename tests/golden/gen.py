@@ -103,3 +103,46 @@ def pauli_matrix(n: int, targets, paulis) -> np.ndarray:
     for q in reversed(range(n)):  # qubit n-1 is the leftmost Kronecker factor
         m = np.kron(m, PAULI[ops.get(q, 0)])
     return m
+
+
+def host_hypot(x: float, y: float) -> float:
+    """|x + iy| as the reference's std::abs(std::complex<double>) rounds it on
+    glibc 2.35 (libstdc++ calls cabs = hypot): C. F. Borges, "An Improved
+    Algorithm for hypot(a, b)", arXiv:1904.09481 (2019), non-FMA form, with a
+    power-of-two rescale above 2^511 / below 2^-511 and the shortcut ax + ay when
+    the operands differ by more than 2^54.  Python floats round every operation
+    to nearest without contraction, as that code does."""
+    import math
+
+    ax, ay = abs(x), abs(y)
+    if ax < ay:
+        ax, ay = ay, ax
+    if math.isinf(ax) or math.isinf(ay):
+        return math.inf
+    if math.isnan(ax) or math.isnan(ay):
+        return math.nan
+    eps, scale, large, tiny = 2.0 ** -54, 2.0 ** -600, 2.0 ** 511, 2.0 ** -511
+
+    def kernel(a: float, b: float) -> float:
+        h = math.sqrt(a * a + b * b)
+        if h <= 2.0 * b:
+            delta = h - b
+            t1 = a * (2.0 * delta - a)
+            t2 = (delta - 2.0 * (a - b)) * delta
+        else:
+            delta = h - a
+            t1 = 2.0 * delta * (a - 2.0 * b)
+            t2 = (4.0 * delta - b) * b + delta * delta
+        return h - (t1 + t2) / (2.0 * h)
+
+    if ax > large:
+        if ay <= ax * eps:
+            return ax + ay
+        return kernel(ax * scale, ay * scale) / scale
+    if ay < tiny:
+        if ax >= ay / eps:
+            return ax + ay
+        return kernel(ax / scale, ay / scale) * scale
+    if ay <= ax * eps:
+        return ax + ay
+    return kernel(ax, ay)

@@ -375,3 +375,203 @@ def test_depolarizing_closed_form(oracle):
                     br = [(b >> k) & 1 for k in reversed(range(n)) if k != q]
                     full[a, b] = red[tuple(ar + br)] / 2
         assert rel(rho.reshape(dim, dim), (1 - p) * R + p * full) < TOL
+
+
+# ---------------------------------------------------------------- DM sampling,
+# the random stream and the per-amplitude probability (src/cppsim/state_dm.hpp:383-408,
+# src/cppsim/utility.hpp:54-102, src/cppsim/state.hpp:915-920)
+
+def _libm_hypot():
+    import ctypes
+
+    libm = ctypes.CDLL("libm.so.6")
+    libm.hypot.restype = ctypes.c_double
+    libm.hypot.argtypes = [ctypes.c_double, ctypes.c_double]
+    return libm.hypot
+
+
+def _hypot_inputs(rng, count):
+    """Operand pairs across the whole double range, with equal, widely varying,
+    zero and tiny/huge operands."""
+    e = rng.integers(-1070, 1020, size=(count, 2))
+    v = rng.random((count, 2)) * np.exp2(e.astype(np.float64))
+    v[: count // 4, 1] = v[: count // 4, 0] * rng.random(count // 4) * np.exp2(-rng.integers(0, 80, count // 4))
+    v[count // 4: count // 4 + 50, 1] = 0.0
+    v[count // 4 + 50: count // 4 + 100, 1] = v[count // 4 + 50: count // 4 + 100, 0]
+    return [(float(a), float(b)) for a, b in v]
+
+
+def test_host_hypot_model_equals_libm_and_is_not_correctly_rounded():
+    """The model in gen.host_hypot reproduces libm's hypot (what std::abs of a
+    complex double calls) on every input tried, and libm's hypot is not the
+    correctly rounded value: a device restatement must follow the model, not
+    compute |z| exactly."""
+    from tests.golden.gen import host_hypot
+
+    hyp = _libm_hypot()
+    rng = np.random.default_rng(31)
+    pairs = _hypot_inputs(rng, 40000)
+    assert all(host_hypot(a, b) == hyp(a, b) for a, b in pairs)
+    # correctly rounded sqrt(a^2 + b^2) by exact rational comparison
+    import math
+
+    def cr(a, b):
+        x = Fraction(a) ** 2 + Fraction(b) ** 2
+        d = math.sqrt(float(x))
+        cands = (math.nextafter(d, 0), d, math.nextafter(d, math.inf))
+        return min(cands, key=lambda c: abs(Fraction(c) ** 2 - x))
+
+    small = [(rng.random(), rng.random() * 2.0 ** -int(rng.integers(0, 30))) for _ in range(20000)]
+    diffs = sum(hyp(a, b) != cr(a, b) for a, b in small)
+    assert diffs > 0  # glibc 2.35's hypot is faithful, not correctly rounded
+
+
+def _brute_dm_samples(diag, uniforms):
+    from tests.golden.gen import host_hypot
+
+    cdf = [0.0]
+    s = 0.0
+    for z in diag:
+        s = s + (abs(z.real) if z.imag == 0.0 else host_hypot(z.real, z.imag))
+        cdf.append(s)
+    return [bisect.bisect_left(cdf, u) - 1 for u in uniforms], cdf
+
+
+def test_dm_sampling_brute_force_complex_diagonal(oracle):
+    """oracle_dm_sampling against a brute-force CDF of |rho_ii| (libm-rounded
+    modulus, IEEE sequential sums) and bisect_left, on a diagonal with nonzero
+    imaginary parts (the hypot path) and uniforms on and beside every step."""
+    rng = np.random.default_rng(32)
+    n = 6
+    dim = 1 << n
+    rho = (rng.standard_normal((dim, dim)) + 1j * rng.standard_normal((dim, dim))) * 0.02
+    diag = rng.random(dim) * np.exp2(-rng.integers(0, 12, dim)) + 1j * rng.standard_normal(dim) * 1e-3
+    diag[5] = 0.0
+    diag[9] = -0.0 + 1e-3j  # purely imaginary
+    diag[11] = -0.25        # negative real part: |rho_ii|
+    rho[np.arange(dim), np.arange(dim)] = diag
+    rho = rho.ravel().copy()
+    us = rng.random(300) * 1.5
+    _, cdf = _brute_dm_samples(diag, [])
+    us = np.concatenate([us, cdf, [np.nextafter(c, 2.0) for c in cdf], [np.nextafter(c, -1.0) for c in cdf]])
+    want, cdf = _brute_dm_samples(diag, us.tolist())
+    got = oracle.dm_sampling(rho, dim, us)
+    assert [int(x) if x < 2**63 else int(x) - 2**64 for x in got] == want
+    assert np.array_equal(oracle.dm_sampling_cdf(rho, dim), np.array(cdf))
+
+
+def test_dm_sampling_known_answers(oracle):
+    """A diagonal density matrix with exactly representable entries has the exact
+    CDF (0, 1/8, 1/2, 1/2, 1): a uniform on a step selects the index below it
+    (lower_bound), one just above selects the next non-empty index, the empty
+    basis state 2 is never selected, and a basis-state density matrix always
+    gives its index (test/cppsim/test_state.cpp:38-47 for the state vector)."""
+    dim = 4
+    rho = np.zeros((dim, dim), dtype=np.complex128)
+    rho[np.arange(dim), np.arange(dim)] = [0.125, 0.375, 0.0, 0.5]
+    rho[0, 3] = rho[3, 0] = 0.2  # off-diagonal entries do not enter
+    r = rho.ravel().copy()
+    assert np.array_equal(oracle.dm_sampling_cdf(r, dim), [0.0, 0.125, 0.5, 0.5, 1.0])
+    us = [0.125, np.nextafter(0.125, 1), 0.5, np.nextafter(0.5, 1), 1.0, 1e-300, 0.0, np.nextafter(1.0, 2)]
+    got = [int(x) for x in oracle.dm_sampling(r, dim, us)]
+    assert got == [0, 1, 1, 3, 3, 0, 2**64 - 1, 4]
+    basis = np.zeros((8, 8), dtype=np.complex128)
+    basis[6, 6] = 1.0
+    assert set(oracle.dm_sampling(basis.ravel().copy(), 8, np.linspace(1e-9, 1.0, 100)).tolist()) == {6}
+
+
+def test_mt19937_64_check_value_and_canonical_uniforms(oracle):
+    """The reference's Random (src/cppsim/utility.hpp:54-102) is std::mt19937_64
+    with uniform_real_distribution<double>(0, 1).  The C++ standard fixes the
+    engine by its check value ([rand.predef]: the 10000th output of a
+    default-constructed mt19937_64, seed 5489, is 9981545732273789042), and
+    generate_canonical with a 64-bit engine is one draw x -> RN(x) / 2^64
+    (clamped below 1)."""
+    raw = oracle.mt64_raw(5489, 10000)
+    assert int(raw[-1]) == 9981545732273789042
+    for seed in (0, 1, 12345, 2**63 + 7):
+        raw = oracle.mt64_raw(seed, 2000)
+        us = oracle.mt64_uniforms(seed, 2000)
+        want = [min(float(int(x)) / 2.0 ** 64, np.nextafter(1.0, 0.0)) for x in raw]
+        assert us.tolist() == want
+
+
+def test_norm_probe_equals_oracle_probability():
+    """The reference's stacked probabilities (sum += std::norm(psi_i),
+    src/cppsim/state.hpp:915-920) compiled with its release flags (gnu++14 -O3
+    -march=native -mfpmath=both, CMakeLists.txt:36,281,349) equal the oracle's
+    fma(re, re, im*im) sums bit for bit.  A plain re*re + im*im or the other
+    contraction order fma(im, im, re*re) would differ."""
+    from oracle import oracle as O
+
+    if not O.NORM_PROBE_SO.exists():
+        pytest.skip("norm probe not built here (make -C oracle norm-probe)")
+    o = O.Oracle()
+    rng = np.random.default_rng(33)
+    diff_plain = diff_swapped = 0
+    for n in (6, 10, 14):
+        psi = random_state(rng, n) * np.exp2(-rng.integers(0, 8, 1 << n))
+        probe = O.norm_probe_stacked_prob(psi)
+        assert np.array_equal(probe, o.sampling_cdf(psi))
+        re, im = psi.real, psi.imag
+        plain = np.concatenate([[0.0], np.cumsum(re * re + im * im)])
+        diff_plain += int(not np.array_equal(plain, probe))
+        swapped = [0.0]
+        s = 0.0
+        for a, b in zip(re.tolist(), im.tolist()):
+            s = s + float(Fraction(b) * Fraction(b) + Fraction(a * a))
+            swapped.append(s)
+        diff_swapped += int(swapped != probe.tolist())
+    assert diff_plain > 0 and diff_swapped > 0
+
+
+def test_parameter_shift_closed_forms_and_finite_differences(oracle):
+    """oracle.parameter_shift_grad (GradCalculator.cpp:8-44) on circuits with a
+    closed-form energy: <Z> after RX(a) on |0> is cos(a) (qulacs RX = exp(+i a X/2)),
+    so d/da = -sin(a); <Z0 Z1> after RY(a) (x) RY(b) is cos(a) cos(b).  Then a
+    random 4-qubit circuit against central differences (step 1e-6)."""
+    from oracle.oracle import parameter_shift_grad
+
+    def e_rx(th):
+        psi = np.array([1, 0], dtype=np.complex128)
+        oracle.pauli_rotation([0], [1], th[0], psi)
+        return oracle.expectation([0], [3], psi)
+
+    for a in (0.3, 1.7, -2.9):
+        g = parameter_shift_grad(e_rx, [a])
+        assert abs(e_rx([a]) - np.cos(a)) < TOL and abs(g[0] + np.sin(a)) < TOL
+
+    def e_ry2(th):
+        psi = np.zeros(4, dtype=np.complex128)
+        psi[0] = 1
+        oracle.pauli_rotation([0], [2], th[0], psi)
+        oracle.pauli_rotation([1], [2], th[1], psi)
+        return oracle.expectation([0, 1], [3, 3], psi)
+
+    a, b = 0.4, -1.1
+    g = parameter_shift_grad(e_ry2, [a, b])
+    assert np.allclose(g, [-np.sin(a) * np.cos(b), -np.cos(a) * np.sin(b)], atol=TOL, rtol=0)
+
+    rng = np.random.default_rng(34)
+    n = 4
+    rots = [([int(rng.integers(n))], [int(rng.integers(1, 4))]) for _ in range(10)] + [([0, 2], [1, 3]), ([1, 3], [2, 2])]
+    ents = [haar_unitary(rng, 2) for _ in range(3)]
+    terms = random_pauli_terms(rng, n, 5)
+
+    def energy(th):
+        psi = np.zeros(1 << n, dtype=np.complex128)
+        psi[0] = 1
+        for i, (tq, pl) in enumerate(rots):
+            oracle.pauli_rotation(tq, pl, th[i], psi)
+            if i % 4 == 3:
+                oracle.dense([i % n, (i + 1) % n], ents[i // 4], psi)
+        return sum(c * oracle.expectation(t, p, psi) for t, p, c in terms)
+
+    th = rng.uniform(0, 2 * np.pi, len(rots))
+    g = parameter_shift_grad(energy, th)
+    h = 1e-6
+    for k in range(th.size):
+        tp, tm = th.copy(), th.copy()
+        tp[k] += h
+        tm[k] -= h
+        assert abs(g[k] - (energy(tp) - energy(tm)) / (2 * h)) < 1e-8
