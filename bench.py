@@ -27,6 +27,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import subprocess
 import sys
 import time
 from pathlib import Path
@@ -106,6 +107,140 @@ def kernel_of(g) -> str:
     }[g.kind]
 
 
+def host_cpu_info() -> dict:
+    """lscpu model, sockets and cores of the host, the CPUs this process may use
+    and the OpenMP placement the CPU baselines run with."""
+    info = {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core", "NUMA node(s)": "numa_nodes"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                v = v.strip()
+                info[keys[k.strip()]] = int(v) if v.isdigit() else v
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_threads() -> int:
+    """OpenMP threads for the CPU baselines: OMP_NUM_THREADS when set (the CPU
+    share of this machine), else the physical cores this process may use."""
+    v = os.environ.get("OMP_NUM_THREADS", "")
+    if v.isdigit() and int(v) > 0:
+        return int(v)
+    info = host_cpu_info()
+    tpc = info.get("threads_per_core", 1) if isinstance(info.get("threads_per_core"), int) else 1
+    return max(1, info["affinity_cpus"] // max(1, tpc))
+
+
+def run_cpu_worker(leg: str, args, threads: int) -> dict:
+    """Runs a CPU-baseline leg in a child process with OMP_PROC_BIND=close,
+    OMP_PLACES=cores and `threads` OpenMP threads (so the binding never touches
+    this process's threads), and returns its JSON."""
+    env = dict(os.environ)
+    env["OMP_NUM_THREADS"] = str(threads)
+    env.setdefault("OMP_PROC_BIND", "close")
+    env.setdefault("OMP_PLACES", "cores")
+    cmd = [sys.executable, str(ROOT / "bench.py"), "--cpu-worker", leg, "--qubits", str(args.qubits), "--gates",
+           str(args.gates), "--seed", str(args.seed), "--cpu-seconds", str(args.cpu_seconds), "--small-count",
+           str(args.small_count), "--small-plain", str(args.small_plain)]
+    r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=900)
+    if r.returncode != 0 or not r.stdout.strip():
+        return {"error": f"cpu worker {leg} failed ({r.returncode}): {r.stderr[-400:]}"}
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    out["omp"] = {k: env.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")}
+    return out
+
+
+def _cpu_kernels():
+    from oracle import oracle as O
+
+    if O.reference_available():
+        return O.Reference(), "reference"
+    return O.Oracle(), "port"
+
+
+def cpu_worker(args) -> dict:
+    """The CPU legs (run by run_cpu_worker in a child process)."""
+    from oracle import oracle as O
+
+    k, kind = _cpu_kernels()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "1"))
+    k.set_num_threads(threads)
+    toucher = O.Oracle()
+    toucher.set_num_threads(threads)
+    if args.cpu_worker == "sweep":
+        from qulacs_amd.workloads import gate_sweep
+
+        return cpu_baseline(args, gate_sweep(args.qubits, args.gates, seed=args.seed), args.qubits, k, kind,
+                            threads, toucher)
+    if args.cpu_worker == "brickwork":
+        from qulacs_amd.workloads import brickwork_circuit
+
+        n = 20
+        gates, obs, u = brickwork_circuit(n, 10, 16, 1024, seed=args.seed + 2)
+        uu = np.ascontiguousarray(u)
+
+        def once():
+            psi = np.empty(1 << n, dtype=np.complex128)
+            toucher.parallel_fill(psi, 0.0)
+            psi[0] = 1.0
+            for kind_g, q, m in gates:
+                if kind_g == "CNOT":
+                    k.two("CNOT", q[0], q[1], psi)
+                else:
+                    k.dense1(q[0], m, psi)
+            e = 0.0
+            for x in obs:
+                tl, pl = _mask_lists(x[0], x[1], n)
+                e += x[3] * k.expectation(tl, pl, psi)
+            toucher.sampling(psi, uu)
+            return e
+
+        once()
+        best = float("inf")
+        for _ in range(3):
+            t0 = time.perf_counter()
+            once()
+            best = min(best, time.perf_counter() - t0)
+        return {"ms": round(best * 1e3, 2), "cores": threads, "kind": kind,
+                "sample": "whole workload (circuit, 16-term expectation, 1024 samples by the in-repo port), best "
+                          "of 3"}
+    if args.cpu_worker == "small":
+        from qulacs_amd.workloads import many_small_circuits
+
+        cs = many_small_circuits(args.small_count, seed=args.seed + 1001)
+        sub = cs[::max(1, len(cs) // args.small_plain)]
+        total_gates = sum(len(c.gates) for c in cs)
+        spent, done_circ, done_gates = 0.0, 0, 0
+        for c in sub:
+            t0 = time.perf_counter()
+            psi = np.empty(1 << c.n, dtype=np.complex128)
+            toucher.parallel_fill(psi, 0.0)
+            psi[0] = 1.0
+            for kind_g, q, m in c.gates:
+                if kind_g == "CZ":
+                    k.two("CZ", q[0], q[1], psi)
+                else:
+                    k.dense1(q[0], m, psi)
+            for bf, pf, rot, coef in c.terms:
+                tl, pl = _mask_lists(bf, pf, c.n)
+                k.expectation(tl, pl, psi)
+            spent += time.perf_counter() - t0
+            done_circ += 1
+            done_gates += len(c.gates)
+            if spent >= args.cpu_seconds / 2:
+                break
+        cpu_s = spent * total_gates / done_gates
+        return {"circuits_per_s": round(len(cs) / cpu_s, 1), "cores": threads, "kind": kind,
+                "sample": f"first {done_circ} of {len(sub)} evenly spaced circuits ({done_gates} gates, "
+                          f"{spent:.1f} s, {threads} OpenMP threads), scaled by gate count to all {len(cs)}"}
+    raise SystemExit(f"unknown cpu worker leg {args.cpu_worker}")
+
+
 def sv_sweep(args, lib_mod, hip):
     from qulacs_amd.workloads import GateLauncher, alg_bytes, gate_sweep
 
@@ -170,14 +305,32 @@ def sv_sweep(args, lib_mod, hip):
     per_launch = dom["bytes"] / dom["count"]
     achieved = per_launch / (dom["mean_ms"] * 1e-3) / 1e9
 
-    # measured device-to-device copy rate for reference (read + write bytes)
-    half = dim * 16 // 2
+    # measured HBM ceilings on this device, from the library's own kernels:
+    # a streaming copy (copy_quantum_state_from_device_to_device: half the state
+    # onto the other half, read + write bytes) and an in-place read-modify-write
+    # pass over the whole state (state_multiply_host by 1, the access pattern of
+    # a dense gate with every qubit >= 3)
+    half_dim = dim // 2
+    one = lib_mod.QmComplex(1.0, 0.0)
     ev2 = hip.create(2)
-    hip.record(ev2[0], hstream)
-    hip.h.hipMemcpyAsync(C.c_void_p(state + half), C.c_void_p(state), half, 3, hstream)
-    hip.record(ev2[1], hstream)
-    copy_ms = hip.elapsed_ms(ev2[0], ev2[1])
+    copy_ms, rmw_ms = [], []
+    for _ in range(5):
+        hip.record(ev2[0], hstream)
+        L.copy_quantum_state_from_device_to_device(C.c_void_p(state + half_dim * 16), C.c_void_p(state), half_dim,
+                                                   stream, 0)
+        hip.record(ev2[1], hstream)
+        copy_ms.append(hip.elapsed_ms(ev2[0], ev2[1]))
+        hip.record(ev2[0], hstream)
+        L.state_multiply_host(one, state, dim, stream, 0)
+        hip.record(ev2[1], hstream)
+        rmw_ms.append(hip.elapsed_ms(ev2[0], ev2[1]))
     hip.destroy(ev2)
+    lib_mod.check()
+    copy_gbps = 2 * half_dim * 16 / (min(copy_ms) * 1e-3) / 1e9
+    rmw_gbps = 2 * dim * 16 / (min(rmw_ms) * 1e-3) / 1e9
+    hbm_measured = {"copy_kernel_GBps": round(copy_gbps, 1), "inplace_rmw_GBps": round(rmw_gbps, 1),
+                    "copy_bytes": 2 * half_dim * 16, "rmw_bytes": 2 * dim * 16, "method": "best of 5, HIP events",
+                    "dominant_kernel_frac_of_rmw": round(achieved / rmw_gbps, 4)}
 
     lib_mod.call("qm_set_synchronous", 1)
     lib_mod.call("release_quantum_state_host", state, 0)
@@ -189,7 +342,7 @@ def sv_sweep(args, lib_mod, hip):
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
                      "traffic": measured_traffic(dom_name, n),
                      "bytes_per_launch": per_launch, "mean_launch_ms": dom["mean_ms"]},
-        "hbm_copy_GBps": round(2 * half / (copy_ms * 1e-3) / 1e9, 1),
+        "hbm_measured": hbm_measured,
     }
 
 
@@ -246,8 +399,62 @@ def dm_sweep(args, lib_mod, hip):
     lib_mod.call("qm_set_synchronous", 1)
     lib_mod.call("release_quantum_state_host", rho, 0)
     lib_mod.call("release_cuda_stream_host", stream, 1, 0)
+
+    # The same noisy circuit through the library's circuit loop
+    # (QuantumCircuit.update_quantum_state on a DensityMatrix): consecutive gates
+    # and channels whose qubits fit in two are composed on the host into one
+    # superoperator and applied in one pass.
+    import qulacs_amd as qa
+
+    circ = qa.QuantumCircuit(n)
+    for g in gates:
+        if g.kind == "CNOT":
+            circ.add_gate(qa.gate.CNOT(g.targets[0], g.targets[1]))
+        elif g.kind == "CZ":
+            circ.add_gate(qa.gate.CZ(g.targets[0], g.targets[1]))
+        else:
+            mat = g.matrix if g.matrix is not None else rotation_matrix(g.kind, g.angle)
+            circ.add_gate(qa.gate.DenseMatrix(list(g.targets), mat))
+        for q in g.targets:
+            circ.add_gate(qa.gate.DepolarizingNoise(q, 0.01))
+    dmst = qa.DensityMatrix(n)
+    qa.set_synchronous(False)
+    circ.update_quantum_state(dmst)  # warm-up (kernel loading)
+    qa.synchronize(dmst)
+    dmst.set_zero_state()
+    qa.synchronize(dmst)
+    t0 = time.perf_counter()
+    circ.update_quantum_state(dmst)
+    qa.synchronize(dmst)
+    loop_ms = (time.perf_counter() - t0) * 1e3
+    qa.set_synchronous(True)
+    tr_loop = dmst.get_squared_norm()
+    # HBM passes the loop issues, by its grouping rule: a run of gates/channels on
+    # at most two qubits together is one pass (a lone gate on one or two qubits
+    # too); a gate on more qubits is two passes (rows, then columns)
+    passes, cur = 0, set()
+    for g in gates:
+        for part in [set(g.targets)] + [{q} for q in g.targets]:
+            if len(part) > 2:
+                passes += (1 if cur else 0) + 2
+                cur = set()
+                continue
+            if len(cur | part) > 2:
+                passes += 1
+                cur = set()
+            cur |= part
+    passes += 1 if cur else 0
+    moved = passes * 32 * dim * dim
     return {"qubits": n, "gates": len(gates), "channels": ops - len(gates), "ms": round(ms, 2),
-            "alg_GBps": round(alg / (ms * 1e-3) / 1e9, 1), "trace_after": tr}
+            "alg_GBps": round(alg / (ms * 1e-3) / 1e9, 1), "trace_after": tr,
+            "circuit_loop": {"ms": round(loop_ms, 2), "passes": passes,
+                             "alg_GBps": round(alg / (loop_ms * 1e-3) / 1e9, 1),
+                             "pass_GBps": round(moved / (loop_ms * 1e-3) / 1e9, 1),
+                             "pass_frac_of_peak": round(moved / (loop_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+                             "trace_after": tr_loop,
+                             "method": "QuantumCircuit.update_quantum_state on a DensityMatrix (gates and "
+                                       "depolarizing channels fused into superoperators), one run after a warm-up; "
+                                       "pass_GBps = passes x 32 B x 4^n / time"}}
 
 
 FP64_PEAK_TFLOPS = 78.6
@@ -256,7 +463,6 @@ FP64_PEAK_TFLOPS = 78.6
 def many_small(args, lib_mod, hip):
     """Many small circuits (n ~ U{8..16}, 4..12 layers of RY RZ + CZ chain, TFIM
     energy each) through the batched executor: one launch per step."""
-    from oracle import oracle as O
     from qulacs_amd.batch import Batch
     from qulacs_amd.workloads import many_small_circuits, small_circuit_alg_flops
 
@@ -341,34 +547,7 @@ def many_small(args, lib_mod, hip):
         "energy_checksum": float(np.sum(energies)),
     }
     if not args.no_cpu_baseline:
-        k, kind = (O.Reference(), "reference") if O.reference_available() else (O.Oracle(), "port")
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-        threads = min(threads, os.cpu_count() or threads)
-        k.set_num_threads(threads)
-        spent, done_circ, done_gates = 0.0, 0, 0
-        for c in sub:
-            t0 = time.perf_counter()
-            psi = np.zeros(1 << c.n, dtype=np.complex128)
-            psi[0] = 1.0
-            for kind_g, q, m in c.gates:
-                if kind_g == "CZ":
-                    k.two("CZ", q[0], q[1], psi)
-                else:
-                    k.dense1(q[0], m, psi)
-            e = 0.0
-            for bf, pf, rot, coef in c.terms:
-                tl, pl = _mask_lists(bf, pf, c.n)
-                e += coef * k.expectation(tl, pl, psi)
-            spent += time.perf_counter() - t0
-            done_circ += 1
-            done_gates += len(c.gates)
-            if spent >= args.cpu_seconds / 2:
-                break
-        cpu_s = spent * total_gates / done_gates
-        out["cpu_baseline"] = {"circuits_per_s": round(len(cs) / cpu_s, 1), "cores": threads, "kind": kind,
-                               "sample": f"first {done_circ} of the {len(sub)} evenly spaced circuits "
-                                         f"({done_gates} gates, {spent:.1f} s, {threads} OpenMP threads), scaled by "
-                                         f"gate count to all {len(cs)}"}
+        out["cpu_baseline"] = run_cpu_worker("small", args, cpu_threads())
     return out
 
 
@@ -511,39 +690,38 @@ def brickwork(args, lib_mod, hip):
     out["batched_energy_matches"] = bool(abs(e_batch - e_gpu) <= 1e-10 * scale)
     out["api_matches"] = bool(abs(e_api - e_gpu) <= 1e-10 * scale and np.array_equal(np.asarray(s_api, dtype=np.uint64), samples))
     if not args.no_cpu_baseline:
-        k, kind = (O.Reference(), "reference") if O.reference_available() else (O.Oracle(), "port")
+        # parity against the CPU (untimed, in this process) ...
+        k, _ = _cpu_kernels()
         o = O.Oracle()
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-        threads = min(threads, os.cpu_count() or threads)
-        k.set_num_threads(threads)
-        best = float("inf")
-        for _ in range(3):
-            t0 = time.perf_counter()
-            psi = np.zeros(dim, dtype=np.complex128)
-            psi[0] = 1.0
-            for kind_g, q, m in gates:
-                if kind_g == "CNOT":
-                    k.two("CNOT", q[0], q[1], psi)
-                else:
-                    k.dense1(q[0], m, psi)
-            e_cpu = 0.0
-            for x in obs:
-                tl, pl = _mask_lists(x[0], x[1], n)
-                e_cpu += x[3] * k.expectation(tl, pl, psi)
-            s_cpu = o.sampling(psi, uu)
-            best = min(best, time.perf_counter() - t0)
-        out["cpu_baseline"] = {"ms": round(best * 1e3, 2), "cores": threads, "kind": kind,
-                               "sample": "whole workload, best of 3 (sampling by the in-repo port)"}
+        psi = np.zeros(dim, dtype=np.complex128)
+        psi[0] = 1.0
+        for kind_g, q, m in gates:
+            if kind_g == "CNOT":
+                k.two("CNOT", q[0], q[1], psi)
+            else:
+                k.dense1(q[0], m, psi)
+        e_cpu = 0.0
+        for x in obs:
+            tl, pl = _mask_lists(x[0], x[1], n)
+            e_cpu += x[3] * k.expectation(tl, pl, psi)
+        s_cpu = o.sampling(psi, uu)
         out["energy_matches"] = bool(abs(e_cpu - e_gpu) <= 1e-10 * scale)
         out["samples_identical"] = bool(np.array_equal(s_cpu, samples))
         out["state_rel_l2"] = float(np.linalg.norm(psi_gpu - psi) / np.linalg.norm(psi))
+        # ... and its timing in child processes with bound OpenMP threads, at the
+        # machine's CPU share and at one thread (the reference csim fixes its
+        # thread count when first used, so each count needs its own process)
+        out["cpu_baseline"] = run_cpu_worker("brickwork", args, cpu_threads())
+        out["cpu_baseline_one_thread"] = run_cpu_worker("brickwork", args, 1)
     return out
 
 
 def serving(args, lib_mod, hip):
     """Poisson stream of mixed-size brickwork circuits through the request
-    server, against the same requests run one after another through the
-    per-gate path (FIFO, same arrival times)."""
+    server, against two sequential baselines over the same requests (FIFO from
+    the same arrival times): the per-gate path (one C ABI call per gate, the
+    reference's circuit loop) and the fused executor one request at a time (the
+    same per-request kernels as the server, without batching or overlap)."""
     import threading
 
     from qulacs_amd.batch import _marshal
@@ -561,7 +739,8 @@ def serving(args, lib_mod, hip):
         prepared.append((arr, keep, (C.c_ulonglong * t)(*[x[0] for x in r.terms]),
                          (C.c_ulonglong * t)(*[x[1] for x in r.terms]), (C.c_uint * t)(*[x[2] for x in r.terms]),
                          (C.c_double * t)(*[x[3] for x in r.terms]), np.ascontiguousarray(r.uniforms)))
-    srv = Server(hbm_budget_bytes=args.serve_budget_gib << 30, large_slots=4, small_max_qubits=16)
+    budget = int(args.serve_budget_gb * 1e9)
+    srv = Server(hbm_budget_bytes=budget, large_slots=4, small_max_qubits=16)
     # warm-up: one small and one large request (kernel loading, allocator pools)
     for r in (min(reqs, key=lambda x: x.n), max(reqs, key=lambda x: x.n)):
         srv.wait(srv.submit(r.n, r.gates[:8], r.terms, r.uniforms[:4]))
@@ -578,7 +757,6 @@ def serving(args, lib_mod, hip):
             ids[i] = L.qm_server_submit(srv.h, r.n, p[0], len(r.gates), p[2], p[3], p[4], p[5], len(r.terms),
                                         p[6].ctypes.data_as(lib_mod.PD), p[6].size)
 
-    t_start = time.perf_counter()
     th = threading.Thread(target=submitter)
     th.start()
     th.join()
@@ -587,22 +765,25 @@ def serving(args, lib_mod, hip):
     for i, r in enumerate(reqs):
         e = C.c_double()
         lat = C.c_double()
+        nflag = C.c_uint()
         out = np.zeros(r.uniforms.size, dtype=np.uint64)
-        L.qm_server_wait(srv.h, ids[i], C.byref(e), out.ctypes.data_as(lib_mod.PI), C.byref(lat))
-        served.append((e.value, out, lat.value))
-    t_end = time.perf_counter()
+        flags = np.zeros(r.uniforms.size, dtype=np.uint8)
+        L.qm_server_wait_checked(srv.h, ids[i], C.byref(e), out.ctypes.data_as(lib_mod.PI),
+                                 flags.ctypes.data_as(lib_mod.PB), C.byref(nflag), C.byref(lat))
+        served.append((e.value, out, flags, lat.value))
     lib_mod.check()
     stats = srv.stats()
     srv.close()
-    lat_served = np.array([x[2] for x in served])
-    makespan = max(submit_at[i] + served[i][2] for i in range(len(reqs)))
+    lat_served = np.array([x[3] for x in served])
+    makespan = max(submit_at[i] + served[i][3] for i in range(len(reqs)))
 
-    # the per-gate path, one request at a time
     stream = lib_mod.call("allocate_cuda_stream_host", 1, 0)
     lib_mod.call("qm_set_synchronous", 0)
+
+    # (1) the per-gate path, one request at a time
     service = []
-    match_e, match_s = 0, 0
-    for r, p in zip(reqs, prepared):
+    match_e, match_s, match_unflagged, flagged_total, diff_unflagged = 0, 0, 0, 0, 0
+    for r, p, sv in zip(reqs, prepared, served):
         t0 = time.perf_counter()
         dim = 1 << r.n
         st = lib_mod.call("allocate_quantum_state_host", dim, 0)
@@ -619,28 +800,66 @@ def serving(args, lib_mod, hip):
                            stream, 0)
         lib_mod.call("release_quantum_state_host", st, 0)
         service.append(time.perf_counter() - t0)
-        i = len(service) - 1
         scale = sum(abs(x[3]) for x in r.terms)
-        match_e += int(abs(e - served[i][0]) <= 1e-10 * scale)
-        match_s += int(np.array_equal(out, served[i][1]))
+        ok = sv[2] == 0
+        match_e += int(abs(e - sv[0]) <= 1e-10 * scale)
+        match_s += int(np.array_equal(out, sv[1]))
+        match_unflagged += int(np.array_equal(out[ok], sv[1][ok]))
+        diff_unflagged += int(np.sum(out[ok] != sv[1][ok]))
+        flagged_total += int((~ok).sum())
+    lib_mod.check()
+
+    # (2) the fused executor, one request at a time (host compile included, as in the server)
+    from qulacs_amd.batch import FusedCircuit
+
+    fused_service = []
+    for r, p in zip(reqs, prepared):
+        t0 = time.perf_counter()
+        dim = 1 << r.n
+        st = lib_mod.call("allocate_quantum_state_host", dim, 0)
+        L.initialize_quantum_state_host(st, dim, stream, 0)
+        if r.n >= 4:
+            fc = FusedCircuit(r.n, r.gates, block_qubits=3)
+            L.qm_circuit_apply(fc.h, st, dim, stream, 0)
+        else:
+            fc = None
+            for kind, q, m in r.gates:
+                if kind == "CNOT":
+                    L.CNOT_gate_host(q[0], q[1], st, dim, stream, 0)
+                else:
+                    L.single_qubit_dense_matrix_gate_host(q[0], _cm(lib_mod, m), st, dim, stream, 0)
+        co = (lib_mod.QmComplex * len(r.terms))(*[lib_mod.QmComplex(x[3], 0.0) for x in r.terms])
+        L.qm_observable_expectation_host(p[2], p[3], p[4], co, len(r.terms), st, dim, stream, 0)
+        out = np.zeros(r.uniforms.size, dtype=np.uint64)
+        L.qm_sampling_host(p[6].ctypes.data_as(lib_mod.PD), p[6].size, out.ctypes.data_as(lib_mod.PI), st, dim,
+                           stream, 0)
+        lib_mod.call("release_quantum_state_host", st, 0)
+        if fc is not None:
+            fc.close()
+        fused_service.append(time.perf_counter() - t0)
     lib_mod.check()
     lib_mod.call("qm_set_synchronous", 1)
     lib_mod.call("release_cuda_stream_host", stream, 1, 0)
-    finish = 0.0
-    lat_plain = []
-    for r, sv in zip(reqs, service):
-        start = max(r.arrival, finish)
-        finish = start + sv
-        lat_plain.append(finish - r.arrival)
-    lat_plain = np.array(lat_plain)
-    plain_makespan = finish - reqs[0].arrival
+
+    def fifo(svc):
+        finish, lat = 0.0, []
+        for r, sv in zip(reqs, svc):
+            start = max(r.arrival, finish)
+            finish = start + sv
+            lat.append(finish - r.arrival)
+        return np.array(lat), finish - reqs[0].arrival
+
+    lat_plain, plain_makespan = fifo(service)
+    lat_fused, fused_makespan = fifo(fused_service)
 
     def pct(a, q):
         return round(float(np.percentile(a, q)) * 1e3, 2)
 
+    n_samples = int(sum(r.uniforms.size for r in reqs))
     return {
         "workload": f"{len(reqs)} Poisson requests at {args.serve_rate}/s, brickwork circuits, n = round(exp(U[ln 10, "
-                    f"ln {args.serve_max_qubits}])), depth U{{20..200}}, 8-term observable, 1000 samples each",
+                    f"ln {args.serve_max_qubits}])), depth U{{20..200}}, 8-term observable, 1000 samples each; "
+                    f"admission budget {args.serve_budget_gb} GB",
         "qubits": [r.n for r in reqs], "gates": int(sum(len(r.gates) for r in reqs)),
         "served": {"circuits_per_s": round(len(reqs) / makespan, 2), "p50_ms": pct(lat_served, 50),
                    "p99_ms": pct(lat_served, 99), "makespan_s": round(makespan, 3), "stats": stats},
@@ -648,8 +867,16 @@ def serving(args, lib_mod, hip):
                              "p99_ms": pct(lat_plain, 99), "service_s_total": round(float(np.sum(service)), 3),
                              "method": "each request alone through the per-gate C ABI path; latency from the same "
                                        "arrival times with first-in first-out queueing"},
+        "fused_sequential": {"circuits_per_s": round(len(reqs) / fused_makespan, 2), "p50_ms": pct(lat_fused, 50),
+                             "p99_ms": pct(lat_fused, 99), "service_s_total": round(float(np.sum(fused_service)), 3),
+                             "method": "each request alone through the fused executor (compile, |0>, circuit, "
+                                       "observable, sampling), one at a time, FIFO from the same arrival times"},
         "speedup_circuits_per_s": round(plain_makespan / makespan, 2),
+        "speedup_over_fused_sequential": round(fused_makespan / makespan, 2),
         "energies_match": f"{match_e}/{len(reqs)}", "samples_identical": f"{match_s}/{len(reqs)}",
+        "unflagged_samples_identical": f"{match_unflagged}/{len(reqs)}",
+        "unflagged_samples_differing": diff_unflagged,
+        "flagged_samples": f"{flagged_total}/{n_samples}",
     }
 
 
@@ -756,21 +983,14 @@ def _cm(lib_mod, m):
     return arr
 
 
-def cpu_baseline(args, gates, n):
-    """Reference csim on the host cores, on a time-boxed prefix of the same sweep."""
-    from oracle import oracle as O
+def cpu_baseline(args, gates, n, k, kind, threads, toucher):
+    """Reference csim on the host cores, on a time-boxed prefix of the same sweep
+    (the state first touched by the OpenMP threads)."""
     from qulacs_amd.workloads import alg_bytes, rotation_matrix
 
-    if O.reference_available():
-        k, kind = O.Reference(), "reference"
-    else:
-        k, kind = O.Oracle(), "port"
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-    threads = min(threads, os.cpu_count() or threads)
-    k.set_num_threads(threads)
     dim = 1 << n
     psi = np.empty(dim, dtype=np.complex128)
-    psi.fill(1.0 / np.sqrt(dim))
+    toucher.parallel_fill(psi, 1.0 / np.sqrt(dim))
     done_bytes = 0
     spent = 0.0
     count = 0
@@ -815,17 +1035,21 @@ def main() -> None:
     ap.add_argument("--small-steps", type=int, default=5)
     ap.add_argument("--small-plain", type=int, default=64)
     ap.add_argument("--no-small", action="store_true")
-    ap.add_argument("--serve-count", type=int, default=48)
+    ap.add_argument("--serve-count", type=int, default=200)
     ap.add_argument("--serve-rate", type=float, default=100.0)
     ap.add_argument("--serve-max-qubits", type=int, default=28)
-    ap.add_argument("--serve-budget-gib", type=int, default=64)
+    ap.add_argument("--serve-budget-gb", type=float, default=288.0)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--adj-batch", type=int, default=256)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--cpu-worker", choices=["sweep", "brickwork", "small"], help=argparse.SUPPRESS)
     ap.add_argument("--only", choices=["sweep", "dm", "brickwork", "small", "adjoint", "serve"],
                     help="run one leg alone (for profiling): 'sweep' prints the full line with the other legs "
                          "left out, the others print their leg's JSON")
     args = ap.parse_args()
+    if args.cpu_worker:
+        print(json.dumps(cpu_worker(args)), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the sweep is a single 30-qubit "
               "state on one device)", file=sys.stderr)
@@ -868,7 +1092,8 @@ def main() -> None:
         print(f"[bench] serving {serve['served']} vs plain {serve['plain_sequential']}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(args, sv["gates"], args.qubits)
+        cpu = run_cpu_worker("sweep", args, cpu_threads())
+        cpu["host"] = host_cpu_info()
         print(f"[bench] cpu baseline {cpu}", file=sys.stderr, flush=True)
     line = {
         "metric": "30q gate-sweep HBM GB/s",
@@ -888,7 +1113,7 @@ def main() -> None:
                    "qubits": args.qubits, "gates": args.gates, "seed": args.seed},
         "roofline": sv["roofline"],
         "cpu_baseline": cpu,
-        "hbm_copy_GBps": sv["hbm_copy_GBps"],
+        "hbm_measured": sv["hbm_measured"],
         "per_kind": sv["per_kind"],
         "per_kernel": sv["per_kernel"],
         "dm_sweep": dm,

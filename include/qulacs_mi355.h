@@ -215,6 +215,21 @@ QM_EXPORT void qm_sampling_host(const double* uniforms, UINT count, ITYPE* out, 
 /* The CDF qm_sampling_host searches: cdf_out[0..dim] (host, dim + 1 doubles),
  * equal bit for bit to the sequential loop's stacked probabilities. */
 QM_EXPORT void qm_sampling_cdf_host(double* cdf_out, void* state, ITYPE dim, void* stream, unsigned int device_number);
+/* Sampling with a boundary check (no reference counterpart; the reference's
+ * contract is that of qm_sampling_host).  Two states that differ by rounding
+ * (a fused circuit against the gate-by-gate loop, or either against the CPU)
+ * give CDFs that differ by a few units in the last place, so a uniform that
+ * lies that close to a CDF step can select a neighbouring index.  Each sample
+ * whose uniform u is within delta = margin * cdf[dim] of the CDF values that
+ * bracket it (cdf[j-1] < u <= cdf[j]) gets flags[c] = 1; every sample with
+ * flags[c] = 0 is the same for any CDF whose entries lie within delta of this
+ * one.  `flagged` (may be NULL) receives the number of flagged samples.
+ * margin = 0 flags nothing; qm_sampling_margin gives the margin for a state
+ * that `gate_count` gates produced. */
+QM_EXPORT void qm_sampling_checked_host(const double* uniforms, UINT count, double margin, ITYPE* out, unsigned char* flags, UINT* flagged, void* state, ITYPE dim, void* stream, unsigned int device_number);
+/* Relative CDF margin covering the rounding difference between two
+ * evaluations of a circuit of `gate_count` gates on a normalised state. */
+QM_EXPORT double qm_sampling_margin(UINT gate_count);
 
 /* ------------------------------------------------------------------ density matrix
  * `state` holds dim*dim complex128 values, row-major rho[y*dim + x]; `dim` is
@@ -240,6 +255,10 @@ QM_EXPORT double dm_expectation_value_multi_qubit_Pauli_operator_partial_list_ho
 QM_EXPORT void dm_normalize_host(double squared_norm, void* state, ITYPE dim, void* stream, unsigned int device_number); /* update_ops_dm.cpp:14-31 */
 /* DM sampling over |rho_ii| (src/cppsim/state_dm.hpp:383-408), same contract as qm_sampling_host. */
 QM_EXPORT void dm_sampling_host(const double* uniforms, UINT count, ITYPE* out, void* state, ITYPE dim, void* stream, unsigned int device_number);
+/* The CDF dm_sampling_host searches (dim + 1 doubles, host), and the sampling
+ * with a boundary check (as qm_sampling_checked_host).  No reference counterpart. */
+QM_EXPORT void dm_sampling_cdf_host(double* cdf_out, void* state, ITYPE dim, void* stream, unsigned int device_number);
+QM_EXPORT void dm_sampling_checked_host(const double* uniforms, UINT count, double margin, ITYPE* out, unsigned char* flags, UINT* flagged, void* state, ITYPE dim, void* stream, unsigned int device_number);
 
 /* ------------------------------------------------------------------ state utilities
  * No GPU counterpart in the reference (its QuantumStateGpu has none of these);
@@ -336,9 +355,10 @@ QM_EXPORT void qm_circuit_stats(const qm_circuit* circuit, ITYPE* stages, ITYPE*
  * d<psi(theta)|H|psi(theta)>/d theta for a circuit of fixed gates (batch kinds
  * on 1..2 qubits) and parametric rotations exp(i coef theta[param] P / 2) with
  * P a Pauli string on 1..4 qubits, for `batch` parameter sets at once.
- * Restates ParametricQuantumCircuit::backprop (src/vqcsim/parametric_circuit.cpp:205-335)
- * with one fused pass per gate on the way back (gradient term, then the
- * inverse applied to both states). */
+ * Restates ParametricQuantumCircuit::backprop (src/vqcsim/parametric_circuit.cpp:205-335):
+ * gates are grouped into passes over blocks of 3 qubits held in registers; on
+ * the way back each pass adds its rotations' gradient terms and applies the
+ * inverses to both states. */
 #define QM_ADJ_ROT 4u
 typedef struct qm_adjoint_gate {
     UINT kind;                 /* QM_BATCH_DENSE/DIAG/CZ/CNOT (fixed) or QM_ADJ_ROT */
@@ -352,6 +372,9 @@ typedef struct qm_adjoint_gate {
 typedef struct qm_adjoint qm_adjoint;
 QM_EXPORT qm_adjoint* qm_adjoint_create(UINT qubit_count, const qm_adjoint_gate* gates, UINT gate_count, UINT param_count);
 QM_EXPORT void qm_adjoint_destroy(qm_adjoint* circuit);
+/* Register-block passes one sweep issues (0: one kernel per gate, for states
+ * below 3 qubits or rotations with more than 3 X/Y qubits) and gates. */
+QM_EXPORT void qm_adjoint_stats(const qm_adjoint* circuit, ITYPE* passes, ITYPE* gates);
 /* Device memory qm_adjoint_run needs for `batch` parameter sets. */
 QM_EXPORT ITYPE qm_adjoint_workspace_bytes(const qm_adjoint* circuit, UINT batch);
 /* theta: device, batch x param_count (row-major); energy: device, batch;
@@ -375,6 +398,12 @@ QM_EXPORT long long qm_server_submit(qm_server* server, UINT qubit_count, const 
 /* Waits for request `id` (once per id): energy, samples (sample_count entries)
  * and the time from submission to completion. */
 QM_EXPORT int qm_server_wait(qm_server* server, long long id, double* energy, ITYPE* samples, double* latency_seconds);
+/* As qm_server_wait, plus the boundary check of each sample (flags: sample_count
+ * bytes, as qm_sampling_checked_host with qm_sampling_margin(gate_count)) and
+ * the number of flagged samples.  A request's samples are computed on the
+ * state its fused circuit produced, which differs from the gate-by-gate state
+ * by rounding; the unflagged samples are the ones both states agree on. */
+QM_EXPORT int qm_server_wait_checked(qm_server* server, long long id, double* energy, ITYPE* samples, unsigned char* flags, UINT* flagged, double* latency_seconds);
 QM_EXPORT void qm_server_stats(qm_server* server, ITYPE* admitted, ITYPE* completed, ITYPE* failed, ITYPE* batches, ITYPE* max_batch, ITYPE* peak_bytes);
 /* Finishes every submitted request, then frees the server. */
 QM_EXPORT void qm_server_destroy(qm_server* server);

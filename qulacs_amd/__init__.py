@@ -71,6 +71,7 @@ from ._core import (  # noqa: F401
     observable,
     quantum_operator,
     get_synchronous,
+    sampling_margin,
     set_circuit_fusion,
     set_synchronous,
     state,

@@ -17,6 +17,7 @@ D = C.c_double
 PU = C.POINTER(C.c_uint)
 PI = C.POINTER(C.c_ulonglong)
 PD = C.POINTER(C.c_double)
+PB = C.POINTER(C.c_ubyte)
 
 
 class QmComplex(C.Structure):
@@ -119,6 +120,8 @@ _SIGS: dict[str, tuple] = {
     "qm_state_add_with_coef_host": (None, [QmComplex, VP, VP, ITYPE, VP, UINT]),
     "qm_observable_expectation_host": (QmComplex, [PI, PI, PU, PC, UINT, VP, ITYPE, VP, UINT]),
     "qm_sampling_host": (None, [PD, UINT, PI, VP, ITYPE, VP, UINT]),
+    "qm_sampling_checked_host": (None, [PD, UINT, D, PI, PB, PU, VP, ITYPE, VP, UINT]),
+    "qm_sampling_margin": (D, [UINT]),
     # density matrix
     "dm_initialize_quantum_state_host": (None, [VP, ITYPE, VP, UINT]),
     "dm_initialize_with_pure_state_host": (None, [VP, VP, ITYPE, VP, UINT]),
@@ -135,6 +138,8 @@ _SIGS: dict[str, tuple] = {
     "dm_normalize_host": (None, [D, VP, ITYPE, VP, UINT]),
     "dm_sampling_host": (None, [PD, UINT, PI, VP, ITYPE, VP, UINT]),
     "qm_sampling_cdf_host": (None, [PD, VP, ITYPE, VP, UINT]),
+    "dm_sampling_cdf_host": (None, [PD, VP, ITYPE, VP, UINT]),
+    "dm_sampling_checked_host": (None, [PD, UINT, D, PI, PB, PU, VP, ITYPE, VP, UINT]),
     # batched circuits
     "qm_batch_create": (VP, []),
     "qm_batch_destroy": (None, [VP]),
@@ -156,10 +161,12 @@ _SIGS: dict[str, tuple] = {
     "qm_adjoint_create": (VP, [UINT, C.POINTER(QmAdjointGate), UINT, UINT]),
     "qm_adjoint_destroy": (None, [VP]),
     "qm_adjoint_workspace_bytes": (ITYPE, [VP, UINT]),
+    "qm_adjoint_stats": (None, [VP, PI, PI]),
     "qm_adjoint_run": (C.c_int, [VP, UINT, VP, PI, PI, PU, PD, UINT, VP, VP, VP, VP, UINT]),
     "qm_server_create": (VP, [ITYPE, UINT, UINT, UINT]),
     "qm_server_submit": (C.c_longlong, [VP, UINT, C.POINTER(QmBatchGate), UINT, PI, PI, PU, PD, UINT, PD, UINT]),
     "qm_server_wait": (C.c_int, [VP, C.c_longlong, PD, PI, PD]),
+    "qm_server_wait_checked": (C.c_int, [VP, C.c_longlong, PD, PI, PB, PU, PD]),
     "qm_server_stats": (None, [VP, PI, PI, PI, PI, PI, PI]),
     "qm_server_destroy": (None, [VP]),
 }

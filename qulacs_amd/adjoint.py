@@ -56,6 +56,12 @@ class AdjointCircuit:
         if not self.h:
             raise _lib.QulacsMI355Error("qm_adjoint_create failed")
 
+    def stats(self) -> dict:
+        """Register-block passes per sweep (0: one kernel per gate) and gates."""
+        p, g = C.c_ulonglong(), C.c_ulonglong()
+        _lib.call("qm_adjoint_stats", self.h, C.byref(p), C.byref(g))
+        return {"passes": int(p.value), "gates": int(g.value)}
+
     def close(self) -> None:
         if self.h:
             _lib.call("qm_adjoint_destroy", self.h)

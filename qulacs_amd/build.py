@@ -40,11 +40,12 @@ HIP_FLAGS = [
     "-Wno-unknown-warning-option",
     f"-I{ROOT / 'include'}",
 ]
-# Per-file device flags.  qm_batch.hip dispatches register-array updates through
-# a switch on a runtime bit position; LLVM's common-code sinking would merge the
-# cases into one access with a computed index and move the array to private
-# (scratch) memory, so it is switched off for that file.
-EXTRA_HIP_FLAGS = {"qm_batch.hip": ["-mllvm", "-simplifycfg-sink-common=false"]}
+# Per-file device flags.  qm_batch.hip and qm_adjoint.hip dispatch register-array
+# updates through a switch on a runtime bit position; LLVM's common-code sinking
+# would merge the cases into one access with a computed index and move the array
+# to private (scratch) memory, so it is switched off for those files.
+EXTRA_HIP_FLAGS = {"qm_batch.hip": ["-mllvm", "-simplifycfg-sink-common=false"],
+                   "qm_adjoint.hip": ["-mllvm", "-simplifycfg-sink-common=false"]}
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fvisibility=hidden", "-Wall", f"-I{ROOT / 'include'}"]
 
 

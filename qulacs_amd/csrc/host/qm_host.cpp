@@ -318,6 +318,16 @@ std::vector<ITYPE> QuantumStateMI355::sampling_with_uniforms(const std::vector<d
     check_abi("sampling");
     return out;
 }
+std::pair<std::vector<ITYPE>, std::vector<unsigned char>> QuantumStateMI355::sampling_with_uniforms_checked(
+    const std::vector<double>& u, double margin) const {
+    std::vector<ITYPE> out(u.size());
+    std::vector<unsigned char> flags(u.size());
+    if (u.empty()) return {out, flags};
+    qm_sampling_checked_host(u.data(), (UINT)u.size(), margin, out.data(), flags.data(), nullptr, data_, dim, stream_,
+                             device_number);
+    check_abi("sampling");
+    return {out, flags};
+}
 std::vector<ITYPE> QuantumStateMI355::sampling(unsigned count) {
     std::vector<double> u(count);
     for (auto& x : u) x = random_.uniform();  // src/cppsim/state.hpp:924
@@ -518,6 +528,16 @@ std::vector<ITYPE> DensityMatrixMI355::sampling_with_uniforms(const std::vector<
     dm_sampling_host(u.data(), (UINT)u.size(), out.data(), data_, dim, stream_, device_number);
     check_abi("sampling");
     return out;
+}
+std::pair<std::vector<ITYPE>, std::vector<unsigned char>> DensityMatrixMI355::sampling_with_uniforms_checked(
+    const std::vector<double>& u, double margin) const {
+    std::vector<ITYPE> out(u.size());
+    std::vector<unsigned char> flags(u.size());
+    if (u.empty()) return {out, flags};
+    dm_sampling_checked_host(u.data(), (UINT)u.size(), margin, out.data(), flags.data(), nullptr, data_, dim, stream_,
+                             device_number);
+    check_abi("sampling");
+    return {out, flags};
 }
 std::vector<ITYPE> DensityMatrixMI355::sampling(unsigned count) {
     std::vector<double> u(count);
@@ -1592,23 +1612,27 @@ const QuantumGateBase* QuantumCircuit::get_gate(size_t i) const {
     if (i >= gates_.size()) throw QulacsError("get_gate: index out of range");
     return gates_[i];
 }
+// Every qubit a gate names must exist in the circuit (src/cppsim/circuit.cpp:144-150).
+void QuantumCircuit::check_gate_indices(const QuantumGateBase& g) const {
+    for (unsigned q : g.get_target_index_list())
+        if (q >= qubit_count_) throw QulacsError("add_gate: target index exceeds the circuit's qubit count");
+    for (unsigned q : g.get_control_index_list())
+        if (q >= qubit_count_) throw QulacsError("add_gate: control index exceeds the circuit's qubit count");
+}
 void QuantumCircuit::add_gate(const QuantumGateBase& g) { add_gate_consume(g.copy()); }
 void QuantumCircuit::add_gate(const QuantumGateBase& g, size_t pos) {
     if (pos > gates_.size()) throw QulacsError("add_gate: position out of range");
+    check_gate_indices(g);
     gates_.insert(gates_.begin() + pos, g.copy());
     touched();
 }
 void QuantumCircuit::add_gate_consume(QuantumGateBase* g) {
-    for (unsigned q : g->get_target_index_list())
-        if (q >= qubit_count_) {
-            delete g;
-            throw QulacsError("add_gate: target index exceeds the circuit's qubit count");
-        }
-    for (unsigned q : g->get_control_index_list())
-        if (q >= qubit_count_) {
-            delete g;
-            throw QulacsError("add_gate: control index exceeds the circuit's qubit count");
-        }
+    try {
+        check_gate_indices(*g);
+    } catch (...) {
+        delete g;
+        throw;
+    }
     gates_.push_back(g);
     touched();
 }

@@ -150,6 +150,9 @@ class QuantumStateMI355 : public QuantumStateBase {
     std::vector<ITYPE> sampling(unsigned count) override;
     // Sampling with caller-supplied uniforms (same indices as the sequential CPU loop).
     std::vector<ITYPE> sampling_with_uniforms(const std::vector<double>& uniforms) const;
+    // samples and their boundary flags (qm_sampling_checked_host)
+    std::pair<std::vector<ITYPE>, std::vector<unsigned char>> sampling_with_uniforms_checked(
+        const std::vector<double>& uniforms, double margin) const;
     std::string to_string() const override;
     // psi[i] *= f(i): f is evaluated on the host for every index, then one device
     // pass multiplies (src/cppsim/state.hpp:885-891, state_gpu.hpp:353).
@@ -179,6 +182,9 @@ class DensityMatrixMI355 : public QuantumStateBase {
     using QuantumStateBase::sampling;
     std::vector<ITYPE> sampling(unsigned count) override;
     std::vector<ITYPE> sampling_with_uniforms(const std::vector<double>& uniforms) const;
+    // samples and their boundary flags (qm_sampling_checked_host)
+    std::pair<std::vector<ITYPE>, std::vector<unsigned char>> sampling_with_uniforms_checked(
+        const std::vector<double>& uniforms, double margin) const;
     std::string to_string() const override;
 };
 
@@ -606,6 +612,8 @@ class QuantumCircuit {
     void add_gate(const QuantumGateBase& g);
     void add_gate(const QuantumGateBase& g, size_t position);
     void add_gate_consume(QuantumGateBase* g);
+    // throws QulacsError when g names a qubit outside the circuit
+    void check_gate_indices(const QuantumGateBase& g) const;
     void remove_gate(size_t i);
     virtual QuantumCircuit* copy() const { return new QuantumCircuit(*this); }
     void update_quantum_state(QuantumStateBase* state);

@@ -332,6 +332,8 @@ PYBIND11_MODULE(_core, m) {
              },
              py::arg("index"))
         .def("sampling_with_uniforms", &QuantumStateMI355::sampling_with_uniforms, py::arg("uniforms"))
+        .def("sampling_with_uniforms_checked", &QuantumStateMI355::sampling_with_uniforms_checked, py::arg("uniforms"),
+             py::arg("margin"))
         .def("data_ptr", [](const QuantumStateMI355& s) { return reinterpret_cast<uintptr_t>(s.data()); })
         .def("stream_ptr", [](const QuantumStateMI355& s) { return reinterpret_cast<uintptr_t>(s.get_cuda_stream()); })
         .def(py::pickle([](const QuantumStateMI355& s) { return ptree::to_json(to_ptree(s)); },
@@ -356,6 +358,8 @@ PYBIND11_MODULE(_core, m) {
              }, py::arg("state"))
         .def("load", [](DensityMatrixMI355& s, const carray& a) { s.load(to_vector(a)); }, py::arg("state"))
         .def("sampling_with_uniforms", &DensityMatrixMI355::sampling_with_uniforms, py::arg("uniforms"))
+        .def("sampling_with_uniforms_checked", &DensityMatrixMI355::sampling_with_uniforms_checked,
+             py::arg("uniforms"), py::arg("margin"))
         .def("data_ptr", [](const DensityMatrixMI355& s) { return reinterpret_cast<uintptr_t>(s.data()); })
         .def(py::pickle([](const DensityMatrixMI355& s) { return ptree::to_json(to_ptree(s)); },
                         [](const std::string& json) {
@@ -837,4 +841,5 @@ PYBIND11_MODULE(_core, m) {
         check_abi("synchronize");
     });
     m.def("get_num_device", &get_num_device);
+    m.def("sampling_margin", &qm_sampling_margin, py::arg("gate_count"));
 }

@@ -1411,12 +1411,7 @@ int qm_circuit_apply(qm_circuit* c, void* state, ITYPE dim, void* stream, unsign
 }  // extern "C"
 
 namespace qm {
-bool circuit_enqueue(qm_circuit* c, void* state, u64 dim, hipStream_t st, unsigned dev) {
-    const char* what = "qm_circuit_apply";
-    if (!c || !state || dim != (1ULL << c->n)) {
-        set_error(QM_ERR_INVALID_ARGUMENT, "%s: null circuit/state or dim is not 2^%u", what, c ? c->n : 0u);
-        return false;
-    }
+bool circuit_prepare(qm_circuit* c, unsigned dev) {
     if (c->device != (int)dev) {
         c->release();
         bool ok = dev_upload(&c->d_passes, c->passes) && dev_upload(&c->d_subs, c->subs) &&
@@ -1427,6 +1422,16 @@ bool circuit_enqueue(qm_circuit* c, void* state, u64 dim, hipStream_t st, unsign
         }
         c->device = (int)dev;
     }
+    return true;
+}
+
+bool circuit_enqueue(qm_circuit* c, void* state, u64 dim, hipStream_t st, unsigned dev) {
+    const char* what = "qm_circuit_apply";
+    if (!c || !state || dim != (1ULL << c->n)) {
+        set_error(QM_ERR_INVALID_ARGUMENT, "%s: null circuit/state or dim is not 2^%u", what, c ? c->n : 0u);
+        return false;
+    }
+    if (!circuit_prepare(c, dev)) return false;
     const size_t lds = sizeof(double2) << c->lq;
     const void* fn = c->K == 2 ? (const void*)k_stage<2> : c->K == 3 ? (const void*)k_stage<3> : (const void*)k_stage<4>;
     if (c->per_cu == 0) {

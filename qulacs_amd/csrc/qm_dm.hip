@@ -6,14 +6,18 @@
 //   * U rho U^dagger is U applied to the row bits followed by conj(U) applied
 //     to the column bits -- two passes of the state-vector gather kernels over
 //     the 2n-bit flat array, each as fast as a state-vector gate of 2n qubits;
-//     a single-qubit unitary is fused into one pass with E = U (x) conj(U) on
-//     (column t, row t + n) (the reference's ext_matrix, update_ops_dm.cpp:41-50);
+//     a unitary on at most two qubits (controls included) is one pass with
+//     E = U (x) conj(U) on the column and row bits of its qubits (for one qubit
+//     the reference's ext_matrix, update_ops_dm.cpp:41-50; the reference applies
+//     larger ones block by block, update_ops_dm.cpp:342-498), a monomial E
+//     (CNOT, CZ, SWAP) through a pass that moves only the entries that change;
 //   * a k-qubit CPTP map sum_i K_i rho K_i^dagger is one pass of a 2k-bit dense
 //     gate whose matrix is the superoperator sum_i K_i (x) conj(K_i), replacing
 //     the copy / gate / scale / add loop of QuantumGate_Probabilistic on a
 //     density matrix (src/cppsim/gate_general.hpp:265-285, about ten passes).
 #include <cmath>
 #include <complex>
+#include <vector>
 
 #include "qm_internal.hpp"
 
@@ -141,6 +145,51 @@ static void dm_unitary(const u32* cl, const u32* cvl, u32 nc, const u32* tl, u32
             }
         const u32 ft[2] = {tl[0], tl[0] + (u32)n};
         dense_any(ft, 2, nullptr, nullptr, 0, e, state, flat, stream, dev, what);
+        return;
+    }
+    if (nt + nc <= 2) {
+        // one pass with the superoperator F (x) conj(F) of the full (controlled)
+        // unitary F on the qubits q = targets then controls; member bits 0..k-1
+        // are the column bits of q, bits k..2k-1 the row bits.  A monomial F (CNOT,
+        // CZ, SWAP, ...) has a monomial superoperator: only changed entries move.
+        const u32 k = nt + nc;
+        const u32 d = 1u << k;
+        u32 q[2];
+        for (u32 i = 0; i < nt; ++i) q[i] = tl[i];
+        for (u32 i = 0; i < nc; ++i) q[nt + i] = cl[i];
+        u32 cpat = 0;
+        for (u32 i = 0; i < nc; ++i) cpat |= cvl[i] << i;
+        const u32 tmask = (1u << nt) - 1;
+        std::vector<cd> F((size_t)d * d, cd(0.0, 0.0));
+        for (u32 x = 0; x < d; ++x) {
+            if ((x >> nt) != cpat) {
+                F[(size_t)x * d + x] = 1.0;
+                continue;
+            }
+            for (u32 yt = 0; yt <= tmask; ++yt) {
+                const u32 y = (x & ~tmask) | yt;
+                const qm_complex e = m[(size_t)yt * (tmask + 1) + (x & tmask)];
+                F[(size_t)y * d + x] = cd(e.re, e.im);
+            }
+        }
+        const u32 D = d * d;
+        std::vector<qm_complex> E((size_t)D * D);
+        for (u32 yr = 0; yr < d; ++yr)
+            for (u32 yc = 0; yc < d; ++yc)
+                for (u32 xr = 0; xr < d; ++xr)
+                    for (u32 xc = 0; xc < d; ++xc) {
+                        const cd v = F[(size_t)yr * d + xr] * std::conj(F[(size_t)yc * d + xc]);
+                        E[(size_t)(yr * d + yc) * D + (xr * d + xc)] = qm_complex{v.real(), v.imag()};
+                    }
+        u32 ft[4];
+        for (u32 i = 0; i < k; ++i) {
+            ft[i] = q[i];
+            ft[k + i] = q[i] + (u32)n;
+        }
+        if (is_monomial(E.data(), 2 * k))
+            monomial_any(ft, 2 * k, E.data(), state, flat, stream, dev, what);
+        else
+            dense_any(ft, 2 * k, nullptr, nullptr, 0, E.data(), state, flat, stream, dev, what);
         return;
     }
     u32 rt[32], rc[32], ct[32];

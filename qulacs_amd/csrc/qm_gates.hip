@@ -512,6 +512,97 @@ void dense_any(const u32* targets, u32 nt, const u32* controls, const u32* cvals
     finish_update(st, what);
 }
 
+// Monomial k-qubit operator (one nonzero per row and column: a permutation of
+// the tuple members with a factor each), e.g. the superoperator of a CNOT or CZ
+// on a density matrix.  Only members that change are read and written:
+// member y receives ph[y] * member src[y].  The member tables are wave-uniform
+// kernel arguments, so every access address is a scalar offset plus the base.
+template <int K>
+struct MonoArgs {
+    double2 ph[1 << K];
+    unsigned char src[1 << K];
+    u32 write;     // members that change
+    u32 unit;      // members whose factor is exactly 1 (no multiply)
+};
+
+template <int K, bool NT>
+__global__ __launch_bounds__(512) void k_monomial(double2* __restrict__ s, u64 n_bases, GatherArgs<K> a,
+                                                   MonoArgs<K> m) {
+    constexpr int N = 1 << K;
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n_bases; i += stride) {
+        const u64 b = base_of<K>(i, a);
+        double2 v[N];
+#pragma unroll
+        for (int y = 0; y < N; ++y)
+            if ((m.write >> y) & 1) v[y] = ld<NT>(s + b + a.offs[m.src[y]]);
+#pragma unroll
+        for (int y = 0; y < N; ++y)
+            if ((m.write >> y) & 1) st<NT>(s + b + a.offs[y], ((m.unit >> y) & 1) ? v[y] : cmul(v[y], m.ph[y]));
+    }
+}
+
+template <int K>
+void launch_monomial(double2* s, u64 dim, const u32* targets, const qm_complex* matrix, hipStream_t st) {
+    constexpr int N = 1 << K;
+    MonoArgs<K> m{};
+    for (int y = 0; y < N; ++y) {
+        int src = -1;
+        for (int x = 0; x < N; ++x)
+            if (matrix[y * N + x].re != 0.0 || matrix[y * N + x].im != 0.0) src = x;
+        m.src[y] = (unsigned char)src;
+        m.ph[y] = make_double2(matrix[y * N + src].re, matrix[y * N + src].im);
+        const bool unit = m.ph[y].x == 1.0 && m.ph[y].y == 0.0;
+        if (unit) m.unit |= 1u << y;
+        if (!(unit && src == y)) m.write |= 1u << y;
+    }
+    if (!m.write) return;
+    GatherArgs<K> a;
+    if (!fill_args<K>(a, targets, nullptr, nullptr, 0)) return;
+    const u64 n_bases = dim >> K;
+    const unsigned grid = grid_for(n_bases, 512);
+    if (a.ins0 >= kLineQubits)
+        k_monomial<K, true><<<grid, 512, 0, st>>>(s, n_bases, a, m);
+    else
+        k_monomial<K, false><<<grid, 512, 0, st>>>(s, n_bases, a, m);
+}
+
+// True when every row and every column of the 2^k x 2^k matrix has exactly one
+// nonzero entry.
+bool is_monomial(const qm_complex* matrix, u32 k) {
+    const u32 N = 1u << k;
+    std::vector<int> col_count(N, 0);
+    for (u32 y = 0; y < N; ++y) {
+        int nz = 0;
+        for (u32 x = 0; x < N; ++x)
+            if (matrix[y * N + x].re != 0.0 || matrix[y * N + x].im != 0.0) {
+                ++nz;
+                ++col_count[x];
+            }
+        if (nz != 1) return false;
+    }
+    for (u32 x = 0; x < N; ++x)
+        if (col_count[x] != 1) return false;
+    return true;
+}
+
+void monomial_any(const u32* targets, u32 nt, const qm_complex* matrix, void* state, u64 dim, void* stream,
+                  unsigned device, const char* what) {
+    if (check_gate(state, dim, targets, nt, nullptr, 0, what) < 0) return;
+    QM_CHECK_ARG(nt >= 1 && nt <= 4 && matrix && is_monomial(matrix, nt), "%s: not a monomial 1..4-qubit operator",
+                 what);
+    if (!use_device(device)) return;
+    hipStream_t st = stream_of(stream);
+    double2* s = (double2*)state;
+    switch (nt) {
+        case 1: launch_monomial<1>(s, dim, targets, matrix, st); break;
+        case 2: launch_monomial<2>(s, dim, targets, matrix, st); break;
+        case 3: launch_monomial<3>(s, dim, targets, matrix, st); break;
+        default: launch_monomial<4>(s, dim, targets, matrix, st); break;
+    }
+    finish_update(st, what);
+}
+
 // Named single-target gate through the gather kernel.
 template <class Op>
 void named1(u32 t, const Op& op, void* state, u64 dim, void* stream, unsigned device, const char* what) {

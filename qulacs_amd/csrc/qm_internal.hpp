@@ -206,6 +206,12 @@ GatherPlan make_plan(const u32* targets, u32 nt, const u32* controls, const u32*
 // split, for density matrices).  Defined in qm_gates.hip.
 void dense_any(const u32* targets, u32 nt, const u32* controls, const u32* cvals, u32 nc, const qm_complex* matrix,
                void* state, u64 dim, void* stream, unsigned device, const char* what);
+// A 1..4-qubit operator with one nonzero per row and column (a permutation of
+// the tuple members with factors): one pass that reads and writes only the
+// members that change.  is_monomial tests a row-major 2^k x 2^k matrix.
+bool is_monomial(const qm_complex* matrix, u32 k);
+void monomial_any(const u32* targets, u32 nt, const qm_complex* matrix, void* state, u64 dim, void* stream,
+                  unsigned device, const char* what);
 
 // ------------------------------------------------------------------ enqueue-only
 // Stream-ordered building blocks that never wait for the stream (the request
@@ -221,7 +227,17 @@ void enqueue_basis(void* state, u64 dim, u64 basis, hipStream_t st);
 bool enqueue_observable(const u64* bf, const u64* pf, const u32* rot, const double* coef, u32 nterms,
                         const void* state, u64 dim, hipStream_t st, double* h_out);
 // lower_bound sampling of one state vector with h_uniforms -> h_out (qm_sampling.hip)
-bool enqueue_sampling(const void* state, u64 dim, const double* h_uniforms, u32 count, u64* h_out, hipStream_t st);
+// With h_flags, also the boundary check of qm_sampling_checked_host (count bytes).
+// The CDF and its work tables live in the stream's grow-on-demand scratch.
+bool enqueue_sampling(const void* state, u64 dim, const double* h_uniforms, u32 count, u64* h_out, hipStream_t st,
+                      double margin = 0.0, unsigned char* h_flags = nullptr);
+// The same in a caller-provided device buffer of sampling_buffer_bytes bytes
+// (nothing is allocated).
+size_t sampling_buffer_bytes(u64 dim, u32 count, bool flags);
+bool enqueue_sampling_in(void* buffer, const void* state, u64 dim, const double* h_uniforms, u32 count, u64* h_out,
+                         hipStream_t st, double margin, unsigned char* h_flags);
+// Uploads a fused circuit's program to `dev` ahead of its first launch.
+bool circuit_prepare(qm_circuit* c, unsigned dev);
 // Many small state vectors (dim <= 2^16 each), one work-group per CDF.
 struct SampleJob {
     const double2* state;
@@ -229,6 +245,8 @@ struct SampleJob {
     const double* h_uniforms;
     u64* h_out;
     u32 count;
+    double margin = 0.0;               // boundary check (qm_sampling_checked_host)
+    unsigned char* h_flags = nullptr;  // count bytes, or nullptr for no check
 };
 bool enqueue_sampling_jobs(const SampleJob* jobs, u32 njobs, hipStream_t st);
 // qm_circuit_apply without the final synchronisation (qm_batch.hip)

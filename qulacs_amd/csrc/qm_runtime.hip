@@ -141,6 +141,31 @@ void enqueue_basis(void* state, u64 dim, u64 basis, hipStream_t st) {
     k_fill_basis<<<grid_for(dim, 256 * 8), 256, 0, st>>>((double2*)state, dim, basis);
 }
 
+// Device-to-device state copy: each thread moves 4 amplitudes per iteration
+// with non-temporal 16-byte accesses (each line is read and written once).
+// State sizes are powers of two; from 256 amplitudes up they split into
+// 256-amplitude groups of four 64-amplitude strips, one strip per load, so
+// consecutive lanes touch consecutive amplitudes.  Smaller states copy one
+// amplitude per thread.
+__global__ __launch_bounds__(256) void k_copy_stream(double2* __restrict__ dst, const double2* __restrict__ src,
+                                                     u64 dim) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    const u64 t0 = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (dim < 256) {
+        for (u64 i = t0; i < dim; i += stride) dst[i] = src[i];
+        return;
+    }
+    const u64 quads = dim >> 2;
+    for (u64 q = t0; q < quads; q += stride) {
+        const u64 base = (q >> 6) * 256 + (q & 63);
+        double2 v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = ld_stream(src + base + 64 * k);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) st_stream(dst + base + 64 * k, v[k]);
+    }
+}
+
 // Counter-based generator (SplitMix64 finaliser on seed and index) feeding a
 // Box-Muller transform; each amplitude draws two independent normals.
 __device__ __forceinline__ u64 mix64(u64 z) {
@@ -295,7 +320,12 @@ void copy_quantum_state_from_device_to_device(void* dst, const void* src, ITYPE 
     QM_CHECK_ARG(dst && src, "copy_quantum_state_from_device_to_device: null pointer");
     if (!use_device(device_number)) return;
     hipStream_t s = stream_of(stream);
-    if (!hip_ok(hipMemcpyAsync(dst, src, (size_t)dim * 16, hipMemcpyDeviceToDevice, s), "d2d copy")) return;
+    if (log2_dim(dim) < 0) {
+        // not a state size: a plain runtime copy of dim amplitudes
+        if (!hip_ok(hipMemcpyAsync(dst, src, (size_t)dim * 16, hipMemcpyDeviceToDevice, s), "d2d copy")) return;
+    } else if (dst != src) {
+        k_copy_stream<<<grid_for(dim >> 2, 256 * 4), 256, 0, s>>>((double2*)dst, (const double2*)src, dim);
+    }
     finish_update(s, "copy_quantum_state_from_device_to_device");
 }
 

@@ -33,9 +33,55 @@ __device__ __forceinline__ double prob_sv(const double2* __restrict__ a, u64 i) 
     const double2 v = a[i];
     return fma(v.x, v.x, v.y * v.y);
 }
+// |x + iy| rounded as the reference's std::abs(std::complex<double>) rounds it.
+// libstdc++ lowers that call to cabs = glibc's hypot, which is not correctly
+// rounded (about 1 result in 1000 differs from the correctly rounded one).
+// glibc 2.35's hypot is the non-FMA form of C. F. Borges, "An Improved
+// Algorithm for hypot(a, b)", arXiv:1904.09481 (2019): h = sqrt(x^2 + y^2)
+// followed by one correction term, with a power-of-two rescale for huge or tiny
+// operands and a shortcut for widely varying ones.  It is restated here with
+// FMA contraction off, so every product and sum rounds as in the host library.
+__device__ __noinline__ double hypot_host(double x, double y) {
+#pragma clang fp contract(off)
+    double ax = fabs(x), ay = fabs(y);
+    if (ax < ay) {
+        const double t = ax;
+        ax = ay;
+        ay = t;
+    }
+    if (!(ax <= 0x1.fffffffffffffp+1023)) return (isinf(ax) || isinf(ay)) ? INFINITY : ax + ay;  // inf / NaN
+    constexpr double kEps = 0x1p-54, kScale = 0x1p-600, kLarge = 0x1p+511, kTiny = 0x1p-511;
+    double scale = 1.0;
+    if (ax > kLarge) {
+        if (ay <= ax * kEps) return ax + ay;
+        ax *= kScale;
+        ay *= kScale;
+        scale = 1.0 / kScale;
+    } else if (ay < kTiny) {
+        if (ax >= ay / kEps) return ax + ay;
+        ax /= kScale;
+        ay /= kScale;
+        scale = kScale;
+    } else if (ay <= ax * kEps) {
+        return ax + ay;
+    }
+    double h = sqrt(ax * ax + ay * ay);
+    double t1, t2;
+    if (h <= 2.0 * ay) {
+        const double delta = h - ay;
+        t1 = ax * (2.0 * delta - ax);
+        t2 = (delta - 2.0 * (ax - ay)) * delta;
+    } else {
+        const double delta = h - ax;
+        t1 = 2.0 * delta * (ax - 2.0 * ay);
+        t2 = (4.0 * delta - ay) * ay + delta * delta;
+    }
+    h -= (t1 + t2) / (2.0 * h);
+    return scale == 1.0 ? h : (scale < 1.0 ? h * scale : h / kScale);
+}
 __device__ __forceinline__ double prob_dm(const double2* __restrict__ a, u64 i, u64 dim) {
     const double2 v = a[i * dim + i];
-    return v.y == 0.0 ? fabs(v.x) : hypot(v.x, v.y);
+    return v.y == 0.0 ? fabs(v.x) : hypot_host(v.x, v.y);
 }
 
 // Block-wide exclusive scan of one u64 per thread (1024 threads, 16 waves).
@@ -470,37 +516,76 @@ void enqueue_cdf(const double2* a, u64 dim, double* cdf, void* work, hipStream_t
 
 // lower_bound(cdf[0..dim], u) - 1, as an unsigned 64-bit index (the reference's
 // -1 for u <= 0 becomes 2^64 - 1 through the same conversion).
-__global__ void k_search(const double* __restrict__ cdf, u64 dim, const double* __restrict__ us, unsigned count,
-                         u64* __restrict__ out) {
-    const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= count) return;
-    const double u = us[c];
+//
+// Boundary check: j = lower_bound(cdf, u) is the first index with cdf[j] >= u.
+// The sample j - 1 is the same for every CDF whose entries lie within delta of
+// these unless u - cdf[j-1] <= delta (j >= 2; cdf[0] = 0 is exact in every CDF)
+// or cdf[j] - u <= delta (1 <= j <= dim).  delta = margin * cdf[dim]; such a
+// sample is flagged (flag 1), any other one is certain (flag 0).  With margin
+// <= 0 nothing is flagged.
+__device__ __forceinline__ u64 search_checked(const double* __restrict__ cdf, u64 dim, double u, double margin,
+                                              unsigned char* flag) {
     u64 lo = 0, hi = dim + 1;  // search [lo, hi) for the first cdf[i] >= u
     while (lo < hi) {
         const u64 mid = lo + (hi - lo) / 2;
         if (cdf[mid] < u) lo = mid + 1;
         else hi = mid;
     }
-    out[c] = lo - 1;
+    if (flag) {
+        unsigned char f = 0;
+        if (margin > 0.0) {
+            const double delta = margin * cdf[dim];
+            if (lo >= 2 && u - cdf[lo - 1] <= delta) f = 1;
+            if (lo >= 1 && lo <= dim && cdf[lo] - u <= delta) f = 1;
+        }
+        *flag = f;
+    }
+    return lo - 1;
 }
 
-bool enqueue_sampling(const void* state, u64 dim, const double* h_uniforms, u32 count, u64* h_out, hipStream_t st) {
-    if (count == 0) return true;
-    Scratch* sc = scratch_for(st);
-    if (!sc) return false;
+__global__ void k_search(const double* __restrict__ cdf, u64 dim, const double* __restrict__ us, unsigned count,
+                         double margin, u64* __restrict__ out, unsigned char* __restrict__ flags) {
+    const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= count) return;
+    out[c] = search_checked(cdf, dim, us[c], margin, flags ? flags + c : nullptr);
+}
+
+size_t sampling_buffer_bytes(u64 dim, u32 count, bool flags) {
     const size_t cdf_bytes = ((size_t)(dim + 1) * sizeof(double) + 255) & ~size_t(255);
     const size_t work_bytes = (cdf_work_bytes(dim) + 255) & ~size_t(255);
     const size_t io_bytes = ((size_t)count * 8 + 255) & ~size_t(255);
-    char* buf = (char*)big_buffer(sc, cdf_bytes + work_bytes + 2 * io_bytes);
-    if (!buf) return false;
+    const size_t flag_bytes = flags ? ((size_t)count + 255) & ~size_t(255) : 0;
+    return cdf_bytes + work_bytes + 2 * io_bytes + flag_bytes;
+}
+
+bool enqueue_sampling_in(void* buffer, const void* state, u64 dim, const double* h_uniforms, u32 count, u64* h_out,
+                         hipStream_t st, double margin, unsigned char* h_flags) {
+    if (count == 0) return true;
+    const size_t cdf_bytes = ((size_t)(dim + 1) * sizeof(double) + 255) & ~size_t(255);
+    const size_t work_bytes = (cdf_work_bytes(dim) + 255) & ~size_t(255);
+    const size_t io_bytes = ((size_t)count * 8 + 255) & ~size_t(255);
+    char* buf = (char*)buffer;
     double* cdf = (double*)buf;
     double* dus = (double*)(buf + cdf_bytes + work_bytes);
     u64* dout = (u64*)((char*)dus + io_bytes);
+    unsigned char* dflags = h_flags ? (unsigned char*)dout + io_bytes : nullptr;
     if (!hip_ok(hipMemcpyAsync(dus, h_uniforms, (size_t)count * 8, hipMemcpyHostToDevice, st), "enqueue_sampling"))
         return false;
     enqueue_cdf<false>((const double2*)state, dim, cdf, buf + cdf_bytes, st);
-    k_search<<<(count + 255) / 256, 256, 0, st>>>(cdf, dim, dus, count, dout);
+    k_search<<<(count + 255) / 256, 256, 0, st>>>(cdf, dim, dus, count, margin, dout, dflags);
+    if (dflags &&
+        !hip_ok(hipMemcpyAsync(h_flags, dflags, count, hipMemcpyDeviceToHost, st), "enqueue_sampling"))
+        return false;
     return hip_ok(hipMemcpyAsync(h_out, dout, (size_t)count * 8, hipMemcpyDeviceToHost, st), "enqueue_sampling");
+}
+
+bool enqueue_sampling(const void* state, u64 dim, const double* h_uniforms, u32 count, u64* h_out, hipStream_t st,
+                      double margin, unsigned char* h_flags) {
+    if (count == 0) return true;
+    Scratch* sc = scratch_for(st);
+    if (!sc) return false;
+    void* buf = big_buffer(sc, sampling_buffer_bytes(dim, count, h_flags != nullptr));
+    return buf && enqueue_sampling_in(buf, state, dim, h_uniforms, count, h_out, st, margin, h_flags);
 }
 
 // One work-group per job: the exact sequential CDF of a small state (the
@@ -511,6 +596,8 @@ struct DevSampleJob {
     double* cdf;
     const double* uniforms;
     u64* out;
+    unsigned char* flags;  // nullptr: no boundary check
+    double margin;
     u32 count, pad;
 };
 
@@ -527,16 +614,8 @@ __global__ __launch_bounds__(kCdfThreads) void k_cdf_jobs(const DevSampleJob* __
         exact_chunk<false>(jb.state, jb.dim, c0, len, s, jb.cdf, P, scan_lds, min_lds, &s_bcast);
     }
     __syncthreads();
-    for (u32 c = threadIdx.x; c < jb.count; c += blockDim.x) {
-        const double u = jb.uniforms[c];
-        u64 lo = 0, hi = jb.dim + 1;
-        while (lo < hi) {
-            const u64 mid = lo + (hi - lo) / 2;
-            if (jb.cdf[mid] < u) lo = mid + 1;
-            else hi = mid;
-        }
-        jb.out[c] = lo - 1;
-    }
+    for (u32 c = threadIdx.x; c < jb.count; c += blockDim.x)
+        jb.out[c] = search_checked(jb.cdf, jb.dim, jb.uniforms[c], jb.margin, jb.flags ? jb.flags + c : nullptr);
 }
 
 bool enqueue_sampling_jobs(const SampleJob* jobs, u32 njobs, hipStream_t st) {
@@ -544,7 +623,8 @@ bool enqueue_sampling_jobs(const SampleJob* jobs, u32 njobs, hipStream_t st) {
     Scratch* sc = scratch_for(st);
     if (!sc) return false;
     size_t cdf_bytes = 0, total = 0;
-    bool contiguous = true;  // host uniforms and outputs laid out job after job
+    bool contiguous = true;  // host uniforms, outputs and flags laid out job after job
+    bool any_flags = false;
     for (u32 j = 0; j < njobs; ++j) {
         if (jobs[j].dim > kMultiBlockMinDim) {
             set_error(QM_ERR_INVALID_ARGUMENT, "enqueue_sampling_jobs: state of %llu amplitudes is too large", jobs[j].dim);
@@ -552,16 +632,21 @@ bool enqueue_sampling_jobs(const SampleJob* jobs, u32 njobs, hipStream_t st) {
         }
         cdf_bytes += ((size_t)(jobs[j].dim + 1) * 8 + 255) & ~size_t(255);
         if (j && (jobs[j].h_uniforms != jobs[j - 1].h_uniforms + jobs[j - 1].count ||
-                  jobs[j].h_out != jobs[j - 1].h_out + jobs[j - 1].count))
+                  jobs[j].h_out != jobs[j - 1].h_out + jobs[j - 1].count ||
+                  (jobs[j].h_flags != nullptr) != (jobs[j - 1].h_flags != nullptr) ||
+                  (jobs[j].h_flags && jobs[j].h_flags != jobs[j - 1].h_flags + jobs[j - 1].count)))
             contiguous = false;
+        any_flags = any_flags || jobs[j].h_flags != nullptr;
         total += jobs[j].count;
     }
     const size_t desc_bytes = ((size_t)njobs * sizeof(DevSampleJob) + 255) & ~size_t(255);
     const size_t io_bytes = (total * 8 + 255) & ~size_t(255);
-    char* buf = (char*)big_buffer(sc, desc_bytes + cdf_bytes + 2 * io_bytes);
+    const size_t flag_bytes = any_flags ? (total + 255) & ~size_t(255) : 0;
+    char* buf = (char*)big_buffer(sc, desc_bytes + cdf_bytes + 2 * io_bytes + flag_bytes);
     if (!buf) return false;
     double* dus = (double*)(buf + desc_bytes + cdf_bytes);
     u64* dout = (u64*)((char*)dus + io_bytes);
+    unsigned char* dflags = any_flags ? (unsigned char*)dout + io_bytes : nullptr;
     std::vector<DevSampleJob> d(njobs);
     size_t off = desc_bytes, io = 0;
     for (u32 j = 0; j < njobs; ++j) {
@@ -572,6 +657,8 @@ bool enqueue_sampling_jobs(const SampleJob* jobs, u32 njobs, hipStream_t st) {
         off += ((size_t)(jobs[j].dim + 1) * 8 + 255) & ~size_t(255);
         d[j].uniforms = dus + io;
         d[j].out = dout + io;
+        d[j].flags = jobs[j].h_flags ? dflags + io : nullptr;
+        d[j].margin = jobs[j].margin;
         if (!contiguous && jobs[j].count &&
             !hip_ok(hipMemcpyAsync(dus + io, jobs[j].h_uniforms, (size_t)jobs[j].count * 8, hipMemcpyHostToDevice, st),
                     "enqueue_sampling_jobs"))
@@ -586,20 +673,29 @@ bool enqueue_sampling_jobs(const SampleJob* jobs, u32 njobs, hipStream_t st) {
         return false;
     k_cdf_jobs<<<njobs, kCdfThreads, 0, st>>>((const DevSampleJob*)buf);
     if (contiguous) {
-        return !total ||
-               hip_ok(hipMemcpyAsync(jobs[0].h_out, dout, total * 8, hipMemcpyDeviceToHost, st), "enqueue_sampling_jobs");
-    }
-    for (u32 j = 0; j < njobs; ++j)
-        if (jobs[j].count && !hip_ok(hipMemcpyAsync(jobs[j].h_out, d[j].out, (size_t)jobs[j].count * 8,
-                                                    hipMemcpyDeviceToHost, st),
-                                     "enqueue_sampling_jobs"))
+        if (!total) return true;
+        if (jobs[0].h_flags &&
+            !hip_ok(hipMemcpyAsync(jobs[0].h_flags, dflags, total, hipMemcpyDeviceToHost, st), "enqueue_sampling_jobs"))
             return false;
+        return hip_ok(hipMemcpyAsync(jobs[0].h_out, dout, total * 8, hipMemcpyDeviceToHost, st), "enqueue_sampling_jobs");
+    }
+    for (u32 j = 0; j < njobs; ++j) {
+        if (!jobs[j].count) continue;
+        if (jobs[j].h_flags && !hip_ok(hipMemcpyAsync(jobs[j].h_flags, d[j].flags, jobs[j].count,
+                                                      hipMemcpyDeviceToHost, st),
+                                       "enqueue_sampling_jobs"))
+            return false;
+        if (!hip_ok(hipMemcpyAsync(jobs[j].h_out, d[j].out, (size_t)jobs[j].count * 8, hipMemcpyDeviceToHost, st),
+                    "enqueue_sampling_jobs"))
+            return false;
+    }
     return true;
 }
 
 template <bool DM>
 void sample(const double* uniforms, unsigned count, ITYPE* out, void* state, u64 dim, void* stream, unsigned dev,
-            const char* what) {
+            const char* what, double margin = 0.0, unsigned char* flags = nullptr, UINT* flagged = nullptr) {
+    if (flagged) *flagged = 0;
     if (!state || log2_dim(dim) < 0) {
         set_error(QM_ERR_INVALID_ARGUMENT, "%s: null state or bad dim", what);
         return;
@@ -609,6 +705,10 @@ void sample(const double* uniforms, unsigned count, ITYPE* out, void* state, u64
         set_error(QM_ERR_INVALID_ARGUMENT, "%s: null uniforms/out", what);
         return;
     }
+    if (!(margin >= 0.0) || margin >= 1.0) {
+        set_error(QM_ERR_INVALID_ARGUMENT, "%s: margin %g outside [0, 1)", what, margin);
+        return;
+    }
     if (!use_device(dev)) return;
     hipStream_t st = stream_of(stream);
     Scratch* sc = scratch_for(st);
@@ -616,16 +716,43 @@ void sample(const double* uniforms, unsigned count, ITYPE* out, void* state, u64
     const size_t cdf_bytes = ((size_t)(dim + 1) * sizeof(double) + 255) & ~size_t(255);
     const size_t work_bytes = (cdf_work_bytes(dim) + 255) & ~size_t(255);
     const size_t io_bytes = ((size_t)count * 8 + 255) & ~size_t(255);
-    char* buf = (char*)big_buffer(sc, cdf_bytes + work_bytes + 2 * io_bytes);
+    const size_t flag_bytes = flags ? ((size_t)count + 255) & ~size_t(255) : 0;
+    char* buf = (char*)big_buffer(sc, cdf_bytes + work_bytes + 2 * io_bytes + flag_bytes);
     if (!buf) return;
     double* cdf = (double*)buf;
     void* work = buf + cdf_bytes;
     double* dus = (double*)(buf + cdf_bytes + work_bytes);
     u64* dout = (u64*)((char*)dus + io_bytes);
+    unsigned char* dflags = flags ? (unsigned char*)dout + io_bytes : nullptr;
     if (!hip_ok(hipMemcpyAsync(dus, uniforms, (size_t)count * 8, hipMemcpyHostToDevice, st), what)) return;
     enqueue_cdf<DM>((const double2*)state, dim, cdf, work, st);
-    k_search<<<(count + 255) / 256, 256, 0, st>>>(cdf, dim, dus, count, dout);
+    k_search<<<(count + 255) / 256, 256, 0, st>>>(cdf, dim, dus, count, margin, dout, dflags);
     if (!hip_ok(hipMemcpyAsync(out, dout, (size_t)count * 8, hipMemcpyDeviceToHost, st), what)) return;
+    if (dflags && !hip_ok(hipMemcpyAsync(flags, dflags, count, hipMemcpyDeviceToHost, st), what)) return;
+    if (!hip_ok(hipStreamSynchronize(st), what)) return;
+    if (flagged && flags) {
+        UINT k = 0;
+        for (unsigned c = 0; c < count; ++c) k += flags[c] ? 1u : 0u;
+        *flagged = k;
+    }
+}
+
+template <bool DM>
+void cdf_to_host(double* cdf_out, void* state, u64 dim, void* stream, unsigned dev, const char* what) {
+    if (!state || !cdf_out || log2_dim(dim) < 0) {
+        set_error(QM_ERR_INVALID_ARGUMENT, "%s: null state/output or bad dim", what);
+        return;
+    }
+    if (!use_device(dev)) return;
+    hipStream_t st = stream_of(stream);
+    Scratch* sc = scratch_for(st);
+    if (!sc) return;
+    const size_t cdf_bytes = ((size_t)(dim + 1) * sizeof(double) + 255) & ~size_t(255);
+    char* buf = (char*)big_buffer(sc, cdf_bytes + cdf_work_bytes(dim));
+    if (!buf) return;
+    enqueue_cdf<DM>((const double2*)state, dim, (double*)buf, buf + cdf_bytes, st);
+    if (!hip_ok(hipMemcpyAsync(cdf_out, buf, (size_t)(dim + 1) * sizeof(double), hipMemcpyDeviceToHost, st), what))
+        return;
     hip_ok(hipStreamSynchronize(st), what);
 }
 
@@ -640,28 +767,50 @@ void qm_sampling_host(const double* uniforms, UINT count, ITYPE* out, void* stat
     sample<false>(uniforms, count, out, state, dim, stream, dev, "qm_sampling_host");
 }
 
-void qm_sampling_cdf_host(double* cdf_out, void* state, ITYPE dim, void* stream, unsigned dev) {
-    const char* what = "qm_sampling_cdf_host";
-    if (!state || !cdf_out || log2_dim(dim) < 0) {
-        set_error(QM_ERR_INVALID_ARGUMENT, "%s: null state/output or bad dim", what);
+void qm_sampling_checked_host(const double* uniforms, UINT count, double margin, ITYPE* out, unsigned char* flags,
+                              UINT* flagged, void* state, ITYPE dim, void* stream, unsigned dev) {
+    if (!flags) {
+        set_error(QM_ERR_INVALID_ARGUMENT, "qm_sampling_checked_host: null flags");
         return;
     }
-    if (!use_device(dev)) return;
-    hipStream_t st = stream_of(stream);
-    Scratch* sc = scratch_for(st);
-    if (!sc) return;
-    const size_t cdf_bytes = ((size_t)(dim + 1) * sizeof(double) + 255) & ~size_t(255);
-    char* buf = (char*)big_buffer(sc, cdf_bytes + cdf_work_bytes(dim));
-    if (!buf) return;
-    enqueue_cdf<false>((const double2*)state, dim, (double*)buf, buf + cdf_bytes, st);
-    if (!hip_ok(hipMemcpyAsync(cdf_out, buf, (size_t)(dim + 1) * sizeof(double), hipMemcpyDeviceToHost, st), what))
-        return;
-    hip_ok(hipStreamSynchronize(st), what);
+    sample<false>(uniforms, count, out, state, dim, stream, dev, "qm_sampling_checked_host", margin, flags, flagged);
+}
+
+void qm_sampling_cdf_host(double* cdf_out, void* state, ITYPE dim, void* stream, unsigned dev) {
+    cdf_to_host<false>(cdf_out, state, dim, stream, dev, "qm_sampling_cdf_host");
 }
 
 void dm_sampling_host(const double* uniforms, UINT count, ITYPE* out, void* state, ITYPE dim, void* stream,
                       unsigned dev) {
     sample<true>(uniforms, count, out, state, dim, stream, dev, "dm_sampling_host");
+}
+
+void dm_sampling_checked_host(const double* uniforms, UINT count, double margin, ITYPE* out, unsigned char* flags,
+                              UINT* flagged, void* state, ITYPE dim, void* stream, unsigned dev) {
+    if (!flags) {
+        set_error(QM_ERR_INVALID_ARGUMENT, "dm_sampling_checked_host: null flags");
+        return;
+    }
+    sample<true>(uniforms, count, out, state, dim, stream, dev, "dm_sampling_checked_host", margin, flags, flagged);
+}
+
+// A gate's update of an amplitude pair (a 2x2 complex product in FMAs) rounds
+// each result to within a few eps of its magnitude, and unitary gates carry
+// earlier errors forward without growth, so two evaluations of G gates differ
+// by ||psi_a - psi_b|| = O(sqrt(G) eps) when the roundings are independent
+// (O(G eps) in the worst case).  A CDF entry moves by at most
+// sum_j | |a_j|^2 - |b_j|^2 | <= 2 ||psi_a - psi_b|| plus a few units in the
+// last place of the running sum.  Measured on brickwork circuits (fused
+// executor, gate-by-gate device path and the CPU loop against each other, 16 to
+// 26 qubits, 900 to 4000 gates), max_i |cdf_a[i] - cdf_b[i]| stays below
+// 25 sqrt(G) eps; the margin is 512 sqrt(G) eps + 64 eps.
+double qm_sampling_margin(UINT gate_count) {
+    const double eps = 0x1p-53;
+    return 512.0 * std::sqrt((double)gate_count) * eps + 64.0 * eps;
+}
+
+void dm_sampling_cdf_host(double* cdf_out, void* state, ITYPE dim, void* stream, unsigned dev) {
+    cdf_to_host<true>(cdf_out, state, dim, stream, dev, "dm_sampling_cdf_host");
 }
 
 }  // extern "C"

@@ -63,7 +63,9 @@ struct Request {
     u64 bytes = 0;
     clk::time_point t_submit, t_admit, t_done;
     double energy = 0.0;
+    double margin = 0.0;  // boundary check of the samples (qm_sampling_margin of the gate count)
     std::vector<u64> samples;
+    std::vector<unsigned char> flags;
     int status = 0;  // 0 queued, 1 admitted, 2 done, 3 failed
     std::string error;
     ~Request() {
@@ -96,10 +98,12 @@ struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t ev = nullptr;
     ReqPtr req;
-    PinnedBuf energy, samples, uniforms;
+    PinnedBuf energy, samples, uniforms, flags;
 };
 
 double seconds(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+u64 large_state_bytes(u32 n) { return ((16ULL << n) + 255) & ~255ULL; }
 
 }  // namespace
 }  // namespace qm
@@ -126,7 +130,7 @@ struct qm_server {
     hipEvent_t bev = nullptr;
     qm_batch* batch = nullptr;
     std::vector<ReqPtr> running_batch, next_batch;
-    PinnedBuf benergy, bsamples, buniforms;
+    PinnedBuf benergy, bsamples, buniforms, bflags;
     u64 in_use = 0;
 
     // statistics (under mu)
@@ -164,20 +168,25 @@ void qm_server::launch_large(Slot& s, const ReqPtr& r) {
     const u64 dim = 1ULL << r->n;
     const u32 count = (u32)r->uniforms.size();
     std::string err;
-    void* state = nullptr;
+    // One stream-ordered allocation holds the state and the sampling CDF with
+    // its tables: r->bytes, the amount admission counted for this request.
+    void* mem = nullptr;
     bool ok = s.energy.reserve(4 * sizeof(double)) && s.samples.reserve((size_t)count * 8 + 8) &&
-              s.uniforms.reserve((size_t)count * 8 + 8);
+              s.uniforms.reserve((size_t)count * 8 + 8) && s.flags.reserve((size_t)count + 8);
     if (ok) {
         std::memcpy(s.uniforms.p, r->uniforms.data(), (size_t)count * 8);
-        ok = hip_ok(hipMallocAsync(&state, r->bytes, s.stream), "request server: state memory");
+        ok = hip_ok(hipMallocAsync(&mem, r->bytes, s.stream), "request server: state memory");
     }
     if (ok) {
+        void* state = mem;
+        void* samp = (char*)mem + large_state_bytes(r->n);
         enqueue_basis(state, dim, 0, s.stream);
         ok = circuit_enqueue(r->circuit, state, dim, s.stream, device) &&
              enqueue_observable(r->bf.data(), r->pf.data(), r->rot.data(), r->coef.data(), (u32)r->coef.size(), state,
                                 dim, s.stream, (double*)s.energy.p) &&
-             enqueue_sampling(state, dim, (const double*)s.uniforms.p, count, (u64*)s.samples.p, s.stream);
-        (void)hipFreeAsync(state, s.stream);
+             enqueue_sampling_in(samp, state, dim, (const double*)s.uniforms.p, count, (u64*)s.samples.p, s.stream,
+                                 r->margin, (unsigned char*)s.flags.p);
+        (void)hipFreeAsync(mem, s.stream);
     }
     if (!ok) err = take_error();
     s.req = r;
@@ -206,7 +215,7 @@ void qm_server::launch_batch() {
     const size_t nb = next_batch.size();
     bool ok = err.empty() && qm_batch_compile(batch, kBatchBlockQubits, 1, device) == QM_OK &&
               benergy.reserve(nb * sizeof(double)) && bsamples.reserve(total * 8 + 8) &&
-              buniforms.reserve(total * 8 + 8);
+              buniforms.reserve(total * 8 + 8) && bflags.reserve(total + 8);
     const double* d_energy = nullptr;
     if (ok) {
         d_energy = qm_batch_run_async(batch, bsptr, device);
@@ -225,6 +234,8 @@ void qm_server::launch_batch() {
             jobs[i].h_uniforms = (const double*)buniforms.p + off;
             jobs[i].h_out = (u64*)bsamples.p + off;
             jobs[i].count = (u32)r->uniforms.size();
+            jobs[i].margin = r->margin;
+            jobs[i].h_flags = (unsigned char*)bflags.p + off;
             off += r->uniforms.size();
         }
         ok = enqueue_sampling_jobs(jobs.data(), (u32)nb, bstream);
@@ -251,6 +262,8 @@ void qm_server::complete_batch(bool ok, const std::string& err) {
         if (ok) {
             r->energy = ((const double*)benergy.p)[i];
             r->samples.assign((const u64*)bsamples.p + off, (const u64*)bsamples.p + off + r->uniforms.size());
+            r->flags.assign((const unsigned char*)bflags.p + off,
+                            (const unsigned char*)bflags.p + off + r->uniforms.size());
         }
         off += r->uniforms.size();
         finish(r, ok, err);
@@ -285,6 +298,8 @@ void qm_server::run() {
                 const double* e = (const double*)s.energy.p;
                 r->energy = e[0] + e[2];
                 r->samples.assign((const u64*)s.samples.p, (const u64*)s.samples.p + r->uniforms.size());
+                r->flags.assign((const unsigned char*)s.flags.p,
+                                (const unsigned char*)s.flags.p + r->uniforms.size());
             }
             finish(r, ok, ok ? std::string() : (r->error.empty() ? std::string(hipGetErrorString(q)) : r->error));
             progress = true;
@@ -390,9 +405,14 @@ long long qm_server_submit(qm_server* s, UINT n, const qm_batch_gate* gates, UIN
     auto r = std::make_shared<Request>();
     r->n = n;
     r->small = n <= std::max<u32>(s->small_max, 3);  // the fused executor needs >= 4 qubits
-    r->bytes = 16ULL << n;
+    // HBM footprint: a small request's state (16 * 2^n; its CDF lives in the
+    // batch stream's scratch, at most 8 * (2^16 + 1) bytes per request); a large
+    // request's one allocation of state + CDF + CDF tables + sample buffers.
+    r->bytes = r->small ? (16ULL << n) : large_state_bytes(n) + sampling_buffer_bytes(1ULL << n, sample_count, true);
+    r->margin = qm_sampling_margin(gate_count);
     if (r->bytes > s->budget) {
-        set_error(QM_ERR_INVALID_ARGUMENT, "%s: a %u-qubit state (%llu bytes) exceeds the budget", what, n, r->bytes);
+        set_error(QM_ERR_INVALID_ARGUMENT, "%s: a %u-qubit request (%llu bytes with its sampling buffers) exceeds the budget", what, n,
+                  r->bytes);
         return -1;
     }
     r->gates.assign(gates, gates + gate_count);
@@ -424,7 +444,9 @@ long long qm_server_submit(qm_server* s, UINT n, const qm_batch_gate* gates, UIN
         if (idx < 0) return -1;
     } else {
         r->circuit = qm_circuit_create(n, r->gates.data(), gate_count, kLargeBlockQubits, 0);
-        if (!r->circuit) return -1;
+        // the program goes to the device here, in the submitting thread, so the
+        // scheduler thread never waits on an upload
+        if (!r->circuit || !use_device(s->device) || !circuit_prepare(r->circuit, s->device)) return -1;
     }
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop) {
@@ -440,7 +462,13 @@ long long qm_server_submit(qm_server* s, UINT n, const qm_batch_gate* gates, UIN
 }
 
 int qm_server_wait(qm_server* s, long long id, double* energy, ITYPE* samples, double* latency_seconds) {
+    return qm_server_wait_checked(s, id, energy, samples, nullptr, nullptr, latency_seconds);
+}
+
+int qm_server_wait_checked(qm_server* s, long long id, double* energy, ITYPE* samples, unsigned char* flags,
+                           UINT* flagged, double* latency_seconds) {
     const char* what = "qm_server_wait";
+    if (flagged) *flagged = 0;
     if (!s) {
         set_error(QM_ERR_INVALID_ARGUMENT, "%s: null server", what);
         return QM_ERR_INVALID_ARGUMENT;
@@ -462,6 +490,12 @@ int qm_server_wait(qm_server* s, long long id, double* energy, ITYPE* samples, d
     }
     if (energy) *energy = r->energy;
     if (samples && !r->samples.empty()) std::memcpy(samples, r->samples.data(), r->samples.size() * 8);
+    if (flags && !r->flags.empty()) std::memcpy(flags, r->flags.data(), r->flags.size());
+    if (flagged) {
+        UINT k = 0;
+        for (unsigned char f : r->flags) k += f ? 1u : 0u;
+        *flagged = k;
+    }
     return QM_OK;
 }
 
@@ -493,6 +527,7 @@ void qm_server_destroy(qm_server* s) {
         sl.energy.release();
         sl.samples.release();
         sl.uniforms.release();
+        sl.flags.release();
     }
     if (s->batch) qm_batch_destroy(s->batch);
     if (s->bev) (void)hipEventDestroy(s->bev);
@@ -500,6 +535,7 @@ void qm_server_destroy(qm_server* s) {
     s->benergy.release();
     s->bsamples.release();
     s->buniforms.release();
+    s->bflags.release();
     delete s;
 }
 

@@ -4,6 +4,7 @@ on one device with continuous batching and admission by HBM footprint.
     srv = Server(hbm_budget_bytes=64 << 30)
     rid = srv.submit(n, gates, terms, uniforms)    # from any thread
     energy, samples, latency = srv.wait(rid)
+    energy, samples, flags, latency = srv.wait_checked(rid)   # with the boundary check
     srv.close()
 
 `gates` and `terms` use the batch forms of qulacs_amd.batch; every request
@@ -48,13 +49,24 @@ class Server:
 
     def wait(self, rid: int):
         """(energy, samples, latency_seconds) of request rid."""
+        e, out, _, lat = self.wait_checked(rid)
+        return e, out, lat
+
+    def wait_checked(self, rid: int):
+        """(energy, samples, flags, latency_seconds) of request rid.  flags[c] = 1
+        marks a sample whose uniform lies within the rounding margin of a CDF
+        step (qm_sampling_checked_host); every other sample is the same for any
+        evaluation of the circuit that differs from this one by rounding."""
         with self._lock:
             count = self._counts.pop(rid)
         e = C.c_double()
         lat = C.c_double()
+        flagged = C.c_uint()
         out = np.zeros(max(1, count), dtype=np.uint64)
-        _lib.call("qm_server_wait", self.h, rid, C.byref(e), out.ctypes.data_as(_lib.PI), C.byref(lat))
-        return e.value, out[:count], lat.value
+        flags = np.zeros(max(1, count), dtype=np.uint8)
+        _lib.call("qm_server_wait_checked", self.h, rid, C.byref(e), out.ctypes.data_as(_lib.PI),
+                  flags.ctypes.data_as(_lib.PB), C.byref(flagged), C.byref(lat))
+        return e.value, out[:count], flags[:count], lat.value
 
     def stats(self) -> dict:
         v = [C.c_ulonglong() for _ in range(6)]

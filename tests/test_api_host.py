@@ -181,6 +181,14 @@ def test_circuit_bookkeeping():
         c.add_X_gate(4)
     with pytest.raises(RuntimeError):
         c.add_gate(gate.CNOT(5, 0))
+    # the positional overload checks the same indices and leaves the circuit as it was
+    before = [c.get_gate(i).get_name() for i in range(c.get_gate_count())]
+    with pytest.raises(RuntimeError):
+        c.add_gate(gate.X(4), 0)
+    with pytest.raises(RuntimeError):
+        c.add_gate(gate.CNOT(7, 1), 1)
+    assert [c.get_gate(i).get_name() for i in range(c.get_gate_count())] == before
+    assert c.calculate_depth() == 3
     cc = c.copy()
     cc.add_Z_gate(0)
     assert cc.get_gate_count() == c.get_gate_count() + 1
@@ -197,6 +205,11 @@ def test_parametric_circuit_bookkeeping():
     assert [c.get_parametric_gate_position(i) for i in range(3)] == [0, 2, 3]
     c.add_gate(gate.X(0), 1)
     assert [c.get_parametric_gate_position(i) for i in range(3)] == [0, 3, 4]
+    with pytest.raises(RuntimeError):
+        c.add_gate(gate.X(3), 1)  # qubit 3 of a 3-qubit circuit: rejected, positions unchanged
+    with pytest.raises(RuntimeError):
+        c.add_parametric_gate(gate.ParametricRX(5, 0.1), 0)
+    assert [c.get_parametric_gate_position(i) for i in range(3)] == [0, 3, 4] and c.get_gate_count() == 6
     c.set_parameter(1, 1.5)
     assert c.get_parameter(1) == 1.5 and c.get_gate(3).get_parameter_value() == 1.5
     c.remove_gate(0)

@@ -93,3 +93,22 @@ def test_batch_window_setter_validation():
         _lib.call("qm_batch_set_local_qubits", b.h, 14)
     _lib.call("qm_batch_set_local_qubits", b.h, 11)
     b.close()
+
+
+def test_adjoint_pass_plan():
+    """The adjoint groups gates into passes over 3-qubit register blocks (host
+    planning, no device work): the 20-qubit, 12-layer hardware-efficient ansatz
+    (480 rotations + 228 CZ) needs about one pass per three qubits per layer, a
+    rotation with four X/Y qubits keeps one kernel per gate, and so does a state
+    below three qubits."""
+    pytest.importorskip("torch")
+    from qulacs_amd.adjoint import AdjointCircuit, hardware_efficient_ansatz
+
+    gates, P = hardware_efficient_ansatz(20, 12)
+    st = AdjointCircuit(20, gates, P).stats()
+    assert st["gates"] == 708 and 80 <= st["passes"] <= 84, st
+    wide = AdjointCircuit(6, [("rot", [0, 1, 2, 3], [1, 2, 1, 1], 0, 1.0)], 1).stats()
+    assert wide["passes"] == 0 and wide["gates"] == 1
+    zrot = AdjointCircuit(6, [("rot", [0, 1, 2, 3], [3, 3, 3, 3], 0, 1.0), ("CZ", [0, 5], None)], 1).stats()
+    assert zrot["passes"] == 1
+    assert AdjointCircuit(2, [("rot", [0], [2], 0, 1.0)], 1).stats()["passes"] == 0

@@ -13,6 +13,7 @@ import qulacs_amd as q  # noqa: E402
 from qulacs_amd import gate  # noqa: E402
 from qulacs_amd.adjoint import AdjointCircuit, expectation, hardware_efficient_ansatz, heisenberg_terms  # noqa: E402
 from tests.batch_util import masks_to_lists  # noqa: E402
+from oracle.oracle import parameter_shift_grad  # noqa: E402
 from tests.golden.gen import haar_unitary  # noqa: E402
 
 pytestmark = pytest.mark.gpu
@@ -35,6 +36,28 @@ def _oracle_energy(oracle, n, gates, th, terms):
         if g[0] == "rot":
             _, qs, ps, p, c = g
             oracle.pauli_rotation(qs, ps, c * th[p], psi)
+        elif g[0] == "dense":
+            oracle.dense(g[1], g[2], psi)
+        elif g[0] == "diag":
+            oracle.controlled_diag([], [], g[1], g[2], psi)
+        else:
+            oracle.two(g[0], g[1][0], g[1][1], psi)
+    e = 0.0
+    for bf, pf, rot, c in terms:
+        t, p = masks_to_lists(bf, pf, n)
+        e += c * oracle.expectation(t, p, psi)
+    return e
+
+
+def _oracle_energy_occ(oracle, n, gates, angles, terms):
+    """The oracle energy with one angle per rotation occurrence (coefficient included)."""
+    it = iter(angles)
+    occ = [(g[0], g[1], g[2], None, next(it)) if g[0] == "rot" else g for g in gates]
+    psi = np.zeros(1 << n, dtype=np.complex128)
+    psi[0] = 1
+    for g in occ:
+        if g[0] == "rot":
+            oracle.pauli_rotation(g[1], g[2], g[4], psi)
         elif g[0] == "dense":
             oracle.dense(g[1], g[2], psi)
         elif g[0] == "diag":
@@ -95,10 +118,15 @@ def test_adjoint_matches_finite_differences_and_backprop(oracle):
     params = [gg[3] for gg in gates if gg[0] == "rot"]
     bp = np.array(hc.backprop(obs))
     shift = np.array(q.GradCalculator().calculate_grad(hc, obs)).real
+    # the oracle's parameter shift over per-occurrence angles (GradCalculator.cpp:8-44)
+    occ_angles = np.array([cf * theta[0][p] for p, cf in zip(params, coefs)])
+    ref_shift = parameter_shift_grad(lambda a: _oracle_energy_occ(oracle, n, gates, a, terms), occ_angles)
     want = np.zeros(P)
     for occ, (p, cf) in enumerate(zip(params, coefs)):
         want[p] += cf * bp[occ]
         assert abs(bp[occ] - shift[occ]) < 1e-9 * scale
+        assert abs(shift[occ] - ref_shift[occ]) < 1e-10 * scale, (occ, shift[occ], ref_shift[occ])
+        assert abs(bp[occ] - ref_shift[occ]) < 1e-10 * scale, (occ, bp[occ], ref_shift[occ])
     assert np.abs(g[0] - want).max() < 1e-10 * scale
 
 
@@ -136,3 +164,36 @@ def test_adjoint_rejects_bad_gates():
         AdjointCircuit(3, [("rot", [0], [1], 2, 1.0)], 1)  # parameter index out of range
     with pytest.raises(QulacsMI355Error):
         AdjointCircuit(4, [("dense", [0, 1, 2], np.eye(8))], 0)
+
+
+def test_adjoint_full_ansatz_shape_against_oracle(oracle):
+    """The 20-qubit, 12-layer hardware-efficient ansatz (480 RY/RZ parameters)
+    with the 57-term Heisenberg chain, two parameter sets: energies against the
+    oracle, and three gradient entries of the first set against central
+    differences (step 1e-6) and the parameter shift of oracle energies."""
+    n, layers, B = 20, 12, 2
+    gates, P = hardware_efficient_ansatz(n, layers)
+    assert P == 480
+    terms = heisenberg_terms(n)
+    assert len(terms) == 57
+    scale = sum(abs(t[3]) for t in terms)
+    rng = np.random.default_rng(21)
+    theta = rng.uniform(0, 2 * np.pi, (B, P))
+    circ = AdjointCircuit(n, gates, P)
+    e, g = circ.run(torch.tensor(theta, device="cuda:0"), terms)
+    e, g = e.cpu().numpy(), g.cpu().numpy()
+    print("energies", e, flush=True)
+    for b in range(B):
+        assert abs(e[b] - _oracle_energy(oracle, n, gates, theta[b], terms)) < 1e-10 * scale
+    h = 1e-6
+    for p in (0, 241, 479):
+        tp, tm = theta[0].copy(), theta[0].copy()
+        tp[p] += h
+        tm[p] -= h
+        fd = (_oracle_energy(oracle, n, gates, tp, terms) - _oracle_energy(oracle, n, gates, tm, terms)) / (2 * h)
+        tp[p] = theta[0][p] + np.pi / 2
+        tm[p] = theta[0][p] - np.pi / 2
+        ps = (_oracle_energy(oracle, n, gates, tp, terms) - _oracle_energy(oracle, n, gates, tm, terms)) / 2
+        print(f"param {p}: adjoint {g[0, p]:+.15f} fd {fd:+.15f} shift {ps:+.15f}", flush=True)
+        assert abs(g[0, p] - fd) < 1e-8 * scale
+        assert abs(g[0, p] - ps) < 1e-10 * scale

@@ -52,3 +52,27 @@ def test_controlled_dense_on_dm_and_sv(targets, controls):
         g.update_quantum_state(dm)
         np.testing.assert_allclose(sv.get_vector(), u @ psi, atol=1e-12)
         np.testing.assert_allclose(dm.get_matrix(), u @ rho @ u.conj().T, atol=1e-12)
+
+
+@pytest.mark.parametrize("n", [6, 9])
+def test_monomial_two_qubit_gates_on_dm(n):
+    """Two-qubit monomial unitaries (CNOT either way, CZ, SWAP, iSWAP, a
+    controlled phase with control value 0) take the one-pass path that moves
+    only the density-matrix entries that change; each equals U rho U^dagger."""
+    rng = np.random.default_rng(40 + n)
+    iswap = np.array([[1, 0, 0, 0], [0, 0, 1j, 0], [0, 1j, 0, 0], [0, 0, 0, 1]])
+    for rep in range(4):
+        a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+        if rep == 0:
+            a, b = 0, n - 1  # lowest and highest qubit: cached and streaming member accesses
+        gs = [gate.CNOT(a, b), gate.CNOT(b, a), gate.CZ(a, b), gate.SWAP(a, b), gate.DenseMatrix([a, b], iswap)]
+        cs = gate.DenseMatrix(b, np.diag([1, 1j]))
+        cs.add_control_qubit(a, 0)
+        gs.append(cs)
+        for g in gs:
+            dm = q.DensityMatrix(n)
+            dm.set_Haar_random_state(rep + 3)
+            rho = dm.get_matrix()
+            u = _full(g, n)
+            g.update_quantum_state(dm)
+            np.testing.assert_allclose(dm.get_matrix(), u @ rho @ u.conj().T, atol=1e-12)
