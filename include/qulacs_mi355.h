@@ -205,6 +205,17 @@ QM_EXPORT void qm_state_add_with_coef_host(qm_complex coef, const void* state_ad
  * allocation, launch and synchronisation per term. */
 QM_EXPORT qm_complex qm_observable_expectation_host(const ITYPE* bit_flip_masks, const ITYPE* phase_flip_masks, const UINT* global_phase_90rot_counts, const qm_complex* coefs, UINT term_count, void* state, ITYPE dim, void* stream, unsigned int device_number);
 
+/* Allocation-free variant of qm_observable_expectation_host, with no
+ * counterpart in the reference: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, same kernels and the same result bit for bit;
+ * it calls no hipMalloc / hipFree.  Its term table, partial sums and results
+ * live in one library-owned device array of about 4 MiB per device, held until
+ * the process ends.  At most 128 terms per call (more is
+ * QM_ERR_INVALID_ARGUMENT).  Calls that overlap on one device share that
+ * array and race: serialise them. */
+QM_EXPORT qm_complex qm_observable_expectation_noalloc_host(const ITYPE* bit_flip_masks, const ITYPE* phase_flip_masks, const UINT* global_phase_90rot_counts, const qm_complex* coefs, UINT term_count, void* state, ITYPE dim, void* stream, unsigned int device_number);
+
 /* Sampling with caller-supplied uniforms: out[c] = lower_bound(cdf, u[c]) - 1
  * over the sequential un-normalised CDF cdf[0] = 0, cdf[i+1] = cdf[i] + |psi_i|^2,
  * exactly as QuantumStateCpu::sampling (src/cppsim/state.hpp:909-930), with

@@ -119,6 +119,7 @@ _SIGS: dict[str, tuple] = {
     "transition_amplitude_multi_qubit_Pauli_operator_whole_list_host": (QmComplex, [PU, UINT, VP, VP, ITYPE, VP, UINT]),
     "qm_state_add_with_coef_host": (None, [QmComplex, VP, VP, ITYPE, VP, UINT]),
     "qm_observable_expectation_host": (QmComplex, [PI, PI, PU, PC, UINT, VP, ITYPE, VP, UINT]),
+    "qm_observable_expectation_noalloc_host": (QmComplex, [PI, PI, PU, PC, UINT, VP, ITYPE, VP, UINT]),
     "qm_sampling_host": (None, [PD, UINT, PI, VP, ITYPE, VP, UINT]),
     "qm_sampling_checked_host": (None, [PD, UINT, D, PI, PB, PU, VP, ITYPE, VP, UINT]),
     "qm_sampling_margin": (D, [UINT]),

@@ -358,6 +358,47 @@ __global__ __launch_bounds__(256) void k_scale_r(double2* __restrict__ s, u64 di
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) st_stream(s + i, cscale(ld_stream(s + i), f));
 }
 
+// ------------------------------------------------------------------ allocation-free observable
+// qm_observable_expectation_noalloc_host runs the same two launches per term
+// group as qm_observable_expectation_host (k_reduce<R_PAULI>, then k_finish),
+// but its term table, coefficients, per-block partials and group results live
+// in `scratch`, a fixed device array, instead of the library's per-stream
+// allocations.  Every block of k_reduce writes its partial slot and k_finish
+// writes every result slot, so nothing in it has to start at zero.  All calls
+// on one device share that device's copy: calls that overlap on one device
+// (other threads, other streams) race.
+constexpr unsigned kNoAllocMaxTerms = 128;
+constexpr size_t kNoAllocTermsOff = 0;
+constexpr size_t kNoAllocCoefsOff = 4096;                                        // >= 128 * sizeof(Term)
+constexpr size_t kNoAllocOutOff = kNoAllocCoefsOff + 2048;                       // 128 * 16 bytes
+constexpr size_t kNoAllocOutDoubles = 2 + 2 * kNoAllocMaxTerms;                  // per group
+constexpr size_t kNoAllocPartialsOff = kNoAllocOutOff + 2 * kNoAllocOutDoubles * 8 + 256 - (2 * kNoAllocOutDoubles * 8) % 256;
+constexpr size_t kNoAllocBytes = kNoAllocPartialsOff + (size_t)kMaxRBlocks * kNoAllocMaxTerms * 2 * sizeof(double);
+static_assert(kNoAllocMaxTerms * sizeof(Term) <= kNoAllocCoefsOff, "term table overlaps the coefficients");
+static_assert(kNoAllocPartialsOff % 256 == 0, "partials not 256-byte aligned");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+unsigned char* g_noalloc_ptr[kNoAllocMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up once per
+// device; the array's name itself is only a host-side stand-in).
+unsigned char* noalloc_scratch(unsigned dev, const char* what) {
+    if (dev >= (unsigned)kNoAllocMaxDevices) {
+        set_error(QM_ERR_UNSUPPORTED, "%s: device %u beyond %d", what, dev, kNoAllocMaxDevices);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        if (!hip_ok(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), what)) return nullptr;
+        g_noalloc_ptr[dev] = (unsigned char*)p;
+    }
+    return g_noalloc_ptr[dev];
+}
+
 }  // namespace qm
 
 using namespace qm;
@@ -545,6 +586,82 @@ qm_complex qm_observable_expectation_host(const ITYPE* bfs, const ITYPE* pfs, co
             total_im += res[1];
         }
     }
+    z.re = total_re;
+    z.im = total_im;
+    return z;
+}
+
+// ------------------------------------------------------------------ allocation-free observable
+qm_complex qm_observable_expectation_noalloc_host(const ITYPE* bfs, const ITYPE* pfs, const UINT* rots,
+                                                 const qm_complex* coefs, UINT nterms, void* state, ITYPE dim,
+                                                 void* stream, unsigned dev) {
+    const char* what = "qm_observable_expectation_noalloc_host";
+    qm_complex z{0, 0};
+    if (!state_ok(state, dim, what)) return z;
+    if (nterms == 0) return z;
+    if (!bfs || !pfs || !rots || !coefs) {
+        set_error(QM_ERR_INVALID_ARGUMENT, "%s: null term arrays", what);
+        return z;
+    }
+    if (nterms > kNoAllocMaxTerms) {
+        set_error(QM_ERR_INVALID_ARGUMENT, "%s: %u terms exceed the %u this entry point holds", what, nterms,
+                  kNoAllocMaxTerms);
+        return z;
+    }
+    const int n = log2_dim(dim);
+    for (UINT k = 0; k < nterms; ++k)
+        if (((bfs[k] | pfs[k]) >> n) != 0) {
+            set_error(QM_ERR_INVALID_ARGUMENT, "%s: term %u acts beyond %d qubits", what, k, n);
+            return z;
+        }
+    // Z-only terms first, then flip terms, each group in input order (the
+    // grouping of the allocating entry point).
+    Term ht[kNoAllocMaxTerms];
+    qm_complex hc[kNoAllocMaxTerms];
+    unsigned count[2] = {0, 0};
+    for (int g = 0; g < 2; ++g)
+        for (UINT k = 0; k < nterms; ++k) {
+            if ((bfs[k] != 0) != (g == 1)) continue;
+            const unsigned pivot = bfs[k] ? 63 - __builtin_clzll(bfs[k]) : 0;
+            ht[count[0] + count[1]] = make_term(bfs[k], pfs[k], rots[k], pivot);
+            hc[count[0] + count[1]] = coefs[k];
+            ++count[g];
+        }
+    const unsigned first[2] = {0, count[0]};
+    if (!use_device(dev)) return z;
+    unsigned char* base = noalloc_scratch(dev, what);
+    if (!base) return z;
+    Term* dterms = (Term*)(base + kNoAllocTermsOff);
+    double2* dcoefs = (double2*)(base + kNoAllocCoefsOff);
+    double* dout = (double*)(base + kNoAllocOutOff);
+    double* partials = (double*)(base + kNoAllocPartialsOff);
+    hipStream_t st = stream_of(stream);
+    if (!hip_ok(hipMemcpyAsync(dterms, ht, nterms * sizeof(Term), hipMemcpyHostToDevice, st), what) ||
+        !hip_ok(hipMemcpyAsync(dcoefs, hc, nterms * sizeof(double2), hipMemcpyHostToDevice, st), what))
+        return z;
+    RArgs r{};
+    r.a = (const double2*)state;
+    r.dim = dim;
+    r.partials = partials;
+    for (int g = 0; g < 2; ++g) {
+        if (!count[g]) continue;
+        const unsigned nb = reduce_blocks(g ? dim / 2 : dim);
+        r.terms = dterms + first[g];
+        k_reduce<R_PAULI><<<dim3(nb, count[g]), kRBlock, 0, st>>>(r);
+        k_finish<<<1, kRBlock, 0, st>>>(partials, nb, count[g], dcoefs + first[g], dout + g * kNoAllocOutDoubles);
+    }
+    double res[2][2] = {{0, 0}, {0, 0}};
+    for (int g = 0; g < 2; ++g)
+        if (count[g] && !hip_ok(hipMemcpyAsync(res[g], dout + g * kNoAllocOutDoubles, 2 * sizeof(double),
+                                               hipMemcpyDeviceToHost, st), what))
+            return z;
+    if (!hip_ok(hipGetLastError(), what) || !hip_ok(hipStreamSynchronize(st), what)) return z;
+    double total_re = 0, total_im = 0;
+    for (int g = 0; g < 2; ++g)
+        if (count[g]) {
+            total_re += res[g][0];
+            total_im += res[g][1];
+        }
     z.re = total_re;
     z.im = total_im;
     return z;
