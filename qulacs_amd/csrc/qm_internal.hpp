@@ -82,7 +82,9 @@ struct Scratch {
     void* big = nullptr;            // grown on demand (sampling CDF etc.)
     size_t big_bytes = 0;
 };
-constexpr size_t kPartialsCap = 1 << 20;  // doubles
+// doubles: a 256-term reduction over 2048 blocks (2^20 partials) plus its
+// per-term results
+constexpr size_t kPartialsCap = (1 << 20) + 4096;
 constexpr size_t kHostResultCap = 1 << 16;
 constexpr size_t kRingBytes = 8 << 20;
 

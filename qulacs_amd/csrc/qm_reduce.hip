@@ -139,34 +139,43 @@ __global__ __launch_bounds__(kRBlock) void k_reduce(RArgs r) {
     }
 }
 
-// Sums partials[term][0..nb) in index order, multiplies by the term's
-// coefficient and writes per-term values plus their ordered total.
-__global__ __launch_bounds__(kRBlock) void k_finish(const double* __restrict__ partials, unsigned nb, unsigned nterms,
-                                                    const double2* __restrict__ coefs, double* __restrict__ out) {
-    __shared__ double lds[32];
-    double total_re = 0.0, total_im = 0.0;
-    for (unsigned t = 0; t < nterms; ++t) {
+// Sums partials[term][0..nb) per term, multiplies by the term's coefficient
+// and writes per-term values plus their total.  One wave per term (16 terms at
+// a time): lane-strided sums in index order, then a fixed shuffle tree; the
+// per-term values are added in term order.  Deterministic, and with no block
+// barriers inside the term loop.
+constexpr int kFinishThreads = 1024;
+__global__ __launch_bounds__(kFinishThreads) void k_finish(const double* __restrict__ partials, unsigned nb,
+                                                           unsigned nterms, const double2* __restrict__ coefs,
+                                                           double* __restrict__ out) {
+    const unsigned lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (unsigned t = wid; t < nterms; t += nw) {
+        const double* p = partials + (u64)t * nb * 2;
         double re = 0.0, im = 0.0;
-        for (unsigned k = threadIdx.x; k < nb; k += blockDim.x) {
-            re += partials[((u64)t * nb + k) * 2];
-            im += partials[((u64)t * nb + k) * 2 + 1];
+        for (unsigned k = lane; k < nb; k += 64) {
+            re += p[2 * k];
+            im += p[2 * k + 1];
         }
-        re = block_sum(re, lds);
-        im = block_sum(im, lds);
+        re = wave_sum(re);
+        im = wave_sum(im);
         if (coefs) {
             const double2 c = coefs[t];
             const double pr = c.x * re - c.y * im, pi = c.x * im + c.y * re;
             re = pr;
             im = pi;
         }
-        total_re += re;
-        total_im += im;
-        if (threadIdx.x == 0) {
+        if (lane == 0) {
             out[2 + 2 * t] = re;
             out[3 + 2 * t] = im;
         }
     }
+    __syncthreads();  // the per-term values written above are visible to the whole block
     if (threadIdx.x == 0) {
+        double total_re = 0.0, total_im = 0.0;
+        for (unsigned t = 0; t < nterms; ++t) {
+            total_re += out[2 + 2 * t];
+            total_im += out[3 + 2 * t];
+        }
         out[0] = total_re;
         out[1] = total_im;
     }
@@ -200,7 +209,7 @@ bool enqueue_reduce(RArgs r, u64 work, const Term* host_terms, unsigned nterms, 
     r.partials = sc->partials;
     double* out = sc->partials + (u64)nb * ny * 2;
     k_reduce<KIND><<<dim3(nb, ny), kRBlock, 0, st>>>(r);
-    k_finish<<<1, kRBlock, 0, st>>>(sc->partials, nb, ny, dcoefs, out);
+    k_finish<<<1, kFinishThreads, 0, st>>>(sc->partials, nb, ny, dcoefs, out);
     *dout = out;
     *ny_out = ny;
     return true;
@@ -349,13 +358,29 @@ __global__ __launch_bounds__(256) void k_add(double2* __restrict__ s, const doub
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) st_stream(s + i, cadd(ld_stream(s + i), ld_stream(a + i)));
 }
-__global__ __launch_bounds__(256) void k_scale_c(double2* __restrict__ s, u64 dim, double2 coef) {
+// Elementwise in-place passes: from 128 amplitudes up, each thread takes two
+// amplitudes 64 apart (both loads issued before either store), so a wave reads
+// 2 KiB of consecutive lines per iteration; smaller states take one per thread.
+template <class F>
+__device__ __forceinline__ void inplace_pass(double2* __restrict__ s, u64 dim, F f) {
     const u64 stride = (u64)gridDim.x * blockDim.x;
-    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) st_stream(s + i, cmul(ld_stream(s + i), coef));
+    const u64 t0 = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (dim < 128) {
+        for (u64 i = t0; i < dim; i += stride) s[i] = f(s[i]);
+        return;
+    }
+    for (u64 i = t0; i < (dim >> 1); i += stride) {
+        const u64 a = ((i >> 6) << 7) | (i & 63);
+        const double2 x = ld_stream(s + a), y = ld_stream(s + a + 64);
+        st_stream(s + a, f(x));
+        st_stream(s + a + 64, f(y));
+    }
 }
-__global__ __launch_bounds__(256) void k_scale_r(double2* __restrict__ s, u64 dim, double f) {
-    const u64 stride = (u64)gridDim.x * blockDim.x;
-    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) st_stream(s + i, cscale(ld_stream(s + i), f));
+__global__ __launch_bounds__(512) void k_scale_c(double2* __restrict__ s, u64 dim, double2 coef) {
+    inplace_pass(s, dim, [coef](double2 v) { return cmul(v, coef); });
+}
+__global__ __launch_bounds__(512) void k_scale_r(double2* __restrict__ s, u64 dim, double f) {
+    inplace_pass(s, dim, [f](double2 v) { return cscale(v, f); });
 }
 
 // ------------------------------------------------------------------ allocation-free observable
@@ -648,7 +673,7 @@ qm_complex qm_observable_expectation_noalloc_host(const ITYPE* bfs, const ITYPE*
         const unsigned nb = reduce_blocks(g ? dim / 2 : dim);
         r.terms = dterms + first[g];
         k_reduce<R_PAULI><<<dim3(nb, count[g]), kRBlock, 0, st>>>(r);
-        k_finish<<<1, kRBlock, 0, st>>>(partials, nb, count[g], dcoefs + first[g], dout + g * kNoAllocOutDoubles);
+        k_finish<<<1, kFinishThreads, 0, st>>>(partials, nb, count[g], dcoefs + first[g], dout + g * kNoAllocOutDoubles);
     }
     double res[2][2] = {{0, 0}, {0, 0}};
     for (int g = 0; g < 2; ++g)
@@ -672,7 +697,7 @@ void normalize_host(double squared_norm, void* state, ITYPE dim, void* stream, u
     if (!state_ok(state, dim, "normalize_host") || !use_device(dev)) return;
     hipStream_t st = stream_of(stream);
     const double f = std::sqrt(1.0 / squared_norm);  // src/csim/update_ops_named_state.cpp:23
-    k_scale_r<<<grid_for(dim, 256 * 4), 256, 0, st>>>((double2*)state, dim, f);
+    k_scale_r<<<grid_for(dim >> 1, 512), 512, 0, st>>>((double2*)state, dim, f);
     finish_update(st, "normalize_host");
 }
 void state_add_host(void* added, void* state, ITYPE dim, void* stream, unsigned dev) {
@@ -694,7 +719,7 @@ void qm_state_add_with_coef_host(qm_complex coef, const void* added, void* state
 void state_multiply_host(qm_complex coef, void* state, ITYPE dim, void* stream, unsigned dev) {
     if (!state_ok(state, dim, "state_multiply_host") || !use_device(dev)) return;
     hipStream_t st = stream_of(stream);
-    k_scale_c<<<grid_for(dim, 256 * 4), 256, 0, st>>>((double2*)state, dim, make_double2(coef.re, coef.im));
+    k_scale_c<<<grid_for(dim >> 1, 512), 512, 0, st>>>((double2*)state, dim, make_double2(coef.re, coef.im));
     finish_update(st, "state_multiply_host");
 }
 

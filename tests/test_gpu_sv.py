@@ -210,6 +210,30 @@ def test_reductions_match_oracle(oracle, n):
         assert abs(z.real - total) <= 1e-10 * scale and abs(z.imag) <= 1e-10 * scale
 
 
+def test_observable_with_hundreds_of_terms_on_23_qubits(oracle):
+    """More than 256 Z-only terms on a state that needs the full 2048 reduction
+    blocks: the terms run in chunks of 256 whose partial sums fill the
+    reduction workspace exactly."""
+    rng = np.random.default_rng(2323)
+    n = 23
+    psi = random_state(rng, n)
+    terms = []
+    for _ in range(260):
+        tg = sorted(rng.choice(n, int(rng.integers(1, 5)), replace=False).tolist())
+        terms.append((tg, [3] * len(tg), float(rng.standard_normal())))
+    terms += random_pauli_terms(rng, n, 12)
+    masks = [pauli_masks(tg, pl) for tg, pl, _ in terms]
+    k = len(terms)
+    bfs = (_lib.C.c_ulonglong * k)(*[m[0] for m in masks])
+    pfs = (_lib.C.c_ulonglong * k)(*[m[1] for m in masks])
+    with DevState.from_host(psi) as s:
+        z = to_complex(_lib.call("qm_observable_expectation_host", bfs, pfs, u32([m[2] for m in masks]),
+                                 cmat([cf for *_, cf in terms]), k, s.ptr, s.dim, None, 0))
+    want = sum(cf * oracle.expectation(tg, pl, psi) for tg, pl, cf in terms)
+    scale = max(abs(want), sum(abs(cf) for *_, cf in terms))
+    assert abs(z.real - want) <= 1e-10 * scale and abs(z.imag) <= 1e-10 * scale, (z, want)
+
+
 def _sampling_states(rng, n):
     dim = 1 << n
     yield "random", random_state(rng, n)
