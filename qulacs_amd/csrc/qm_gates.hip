@@ -346,10 +346,194 @@ struct OpDiag {  // member x *= d[x]
     }
 };
 
+// ------------------------------------------------------------------ low-qubit gates
+// Gates with a target or control below bit 3, i.e. inside one 128-byte line.
+// The gather kernel gives each lane one tuple, so there each access is 16 bytes
+// at a 32..64-byte stride, a line is visited by several instructions and only
+// the changed 16-byte pieces are written back.  Here consecutive lanes take
+// consecutive amplitudes instead: every wave instruction reads or writes whole
+// lines (eight lanes per line), a lane fetches the members of its tuple that
+// share its line from the lanes holding them (__shfl), and every line the gate
+// touches is written back whole, unchanged amplitudes included, with streaming
+// accesses.  Tuple bits in LM are the targets below bit 3 (lane bits); the
+// other targets are held in registers as in the gather kernel.  Controls below
+// bit 3 select per lane between the new and the old value; controls from bit 3
+// up are fixed bits of the enumeration, as are the high targets (inserted
+// zeros).
+template <int K>
+struct LaneArgs {
+    u64 low[kMaxIns];      // (1 << p) - 1 for the ascending positions p >= 3 of high targets and controls
+    int n_ins;
+    u64 cmask;             // controls >= 3 that must be 1
+    u64 offs[1 << K];      // member offsets from the high targets (entries with no LM bit)
+    unsigned char pos[K];  // target position of each tuple bit
+    u32 lc_mask, lc_val;   // controls below bit 3: a lane takes the new value iff (j & lc_mask) == lc_val
+};
+
+__device__ __forceinline__ double2 shfl2(double2 v, int src) {
+    return make_double2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
+}
+
+// Whether the lines of high member combination x (no LM bit) are touched:
+// some member sharing them is read or written.
+template <int K, unsigned LM, class Op>
+__host__ __device__ constexpr bool lane_touched(int x) {
+    for (int l = 0; l < (1 << K); ++l)
+        if ((l & ~(int)LM) == 0 && ((((Op::READ | Op::WRITE) >> (l | x)) & 1) != 0)) return true;
+    return false;
+}
+
+// U items per thread, all loaded before any is used.  Requires n_items to be a
+// multiple of 64: whether an item exists is then uniform across a wave, so
+// every lane takes part in every shuffle.
+template <int K, unsigned LM, int U, class Op>
+__global__ __launch_bounds__(512) void k_lane(double2* __restrict__ s, u64 n_items, LaneArgs<K> a, Op op) {
+    constexpr int N = 1 << K;
+    const u64 stride = (u64)gridDim.x * blockDim.x * U;
+    const int lane = threadIdx.x & 63;
+    for (u64 i0 = (u64)blockIdx.x * blockDim.x * U + threadIdx.x; i0 < n_items; i0 += stride) {
+        u64 j[U];
+        unsigned own[U];
+        double2 h[U][N];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            u64 jj = i0 + (u64)u * blockDim.x;
+            if (jj >= n_items) continue;  // wave-uniform
+#pragma unroll
+            for (int k = 0; k < kMaxIns; ++k)
+                if (k < a.n_ins) {
+                    const u64 lo = a.low[k];
+                    jj = (jj & lo) | ((jj & ~lo) << 1);
+                }
+            jj |= a.cmask;
+            j[u] = jj;
+            // this lane's tuple bits below bit 3: its own member within the line group
+            unsigned o = 0;
+#pragma unroll
+            for (int t = 0; t < K; ++t)
+                if ((LM >> t) & 1) o |= (unsigned)((jj >> a.pos[t]) & 1) << t;
+            own[u] = o;
+#pragma unroll
+            for (int x = 0; x < N; ++x)
+                if ((x & LM) == 0 && lane_touched<K, LM, Op>(x)) h[u][x] = ld_stream(s + jj + a.offs[x]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (i0 + (u64)u * blockDim.x >= n_items) continue;  // wave-uniform
+            double2 v[N];
+#pragma unroll
+            for (int x = 0; x < N; ++x) {
+                v[x] = make_double2(0.0, 0.0);
+                if (!((Op::READ >> x) & 1)) continue;
+                // member x sits in the lane whose low tuple bits are x & LM
+                const unsigned d = own[u] ^ (x & LM);
+                int dl = 0;
+#pragma unroll
+                for (int t = 0; t < K; ++t)
+                    if ((LM >> t) & 1) dl |= (int)((d >> t) & 1) << a.pos[t];
+                v[x] = shfl2(h[u][x & ~LM], lane ^ dl);
+            }
+            op.apply(v);
+            const bool ctrl = ((u32)j[u] & a.lc_mask) == a.lc_val;
+#pragma unroll
+            for (int x = 0; x < N; ++x) {
+                if ((x & LM) != 0 || !lane_touched<K, LM, Op>(x)) continue;
+                double2 out = v[x];
+                bool wr = (Op::WRITE >> x) & 1;
+#pragma unroll
+                for (int l = 1; l < N; ++l)
+                    if ((l & ~LM) == 0 && own[u] == (unsigned)l) {
+                        out = v[l | x];
+                        wr = (Op::WRITE >> (l | x)) & 1;
+                    }
+                st_stream(s + j[u] + a.offs[x], (ctrl && wr) ? out : h[u][x]);
+            }
+        }
+    }
+}
+
+// Sets up k_lane for a gate with K targets; returns false when the gate has no
+// qubit below bit 3 or the state is too small (fewer than 64 work items).
+template <int K>
+bool fill_lane_args(LaneArgs<K>& a, unsigned& lm, u64& n_items, u64 dim, const u32* targets, const u32* controls,
+                    const u32* cvals, u32 nc) {
+    u64 hi_bits = 0;
+    bool any_low = false;
+    lm = 0;
+    a.cmask = 0;
+    a.lc_mask = a.lc_val = 0;
+    for (int t = 0; t < K; ++t) {
+        a.pos[t] = (unsigned char)targets[t];
+        if (targets[t] < kLineQubits) {
+            lm |= 1u << t;
+            any_low = true;
+        } else {
+            hi_bits |= 1ULL << targets[t];
+        }
+    }
+    for (u32 c = 0; c < nc; ++c) {
+        if (controls[c] < kLineQubits) {
+            any_low = true;
+            a.lc_mask |= 1u << controls[c];
+            if (cvals[c]) a.lc_val |= 1u << controls[c];
+        } else {
+            hi_bits |= 1ULL << controls[c];
+            if (cvals[c]) a.cmask |= 1ULL << controls[c];
+        }
+    }
+    if (!any_low) return false;
+    a.n_ins = 0;
+    for (int b = 0; b < 64; ++b)
+        if ((hi_bits >> b) & 1) {
+            if (a.n_ins == kMaxIns) return false;
+            a.low[a.n_ins++] = (1ULL << b) - 1;
+        }
+    for (int k = a.n_ins; k < kMaxIns; ++k) a.low[k] = 0;
+    n_items = dim >> a.n_ins;
+    if (n_items < 64) return false;
+    for (int x = 0; x < (1 << K); ++x) {
+        u64 o = 0;
+        for (int t = 0; t < K; ++t)
+            if (((x >> t) & 1) && !((lm >> t) & 1)) o |= 1ULL << targets[t];
+        a.offs[x] = o;
+    }
+    return true;
+}
+
+template <int K, class Op>
+bool launch_lane(double2* s, u64 dim, const u32* targets, const u32* controls, const u32* cvals, u32 nc,
+                 const Op& op, hipStream_t st) {
+    LaneArgs<K> a;
+    unsigned lm = 0;
+    u64 n_items = 0;
+    if (!fill_lane_args<K>(a, lm, n_items, dim, targets, controls, cvals, nc)) return false;
+    constexpr int U = 2;
+    const unsigned block = 512;
+    const unsigned grid = grid_for(n_items, (u64)block * U);
+    switch (lm) {
+        case 0: k_lane<K, 0, U, Op><<<grid, block, 0, st>>>(s, n_items, a, op); break;
+        case 1: k_lane<K, 1, U, Op><<<grid, block, 0, st>>>(s, n_items, a, op); break;
+        case 2: if constexpr (K >= 2) k_lane<K, 2, U, Op><<<grid, block, 0, st>>>(s, n_items, a, op); break;
+        default: if constexpr (K >= 2) k_lane<K, 3, U, Op><<<grid, block, 0, st>>>(s, n_items, a, op); break;
+    }
+    return true;
+}
+
 // ------------------------------------------------------------------ generic driver
 template <int K, int U, class Op>
 void launch_gather(double2* s, u64 dim, const u32* targets, const u32* controls, const u32* cvals, u32 nc,
                    const Op& op, hipStream_t st) {
+    // The line kernel pays off where the gather kernel would touch lines
+    // partially: operations that read or write only some tuple members (CZ,
+    // phases, projections) and controls below bit 3.  A full dense update with
+    // a low target streams whole lines through the gather kernel's partner loads
+    // already, and the line kernel would repeat its arithmetic per lane.
+    constexpr unsigned kAll = (1u << (1 << K)) - 1;
+    bool low_control = false;
+    for (u32 c = 0; c < nc; ++c) low_control |= controls[c] < kLineQubits;
+    if (K <= 2 && (Op::READ != kAll || Op::WRITE != kAll || low_control) &&
+        launch_lane<K>(s, dim, targets, controls, cvals, nc, op, st))
+        return;
     GatherArgs<K> a;
     if (!fill_args<K>(a, targets, controls, cvals, nc)) return;
     const u64 n_bases = dim >> (K + nc);
