@@ -28,6 +28,10 @@ namespace qm {
 constexpr int kCdfThreads = 1024;
 constexpr int kCdfPer = 4;                       // elements per thread per chunk
 constexpr int kCdfChunk = kCdfThreads * kCdfPer; // 4096 (32 KiB of LDS)
+// A block scan that stops within kSerialTrigger elements of where it started is
+// followed by kSerialRun one-by-one additions (see exact_chunk).
+constexpr int kSerialTrigger = 64;
+constexpr int kSerialRun = 512;
 
 __device__ __forceinline__ double prob_sv(const double2* __restrict__ a, u64 i) {
     const double2 v = a[i];
@@ -139,9 +143,38 @@ __device__ __forceinline__ void exact_chunk(const double2* __restrict__ a, u64 d
     for (int j = threadIdx.x; j < len; j += kCdfThreads)
         P[j] = DM ? prob_dm(a, c0 + j, dim) : prob_sv(a, c0 + j);
     __syncthreads();
-    int pos = 0;  // first element of the chunk not yet summed
+    int pos = 0;     // first element of the chunk not yet summed
+    int serial = 0;  // elements to add one by one before the next block scan
     while (pos < len) {
         const int lo = threadIdx.x * kCdfPer;  // this thread's elements: lo .. lo+kCdfPer-1
+        if (serial > 0) {
+            // Crossings close together (a small running sum doubling every few
+            // elements): the next elements are added one at a time by one
+            // thread, the same IEEE additions as the sequential loop, which is
+            // cheaper than a block scan per crossing.
+            const int end = min(len, pos + serial);
+            if (threadIdx.x == 0) {
+                double t = s;
+                for (int j0 = pos; j0 < end; j0 += 16) {
+                    double v[16];  // loaded together so the LDS latency is paid once per 16
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) v[k] = j0 + k < end ? P[j0 + k] : 0.0;
+#pragma unroll
+                    for (int k = 0; k < 16; ++k)
+                        if (j0 + k < end) {
+                            t = t + v[k];
+                            cdf[c0 + j0 + k + 1] = t;
+                        }
+                }
+                s_bcast = t;
+            }
+            __syncthreads();
+            s = s_bcast;
+            __syncthreads();
+            pos = end;
+            serial = 0;
+            continue;
+        }
         if (!isfinite(s)) {
             // inf/NaN: finish the chunk with plain sequential additions
             // (only reachable from a non-finite amplitude).
@@ -241,6 +274,7 @@ __device__ __forceinline__ void exact_chunk(const double2* __restrict__ a, u64 d
             __syncthreads();
             s = s_bcast;
             __syncthreads();
+            if (first - pos < kSerialTrigger) serial = kSerialRun;
             pos = first + 1;
         }
     }
