@@ -429,20 +429,36 @@ def dm_sweep(args, lib_mod, hip):
     loop_ms = (time.perf_counter() - t0) * 1e3
     qa.set_synchronous(True)
     tr_loop = dmst.get_squared_norm()
-    # HBM passes the loop issues, by its grouping rule: a run of gates/channels on
-    # at most two qubits together is one pass (a lone gate on one or two qubits
-    # too); a gate on more qubits is two passes (rows, then columns)
-    passes, cur = 0, set()
+    # HBM passes the loop issues, by its grouping rule: three to five
+    # single-qubit channels in a row on distinct qubits are one pass; otherwise a
+    # run of gates/channels on at most two qubits together is one pass (a lone
+    # gate on one or two qubits too); a gate on more qubits is two passes (rows,
+    # then columns)
+    seq = []
     for g in gates:
-        for part in [set(g.targets)] + [{q} for q in g.targets]:
-            if len(part) > 2:
-                passes += (1 if cur else 0) + 2
-                cur = set()
-                continue
-            if len(cur | part) > 2:
-                passes += 1
-                cur = set()
-            cur |= part
+        seq.append(("gate", set(g.targets)))
+        seq += [("chan", {q}) for q in g.targets]
+    passes, cur, k = 0, set(), 0
+    while k < len(seq):
+        run = 0
+        while k + run < len(seq) and run < 5 and seq[k + run][0] == "chan" and \
+                not any(seq[k + run][1] <= seq[k + j][1] for j in range(run)):
+            run += 1
+        if run >= 3:
+            passes += (1 if cur else 0) + 1
+            cur = set()
+            k += run
+            continue
+        part = seq[k][1]
+        k += 1
+        if len(part) > 2:
+            passes += (1 if cur else 0) + 2
+            cur = set()
+            continue
+        if len(cur | part) > 2:
+            passes += 1
+            cur = set()
+        cur |= part
     passes += 1 if cur else 0
     moved = passes * 32 * dim * dim
     return {"qubits": n, "gates": len(gates), "channels": ops - len(gates), "ms": round(ms, 2),

@@ -258,6 +258,14 @@ QM_EXPORT void dm_superoperator_host(const UINT* target_qubit_index_list, UINT t
 /* Single-qubit depolarizing channel with probability p (DepolarizingNoise,
  * src/cppsim/gate_factory.cpp:454-467) in one pass. */
 QM_EXPORT void dm_depolarizing_host(UINT target_qubit_index, double p, void* state, ITYPE dim, void* stream, unsigned int device_number);
+/* Single-qubit channels on distinct qubits (1..4 of them), each given by its
+ * 4 x 4 superoperator (row index = row bit * 2 + column bit, as for
+ * dm_superoperator_host), applied together in one pass.  Each superoperator
+ * must map {00, 11} and {01, 10} onto themselves (Pauli channels, amplitude and
+ * phase damping); anything else is QM_ERR_INVALID_ARGUMENT.  Replaces one
+ * QuantumGate_Probabilistic / CPTP update per channel on a density matrix
+ * (src/cppsim/gate_general.hpp:265-285), e.g. the noise after a multi-qubit gate. */
+QM_EXPORT void qm_dm_single_qubit_channels_host(const UINT* qubits, UINT count, const qm_complex* superops, void* state, ITYPE dim, void* stream, unsigned int device_number);
 QM_EXPORT double dm_state_norm_squared_host(void* state, ITYPE dim, void* stream, unsigned int device_number); /* stat_ops_dm.cpp:12-26 */
 QM_EXPORT double dm_measurement_distribution_entropy_host(void* state, ITYPE dim, void* stream, unsigned int device_number); /* stat_ops_dm.cpp:28-47 */
 QM_EXPORT double dm_M0_prob_host(UINT target_qubit_index, void* state, ITYPE dim, void* stream, unsigned int device_number); /* stat_ops_dm.cpp:49-63 */

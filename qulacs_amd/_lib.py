@@ -131,6 +131,7 @@ _SIGS: dict[str, tuple] = {
     "dm_multi_qubit_control_multi_qubit_dense_matrix_gate_host": (None, [PU, PU, UINT, PU, UINT, PC, VP, ITYPE, VP, UINT]),
     "dm_superoperator_host": (None, [PU, UINT, PC, VP, ITYPE, VP, UINT]),
     "dm_depolarizing_host": (None, [UINT, D, VP, ITYPE, VP, UINT]),
+    "qm_dm_single_qubit_channels_host": (None, [PU, UINT, PC, VP, ITYPE, VP, UINT]),
     "dm_state_norm_squared_host": (D, [VP, ITYPE, VP, UINT]),
     "dm_measurement_distribution_entropy_host": (D, [VP, ITYPE, VP, UINT]),
     "dm_M0_prob_host": (D, [UINT, VP, ITYPE, VP, UINT]),

@@ -1788,11 +1788,28 @@ bool dm_fusable(const QuantumGateBase* g) {
 }
 }  // namespace
 
-// Density matrices: consecutive gates and channels whose qubits fit in 3 together are
+// Density matrices: consecutive gates and channels whose qubits fit in 2 together are
 // composed on the host into one superoperator and applied in one pass
-// (dm_superoperator_host); a lone gate keeps its own entry point.
+// (dm_superoperator_host); a lone gate keeps its own entry point.  Three to five
+// single-qubit channels in a row on distinct qubits (the noise after a
+// multi-qubit gate) are one pass of qm_dm_single_qubit_channels_host when each
+// maps {00, 11} and {01, 10} onto themselves.
 namespace {
 constexpr size_t kDmFuseQubits = 2;
+constexpr size_t kDmChannelRun = 5;  // channels per qm_dm_single_qubit_channels_host call
+
+// Single-qubit channel whose 4 x 4 superoperator only links 00 with 11 and 01
+// with 10 (row bit * 2 + column bit); its superoperator goes to *S.
+bool orbit_channel(const QuantumGateBase* g, ComplexMatrix* S) {
+    if (!g->is_noise() || g->get_target_index_list().size() != 1 || !g->get_control_index_list().empty() ||
+        !dm_fusable(g))
+        return false;
+    if (!superop_on(g, {g->get_target_index_list()[0]}, S)) return false;
+    static const int cross[8][2] = {{0, 1}, {0, 2}, {3, 1}, {3, 2}, {1, 0}, {1, 3}, {2, 0}, {2, 3}};
+    for (const auto& rc : cross)
+        if ((*S)(rc[0], rc[1]) != CPPCTYPE(0.0, 0.0)) return false;
+    return true;
+}
 }  // namespace
 void QuantumCircuit::update_dm_fused(DensityMatrixMI355* d) {
     std::vector<unsigned> q;
@@ -1814,6 +1831,27 @@ void QuantumCircuit::update_dm_fused(DensityMatrixMI355* d) {
         count = 0;
     };
     for (size_t k = 0; k < gates_.size(); ++k) {
+        {
+            std::vector<unsigned> cq;
+            std::vector<CPPCTYPE> sops;
+            ComplexMatrix S;
+            while (k + cq.size() < gates_.size() && cq.size() < kDmChannelRun &&
+                   orbit_channel(gates_[k + cq.size()], &S)) {
+                const unsigned t = gates_[k + cq.size()]->get_target_index_list()[0];
+                if (std::find(cq.begin(), cq.end(), t) != cq.end()) break;
+                cq.push_back(t);
+                for (size_t r = 0; r < 4; ++r)
+                    for (size_t c = 0; c < 4; ++c) sops.push_back(S(r, c));
+            }
+            if (cq.size() >= 3) {
+                flush();
+                qm_dm_single_qubit_channels_host(cq.data(), (UINT)cq.size(), qc(sops.data()), d->data(), d->dim,
+                                                 d->get_cuda_stream(), d->device_number);
+                check_abi("update_quantum_state");
+                k += cq.size() - 1;
+                continue;
+            }
+        }
         const QuantumGateBase* g = gates_[k];
         const auto qs = sorted_union({g->get_target_index_list(), g->get_control_index_list()});
         if (qs.size() > kDmFuseQubits || !dm_fusable(g)) {

@@ -15,6 +15,7 @@
 //     gate whose matrix is the superoperator sum_i K_i (x) conj(K_i), replacing
 //     the copy / gate / scale / add loop of QuantumGate_Probabilistic on a
 //     density matrix (src/cppsim/gate_general.hpp:265-285, about ten passes).
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <vector>
@@ -202,6 +203,145 @@ static void dm_unitary(const u32* cl, const u32* cvl, u32 nc, const u32* tl, u32
     dense_any(ct, nt, nc ? cl : nullptr, cvl, nc, mc.data(), state, flat, stream, dev, what);
 }
 
+// ------------------------------------------------------------------ single-qubit channels in one pass
+// A single-qubit channel whose superoperator S (4 x 4 over member = row bit * 2
+// + column bit) maps {00, 11} onto itself and {01, 10} onto itself -- Pauli
+// channels (depolarizing, dephasing, bit/phase flips), amplitude and phase
+// damping -- only mixes rho[y][x] with rho[y ^ 2^q][x ^ 2^q].  For channels on
+// distinct qubits q_0..q_{K-1} the 2^K entries reached from one entry by such
+// flips ("an orbit") are closed under all K channels, so one thread loads an
+// orbit, applies the K channels as butterflies in registers and stores it: K
+// channels in one pass instead of one pass each.
+//
+// Each orbit is named by one representative entry.  For a channel on a qubit q
+// >= 3 the representative has column bit q = 0 and its row bit q is the
+// channel's "class" (row bit == column bit or not), taken from the top bits of
+// the work index so that it is the same across a wave.  For q < 3 (inside a
+// 128-byte line) the representative has row bit q = 0 and the column bit q is a
+// free bit, so the two halves of every line are handled by neighbouring lanes;
+// the class is then that column bit.  Every wave instruction covers whole
+// lines, and the accesses stream.
+constexpr int kOrbitMax = 5;  // an orbit of 2^K complex values stays in registers
+template <int K>
+struct OrbitArgs {
+    u64 ins[2 * K];         // (1 << p) - 1 for the ascending inserted positions
+    int n_ins;
+    u64 colbit[K], rowbit[K];
+    u64 free_items;         // work items per class word
+    double2 blk[K][2][4];   // [channel][class][2 x 2 block: representative's member first]
+};
+
+// Channels 0..L-1 are on qubits below 3 ("low"), L..K-1 on qubits >= 3; the
+// class word holds the high channels' classes, bit k - L for channel k.
+template <int K, int L>
+__global__ __launch_bounds__(256) void k_dm_orbit(double2* __restrict__ s, u64 n_orbits, OrbitArgs<K> a) {
+    constexpr int N = 1 << K;
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n_orbits; i += stride) {
+        // free_items is a multiple of 64, so the class word is uniform across the wave
+        const unsigned word = __builtin_amdgcn_readfirstlane((unsigned)(i / a.free_items));
+        u64 b = i % a.free_items;
+#pragma unroll
+        for (int k = 0; k < 2 * K; ++k)
+            if (k < a.n_ins) b = (b & a.ins[k]) | ((b & ~a.ins[k]) << 1);
+#pragma unroll
+        for (int k = L; k < K; ++k)
+            if ((word >> (k - L)) & 1) b |= a.rowbit[k];
+        // Member x is b ^ (XOR over set bits k of x of colbit | rowbit).  The
+        // flipped bits of b are the same in every lane of the wave except the
+        // column bits of low channels, so member x = lane 0's b + a uniform delta
+        // + a small per-lane offset that depends on x's low-channel bits only.
+        const u64 b0 = ((u64)__builtin_amdgcn_readfirstlane((unsigned)(b >> 32)) << 32) |
+                       (u64)__builtin_amdgcn_readfirstlane((unsigned)b);
+        unsigned lane_bytes[1 << L];
+#pragma unroll
+        for (int xl = 0; xl < (1 << L); ++xl) {
+            long long adj = (long long)(b - b0);
+#pragma unroll
+            for (int k = 0; k < L; ++k)
+                if ((xl >> k) & 1) adj += (b & a.colbit[k]) ? -(long long)a.colbit[k] : (long long)a.colbit[k];
+            lane_bytes[xl] = (unsigned)(adj * (long long)sizeof(double2));
+        }
+        auto member = [&](int x) {
+            long long uni = 0;
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if ((x >> k) & 1) {
+                    if (k < L)
+                        uni += (long long)a.rowbit[k];
+                    else
+                        uni += (long long)a.colbit[k] +
+                               (((word >> (k - L)) & 1) ? -(long long)a.rowbit[k] : (long long)a.rowbit[k]);
+                }
+            return (double2*)((char*)(s + (long long)b0 + uni) + lane_bytes[x & ((1 << L) - 1)]);
+        };
+        double2 v[N];
+#pragma unroll
+        for (int x = 0; x < N; ++x) v[x] = ld_stream(member(x));
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            // class: for a low channel the representative's column bit, else the word bit
+            const int c = k < L ? ((b & a.colbit[k]) ? 1 : 0) : (int)((word >> (k - L)) & 1);
+            const double2 m0 = a.blk[k][c][0], m1 = a.blk[k][c][1], m2 = a.blk[k][c][2], m3 = a.blk[k][c][3];
+#pragma unroll
+            for (int x = 0; x < N; ++x) {
+                if ((x >> k) & 1) continue;
+                const double2 p = v[x], q = v[x | (1 << k)];
+                v[x] = cmac(cmul(m0, p), m1, q);
+                v[x | (1 << k)] = cmac(cmul(m2, p), m3, q);
+            }
+        }
+#pragma unroll
+        for (int x = 0; x < N; ++x) st_stream(member(x), v[x]);
+    }
+}
+
+// Channels in `q` (any order); returns false when the matrix is too small for
+// whole-wave classes (the caller then applies the channels one by one).
+template <int K>
+bool launch_orbit(double2* s, u64 flat, int n, const u32* q, const qm_complex* S, hipStream_t st) {
+    // low channels first (the channels act on distinct qubits and commute)
+    int order[K], L = 0;
+    for (int k = 0; k < K; ++k)
+        if (q[k] < kLineQubits) order[L++] = k;
+    int h = L;
+    for (int k = 0; k < K; ++k)
+        if (q[k] >= kLineQubits) order[h++] = k;
+    OrbitArgs<K> a;
+    u32 pos[2 * K];
+    int np = 0;
+    for (int j = 0; j < K; ++j) {
+        const u32 qq = q[order[j]];
+        a.colbit[j] = 1ULL << qq;
+        a.rowbit[j] = 1ULL << (qq + n);
+        pos[np++] = qq + n;
+        if (j >= L) pos[np++] = qq;
+        const qm_complex* m = S + 16 * order[j];
+        auto c = [&](int r, int col) { return make_double2(m[4 * r + col].re, m[4 * r + col].im); };
+        // class 0: representative 00, partner 11
+        a.blk[j][0][0] = c(0, 0); a.blk[j][0][1] = c(0, 3); a.blk[j][0][2] = c(3, 0); a.blk[j][0][3] = c(3, 3);
+        if (j >= L) {  // class 1: representative 10 (row 1, column 0), partner 01
+            a.blk[j][1][0] = c(2, 2); a.blk[j][1][1] = c(2, 1); a.blk[j][1][2] = c(1, 2); a.blk[j][1][3] = c(1, 1);
+        } else {       // class 1: representative 01 (row 0, column 1), partner 10
+            a.blk[j][1][0] = c(1, 1); a.blk[j][1][1] = c(1, 2); a.blk[j][1][2] = c(2, 1); a.blk[j][1][3] = c(2, 2);
+        }
+    }
+    std::sort(pos, pos + np);
+    a.n_ins = np;
+    for (int k = 0; k < 2 * K; ++k) a.ins[k] = k < np ? (1ULL << pos[k]) - 1 : 0;
+    const u64 n_orbits = flat >> K;
+    a.free_items = n_orbits >> (K - L);
+    if (a.free_items % 64 != 0) return false;
+    const unsigned grid = grid_for(n_orbits, 256);
+    switch (L) {
+        case 0: k_dm_orbit<K, 0><<<grid, 256, 0, st>>>(s, n_orbits, a); break;
+        case 1: k_dm_orbit<K, 1><<<grid, 256, 0, st>>>(s, n_orbits, a); break;
+        case 2: if constexpr (K >= 2) k_dm_orbit<K, 2><<<grid, 256, 0, st>>>(s, n_orbits, a); break;
+        default: if constexpr (K >= 3) k_dm_orbit<K, 3><<<grid, 256, 0, st>>>(s, n_orbits, a); break;
+    }
+    return true;
+}
+
 }  // namespace qm
 
 using namespace qm;
@@ -267,6 +407,43 @@ void dm_depolarizing_host(UINT t, double p, void* state, ITYPE dim, void* stream
         {b, 0}, {0, 0}, {0, 0}, {a, 0},
     };
     dm_superoperator_host(&t, 1, S, state, dim, stream, dev);
+}
+
+void qm_dm_single_qubit_channels_host(const UINT* qubits, UINT count, const qm_complex* superops, void* state,
+                                      ITYPE dim, void* stream, unsigned dev) {
+    const char* what = "qm_dm_single_qubit_channels_host";
+    int n;
+    if (!dm_ok(state, dim, &n, what)) return;
+    QM_CHECK_ARG(qubits && superops && count >= 1 && count <= (UINT)kOrbitMax && count <= (UINT)n,
+                 "%s: %u channels (1..%d, at most the qubit count)", what, count, kOrbitMax);
+    if (!check_qubits(qubits, count, n, what)) return;
+    for (UINT k = 0; k < count; ++k) {
+        const qm_complex* m = superops + 16 * k;
+        // entries linking {00, 11} with {01, 10} must vanish
+        static const int cross[8][2] = {{0, 1}, {0, 2}, {3, 1}, {3, 2}, {1, 0}, {1, 3}, {2, 0}, {2, 3}};
+        for (const auto& rc : cross) {
+            const qm_complex z = m[4 * rc[0] + rc[1]];
+            QM_CHECK_ARG(z.re == 0.0 && z.im == 0.0, "%s: channel %u mixes diagonal and off-diagonal entries", what, k);
+        }
+    }
+    if (!use_device(dev)) return;
+    hipStream_t st = stream_of(stream);
+    double2* s = (double2*)state;
+    const u64 flat = dim * dim;
+    bool done = false;
+    switch (count) {
+        case 1: done = launch_orbit<1>(s, flat, n, qubits, superops, st); break;
+        case 2: done = launch_orbit<2>(s, flat, n, qubits, superops, st); break;
+        case 3: done = launch_orbit<3>(s, flat, n, qubits, superops, st); break;
+        case 4: done = launch_orbit<4>(s, flat, n, qubits, superops, st); break;
+        default: done = launch_orbit<5>(s, flat, n, qubits, superops, st); break;
+    }
+    if (!done) {  // small matrices: one superoperator pass per channel
+        for (UINT k = 0; k < count; ++k)
+            dm_superoperator_host(qubits + k, 1, superops + 16 * k, state, dim, stream, dev);
+        return;
+    }
+    finish_update(st, what);
 }
 
 double dm_state_norm_squared_host(void* state, ITYPE dim, void* stream, unsigned dev) {
