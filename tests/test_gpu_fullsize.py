@@ -199,6 +199,11 @@ def test_33q_index_math_above_32_bits():
     steps.append((gate.PauliRotation([4, 32], [1, 3], th), [4, 32],
                   np.cos(th / 2) * np.eye(4) + 1j * np.sin(th / 2) * pz, ()))
     steps.append((gate.CZ(32, 31), [31], np.diag([1.0, -1.0]).astype(complex), (32,)))
+    # qubits below 3 with a partial operation or a control there: the line kernel
+    steps.append((gate.CZ(1, 32), [32], np.diag([1.0, -1.0]).astype(complex), (1,)))
+    steps.append((gate.CNOT(2, 31), [31], X, (2,)))
+    steps.append((gate.S(0), [0], np.diag([1.0, 1j]), ()))
+    steps.append((gate.CZ(0, 1), [1], np.diag([1.0, -1.0]).astype(complex), (0,)))
     for g, t, m, ctl in steps:
         print("gate", g.get_name(), t, flush=True)
         g.update_quantum_state(s)
