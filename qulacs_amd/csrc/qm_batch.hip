@@ -54,6 +54,9 @@ constexpr u32 kLowFixed = 4;    // qubits 0..3 are local in every stage
 // 128 KiB tile still has 4 waves per SIMD to hide LDS latency.
 template <int K>
 constexpr u32 block_threads() { return K >= 4 ? 512u : 1024u; }
+// register blocks per thread in the pass loop (see run_passes)
+template <int K>
+constexpr int batch_blocks() { return 1; }
 constexpr u32 kMaxBlockThreads = 1024;
 constexpr u32 kMinQubits = 4;   // smaller circuits run on 4 qubits (extra qubits stay |0>)
 
@@ -89,7 +92,9 @@ __host__ __forceinline__ void set_byte(u64& w, u32 i, u32 v) { w = (w & ~(0xffUL
 struct DPass {
     unsigned char pos[4];  // block bits (local positions, ascending)
     u32 nsub, sub0;
+    u32 flags;             // kPassHasDiag: a DIAG sub-operation (one register block per thread)
 };
+constexpr u32 kPassHasDiag = 1;
 struct DSub {
     u32 type;
     u32 a;     // packed bytes: D1/D2 block-bit indices; DIAG/CZ local positions
@@ -211,16 +216,26 @@ __device__ __forceinline__ void pair_update2(double2& x0, double2& y0, double2& 
           [m3r] "s"(m3.x), [m3i] "s"(m3.y));
 }
 
-template <int K, int P>
-__device__ __forceinline__ void sub_d1(double2* v, const double2* mp) {
+// One 2x2 matrix on block bit P of each of the G register blocks of a thread.
+// With two blocks, the pair updates interleave pair q of block 0 with pair q of
+// block 1; with one block, pairs q and q + 1.
+template <int K, int P, int G>
+__device__ __forceinline__ void sub_d1(double2 (&v)[G][1 << K], const double2* mp) {
     const double2 m0 = mp[0], m1 = mp[1], m2 = mp[2], m3 = mp[3];
-    // the 2^(K-1) pairs (j, j | 2^P), two at a time
+    // the 2^(K-1) pairs (j, j | 2^P)
     constexpr int NP = 1 << (K - 1);
     auto pair_j = [](int q) { return ((q >> P) << (P + 1)) | (q & ((1 << P) - 1)); };
+    if constexpr (G == 2) {
 #pragma unroll
-    for (int q = 0; q < NP; q += 2)
-        pair_update2(v[pair_j(q)], v[pair_j(q) | (1 << P)], v[pair_j(q + 1)], v[pair_j(q + 1) | (1 << P)], m0, m1, m2,
-                     m3);
+        for (int q = 0; q < NP; ++q)
+            pair_update2(v[0][pair_j(q)], v[0][pair_j(q) | (1 << P)], v[1][pair_j(q)], v[1][pair_j(q) | (1 << P)], m0,
+                         m1, m2, m3);
+    } else {
+#pragma unroll
+        for (int q = 0; q < NP; q += 2)
+            pair_update2(v[0][pair_j(q)], v[0][pair_j(q) | (1 << P)], v[0][pair_j(q + 1)],
+                         v[0][pair_j(q + 1) | (1 << P)], m0, m1, m2, m3);
+    }
 }
 
 template <int K, int P0, int P1>
@@ -332,124 +347,167 @@ __device__ __forceinline__ u64 tile_index(u64 tb, const TI* hi_tab, u32 i) {
     return tb + hi_tab[i >> kLowFixed] + (i & ((1u << kLowFixed) - 1));
 }
 
-// One LAYER sub-operation on the register block of the thread whose block
-// base (local index with zeros at the block positions) is base.
-template <int K>
-__device__ __forceinline__ void apply_layer(double2* v, const DSub& sub, u32 base, const double2* __restrict__ pool) {
+// One LAYER sub-operation on the G register blocks of a thread, block g with
+// block base base[g] (local index with zeros at the block positions).  The
+// sub-operation is decoded once for all blocks.
+template <int K, int G>
+__device__ __forceinline__ void apply_layer(double2 (&v)[G][1 << K], const DSub& sub, const u32 (&base)[G],
+                                            const double2* __restrict__ pool) {
     constexpr int D = 1 << K;
     const u32 nit = (sub.type >> 12) & 7u, d1 = (sub.type >> 8) & 15u;
     if (nit || (sub.type >> 16)) {
-        // bit j of sgn: amplitude j of the register block changes sign; base is
-        // zero at the block positions, so a local position p outside the block
-        // is (base >> p) & 1 for the whole block
-        u32 sgn = sub.type >> 16;
+        // bit j of sgn[g]: amplitude j of block g changes sign; base is zero at
+        // the block positions, so a local position p outside the block is
+        // (base >> p) & 1 for the whole block
+        u32 sgn[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) sgn[g] = sub.type >> 16;
         for (u32 c = 0; c < nit; ++c) {
             const u32 w = (c < 2 ? sub.a : sub.m) >> (16 * (c & 1));
-            const u32 b0 = 0u - ((base >> (w & 0xffu)) & 1u);
-            if (w & 0x8000u)
-                sgn ^= b0 & (0u - ((base >> ((w >> 8) & 0x7fu)) & 1u));
-            else
-                sgn ^= b0 & (u32)(kBlockPatterns >> (16 * ((w >> 8) & 3u)));
+            const u32 p0 = w & 0xffu;
+            if (w & 0x8000u) {
+                const u32 p1 = (w >> 8) & 0x7fu;
+#pragma unroll
+                for (int g = 0; g < G; ++g) sgn[g] ^= 0u - ((base[g] >> p0) & (base[g] >> p1) & 1u);
+            } else {
+                const u32 pat = (u32)(kBlockPatterns >> (16 * ((w >> 8) & 3u)));
+#pragma unroll
+                for (int g = 0; g < G; ++g) sgn[g] ^= (0u - ((base[g] >> p0) & 1u)) & pat;
+            }
         }
 #pragma unroll
-        for (int j = 0; j < D; ++j) {
-            // flip the sign bits of both components where bit j of sgn is set
-            const int f = (int)((sgn << (31 - j)) & 0x80000000u);
-            v[j].x = __hiloint2double(__double2hiint(v[j].x) ^ f, __double2loint(v[j].x));
-            v[j].y = __hiloint2double(__double2hiint(v[j].y) ^ f, __double2loint(v[j].y));
-        }
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                // flip the sign bits of both components where bit j of sgn is set
+                const int f = (int)((sgn[g] << (31 - j)) & 0x80000000u);
+                v[g][j].x = __hiloint2double(__double2hiint(v[g][j].x) ^ f, __double2loint(v[g][j].x));
+                v[g][j].y = __hiloint2double(__double2hiint(v[g][j].y) ^ f, __double2loint(v[g][j].y));
+            }
     }
     const double2* mp = pool + sub.data;
     if (d1 & 1u) {
-        sub_d1<K, 0>(v, mp);
+        sub_d1<K, 0, G>(v, mp);
         mp += 4;
     }
     if (d1 & 2u) {
-        sub_d1<K, 1>(v, mp);
+        sub_d1<K, 1, G>(v, mp);
         mp += 4;
     }
     if constexpr (K > 2)
         if (d1 & 4u) {
-            sub_d1<K, 2>(v, mp);
+            sub_d1<K, 2, G>(v, mp);
             mp += 4;
         }
     if constexpr (K > 3)
-        if (d1 & 8u) sub_d1<K, 3>(v, mp);
+        if (d1 & 8u) sub_d1<K, 3, G>(v, mp);
 }
 
-// Applies passes [pass0, pass0 + npass) to the 2^lq amplitudes in LDS; ends
-// with a barrier after every pass.
-template <int K>
+// The sub-operations of one pass on the G register blocks of groups gi0,
+// gi0 + T, ..., read from and written back to LDS.  soff[j]: swizzled slot
+// offset of register-block index j (slot_of is linear over GF(2), so the slot
+// of base | off[j] is slot_of(base) ^ soff[j]).
+template <int K, u32 T, int G>
+__device__ __forceinline__ void pass_groups(double2* st, const DPass& ps, const u32 (&soff)[1 << K], u32 gi0,
+                                            const DSub* __restrict__ subs, const double2* __restrict__ pool) {
+    constexpr int D = 1 << K;
+    u32 base[G], sb[G];
+    double2 v[G][D];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        u32 b = gi0 + (u32)g * T;
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            const u32 q = ps.pos[i];
+            b = (b & ((1u << q) - 1)) | ((b >> q) << (q + 1));
+        }
+        base[g] = b;
+        sb[g] = slot_of(b);
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int j = 0; j < D; ++j) v[g][j] = st[sb[g] ^ soff[j]];
+    // LAYER sub-operations run in an inner loop of their own and update the
+    // register blocks in place, so the blocks stay at fixed registers there;
+    // the other kinds are handled between runs of LAYERs.
+    u32 k = 0;
+    while (k < ps.nsub) {
+        DSub sub = subs[ps.sub0 + k];
+        while ((sub.type & 0xffu) == SUB_LAYER) {
+            apply_layer<K, G>(v, sub, base, pool);
+            if (++k == ps.nsub) break;
+            sub = subs[ps.sub0 + k];
+        }
+        if (k == ps.nsub) break;
+        const u32 kind = sub.type & 0xffu;
+        const double2* mp = pool + sub.data;
+        if (kind == SUB_D2) {
+#pragma unroll
+            for (int g = 0; g < G; ++g) apply_d2<K>(v[g], byte_at(sub.a, 0), byte_at(sub.a, 1), mp);
+        } else if (kind == SUB_DK) {
+#pragma unroll
+            for (int g = 0; g < G; ++g) sub_dk<K>(v[g], mp);
+        } else if (G == 1 && kind == SUB_DIAG) {
+            // (passes with a DIAG run with one block per thread: the per-lane
+            // reads of the diagonal for two blocks exhaust the registers)
+            // bit i of the diagonal index: local position p_i, i.e. bit p_i of
+            // base or, for a block position, bit j of the block pattern
+            u32 pat[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const u32 p = byte_at(sub.a, i);
+                u32 pt = 0;
+#pragma unroll
+                for (int b = 0; b < K; ++b) pt |= (ps.pos[b] == p) ? (u32)(kBlockPatterns >> (16 * b)) & 0xffffu : 0u;
+                pat[i] = i < (int)sub.m ? pt : 0u;
+            }
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                u32 tbit[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) tbit[i] = i < (int)sub.m ? (base[g] >> byte_at(sub.a, i)) & 1u : 0u;
+#pragma unroll
+                for (int j = 0; j < D; ++j) {
+                    u32 idx = 0;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) idx |= ((tbit[i] | (pat[i] >> j)) & 1u) << i;
+                    cmul_inplace(v[g][j], mp[idx]);
+                }
+            }
+        }
+        ++k;
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int j = 0; j < D; ++j) st[sb[g] ^ soff[j]] = v[g][j];
+}
+
+// Applies passes [pass0, pass0 + npass) to the 2^lq amplitudes in LDS with a
+// work-group of T threads; ends with a barrier after every pass.  With G = 2
+// and exactly 2 T groups of 2^K amplitudes, every thread holds two register
+// blocks and decodes each sub-operation once for both.
+template <int K, u32 T, int G>
 __device__ __forceinline__ void run_passes(double2* st, u32 lq, const DPass* __restrict__ passes, u32 pass0,
                                            u32 npass, const DSub* __restrict__ subs,
                                            const double2* __restrict__ pool) {
     constexpr int D = 1 << K;
-    const u32 L = 1u << lq;
+    const u32 ngroups = (1u << lq) >> K;
     for (u32 p = 0; p < npass; ++p) {
         const DPass ps = passes[pass0 + p];
-        u32 off[D];
+        u32 soff[D];
 #pragma unroll
         for (int j = 0; j < D; ++j) {
             u32 o = 0;
 #pragma unroll
             for (int i = 0; i < K; ++i) o |= (u32)((j >> i) & 1) << ps.pos[i];
-            off[j] = o;
+            soff[j] = slot_of(o);
         }
-        for (u32 g = threadIdx.x; g < (L >> K); g += blockDim.x) {
-            u32 base = g;
-#pragma unroll
-            for (int i = 0; i < K; ++i) {
-                const u32 q = ps.pos[i];
-                base = (base & ((1u << q) - 1)) | ((base >> q) << (q + 1));
-            }
-            const u32 sb = slot_of(base);
-            double2 v[D];
-#pragma unroll
-            for (int j = 0; j < D; ++j) v[j] = st[sb ^ slot_of(off[j])];
-            // LAYER sub-operations run in an inner loop of their own and update the
-            // register block in place, so the block stays at fixed registers there;
-            // the other kinds are handled between runs of LAYERs.
-            u32 k = 0;
-            while (k < ps.nsub) {
-                DSub sub = subs[ps.sub0 + k];
-                while ((sub.type & 0xffu) == SUB_LAYER) {
-                    apply_layer<K>(v, sub, base, pool);
-                    if (++k == ps.nsub) break;
-                    sub = subs[ps.sub0 + k];
-                }
-                if (k == ps.nsub) break;
-                const u32 kind = sub.type & 0xffu;
-                const double2* mp = pool + sub.data;
-                if (kind == SUB_D2) {
-                    apply_d2<K>(v, byte_at(sub.a, 0), byte_at(sub.a, 1), mp);
-                } else if (kind == SUB_DK) {
-                    sub_dk<K>(v, mp);
-                } else if (kind == SUB_DIAG) {
-                    // bit i of the diagonal index: local position p_i, i.e. bit p_i of
-                    // base or, for a block position, bit j of the block pattern
-                    u32 tbit[4], pat[4];
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const u32 p = byte_at(sub.a, i);
-                        tbit[i] = i < (int)sub.m ? (base >> p) & 1u : 0u;
-                        u32 pt = 0;
-#pragma unroll
-                        for (int b = 0; b < K; ++b)
-                            pt |= (ps.pos[b] == p) ? (u32)(kBlockPatterns >> (16 * b)) & 0xffffu : 0u;
-                        pat[i] = i < (int)sub.m ? pt : 0u;
-                    }
-#pragma unroll
-                    for (int j = 0; j < D; ++j) {
-                        u32 idx = 0;
-#pragma unroll
-                        for (int i = 0; i < 4; ++i) idx |= ((tbit[i] | (pat[i] >> j)) & 1u) << i;
-                        cmul_inplace(v[j], mp[idx]);
-                    }
-                }
-                ++k;
-            }
-#pragma unroll
-            for (int j = 0; j < D; ++j) st[sb ^ slot_of(off[j])] = v[j];
+        if (G == 2 && ngroups == 2 * T && !(ps.flags & kPassHasDiag)) {
+            pass_groups<K, T, G>(st, ps, soff, threadIdx.x, subs, pool);
+        } else {
+            for (u32 g = threadIdx.x; g < ngroups; g += T) pass_groups<K, T, 1>(st, ps, soff, g, subs, pool);
         }
         __syncthreads();
     }
@@ -457,40 +515,41 @@ __device__ __forceinline__ void run_passes(double2* st, u32 lq, const DPass* __r
 
 // Tiles of one stage of a batched circuit held in device memory (every stage
 // after the first), the next tile loaded into registers while the passes run
-// on the current one.  P amplitudes per thread, T threads.
+// on the current one.  P amplitudes per thread, T threads.  The loop is laid
+// out as in stage_tiles: unconditional loads of the next tile at the top, the
+// LDS fill from them after the stores of the current tile.
 constexpr u32 kBatchTile = 1u << kMaxLocal;
-template <int K, u32 P, u32 T>
+template <int K, u32 P, u32 T, int G>
 __device__ __forceinline__ void batch_tiles(double2* st, double2* __restrict__ gs, const u32* hi_tab, u32 lq, u32 tl,
                                             const DStage& sg, const DPass* __restrict__ passes,
                                             const DSub* __restrict__ subs, const double2* __restrict__ pool) {
     const u32 tid = threadIdx.x, ntiles = 1u << tl;
-    double rx[P], ry[P];
     {
         const u64 tb = tile_base(0, tl, sg);
 #pragma unroll
-        for (u32 k = 0; k < P; ++k) {
-            const double2 a = gs[tile_index(tb, hi_tab, tid + k * T)];
-            rx[k] = a.x;
-            ry[k] = a.y;
-        }
-    }
-    for (u32 t = 0; t < ntiles; ++t) {
-#pragma unroll
-        for (u32 k = 0; k < P; ++k) st[slot_of(tid + k * T)] = make_double2(rx[k], ry[k]);
+        for (u32 k = 0; k < P; ++k) st[slot_of(tid + k * T)] = gs[tile_index(tb, hi_tab, tid + k * T)];
         __syncthreads();
-        if (t + 1 < ntiles) {
+    }
+    for (u32 t = 0;; ++t) {
+        double rx[P], ry[P];
+        const bool more = t + 1 < ntiles;
+        {
             const u64 tb = tile_base(t + 1, tl, sg);
 #pragma unroll
             for (u32 k = 0; k < P; ++k) {
-                const double2 a = gs[tile_index(tb, hi_tab, tid + k * T)];
+                const double2 a = gs[more ? tile_index(tb, hi_tab, tid + k * T) : 0];
                 rx[k] = a.x;
                 ry[k] = a.y;
             }
         }
-        run_passes<K>(st, lq, passes, sg.pass0, sg.npass, subs, pool);
+        run_passes<K, T, G>(st, lq, passes, sg.pass0, sg.npass, subs, pool);
         const u64 tb = tile_base(t, tl, sg);
 #pragma unroll
         for (u32 k = 0; k < P; ++k) gs[tile_index(tb, hi_tab, tid + k * T)] = st[slot_of(tid + k * T)];
+        __syncthreads();
+        if (!more) break;
+#pragma unroll
+        for (u32 k = 0; k < P; ++k) st[slot_of(tid + k * T)] = make_double2(rx[k], ry[k]);
         __syncthreads();
     }
 }
@@ -527,9 +586,9 @@ __global__ __launch_bounds__(block_threads<K>()) void k_batch(
                 build_hi_tab(hi_tab, lq, sg);
                 __syncthreads();
             }
-            if (K <= 3 && tl > 0 && s > 0 && L == kBatchTile * bd) {  // K = 4 has no registers to spare
-                batch_tiles<K, kBatchTile / block_threads<K>(), block_threads<K>()>(st, gs, hi_tab, lq, tl, sg,
-                                                                                     passes, subs, pool);
+            if (K <= 3 && tl > 0 && s > 0 && L == kBatchTile) {  // K = 4 has no registers to spare
+                batch_tiles<K, kBatchTile / block_threads<K>(), block_threads<K>(), batch_blocks<K>()>(
+                    st, gs, hi_tab, lq, tl, sg, passes, subs, pool);
                 continue;
             }
             for (u32 t = 0; t < (1u << tl); ++t) {
@@ -541,7 +600,7 @@ __global__ __launch_bounds__(block_threads<K>()) void k_batch(
                     for (u32 i = tid; i < L; i += bd) st[slot_of(i)] = gs[tile_index(tb, hi_tab, i)];
                 }
                 __syncthreads();
-                run_passes<K>(st, lq, passes, sg.pass0, sg.npass, subs, pool);
+                run_passes<K, block_threads<K>(), batch_blocks<K>()>(st, lq, passes, sg.pass0, sg.npass, subs, pool);
                 if (tl > 0) {
                     for (u32 i = tid; i < L; i += bd) gs[tile_index(tb, hi_tab, i)] = st[slot_of(i)];
                 } else if (keep) {
@@ -565,47 +624,64 @@ __global__ __launch_bounds__(block_threads<K>()) void k_batch(
 // touch the same amplitudes.  The next tile is loaded into registers while the
 // passes run on the current one (a barrier does not wait for global loads), so
 // the HBM traffic of consecutive tiles overlaps the arithmetic.
+//
+// Work-group size and register blocks per thread: at K = 3, 512 threads with
+// two 8-amplitude blocks each (a 13-qubit tile is 1024 groups), so every
+// sub-operation is decoded once per two blocks and the FP64 chains of the two
+// interleave; K = 2: 1024 threads with two 4-amplitude blocks; K = 4: 512
+// threads with one 16-amplitude block.
 template <int K>
-constexpr u32 stage_threads() { return K >= 4 ? 512u : 1024u; }
+constexpr u32 stage_threads() { return K == 2 ? 1024u : 512u; }
+template <int K>
+constexpr int stage_blocks() { return K <= 3 ? 2 : 1; }
 
-// Tiles of exactly P * kStageThreads amplitudes, P per thread, with prefetch.
+// Tiles of exactly P * T amplitudes, P per thread, with prefetch.  Each
+// iteration issues the loads of the next tile, runs the passes on the tile in
+// LDS, stores it and only then moves the prefetched registers into LDS, so the
+// wait for the loads follows the stores in program order and lets the stores
+// stay in flight (the vector memory counter returns in order); with the LDS
+// fill at the top of the loop the compiler's wait there has to hold for the
+// first entry too and would drain the previous tile's stores.  The loads are
+// unconditional (after the last tile every lane reads amplitude 0, one cache
+// line, and the values are not used): a conditional load merged with the old
+// values makes the compiler copy the loaded registers, i.e. wait, at once.
 template <int K, u32 P, u32 T = stage_threads<K>()>
 __device__ __forceinline__ void stage_tiles(double2* __restrict__ state, double2* st, const u64* hi_tab, u32 lq, u32 tl,
                                             const DStage& sg, const DPass* __restrict__ passes,
                                             const DSub* __restrict__ subs, const double2* __restrict__ pool) {
     const u32 tid = threadIdx.x;
     const u64 ntiles = 1ULL << tl;
-    double rx[P], ry[P];  // the prefetched tile (separate scalars keep it in registers)
     u64 t = blockIdx.x;
     if (t >= ntiles) return;
     {
         const u64 tb = tile_base(t, tl, sg);
 #pragma unroll
-        for (u32 k = 0; k < P; ++k) {
-            const double2 a = state[tile_index(tb, hi_tab, tid + k * T)];
-            rx[k] = a.x;
-            ry[k] = a.y;
-        }
-    }
-    for (; t < ntiles; t += gridDim.x) {
-#pragma unroll
-        for (u32 k = 0; k < P; ++k) st[slot_of(tid + k * T)] = make_double2(rx[k], ry[k]);
+        for (u32 k = 0; k < P; ++k) st[slot_of(tid + k * T)] = state[tile_index(tb, hi_tab, tid + k * T)];
         __syncthreads();
+    }
+    for (;;) {
+        double rx[P], ry[P];  // the prefetched tile (separate scalars keep it in registers)
         const u64 tn = t + gridDim.x;
-        if (tn < ntiles) {
+        const bool more = tn < ntiles;
+        {
             const u64 tb = tile_base(tn, tl, sg);
 #pragma unroll
             for (u32 k = 0; k < P; ++k) {
-                const double2 a = state[tile_index(tb, hi_tab, tid + k * T)];
-                rx[k] = a.x;
-                ry[k] = a.y;
+                const double2 x = state[more ? tile_index(tb, hi_tab, tid + k * T) : 0];
+                rx[k] = x.x;
+                ry[k] = x.y;
             }
         }
-        run_passes<K>(st, lq, passes, sg.pass0, sg.npass, subs, pool);
+        run_passes<K, T, stage_blocks<K>()>(st, lq, passes, sg.pass0, sg.npass, subs, pool);
         const u64 tb = tile_base(t, tl, sg);
 #pragma unroll
         for (u32 k = 0; k < P; ++k) state[tile_index(tb, hi_tab, tid + k * T)] = st[slot_of(tid + k * T)];
+        if (!more) break;
+        __syncthreads();  // the tile has been read out of LDS
+#pragma unroll
+        for (u32 k = 0; k < P; ++k) st[slot_of(tid + k * T)] = make_double2(rx[k], ry[k]);
         __syncthreads();
+        t = tn;
     }
 }
 
@@ -619,22 +695,33 @@ __global__ __launch_bounds__(stage_threads<K>()) void k_stage(double2* __restric
     const u32 L = 1u << lq;
     build_hi_tab(hi_tab, lq, sg);
     __syncthreads();
+    constexpr u32 T = stage_threads<K>();
     // the prefetched tile needs 4 P registers per thread on top of the register
-    // block: it fits beside K <= 3 blocks only
-    if (K <= 3 && L == 8 * stage_threads<K>()) {
-        stage_tiles<K, 8>(state, st, hi_tab, lq, tl, sg, passes, subs, pool);
-    } else if (K <= 3 && L == 4 * stage_threads<K>()) {
-        stage_tiles<K, 4>(state, st, hi_tab, lq, tl, sg, passes, subs, pool);
-    } else {
-        const u64 ntiles = 1ULL << tl;
-        for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
-            const u64 tb = tile_base(t, tl, sg);
-            for (u32 i = threadIdx.x; i < L; i += blockDim.x) st[slot_of(i)] = state[tile_index(tb, hi_tab, i)];
-            __syncthreads();
-            run_passes<K>(st, lq, passes, sg.pass0, sg.npass, subs, pool);
-            for (u32 i = threadIdx.x; i < L; i += blockDim.x) state[tile_index(tb, hi_tab, i)] = st[slot_of(i)];
-            __syncthreads();
+    // blocks: it fits beside K <= 3 blocks only
+    if constexpr (K == 3) {
+        if (L == 16 * T) {
+            stage_tiles<K, 16>(state, st, hi_tab, lq, tl, sg, passes, subs, pool);
+            return;
         }
+    }
+    if constexpr (K <= 3) {
+        if (L == 8 * T) {
+            stage_tiles<K, 8>(state, st, hi_tab, lq, tl, sg, passes, subs, pool);
+            return;
+        }
+        if (L == 4 * T) {
+            stage_tiles<K, 4>(state, st, hi_tab, lq, tl, sg, passes, subs, pool);
+            return;
+        }
+    }
+    const u64 ntiles = 1ULL << tl;
+    for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const u64 tb = tile_base(t, tl, sg);
+        for (u32 i = threadIdx.x; i < L; i += T) st[slot_of(i)] = state[tile_index(tb, hi_tab, i)];
+        __syncthreads();
+        run_passes<K, T, stage_blocks<K>()>(st, lq, passes, sg.pass0, sg.npass, subs, pool);
+        for (u32 i = threadIdx.x; i < L; i += T) state[tile_index(tb, hi_tab, i)] = st[slot_of(i)];
+        __syncthreads();
     }
 }
 
@@ -865,6 +952,7 @@ void schedule_passes(Flat& f, std::vector<HOp> ops, u32 lq, u32 K, DStage& sg) {
             flush();
             DSub sb{};
             if (o.type == 1) {
+                ps.flags |= kPassHasDiag;
                 sb.type = SUB_DIAG;
                 sb.m = (u32)o.q.size();
                 for (size_t k = 0; k < o.q.size(); ++k) sb.a |= o.q[k] << (8 * k);
@@ -1079,7 +1167,7 @@ bool batch_prepare_launch(qm_batch* b, unsigned dev, const char* what) {
     const void* fn = b->K == 2 ? (const void*)k_batch<2> : b->K == 3 ? (const void*)k_batch<3> : (const void*)k_batch<4>;
     if (!hip_ok(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), what)) return false;
     int per_cu = 0, cus = 0;
-    const int threads = (int)(b->K >= 4 ? block_threads<4>() : block_threads<3>());
+    const int threads = (int)(b->K == 2 ? block_threads<2>() : b->K == 3 ? block_threads<3>() : block_threads<4>());
     if (!hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, lds), what)) return false;
     if (!hip_ok(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, (int)dev), what)) return false;
     if (per_cu < 1) {
@@ -1437,7 +1525,7 @@ bool circuit_enqueue(qm_circuit* c, void* state, u64 dim, hipStream_t st, unsign
     if (c->per_cu == 0) {
         int per_cu = 0, cus = 0;
         if (!hip_ok(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), what) ||
-            !hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)(c->K >= 4 ? stage_threads<4>() : stage_threads<3>()), lds), what) ||
+            !hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)(c->K == 2 ? stage_threads<2>() : c->K == 3 ? stage_threads<3>() : stage_threads<4>()), lds), what) ||
             !hip_ok(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, (int)dev), what))
             return false;
         if (per_cu < 1) {

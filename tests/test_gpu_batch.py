@@ -82,7 +82,7 @@ def test_batch_errors():
 
 
 @pytest.mark.parametrize("n,block,local", [(4, 4, 13), (8, 2, 8), (13, 3, 13), (14, 4, 13), (16, 4, 10),
-                                           (20, 3, 12), (22, 4, 13)])
+                                           (20, 3, 12), (22, 4, 13), (18, 3, 13), (16, 2, 13), (15, 2, 12)])
 def test_fused_circuit_matches_oracle(oracle, n, block, local):
     from qulacs_amd import _lib
     from qulacs_amd.batch import FusedCircuit
