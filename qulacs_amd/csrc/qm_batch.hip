@@ -63,17 +63,19 @@ constexpr u32 kMinQubits = 4;   // smaller circuits run on 4 qubits (extra qubit
 // Sub-operation kinds (low byte of DSub::type).
 // LAYER: CZ gates, then one 2x2 matrix on each block bit of a mask.  type bits
 //   8..11: the block-bit mask; bits 16..31: the sign pattern over the register
-//   block of the CZs between two block bits; bits 12..14: the number (<= 4) of
-//   other CZs, 16 bits each in (c < 2 ? a : m) >> 16 (c & 1): bit 15 clear: a
-//   block bit (bits 8..9) and a local position outside the block (bits 0..7);
-//   set: two local positions outside the block (bits 0..7, 8..14).
+//   block of the CZs between two block bits; bit 15: a and m hold, for block
+//   bit i, the mask of local positions outside the block that share a CZ with
+//   it (16 bits each: a = M0 | M1 << 16, m = M2 | M3 << 16), so the sign of
+//   register slot j changes with the parity of base & M_i where bit i of j is
+//   set; bits 12..14: the number (<= 4) of CZs between two positions outside
+//   the block, byte c of pairs = p | q << 4.
 //   data: the matrices (4 complex each) of the mask's bits in ascending order.
 // D2: 4x4 on block bits byte 0 / byte 1 of a; DK: 2^K x 2^K on the block;
 // DIAG: diagonal on the m local positions in the bytes of a.
 enum : u32 { SUB_LAYER = 0, SUB_D2 = 1, SUB_DK = 2, SUB_DIAG = 3 };
 // bits 16 i .. 16 i + 15: the register-block indices j (< 16) with bit i set
 constexpr u64 kBlockPatterns = 0xFF00F0F0CCCCAAAAull;
-constexpr u32 kCzItemsMax = 4;
+constexpr u32 kCzPairsMax = 4;
 
 struct DCirc {
     u32 n, lq, nstage, stage0, term0, nterm;
@@ -93,13 +95,16 @@ struct DPass {
     unsigned char pos[4];  // block bits (local positions, ascending)
     u32 nsub, sub0;
     u32 flags;             // kPassHasDiag: a DIAG sub-operation (one register block per thread)
+    unsigned short soff[16];  // slot_of(offset of register-block index j), computed on the host
 };
 constexpr u32 kPassHasDiag = 1;
 struct DSub {
     u32 type;
-    u32 a;     // packed bytes: D1/D2 block-bit indices; DIAG/CZ local positions
-    u32 m;     // DIAG: qubit count
-    u32 data;  // offset into the complex pool
+    u32 a;      // D2: block-bit indices (bytes); DIAG: local positions (bytes); LAYER: CZ masks
+    u32 m;      // DIAG: qubit count; LAYER: CZ masks
+    u32 data;   // offset into the complex pool
+    u32 pairs;  // LAYER: CZs between two positions outside the block
+    u32 pad[3];
 };
 struct DTerm {
     u64 bf, pf;
@@ -354,26 +359,28 @@ template <int K, int G>
 __device__ __forceinline__ void apply_layer(double2 (&v)[G][1 << K], const DSub& sub, const u32 (&base)[G],
                                             const double2* __restrict__ pool) {
     constexpr int D = 1 << K;
-    const u32 nit = (sub.type >> 12) & 7u, d1 = (sub.type >> 8) & 15u;
-    if (nit || (sub.type >> 16)) {
+    const u32 npair = (sub.type >> 12) & 7u, d1 = (sub.type >> 8) & 15u;
+    if (sub.type >> 12) {
         // bit j of sgn[g]: amplitude j of block g changes sign; base is zero at
         // the block positions, so a local position p outside the block is
         // (base >> p) & 1 for the whole block
         u32 sgn[G];
 #pragma unroll
         for (int g = 0; g < G; ++g) sgn[g] = sub.type >> 16;
-        for (u32 c = 0; c < nit; ++c) {
-            const u32 w = (c < 2 ? sub.a : sub.m) >> (16 * (c & 1));
-            const u32 p0 = w & 0xffu;
-            if (w & 0x8000u) {
-                const u32 p1 = (w >> 8) & 0x7fu;
+        if (sub.type & 0x8000u) {
 #pragma unroll
-                for (int g = 0; g < G; ++g) sgn[g] ^= 0u - ((base[g] >> p0) & (base[g] >> p1) & 1u);
-            } else {
-                const u32 pat = (u32)(kBlockPatterns >> (16 * ((w >> 8) & 3u)));
+            for (int i = 0; i < K; ++i) {
+                const u32 mi = ((i < 2 ? sub.a : sub.m) >> (16 * (i & 1))) & 0xffffu;
+                constexpr u32 kPat[4] = {(u32)(kBlockPatterns & 0xffffu), (u32)((kBlockPatterns >> 16) & 0xffffu),
+                                         (u32)((kBlockPatterns >> 32) & 0xffffu), (u32)(kBlockPatterns >> 48)};
 #pragma unroll
-                for (int g = 0; g < G; ++g) sgn[g] ^= (0u - ((base[g] >> p0) & 1u)) & pat;
+                for (int g = 0; g < G; ++g) sgn[g] ^= (0u - ((u32)__popc(base[g] & mi) & 1u)) & kPat[i];
             }
+        }
+        for (u32 c = 0; c < npair; ++c) {
+            const u32 w = sub.pairs >> (8 * c), p0 = w & 15u, p1 = (w >> 4) & 15u;
+#pragma unroll
+            for (int g = 0; g < G; ++g) sgn[g] ^= 0u - ((base[g] >> p0) & (base[g] >> p1) & 1u);
         }
 #pragma unroll
         for (int g = 0; g < G; ++g)
@@ -498,12 +505,7 @@ __device__ __forceinline__ void run_passes(double2* st, u32 lq, const DPass* __r
         const DPass ps = passes[pass0 + p];
         u32 soff[D];
 #pragma unroll
-        for (int j = 0; j < D; ++j) {
-            u32 o = 0;
-#pragma unroll
-            for (int i = 0; i < K; ++i) o |= (u32)((j >> i) & 1) << ps.pos[i];
-            soff[j] = slot_of(o);
-        }
+        for (int j = 0; j < D; ++j) soff[j] = ps.soff[j];
         if (G == 2 && ngroups == 2 * T && !(ps.flags & kPassHasDiag)) {
             pass_groups<K, T, G>(st, ps, soff, threadIdx.x, subs, pool);
         } else {
@@ -891,6 +893,11 @@ void schedule_passes(Flat& f, std::vector<HOp> ops, u32 lq, u32 K, DStage& sg) {
                 if ((block >> p) & 1) bpos[k++] = p;
         }
         for (u32 i = 0; i < 4; ++i) ps.pos[i] = (unsigned char)bpos[i];
+        for (u32 j = 0; j < (1u << K); ++j) {
+            u32 o = 0;
+            for (u32 i = 0; i < K; ++i) o |= ((j >> i) & 1u) << bpos[i];
+            ps.soff[j] = (unsigned short)slot_of(o);
+        }
         ps.sub0 = (u32)f.subs.size();
         auto bit_of = [&](u32 p) -> u32 {
             for (u32 i = 0; i < K; ++i)
@@ -900,14 +907,17 @@ void schedule_passes(Flat& f, std::vector<HOp> ops, u32 lq, u32 K, DStage& sg) {
         // Consecutive CZ gates, then consecutive single-qubit matrices on distinct
         // block bits (these commute with each other), become one LAYER.
         std::vector<const std::vector<cd>*> pend_d1(K, nullptr);
-        u32 d1mask = 0, cz_const = 0;
-        std::vector<u32> cz_items;
+        u32 d1mask = 0, cz_const = 0, npair = 0, pairs = 0;
+        u32 czmask[4] = {0, 0, 0, 0};
         bool cz_any = false;
         auto flush = [&]() {
             if (d1mask == 0 && !cz_any) return;
             DSub sb{};
-            sb.type = SUB_LAYER | (d1mask << 8) | ((u32)cz_items.size() << 12) | (cz_const << 16);
-            for (u32 c = 0; c < cz_items.size(); ++c) (c < 2 ? sb.a : sb.m) |= cz_items[c] << (16 * (c & 1));
+            const bool masks = (czmask[0] | czmask[1] | czmask[2] | czmask[3]) != 0;
+            sb.type = SUB_LAYER | (d1mask << 8) | (npair << 12) | (masks ? 0x8000u : 0u) | (cz_const << 16);
+            sb.a = czmask[0] | (czmask[1] << 16);
+            sb.m = czmask[2] | (czmask[3] << 16);
+            sb.pairs = pairs;
             sb.data = (u32)f.pool.size();
             for (u32 b = 0; b < K; ++b)
                 if (pend_d1[b]) pool_put(f, *pend_d1[b]);
@@ -915,7 +925,9 @@ void schedule_passes(Flat& f, std::vector<HOp> ops, u32 lq, u32 K, DStage& sg) {
             std::fill(pend_d1.begin(), pend_d1.end(), nullptr);
             d1mask = 0;
             cz_const = 0;
-            cz_items.clear();
+            npair = 0;
+            pairs = 0;
+            std::fill(czmask, czmask + 4, 0u);
             cz_any = false;
         };
         const u32 jmask = (1u << (1u << K)) - 1;
@@ -924,12 +936,12 @@ void schedule_passes(Flat& f, std::vector<HOp> ops, u32 lq, u32 K, DStage& sg) {
             if (in0 && in1) {
                 cz_const ^= (u32)(kBlockPatterns >> (16 * bit_of(p0))) & (u32)(kBlockPatterns >> (16 * bit_of(p1))) &
                             jmask;
+            } else if (in0 || in1) {
+                czmask[bit_of(in0 ? p0 : p1)] ^= 1u << (in0 ? p1 : p0);
             } else {
-                if (cz_items.size() == kCzItemsMax) flush();
-                if (in0 || in1)
-                    cz_items.push_back((in0 ? p1 : p0) | (bit_of(in0 ? p0 : p1) << 8));
-                else
-                    cz_items.push_back(p0 | (p1 << 8) | 0x8000u);
+                if (npair == kCzPairsMax) flush();
+                pairs |= (p0 | (p1 << 4)) << (8 * npair);
+                ++npair;
             }
             cz_any = true;
         };
