@@ -835,7 +835,7 @@ def serving(args, lib_mod, hip):
         st = lib_mod.call("allocate_quantum_state_host", dim, 0)
         L.initialize_quantum_state_host(st, dim, stream, 0)
         if r.n >= 4:
-            fc = FusedCircuit(r.n, r.gates, block_qubits=3)
+            fc = FusedCircuit(r.n, r.gates, block_qubits=4)
             L.qm_circuit_apply(fc.h, st, dim, stream, 0)
         else:
             fc = None

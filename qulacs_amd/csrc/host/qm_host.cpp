@@ -1887,18 +1887,16 @@ void QuantumCircuit::build_plan() {
         std::deque<std::vector<qm_complex>> store;
         std::vector<qm_batch_gate> lowered;
         size_t j = i, members = 0;
-        bool wide = false;
         for (; j < n; ++j) {
             qm_batch_gate bg{};
             bool skip = false;
             if (!lower_gate(gates_[j], &bg, &store, &skip)) break;
             ++members;
             if (skip) continue;
-            wide = wide || (bg.kind == QM_BATCH_DENSE && bg.qubit_count == 4);
             lowered.push_back(bg);
         }
         if (members >= min_fused_run(qubit_count_)) {
-            qm_circuit* prog = qm_circuit_create(qubit_count_, lowered.data(), (UINT)lowered.size(), wide ? 4 : 3, 0);
+            qm_circuit* prog = qm_circuit_create(qubit_count_, lowered.data(), (UINT)lowered.size(), 4, 0);
             if (prog) {
                 plan_.push_back({i, j, prog});
                 i = j;

@@ -588,7 +588,7 @@ __global__ __launch_bounds__(block_threads<K>()) void k_batch(
                 build_hi_tab(hi_tab, lq, sg);
                 __syncthreads();
             }
-            if (K <= 3 && tl > 0 && s > 0 && L == kBatchTile) {  // K = 4 has no registers to spare
+            if (tl > 0 && s > 0 && L == kBatchTile) {
                 batch_tiles<K, kBatchTile / block_threads<K>(), block_threads<K>(), batch_blocks<K>()>(
                     st, gs, hi_tab, lq, tl, sg, passes, subs, pool);
                 continue;
@@ -699,8 +699,8 @@ __global__ __launch_bounds__(stage_threads<K>()) void k_stage(double2* __restric
     __syncthreads();
     constexpr u32 T = stage_threads<K>();
     // the prefetched tile needs 4 P registers per thread on top of the register
-    // blocks: it fits beside K <= 3 blocks only
-    if constexpr (K == 3) {
+    // blocks (at K = 4: 64 beside ~165, within the 256 of a 512-thread group)
+    if constexpr (K >= 3) {
         if (L == 16 * T) {
             stage_tiles<K, 16>(state, st, hi_tab, lq, tl, sg, passes, subs, pool);
             return;
