@@ -97,13 +97,21 @@ def measured_traffic(kernel: str, qubits: int):
 
 def kernel_of(g) -> str:
     """Name of the kernel instantiation that executes gate g (qulacs_amd/csrc/qm_gates.hip): the
-    library streams with non-temporal accesses when no acted-on qubit is below bit 3."""
+    library streams with non-temporal accesses when no acted-on qubit is below bit 3; CZ with a
+    qubit below 3 and CNOT with a control below 3 run the line kernel (k_lane, LM = 1 when the
+    target is below 3); CZ on higher qubits takes two tuples per thread."""
     nt = "true" if min(g.targets) >= 3 else "false"
+    if g.kind == "CZ":
+        if min(g.targets) < 3:
+            return f"k_lane<1, {1 if g.targets[1] < 3 else 0}u, 2, OpNeg1>"
+        return "k_gather<1, 2, true, OpNeg1>"
+    if g.kind == "CNOT" and g.targets[0] < 3:
+        return f"k_lane<1, {1 if g.targets[1] < 3 else 0}u, 2, OpSwap01>"
     return {
         "dense1": f"k_gather<1, 1, {nt}, OpDenseArg<1>>", "RX": f"k_gather<1, 1, {nt}, OpDenseArg<1>>",
         "RY": f"k_gather<1, 1, {nt}, OpDenseArg<1>>", "RZ": f"k_gather<1, 1, {nt}, OpDiag<1>>",
         "dense2": f"k_gather<2, 1, {nt}, OpDenseArg<2>>", "dense5": f"k_dense_mfma<5, {nt}>",
-        "CNOT": f"k_gather<1, 1, {nt}, OpSwap01>", "CZ": f"k_gather<1, 1, {nt}, OpNeg1>",
+        "CNOT": f"k_gather<1, 1, {nt}, OpSwap01>",
     }[g.kind]
 
 
