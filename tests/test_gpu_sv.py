@@ -121,6 +121,26 @@ def test_random_circuits_match_oracle(oracle, n, seed):
         assert rel(s.download(), ref) < STATE_TOL
 
 
+@pytest.mark.parametrize("k", [4, 5])
+def test_dense_on_matrix_cores_with_a_partial_last_group(oracle, k):
+    """The 4- and 5-qubit dense kernels take tuples 16 at a time; with fewer than
+    16 tuples (or a count that is not a multiple of 16) the lanes past the end
+    hold copies of the last tuple.  Every state size from k to k + 5 qubits,
+    targets in random order, against the oracle."""
+    rng = np.random.default_rng(50 + k)
+    for n in range(k, k + 6):
+        dim = 1 << n
+        psi = random_state(rng, n)
+        ref = psi.copy()
+        with DevState.from_host(psi) as s:
+            for _ in range(3):
+                tg = rng.choice(n, k, replace=False).tolist()
+                u = haar_unitary(rng, k)
+                oracle.dense(tg, u, ref)
+                _lib.call("multi_qubit_dense_matrix_gate_host", u32(tg), k, cmat(u), s.ptr, dim, None, 0)
+            assert rel(s.download(), ref) < STATE_TOL, (k, n)
+
+
 @pytest.mark.parametrize("n", [8, 14])
 def test_every_target_position_dense_and_named(oracle, n):
     """Each single-qubit kind on every target, each 2q dense on low/low, low/high and high/high pairs."""
