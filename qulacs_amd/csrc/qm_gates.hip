@@ -586,6 +586,23 @@ void launch_dense_mfma(double2* s, u64 dim, const u32* targets, const u32* contr
         k_dense_mfma<K, false><<<grid, 256, 0, st>>>(s, n_bases, a, dmat);
 }
 
+// 4-qubit dense: the register-tuple kernel (one tuple per thread, FP64 FMAs)
+// or the matrix-core kernel.  Measured on a 30-qubit state (34 target sets):
+// the register-tuple kernel is 0-20 % faster on every scattered target set
+// (5.6-6.9 ms against 6.7-8.1 ms) and on runs of four adjacent qubits from 4
+// to 17, but slower when all four targets lie below qubit 6 (the members of a
+// tuple share 128-byte lines and each lane's loads are 256 bytes apart:
+// 7.1-9.9 ms against 6.8-7.5 ms) or when three or more lie in 18..25 (tuple
+// members 4-512 MiB apart: up to 9.1 ms against 6.9 ms).
+bool dense4_on_valu(const u32* targets) {
+    u32 mid = 0, hi = 0;
+    for (int i = 0; i < 4; ++i) {
+        mid += targets[i] >= 18 && targets[i] <= 25;
+        hi = std::max(hi, targets[i]);
+    }
+    return hi >= 6 && mid < 3;
+}
+
 // Validates state/dim/targets/controls; returns n (qubit count) or -1.
 int check_gate(void* state, u64 dim, const u32* targets, u32 nt, const u32* controls, u32 nc, const char* what) {
     const int n = log2_dim(dim);
@@ -653,7 +670,9 @@ void dense_any(const u32* targets, u32 nt, const u32* controls, const u32* cvals
             const size_t nel = (size_t)1 << (2 * nt);
             const double2* dm = (const double2*)upload(sc, matrix, nel * sizeof(double2));
             if (!dm) return;
-            if (nt == 4)
+            if (nt == 4 && dense4_on_valu(targets))
+                launch_dense_stream<4>(s, dim, targets, controls, cvals, nc, MatInPtr<4>{dm}, st);
+            else if (nt == 4)
                 launch_dense_mfma<4>(s, dim, targets, controls, cvals, nc, dm, st);
             else
                 launch_dense_mfma<5>(s, dim, targets, controls, cvals, nc, dm, st);
