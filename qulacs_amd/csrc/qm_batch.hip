@@ -8,7 +8,7 @@
 // reduction per term).
 //
 // Design:
-//   * one work-group (512 threads) per circuit, circuits taken from a queue
+//   * one work-group (1024 threads, 512 at K = 4) per circuit, circuits taken from a queue
 //     (an atomic counter) in descending-cost order, so one launch covers a batch
 //     of uneven sizes; every work-group leaves the loop once the queue is empty;
 //   * the state lives in LDS: 2^lq complex128 with lq = min(n, 13) (128 KiB at
@@ -54,9 +54,6 @@ constexpr u32 kLowFixed = 4;    // qubits 0..3 are local in every stage
 // 128 KiB tile still has 4 waves per SIMD to hide LDS latency.
 template <int K>
 constexpr u32 block_threads() { return K >= 4 ? 512u : 1024u; }
-// register blocks per thread in the pass loop (see run_passes)
-template <int K>
-constexpr int batch_blocks() { return 1; }
 constexpr u32 kMaxBlockThreads = 1024;
 constexpr u32 kMinQubits = 4;   // smaller circuits run on 4 qubits (extra qubits stay |0>)
 
@@ -589,8 +586,9 @@ __global__ __launch_bounds__(block_threads<K>()) void k_batch(
                 __syncthreads();
             }
             if (tl > 0 && s > 0 && L == kBatchTile) {
-                batch_tiles<K, kBatchTile / block_threads<K>(), block_threads<K>(), batch_blocks<K>()>(
-                    st, gs, hi_tab, lq, tl, sg, passes, subs, pool);
+                // one register block per thread (two measured slower here)
+                batch_tiles<K, kBatchTile / block_threads<K>(), block_threads<K>(), 1>(st, gs, hi_tab, lq, tl,
+                                                                                        sg, passes, subs, pool);
                 continue;
             }
             for (u32 t = 0; t < (1u << tl); ++t) {
@@ -602,7 +600,7 @@ __global__ __launch_bounds__(block_threads<K>()) void k_batch(
                     for (u32 i = tid; i < L; i += bd) st[slot_of(i)] = gs[tile_index(tb, hi_tab, i)];
                 }
                 __syncthreads();
-                run_passes<K, block_threads<K>(), batch_blocks<K>()>(st, lq, passes, sg.pass0, sg.npass, subs, pool);
+                run_passes<K, block_threads<K>(), 1>(st, lq, passes, sg.pass0, sg.npass, subs, pool);
                 if (tl > 0) {
                     for (u32 i = tid; i < L; i += bd) gs[tile_index(tb, hi_tab, i)] = st[slot_of(i)];
                 } else if (keep) {
