@@ -575,6 +575,58 @@ def many_small(args, lib_mod, hip):
     return out
 
 
+def fused_large(args, lib_mod, hip):
+    """The fused stage executor alone on one large state: an n-qubit brickwork circuit
+    (default 26 qubits, 40 layers: a Haar 2x2 on every qubit, then a CNOT brick) from |0>,
+    at register-block widths 3 and 4 with the 13-qubit window, timed with HIP events on the
+    launch stream (best of three runs after a warm-up).  Reports the stage count, the state
+    traffic (2 x 16 B x 2^n per stage) and the FP64 rate of the fused arithmetic (one 2x2
+    complex matrix per qubit per layer: 16 flops per amplitude) against the 78.6 TFLOP/s
+    vector peak; the two widths must agree (|<psi_3|psi_4>| within 1e-10 of 1)."""
+    from qulacs_amd.batch import FusedCircuit
+    from qulacs_amd.workloads import brickwork_circuit
+
+    n, layers = args.fused_qubits, args.fused_layers
+    dim = 1 << n
+    gates, _, _ = brickwork_circuit(n, layers, 1, 1, seed=args.seed + 5)
+    L = lib_mod.lib()
+    stream = lib_mod.call("allocate_cuda_stream_host", 1, 0)
+    hstream = C.c_void_p.from_address(stream).value  # the hipStream_t behind the handle
+    flops = 16.0 * layers * n * dim
+    out = {"workload": f"{n}-qubit brickwork, {layers} layers ({n * layers} Haar 1q + CNOT bricks), from |0>, "
+                       "13-qubit window", "qubits": n, "layers": layers, "gates": len(gates)}
+    states = {}
+    lib_mod.call("qm_set_synchronous", 0)
+    for K in (3, 4):
+        fc = FusedCircuit(n, gates, block_qubits=K)
+        st = lib_mod.call("allocate_quantum_state_host", dim, 0)
+        e0, e1 = hip.create(2)
+        L.initialize_quantum_state_host(st, dim, stream, 0)
+        L.qm_circuit_apply(fc.h, st, dim, stream, 0)
+        best = float("inf")
+        for _ in range(3):
+            L.initialize_quantum_state_host(st, dim, stream, 0)
+            hip.record(e0, hstream)
+            L.qm_circuit_apply(fc.h, st, dim, stream, 0)
+            hip.record(e1, hstream)
+            best = min(best, hip.elapsed_ms(e0, e1))
+        hip.destroy([e0, e1])
+        lib_mod.check()
+        stats = fc.stats()
+        traffic = stats["stages"] * 2.0 * 16 * dim
+        out[f"K{K}"] = {"ms": round(best, 2), **stats, "state_traffic_GBps": round(traffic / best / 1e6, 1),
+                        "fp64_TFLOPs": round(flops / best / 1e9, 2), "fp64_frac_of_peak": round(flops / best / 1e9 / 78.6, 4)}
+        states[K] = st
+        fc.close()
+    lib_mod.call("qm_set_synchronous", 1)
+    z = lib_mod.call("inner_product_host", states[3], states[4], dim, stream, 0)
+    out["widths_agree"] = bool(abs(abs(complex(z.re, z.im)) - 1.0) < 1e-10)
+    for st in states.values():
+        lib_mod.call("release_quantum_state_host", st, 0)
+    lib_mod.call("release_cuda_stream_host", stream, 1, 0)
+    return out
+
+
 def brickwork(args, lib_mod, hip):
     """20-qubit brickwork (10 layers of Haar 1q on every qubit + a CNOT brick),
     16-term observable and 1024 samples: the per-gate path end to end, the fused
@@ -1064,10 +1116,13 @@ def main() -> None:
     ap.add_argument("--serve-max-qubits", type=int, default=28)
     ap.add_argument("--serve-budget-gb", type=float, default=288.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--fused-qubits", type=int, default=26)
+    ap.add_argument("--fused-layers", type=int, default=40)
+    ap.add_argument("--no-fused", action="store_true")
     ap.add_argument("--adj-batch", type=int, default=256)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--cpu-worker", choices=["sweep", "brickwork", "small"], help=argparse.SUPPRESS)
-    ap.add_argument("--only", choices=["sweep", "dm", "brickwork", "small", "adjoint", "serve"],
+    ap.add_argument("--only", choices=["sweep", "dm", "brickwork", "fused", "small", "adjoint", "serve"],
                     help="run one leg alone (for profiling): 'sweep' prints the full line with the other legs "
                          "left out, the others print their leg's JSON")
     args = ap.parse_args()
@@ -1083,10 +1138,10 @@ def main() -> None:
 
     hip = HipEvents()
     if args.only == "sweep":
-        args.no_dm = args.no_small = args.no_serve = args.no_adjoint = True
+        args.no_dm = args.no_small = args.no_serve = args.no_adjoint = args.no_fused = True
     elif args.only:
-        leg = {"dm": dm_sweep, "brickwork": brickwork, "small": many_small, "adjoint": adjoint,
-               "serve": serving}[args.only]
+        leg = {"dm": dm_sweep, "brickwork": brickwork, "fused": fused_large, "small": many_small,
+               "adjoint": adjoint, "serve": serving}[args.only]
         print(json.dumps({args.only: leg(args, _lib, hip)}, default=str), flush=True)
         return
     print(f"[bench] {args.qubits}-qubit sweep, {args.gates} gates, warmup {args.warmup}, steps {args.steps}",
@@ -1101,6 +1156,11 @@ def main() -> None:
     if args.only != "sweep":
         c20 = brickwork(args, _lib, hip)
         print(f"[bench] 20q brickwork {c20['gpu_ms']} ms", file=sys.stderr, flush=True)
+    fused = None
+    if not args.no_fused:
+        fused = fused_large(args, _lib, hip)
+        print(f"[bench] fused {args.fused_qubits}q brickwork K3 {fused['K3']['ms']} ms, K4 {fused['K4']['ms']} ms",
+              file=sys.stderr, flush=True)
     small = None
     if not args.no_small:
         small = many_small(args, _lib, hip)
@@ -1142,6 +1202,7 @@ def main() -> None:
         "per_kernel": sv["per_kernel"],
         "dm_sweep": dm,
         "brickwork_20q": c20,
+        "fused_executor": fused,
         "many_small_circuits": small,
         "serving": serve,
         "adjoint": adj,
