@@ -141,6 +141,30 @@ def test_dense_on_matrix_cores_with_a_partial_last_group(oracle, k):
             assert rel(s.download(), ref) < STATE_TOL, (k, n)
 
 
+@pytest.mark.parametrize("k", [4, 5])
+def test_controlled_dense_on_four_and_five_targets(oracle, k):
+    """4- and 5-target dense gates with one or two controls (either control value),
+    target sets both scattered and clustered (the 4-target case takes the register
+    kernel or the matrix-core kernel by target set), against the oracle."""
+    rng = np.random.default_rng(70 + k)
+    n = 12
+    dim = 1 << n
+    psi = random_state(rng, n)
+    ref = psi.copy()
+    with DevState.from_host(psi) as s:
+        for tg in ([0, 1, 2, 3, 4][:k], [6, 7, 8, 9, 10][:k], rng.choice(n, k, replace=False).tolist(),
+                   rng.choice(n, k, replace=False).tolist()):
+            rest = [q for q in range(n) if q not in tg]
+            for nc in (1, 2):
+                cl = rng.choice(rest, nc, replace=False).tolist()
+                cv = rng.integers(0, 2, nc).tolist()
+                u = haar_unitary(rng, k)
+                oracle.controlled_dense(cl, cv, tg, u, ref)
+                _lib.call("multi_qubit_control_multi_qubit_dense_matrix_gate_host", u32(cl), u32(cv), nc, u32(tg), k,
+                          cmat(u), s.ptr, dim, None, 0)
+        assert rel(s.download(), ref) < STATE_TOL
+
+
 @pytest.mark.parametrize("n", [8, 14])
 def test_every_target_position_dense_and_named(oracle, n):
     """Each single-qubit kind on every target, each 2q dense on low/low, low/high and high/high pairs."""
