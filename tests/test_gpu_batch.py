@@ -129,9 +129,12 @@ def test_graph_replays_match_eager_runs():
     b.close()
 
 
-def test_fused_circuit_above_32_qubits_round_trip():
+@pytest.mark.parametrize("block", [3, 4])
+def test_fused_circuit_above_32_qubits_round_trip(block):
     """33 qubits (128 GiB): offsets of window qubits >= 32 exceed 32 bits.  U then
-    U^dagger from |0...0> must return |0...0>: P(q = 0) = 1 for every probed qubit."""
+    U^dagger from |0...0> must return |0...0>: P(q = 0) = 1 for every probed qubit.
+    Both register-block widths that prefetch the next tile (3: two blocks per thread,
+    4: one 16-amplitude block)."""
     from qulacs_amd import _lib
     from qulacs_amd.batch import FusedCircuit
 
@@ -139,7 +142,7 @@ def test_fused_circuit_above_32_qubits_round_trip():
     rng = np.random.default_rng(3333)
     gates = [("dense", [32], haar_unitary(rng, 1)), ("CNOT", [32, 0], None), ("CZ", [31, 32], None),
              ("dense", [32, 5], haar_unitary(rng, 2))]
-    gates += random_batch_circuit(rng, n, 20, max_dense=3)
+    gates += random_batch_circuit(rng, n, 20, max_dense=block)
     inverse = []
     for kind, q, m in reversed(gates):
         if kind == "dense":
@@ -148,7 +151,7 @@ def test_fused_circuit_above_32_qubits_round_trip():
             inverse.append((kind, q, np.conj(np.asarray(m))))
         else:
             inverse.append((kind, q, m))
-    fc = FusedCircuit(n, gates + inverse, block_qubits=3)
+    fc = FusedCircuit(n, gates + inverse, block_qubits=block)
     dim = 1 << n
     print("fused 33 qubits", fc.stats(), flush=True)
     try:
