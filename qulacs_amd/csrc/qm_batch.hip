@@ -92,7 +92,7 @@ struct DPass {
     unsigned char pos[4];  // block bits (local positions, ascending)
     u32 nsub, sub0;
     u32 flags;             // kPassHasDiag: a DIAG sub-operation (one register block per thread)
-    unsigned short soff[16];  // slot_of(offset of register-block index j), computed on the host
+    unsigned short sbit[4];   // slot_of(1 << pos[i]), computed on the host
 };
 constexpr u32 kPassHasDiag = 1;
 struct DSub {
@@ -408,14 +408,16 @@ __device__ __forceinline__ void apply_layer(double2 (&v)[G][1 << K], const DSub&
 }
 
 // The sub-operations of one pass on the G register blocks of groups gi0,
-// gi0 + T, ..., read from and written back to LDS.  soff[j]: swizzled slot
-// offset of register-block index j (slot_of is linear over GF(2), so the slot
-// of base | off[j] is slot_of(base) ^ soff[j]).
+// gi0 + T, ..., read from and written back to LDS.  slot_of is linear over
+// GF(2), so the slot of base | off[j] is slot_of(base) ^ (XOR of sbit[i] over
+// the bits i of j): each lane forms its 2^K slots from the K per-bit offsets
+// (the empty asm keeps every partial XOR in a vector register, so the compiler
+// does not form the 2^K uniform combinations in scalar registers, which spill).
 template <int K, u32 T, int G>
-__device__ __forceinline__ void pass_groups(double2* st, const DPass& ps, const u32 (&soff)[1 << K], u32 gi0,
+__device__ __forceinline__ void pass_groups(double2* st, const DPass& ps, const u32 (&sbit)[K], u32 gi0,
                                             const DSub* __restrict__ subs, const double2* __restrict__ pool) {
     constexpr int D = 1 << K;
-    u32 base[G], sb[G];
+    u32 base[G], slot[G][D];
     double2 v[G][D];
 #pragma unroll
     for (int g = 0; g < G; ++g) {
@@ -426,12 +428,21 @@ __device__ __forceinline__ void pass_groups(double2* st, const DPass& ps, const 
             b = (b & ((1u << q) - 1)) | ((b >> q) << (q + 1));
         }
         base[g] = b;
-        sb[g] = slot_of(b);
+        slot[g][0] = slot_of(b);
+#pragma unroll
+        for (int j = 1; j < D; ++j) {
+            const int lb = __builtin_ctz(j);
+            slot[g][j] = slot[g][j & (j - 1)] ^ sbit[lb];
+            // one block per thread: keep the chain per lane (see above); two
+            // blocks: the 2^K uniform offsets are formed once and shared, which
+            // costs fewer vector registers where both blocks are live
+            if constexpr (G == 1) asm volatile("" : "+v"(slot[g][j]));
+        }
     }
 #pragma unroll
     for (int g = 0; g < G; ++g)
 #pragma unroll
-        for (int j = 0; j < D; ++j) v[g][j] = st[sb[g] ^ soff[j]];
+        for (int j = 0; j < D; ++j) v[g][j] = st[slot[g][j]];
     // LAYER sub-operations run in an inner loop of their own and update the
     // register blocks in place, so the blocks stay at fixed registers there;
     // the other kinds are handled between runs of LAYERs.
@@ -485,7 +496,7 @@ __device__ __forceinline__ void pass_groups(double2* st, const DPass& ps, const 
 #pragma unroll
     for (int g = 0; g < G; ++g)
 #pragma unroll
-        for (int j = 0; j < D; ++j) st[sb[g] ^ soff[j]] = v[g][j];
+        for (int j = 0; j < D; ++j) st[slot[g][j]] = v[g][j];
 }
 
 // Applies passes [pass0, pass0 + npass) to the 2^lq amplitudes in LDS with a
@@ -500,13 +511,15 @@ __device__ __forceinline__ void run_passes(double2* st, u32 lq, const DPass* __r
     const u32 ngroups = (1u << lq) >> K;
     for (u32 p = 0; p < npass; ++p) {
         const DPass ps = passes[pass0 + p];
-        u32 soff[D];
+        u32 sbit[K];
 #pragma unroll
-        for (int j = 0; j < D; ++j) soff[j] = ps.soff[j];
+        for (int i = 0; i < K; ++i) sbit[i] = ps.sbit[i];
         if (G == 2 && ngroups == 2 * T && !(ps.flags & kPassHasDiag)) {
-            pass_groups<K, T, G>(st, ps, soff, threadIdx.x, subs, pool);
+            pass_groups<K, T, G>(st, ps, sbit, threadIdx.x, subs, pool);
+        } else if (ngroups == T) {
+            pass_groups<K, T, 1>(st, ps, sbit, threadIdx.x, subs, pool);
         } else {
-            for (u32 g = threadIdx.x; g < ngroups; g += T) pass_groups<K, T, 1>(st, ps, soff, g, subs, pool);
+            for (u32 g = threadIdx.x; g < ngroups; g += T) pass_groups<K, T, 1>(st, ps, sbit, g, subs, pool);
         }
         __syncthreads();
     }
@@ -891,11 +904,7 @@ void schedule_passes(Flat& f, std::vector<HOp> ops, u32 lq, u32 K, DStage& sg) {
                 if ((block >> p) & 1) bpos[k++] = p;
         }
         for (u32 i = 0; i < 4; ++i) ps.pos[i] = (unsigned char)bpos[i];
-        for (u32 j = 0; j < (1u << K); ++j) {
-            u32 o = 0;
-            for (u32 i = 0; i < K; ++i) o |= ((j >> i) & 1u) << bpos[i];
-            ps.soff[j] = (unsigned short)slot_of(o);
-        }
+        for (u32 i = 0; i < K; ++i) ps.sbit[i] = (unsigned short)slot_of(1u << bpos[i]);
         ps.sub0 = (u32)f.subs.size();
         auto bit_of = [&](u32 p) -> u32 {
             for (u32 i = 0; i < K; ++i)
