@@ -90,7 +90,7 @@ __device__ __forceinline__ double prob_dm(const double2* __restrict__ a, u64 i, 
 
 // Block-wide exclusive scan of one u64 per thread (1024 threads, 16 waves).
 __device__ __forceinline__ u64 block_excl_scan(u64 v, u64* lds /* >= 16 */, u64* total) {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     u64 x = v;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
@@ -115,21 +115,97 @@ __device__ __forceinline__ u64 block_excl_scan(u64 v, u64* lds /* >= 16 */, u64*
     return excl;
 }
 
-__device__ __forceinline__ int block_min_int(int v, int* lds /* >= 16 */) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) lds[wid] = v;
+// One exchange of per-wave partials across the 16 waves of a 1024-thread block
+// with a single barrier.  Exchanges alternate between two LDS slots (r & 1): a
+// wave that writes a slot again, two exchanges later, has passed the barrier of
+// the exchange in between, which every wave reaches only after it has read the
+// slot.  Every thread of the block runs the same sequence of exchanges.
+constexpr int kCdfWaves = kCdfThreads / 64;
+struct Xchg {
+    u64 sum[kCdfWaves];
+    u64 val[kCdfWaves];
+    int mn[kCdfWaves];
+    double s;
+    int first;
+};
+
+struct XScan {
+    u64 excl;       // sum of v over the threads before this one
+    u64 total;      // sum of v over the block
+    u64 wave_incl;  // in lane w < 16: sum of v over waves 0..w
+    int first;      // m of the first thread with m != INT_MAX (INT_MAX if none)
+};
+
+// Cross-lane steps on the VALU (DPP) and the scalar unit (ballot, readlane)
+// rather than through the LDS crossbar: the exchanges sit on the serial path
+// of the walk, where their latency is the cost.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ u64 dpp_u64(u64 x) {
+    // lanes without a source (or in rows left out by ROW_MASK) read 0
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(u32)x, CTRL, ROW_MASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(u32)(x >> 32), CTRL, ROW_MASK, 0xf, false);
+    return ((u64)(u32)hi << 32) | (u32)lo;
+}
+// inclusive scan within each row of 16 lanes
+__device__ __forceinline__ u64 row_incl_scan(u64 x) {
+    x += dpp_u64<0x111>(x);  // row_shr:1
+    x += dpp_u64<0x112>(x);  // row_shr:2
+    x += dpp_u64<0x114>(x);  // row_shr:4
+    x += dpp_u64<0x118>(x);  // row_shr:8
+    return x;
+}
+// inclusive scan over the 64 lanes of a wave
+__device__ __forceinline__ u64 wave_incl_scan(u64 x) {
+    x = row_incl_scan(x);
+    x += dpp_u64<0x142, 0xa>(x);  // row_bcast:15: lane 15 into row 1, lane 47 into row 3
+    x += dpp_u64<0x143, 0xc>(x);  // row_bcast:31: lane 31 into rows 2 and 3
+    return x;
+}
+__device__ __forceinline__ u64 readlane_u64(u64 x, int l) {
+    const u32 lo = (u32)__builtin_amdgcn_readlane((int)(u32)x, l);
+    const u32 hi = (u32)__builtin_amdgcn_readlane((int)(u32)(x >> 32), l);
+    return ((u64)hi << 32) | lo;
+}
+// m of the lowest lane with m != INT_MAX (INT_MAX if none).  Every use passes
+// an element or thread index that grows with the lane, so this is the minimum.
+__device__ __forceinline__ int wave_first_set(int m) {
+    const u64 b = __ballot(m != INT_MAX);
+    return b ? __builtin_amdgcn_readlane(m, __ffsll((unsigned long long)b) - 1) : INT_MAX;
+}
+
+// Exclusive scan of v and the first m != INT_MAX over the block (m must grow
+// with the thread index where it is set).  The sums must not exceed 2^64 - 1.
+__device__ __forceinline__ XScan xchg_scan_first(u64 v, int m, Xchg* X, int& r) {
+    const int lane = threadIdx.x & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const u64 x = wave_incl_scan(v);
+    const int mf = wave_first_set(m);
+    Xchg& slot = X[r & 1];
+    if (lane == 63) slot.sum[wid] = x;
+    if (lane == 0) slot.mn[wid] = mf;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        int m = lds[0];
-        for (int w = 1; w < kCdfThreads / 64; ++w) m = min(m, lds[w]);
-        lds[16] = m;
+    ++r;
+    const u64 t = row_incl_scan(lane < kCdfWaves ? slot.sum[lane] : 0);  // lanes 0..15: over waves 0..lane
+    const int tm = lane < kCdfWaves ? slot.mn[lane] : INT_MAX;
+    XScan o;
+    o.total = readlane_u64(t, kCdfWaves - 1);
+    o.excl = (wid == 0 ? 0 : readlane_u64(t, wid - 1)) + (x - v);
+    o.wave_incl = t;
+    o.first = wave_first_set(tm);
+    return o;
+}
+
+// One thread (owner) hands (s, first) to the whole block.
+__device__ __forceinline__ void xchg_bcast(bool owner, double& s, int& first, Xchg* X, int& r) {
+    Xchg& slot = X[r & 1];
+    if (owner) {
+        slot.s = s;
+        slot.first = first;
     }
     __syncthreads();
-    const int r = lds[16];
-    __syncthreads();
-    return r;
+    ++r;
+    s = slot.s;
+    first = slot.first;
 }
 
 // Sums chunk [c0, c0 + len) onto the running sum s exactly as the sequential
@@ -137,59 +213,60 @@ __device__ __forceinline__ int block_min_int(int v, int* lds /* >= 16 */) {
 // 1024-thread block with the same s; returns with the same s in every thread.
 template <bool DM>
 __device__ __forceinline__ void exact_chunk(const double2* __restrict__ a, u64 dim, u64 c0, int len, double& s,
-                                            double* __restrict__ cdf, double* P, u64* scan_lds, int* min_lds,
-                                            double* s_bcast_p) {
-    double& s_bcast = *s_bcast_p;
+                                            double* __restrict__ cdf, double* P, Xchg* X, int& r) {
+    const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (int j = threadIdx.x; j < len; j += kCdfThreads)
         P[j] = DM ? prob_dm(a, c0 + j, dim) : prob_sv(a, c0 + j);
     __syncthreads();
+    const u64 LIM = 1ULL << 53;
+    const int lo = threadIdx.x * kCdfPer;  // this thread's elements: lo .. lo+kCdfPer-1
     int pos = 0;     // first element of the chunk not yet summed
     int serial = 0;  // elements to add one by one before the next block scan
     while (pos < len) {
-        const int lo = threadIdx.x * kCdfPer;  // this thread's elements: lo .. lo+kCdfPer-1
-        if (serial > 0) {
-            // Crossings close together (a small running sum doubling every few
-            // elements): the next elements are added one at a time by one
-            // thread, the same IEEE additions as the sequential loop, which is
-            // cheaper than a block scan per crossing.
-            const int end = min(len, pos + serial);
+        if (serial > 0 || !isfinite(s)) {
+            // One thread adds the next elements one at a time, the same IEEE
+            // additions as the sequential loop, and leaves the sums in P for the
+            // block to store.  Used after crossings close together (a small
+            // running sum doubling every few elements), where it is cheaper than
+            // a block scan per crossing, and to finish a chunk after an inf/NaN
+            // (only reachable from a non-finite amplitude).
+            const int end = isfinite(s) ? min(len, pos + serial) : len;
             if (threadIdx.x == 0) {
                 double t = s;
-                for (int j0 = pos; j0 < end; j0 += 16) {
-                    double v[16];  // loaded together so the LDS latency is paid once per 16
+                int j = pos;
+                double v[16];
+                if (j + 16 <= end) {
 #pragma unroll
-                    for (int k = 0; k < 16; ++k) v[k] = j0 + k < end ? P[j0 + k] : 0.0;
-#pragma unroll
-                    for (int k = 0; k < 16; ++k)
-                        if (j0 + k < end) {
-                            t = t + v[k];
-                            cdf[c0 + j0 + k + 1] = t;
-                        }
+                    for (int k = 0; k < 16; ++k) v[k] = P[j + k];
                 }
-                s_bcast = t;
+                for (; j + 16 <= end; j += 16) {
+                    double w[16];  // the next 16, loaded before this block's sums are stored
+                    if (j + 32 <= end) {
+#pragma unroll
+                        for (int k = 0; k < 16; ++k) w[k] = P[j + 16 + k];
+                    }
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) {
+                        t = t + v[k];
+                        v[k] = t;
+                    }
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) P[j + k] = v[k];
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) v[k] = w[k];
+                }
+                for (; j < end; ++j) {
+                    t = t + P[j];
+                    P[j] = t;
+                }
+                s = t;
             }
-            __syncthreads();
-            s = s_bcast;
-            __syncthreads();
+            int unused = 0;
+            xchg_bcast(threadIdx.x == 0, s, unused, X, r);
+            for (int j = pos + threadIdx.x; j < end; j += kCdfThreads) cdf[c0 + j + 1] = P[j];
             pos = end;
             serial = 0;
             continue;
-        }
-        if (!isfinite(s)) {
-            // inf/NaN: finish the chunk with plain sequential additions
-            // (only reachable from a non-finite amplitude).
-            if (threadIdx.x == 0) {
-                double t = s;
-                for (int j = pos; j < len; ++j) {
-                    t = t + P[j];
-                    cdf[c0 + j + 1] = t;
-                }
-                s_bcast = t;
-            }
-            __syncthreads();
-            s = s_bcast;
-            __syncthreads();
-            break;
         }
         if (s == 0.0) {
             // fl(0 + p) = p: the sum stays 0 up to the first positive p.
@@ -198,7 +275,7 @@ __device__ __forceinline__ void exact_chunk(const double2* __restrict__ a, u64 d
                 const int j = lo + k;
                 if (j >= pos && j < len && P[j] != 0.0) { first = j; break; }
             }
-            first = block_min_int(first, min_lds);
+            first = xchg_scan_first(0, first, X, r).first;
             const int stop = first == INT_MAX ? len : first;
             for (int k = 0; k < kCdfPer; ++k) {
                 const int j = lo + k;
@@ -214,42 +291,56 @@ __device__ __forceinline__ void exact_chunk(const double2* __restrict__ a, u64 d
         if (e < -1022) e = -1022;
         const double u = ldexp(1.0, e - 52);
         const u64 S = (u64)(s / u);  // exact: s is a multiple of u below 2^53 u
-        const u64 LIM = 1ULL << 53;
-        // per-element integer increments; ties are marked special
+        // per-element integer increments, at most 2^53 each (a larger one
+        // certainly crosses the binade); the first tie
         u64 R[kCdfPer];
-        bool tie[kCdfPer];
         u64 mine = 0;
+        int tie = INT_MAX;
 #pragma unroll
         for (int k = 0; k < kCdfPer; ++k) {
             const int j = lo + k;
             R[k] = 0;
-            tie[k] = false;
             if (j >= pos && j < len) {
                 double q = ldexp(P[j], 52 - e);  // exact scaling by a power of two (or +inf)
-                if (!(q < 0x1p62)) q = 0x1p62;   // certain binade crossing; keeps sums in range
+                if (!(q < 0x1p53)) q = 0x1p53;
                 const double fl = floor(q);
-                const double fr = q - fl;        // exact for q < 2^53; 0 above
+                const double fr = q - fl;        // exact for q < 2^53
                 R[k] = (u64)fl + (fr > 0.5 ? 1 : 0);
-                tie[k] = (fr == 0.5);
+                if (fr == 0.5 && tie == INT_MAX) tie = j;
             }
             mine += R[k];
         }
-        u64 total;
-        const u64 excl = block_excl_scan(mine, scan_lds, &total);
-        // first special element: a tie or the first partial sum reaching 2^53
-        int first = INT_MAX;
-        u64 run = S + excl;
+        // A thread whose sum reaches 2^53 holds a crossing; clamping keeps the
+        // block sum below 2^63 and changes no sum before the first crossing.
+        if (mine > LIM) mine = LIM;
+        const XScan xs = xchg_scan_first(mine, tie, X, r);
+        const int first_tie = xs.first;
+        // The partial sums S + (increments up to element j) never decrease, so
+        // the first element reaching 2^53 (the first crossing) lies in the
+        // first wave whose inclusive sum reaches it.
+        int wcross = kCdfWaves;
+        if (S + xs.total >= LIM) wcross = __ffsll((long long)__ballot(lane < kCdfWaves && S + xs.wave_incl >= LIM)) - 1;
+        // first crossing as seen from this thread's wave: after all of its
+        // elements (INT_MAX), before all of them (-1), or the element itself
+        int cross = wid < wcross ? INT_MAX : -1;
+        u64 run = S + xs.excl;
+        if (wid == wcross) {  // wave-uniform
+            int mc = INT_MAX;
 #pragma unroll
-        for (int k = 0; k < kCdfPer; ++k) {
-            const int j = lo + k;
-            if (j >= pos && j < len && first == INT_MAX) {
-                if (tie[k] || run + R[k] >= LIM) first = j;
-                run += R[k];
+            for (int k = 0; k < kCdfPer; ++k) {
+                const int j = lo + k;
+                if (j >= pos && j < len && mc == INT_MAX) {
+                    if (run + R[k] >= LIM) mc = j;
+                    run += R[k];
+                }
             }
+            cross = wave_first_set(mc);
         }
-        first = block_min_int(first, min_lds);
-        const int stop = first == INT_MAX ? len : first;
-        run = S + excl;
+        // the first special element (tie or crossing) as seen from this
+        // thread: exact for the thread that owns it and for every thread with
+        // elements before it, and before all elements of any other thread
+        const int first = min(first_tie, cross);
+        run = S + xs.excl;
         u64 before_first = 0;  // S + increments strictly before `first` (owned by its thread)
 #pragma unroll
         for (int k = 0; k < kCdfPer; ++k) {
@@ -257,25 +348,23 @@ __device__ __forceinline__ void exact_chunk(const double2* __restrict__ a, u64 d
             if (j >= pos && j < len) {
                 if (j == first) before_first = run;
                 run += R[k];
-                if (j < stop) cdf[c0 + j + 1] = (double)run * u;
+                if (j < first) cdf[c0 + j + 1] = (double)run * u;
             }
         }
-        if (first == INT_MAX) {
-            s = (double)(S + total) * u;
+        if (first_tie == INT_MAX && wcross == kCdfWaves) {
+            s = (double)(S + xs.total) * u;
             pos = len;
         } else {
             // The owning thread performs the one IEEE addition exactly as the CPU does.
-            if (lo <= first && first < lo + kCdfPer) {
-                const double sb = (double)before_first * u;
-                const double sn = sb + P[first];
-                cdf[c0 + first + 1] = sn;
-                s_bcast = sn;
+            const bool owner = lo <= first && first < lo + kCdfPer;
+            int f = first;
+            if (owner) {
+                s = (double)before_first * u + P[first];
+                cdf[c0 + first + 1] = s;
             }
-            __syncthreads();
-            s = s_bcast;
-            __syncthreads();
-            if (first - pos < kSerialTrigger) serial = kSerialRun;
-            pos = first + 1;
+            xchg_bcast(owner, s, f, X, r);
+            if (f - pos < kSerialTrigger) serial = kSerialRun;
+            pos = f + 1;
         }
     }
     __syncthreads();
@@ -285,14 +374,13 @@ template <bool DM>
 __global__ __launch_bounds__(kCdfThreads) void k_cdf_exact(const double2* __restrict__ a, u64 dim,
                                                            double* __restrict__ cdf) {
     __shared__ double P[kCdfChunk];
-    __shared__ u64 scan_lds[17];
-    __shared__ int min_lds[17];
-    __shared__ double s_bcast;
+    __shared__ Xchg X[2];
+    int r = 0;
     double s = 0.0;  // exact running sum, identical in every thread
     if (threadIdx.x == 0) cdf[0] = 0.0;
     for (u64 c0 = 0; c0 < dim; c0 += kCdfChunk) {
         const int len = (int)min((u64)kCdfChunk, dim - c0);
-        exact_chunk<DM>(a, dim, c0, len, s, cdf, P, scan_lds, min_lds, &s_bcast);
+        exact_chunk<DM>(a, dim, c0, len, s, cdf, P, X, r);
     }
 }
 
@@ -393,7 +481,7 @@ __global__ __launch_bounds__(kIncrThreads) void k_chunk_incr(const double2* __re
         acc += __shfl_xor(acc, off, 64);
         bad |= __shfl_xor(bad, off, 64);
     }
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (lane == 0) {
         lds_u[wid] = acc;
         lds_i[wid] = bad;
@@ -418,13 +506,31 @@ __global__ __launch_bounds__(kCdfThreads) void k_cdf_walk(const double2* __restr
                                                           double* __restrict__ start, int* __restrict__ exact,
                                                           double* __restrict__ cdf) {
     __shared__ double P[kCdfChunk];
-    __shared__ u64 scan_lds[17];
-    __shared__ int min_lds[17];
-    __shared__ double s_bcast;
+    __shared__ Xchg X[2];
+    int r = 0;
+    const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const u64 LIM = 1ULL << 53;
     double s = 0.0;  // exact running sum, identical in every thread
     if (threadIdx.x == 0) cdf[0] = 0.0;
+    // chunk summaries of this thread's chunk in the window, loaded one window
+    // ahead (the next window's loads are in flight during an exact chunk)
+    double tT = 0.0;
+    int tslow = 0, tE = 0;
+    u64 tN = 0;
+    auto load = [&](u64 cc) {
+        const u64 ci = cc + threadIdx.x;
+        tT = 0.0;
+        tslow = tE = 0;
+        tN = 0;
+        if (ci < M) {
+            tT = T[ci];
+            tslow = slow[ci];
+            tE = E[ci];
+            tN = N[ci];
+        }
+    };
     u64 c = 0;
+    load(c);
     while (c < M) {
         const u64 ci = c + threadIdx.x;
         int es = INT_MIN;
@@ -436,42 +542,61 @@ __global__ __launch_bounds__(kCdfThreads) void k_cdf_walk(const double2* __restr
             S = (u64)(s / u);
         }
         bool ok = false;
-        u64 v = 0;
+        u64 v = 0;  // below 2^53 (a chunk whose sum reaches it is slow)
         if (ci < M && isfinite(s)) {
-            if (T[ci] == 0.0) {
+            if (tT == 0.0) {
                 ok = true;
-            } else if (s != 0.0 && !slow[ci] && E[ci] == es) {
+            } else if (s != 0.0 && !tslow && tE == es) {
                 ok = true;
-                v = N[ci];
+                v = tN;
             }
         }
-        const int first_bad = block_min_int(ok ? INT_MAX : (int)threadIdx.x, min_lds);
-        if ((int)threadIdx.x >= first_bad) v = 0;
-        u64 total;
-        const u64 excl = block_excl_scan(v, scan_lds, &total);
-        const int cross =
-            ((int)threadIdx.x < first_bad && S + excl + v >= LIM) ? (int)threadIdx.x : INT_MAX;
-        const int first_cross = block_min_int(cross, min_lds);
-        // chunks c .. c + stop - 1 are fast (at most the window's kCdfThreads)
-        const int stop = min(min(first_bad, first_cross), kCdfThreads);
-        if ((int)threadIdx.x < stop) {
-            start[ci] = s == 0.0 ? 0.0 : (double)(S + excl) * u;
+        const XScan xs = xchg_scan_first(v, ok ? INT_MAX : (int)threadIdx.x, X, r);
+        const int first_bad = xs.first;
+        // Chunk ci is fast when every chunk up to it is ok and the sum stays
+        // below 2^53 units through it; the sums never decrease, so the fast
+        // chunks are the first `stop` of the window.
+        const bool fast = (int)threadIdx.x < first_bad && S + xs.excl + v < LIM;
+        if (fast) {
+            start[ci] = s == 0.0 ? 0.0 : (double)(S + xs.excl) * u;
             exact[ci] = 0;
         }
-        if (stop > 0) {
-            if ((int)threadIdx.x == stop - 1) s_bcast = s == 0.0 ? 0.0 : (double)(S + excl + v) * u;
+        // stop and the sum in front of chunk c + stop: from the first wave
+        // holding a chunk that is not fast
+        {
+            const u64 nf = __ballot(!fast);
+            const int fl = nf ? __ffsll((unsigned long long)nf) - 1 : 0;
+            const u64 at = readlane_u64(S + xs.excl, fl);
+            Xchg& slot = X[r & 1];
+            if (lane == 0) {
+                slot.mn[wid] = nf ? wid * 64 + fl : INT_MAX;
+                slot.val[wid] = at;
+            }
             __syncthreads();
-            s = s_bcast;
-            __syncthreads();
+            ++r;
         }
+        const Xchg& got = X[(r - 1) & 1];
+        const int tm = lane < kCdfWaves ? got.mn[lane] : INT_MAX;
+        const u64 tv = lane < kCdfWaves ? got.val[lane] : 0;
+        const u64 has = __ballot(tm != INT_MAX);
+        int stop = kCdfThreads;
+        u64 Sstop = S + xs.total;
+        if (has) {
+            const int w = __ffsll((unsigned long long)has) - 1;
+            stop = __builtin_amdgcn_readlane(tm, w);
+            Sstop = readlane_u64(tv, w);
+        }
+        if (stop > 0 && s != 0.0) s = (double)Sstop * u;
         c += (u64)stop;
-        if (stop < kCdfThreads && c < M) {
+        const bool exact_next = stop < kCdfThreads && c < M;
+        load(c + (exact_next ? 1 : 0));
+        if (exact_next) {
             if (threadIdx.x == 0) {
                 start[c] = s;
                 exact[c] = 1;
             }
             const int len = (int)min((u64)kCdfChunk, dim - c * kCdfChunk);
-            exact_chunk<DM>(a, dim, c * kCdfChunk, len, s, cdf, P, scan_lds, min_lds, &s_bcast);
+            exact_chunk<DM>(a, dim, c * kCdfChunk, len, s, cdf, P, X, r);
             c += 1;
         }
     }
@@ -637,15 +762,14 @@ struct DevSampleJob {
 
 __global__ __launch_bounds__(kCdfThreads) void k_cdf_jobs(const DevSampleJob* __restrict__ jobs) {
     __shared__ double P[kCdfChunk];
-    __shared__ u64 scan_lds[17];
-    __shared__ int min_lds[17];
-    __shared__ double s_bcast;
+    __shared__ Xchg X[2];
+    int r = 0;
     const DevSampleJob jb = jobs[blockIdx.x];
     double s = 0.0;
     if (threadIdx.x == 0) jb.cdf[0] = 0.0;
     for (u64 c0 = 0; c0 < jb.dim; c0 += kCdfChunk) {
         const int len = (int)min((u64)kCdfChunk, jb.dim - c0);
-        exact_chunk<false>(jb.state, jb.dim, c0, len, s, jb.cdf, P, scan_lds, min_lds, &s_bcast);
+        exact_chunk<false>(jb.state, jb.dim, c0, len, s, jb.cdf, P, X, r);
     }
     __syncthreads();
     for (u32 c = threadIdx.x; c < jb.count; c += blockDim.x)

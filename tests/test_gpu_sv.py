@@ -294,6 +294,19 @@ def _sampling_states(rng, n):
     # short-mantissa probabilities: ties in the sequential rounding
     t = np.sqrt(np.resize([3.0, 1.0, 5.0, 7.0], dim) * 2.0 ** -(n + 2)).astype(np.complex128)
     yield "ties", t
+    # growing amplitudes: the running sum crosses a binade every few dozen
+    # elements over the whole array, not only at its start
+    g = random_state(rng, n) * np.exp(np.arange(dim) / (dim / 30))
+    yield "growing", g / np.linalg.norm(g)
+    # spikes: runs of four consecutive elements each at least twice the running
+    # sum in front of it (several binade crossings inside one group of four),
+    # on a floor of tiny values with short mantissas (ties)
+    sp = np.sqrt(np.resize([3.0, 1.0, 5.0, 7.0], dim) * 2.0 ** -40).astype(np.complex128)
+    step = max(dim // 64, 16)
+    for k, i in enumerate(range(step // 2, dim - 4, step)):
+        i -= i % 4
+        sp[i: i + 4] = 2.0 ** (4 * k) * np.array([1.0, 2.0, 4.0, 8.0]) * (1 + 0.3j)
+    yield "spikes", sp / np.linalg.norm(sp)
 
 
 @pytest.mark.parametrize("n", [6, 12, 18])
