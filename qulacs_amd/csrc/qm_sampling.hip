@@ -123,8 +123,11 @@ __device__ __forceinline__ u64 block_excl_scan(u64 v, u64* lds /* >= 16 */, u64*
 constexpr int kCdfWaves = kCdfThreads / 64;
 struct Xchg {
     u64 sum[kCdfWaves];
+    u64 sum2[kCdfWaves];
     u64 val[kCdfWaves];
+    double dv[kCdfWaves];
     int mn[kCdfWaves];
+    int ok[kCdfWaves];
     double s;
     int first;
 };
@@ -132,6 +135,7 @@ struct Xchg {
 struct XScan {
     u64 excl;       // sum of v over the threads before this one
     u64 total;      // sum of v over the block
+    u64 total2;     // sum of v2 over the block (when asked for)
     u64 wave_incl;  // in lane w < 16: sum of v over waves 0..w
     int first;      // m of the first thread with m != INT_MAX (INT_MAX if none)
 };
@@ -166,6 +170,26 @@ __device__ __forceinline__ u64 readlane_u64(u64 x, int l) {
     const u32 hi = (u32)__builtin_amdgcn_readlane((int)(u32)(x >> 32), l);
     return ((u64)hi << 32) | lo;
 }
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ double dpp_f64(double x) {
+    return __longlong_as_double((long long)dpp_u64<CTRL, ROW_MASK>((u64)__double_as_longlong(x)));
+}
+__device__ __forceinline__ double row_incl_scan_f64(double x) {
+    x += dpp_f64<0x111>(x);
+    x += dpp_f64<0x112>(x);
+    x += dpp_f64<0x114>(x);
+    x += dpp_f64<0x118>(x);
+    return x;
+}
+__device__ __forceinline__ double wave_incl_scan_f64(double x) {
+    x = row_incl_scan_f64(x);
+    x += dpp_f64<0x142, 0xa>(x);
+    x += dpp_f64<0x143, 0xc>(x);
+    return x;
+}
+__device__ __forceinline__ double readlane_f64(double x, int l) {
+    return __longlong_as_double((long long)readlane_u64((u64)__double_as_longlong(x), l));
+}
 // m of the lowest lane with m != INT_MAX (INT_MAX if none).  Every use passes
 // an element or thread index that grows with the lane, so this is the minimum.
 __device__ __forceinline__ int wave_first_set(int m) {
@@ -173,9 +197,12 @@ __device__ __forceinline__ int wave_first_set(int m) {
     return b ? __builtin_amdgcn_readlane(m, __ffsll((unsigned long long)b) - 1) : INT_MAX;
 }
 
-// Exclusive scan of v and the first m != INT_MAX over the block (m must grow
-// with the thread index where it is set).  The sums must not exceed 2^64 - 1.
-__device__ __forceinline__ XScan xchg_scan_first(u64 v, int m, Xchg* X, int& r) {
+// Exclusive scan of v and the first m != INT_MAX over a block of NW waves (m
+// must grow with the thread index where it is set), and with TWO the block sum
+// of v2.  The sums must not exceed 2^64 - 1.
+template <int NW = kCdfWaves, bool TWO = false>
+__device__ __forceinline__ XScan xchg_scan_first(u64 v, int m, Xchg* X, int& r, u64 v2 = 0) {
+    static_assert(NW >= 1 && NW <= kCdfWaves, "at most 16 waves");
     const int lane = threadIdx.x & 63;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const u64 x = wave_incl_scan(v);
@@ -183,12 +210,18 @@ __device__ __forceinline__ XScan xchg_scan_first(u64 v, int m, Xchg* X, int& r) 
     Xchg& slot = X[r & 1];
     if (lane == 63) slot.sum[wid] = x;
     if (lane == 0) slot.mn[wid] = mf;
+    if constexpr (TWO) {
+        const u64 x2 = wave_incl_scan(v2);
+        if (lane == 63) slot.sum2[wid] = x2;
+    }
     __syncthreads();
     ++r;
-    const u64 t = row_incl_scan(lane < kCdfWaves ? slot.sum[lane] : 0);  // lanes 0..15: over waves 0..lane
-    const int tm = lane < kCdfWaves ? slot.mn[lane] : INT_MAX;
+    const u64 t = row_incl_scan(lane < NW ? slot.sum[lane] : 0);  // lanes 0..NW-1: over waves 0..lane
+    const int tm = lane < NW ? slot.mn[lane] : INT_MAX;
     XScan o;
-    o.total = readlane_u64(t, kCdfWaves - 1);
+    o.total = readlane_u64(t, NW - 1);
+    o.total2 = 0;
+    if constexpr (TWO) o.total2 = readlane_u64(row_incl_scan(lane < NW ? slot.sum2[lane] : 0), NW - 1);
     o.excl = (wid == 0 ? 0 : readlane_u64(t, wid - 1)) + (x - v);
     o.wave_incl = t;
     o.first = wave_first_set(tm);
@@ -275,7 +308,7 @@ __device__ __forceinline__ void exact_chunk(const double2* __restrict__ a, u64 d
                 const int j = lo + k;
                 if (j >= pos && j < len && P[j] != 0.0) { first = j; break; }
             }
-            first = xchg_scan_first(0, first, X, r).first;
+            first = xchg_scan_first<>(0, first, X, r).first;
             const int stop = first == INT_MAX ? len : first;
             for (int k = 0; k < kCdfPer; ++k) {
                 const int j = lo + k;
@@ -313,7 +346,7 @@ __device__ __forceinline__ void exact_chunk(const double2* __restrict__ a, u64 d
         // A thread whose sum reaches 2^53 holds a crossing; clamping keeps the
         // block sum below 2^63 and changes no sum before the first crossing.
         if (mine > LIM) mine = LIM;
-        const XScan xs = xchg_scan_first(mine, tie, X, r);
+        const XScan xs = xchg_scan_first<>(mine, tie, X, r);
         const int first_tie = xs.first;
         // The partial sums S + (increments up to element j) never decrease, so
         // the first element reaching 2^53 (the first crossing) lies in the
@@ -387,20 +420,55 @@ __global__ __launch_bounds__(kCdfThreads) void k_cdf_exact(const double2* __rest
 // ------------------------------------------------------------------ multi-block CDF
 // The same sums for large states, spread over the whole GPU:
 //   A (grid): approximate chunk totals T[c] (used only to predict);
-//   B (1 block): predicted binade E[c] of the running sum at each chunk start;
-//   C (grid): per chunk, the integer increments in the predicted binade, their
-//      total N[c], and slow[c] if any increment is a tie or would cross the binade;
-//   D (1 block): the exact running sum walks the chunks 1024 at a time: a chunk
-//      is fast when it is all zeros or when it is not slow, its predicted binade
-//      equals the running sum's binade and the sum stays below 2^53 units; fast
-//      chunks only record their exact start value, any other chunk is summed
-//      element by element (exact_chunk) before the walk continues;
-//   E (grid): every fast chunk writes its CDF values from its exact start.
-// A wrong prediction only turns a chunk slow; every value written equals the
-// sequential loop's.
+//   B (1 block): predicted running sum in front of each chunk and its binade E[c];
+//   C (grid): per chunk, the integer increments in the predicted binade and
+//      their total N[c], and the chunk's kind: plain (no tie, no crossing
+//      predicted), one tie (its position and the parity that decides its
+//      rounding), one predicted binade crossing (its position, the increments
+//      before it, its own probability and the increments after it in the next
+//      binade), or anything else;
+//   D (1 block): the exact running sum walks the chunks 1024 at a time: plain
+//      chunks in the running sum's binade are fast while the sum stays below
+//      2^53 units and only record their exact start value; at the first other
+//      chunk the walk settles a single tie or a single crossing from the
+//      precomputed values (one IEEE addition for the crossing) when the exact
+//      start confirms them, and sums the chunk element by element
+//      (exact_chunk) otherwise;
+//   E (grid): every chunk not summed by the walk writes its CDF values from its
+//      exact start.
+// A wrong prediction only sends a chunk to exact_chunk; every value written
+// equals the sequential loop's.
 constexpr int kPredictThreads = 1024;
 constexpr int kIncrThreads = 256;
+constexpr int kIncrWaves = kIncrThreads / 64;
+constexpr int kIncrPer = kCdfChunk / kIncrThreads;  // elements per thread (16)
 constexpr u64 kMultiBlockMinDim = 1ULL << 16;
+
+// chunk kinds (stage C)
+constexpr int kChunkPlain = 0;
+constexpr int kChunkTie = 1;
+constexpr int kChunkCross = 2;
+constexpr int kChunkExact = 3;
+// how the walk left a chunk (stage D)
+constexpr int kStartPlain = 0;  // fill from the start value in the predicted binade
+constexpr int kStartExact = 1;  // summed by the walk
+constexpr int kStartCross = 2;  // fill around the crossing settled by the walk
+constexpr int kStartTie = 3;    // fill with the tie rounded from the start's parity
+
+struct ChunkSums {
+    double* T;      // approximate chunk totals
+    double* pst;    // predicted running sum in front of each chunk
+    double* ep;     // crossing chunk: probability of the crossing element; after the walk, the sum through it
+    double* start;  // exact running sum in front of each chunk (walk)
+    u64* N;         // increments in the predicted binade (a tie counted rounded down)
+    u64* eA;        // tie: parity of the increments before it plus its floor; crossing: increments before it
+    u64* eR;        // crossing: the crossing element's increment
+    u64* eB;        // crossing: increments after it, in the next binade
+    int* E;         // predicted binade of the sum in front (INT_MIN: zero)
+    int* kind;      // kChunk*
+    int* ej;        // tie / crossing: element index within the chunk
+    int* exact;     // kStart*
+};
 
 template <bool DM>
 __device__ __forceinline__ double prob_at(const double2* __restrict__ a, u64 i, u64 dim) {
@@ -419,92 +487,157 @@ __global__ __launch_bounds__(kIncrThreads) void k_chunk_total(const double2* __r
     if (threadIdx.x == 0) T[blockIdx.x] = acc;
 }
 
-__global__ __launch_bounds__(kPredictThreads) void k_chunk_predict(const double* __restrict__ T, u64 M,
-                                                                   int* __restrict__ E) {
-    __shared__ double part[kPredictThreads];
+__global__ __launch_bounds__(kPredictThreads) void k_chunk_predict(ChunkSums cs, u64 M) {
+    static_assert(kPredictThreads == kCdfThreads, "the block scan below assumes 16 waves");
+    __shared__ double wsum[kCdfWaves];
     const u64 seg = (M + kPredictThreads - 1) / kPredictThreads;
     const u64 lo = (u64)threadIdx.x * seg, hi = min(lo + seg, M);
     double local = 0.0;
-    for (u64 c = lo; c < hi; ++c) local += T[c];
-    part[threadIdx.x] = local;
+    for (u64 c = lo; c < hi; ++c) local += cs.T[c];
+    // Exclusive block scan of the segment sums.  The order of these additions
+    // only moves a prediction: a chunk predicted wrongly is summed exactly by
+    // the walk.
+    const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const double x = wave_incl_scan_f64(local);
+    if (lane == 63) wsum[wid] = x;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double run = 0.0;
-        for (int t = 0; t < kPredictThreads; ++t) {
-            const double v = part[t];
-            part[t] = run;
-            run += v;
-        }
-    }
-    __syncthreads();
-    double run = part[threadIdx.x];
+    const double t = row_incl_scan_f64(lane < kCdfWaves ? wsum[lane] : 0.0);
+    double run = (wid == 0 ? 0.0 : readlane_f64(t, wid - 1)) + dpp_f64<0x138>(x);  // wave_shr:1
     for (u64 c = lo; c < hi; ++c) {
-        E[c] = run == 0.0 ? INT_MIN : max(ilogb(run), -1022);
-        run += T[c];
+        cs.pst[c] = run;
+        cs.E[c] = run == 0.0 ? INT_MIN : max(ilogb(run), -1022);
+        run += cs.T[c];
     }
 }
 
 template <bool DM>
-__global__ __launch_bounds__(kIncrThreads) void k_chunk_incr(const double2* __restrict__ a, u64 dim,
-                                                             const int* __restrict__ E, const double* __restrict__ T,
-                                                             u64* __restrict__ N, int* __restrict__ slow) {
-    __shared__ u64 lds_u[kIncrThreads / 64];
-    __shared__ int lds_i[kIncrThreads / 64];
+__global__ __launch_bounds__(kIncrThreads) void k_chunk_incr(const double2* __restrict__ a, u64 dim, ChunkSums cs) {
+    __shared__ Xchg X[2];
+    int r = 0;
     const u64 c = blockIdx.x;
-    const double tc = T[c];
-    const int e = E[c];
+    const double tc = cs.T[c];
+    const int e = cs.E[c];
     if (tc == 0.0 || !isfinite(tc) || e == INT_MIN) {
         if (threadIdx.x == 0) {
-            N[c] = 0;
-            slow[c] = tc == 0.0 ? 0 : 1;
+            cs.N[c] = 0;
+            cs.kind[c] = tc == 0.0 ? kChunkPlain : kChunkExact;
         }
         return;
     }
     const u64 c0 = c * kCdfChunk;
     const int len = (int)min((u64)kCdfChunk, dim - c0);
     const u64 LIM = 1ULL << 53;
-    u64 acc = 0;
-    int bad = 0;
-    for (int j = threadIdx.x; j < len; j += blockDim.x) {
-        const double q = ldexp(prob_at<DM>(a, c0 + j, dim), 52 - e);
-        if (!(q < 0x1p53)) {
-            bad = 1;
-            continue;
-        }
-        const double fl = floor(q), fr = q - fl;
-        if (fr == 0.5) bad = 1;
-        acc += (u64)fl + (fr > 0.5 ? 1 : 0);
-    }
-    if (acc >= LIM) bad = 1;  // keeps every per-thread value below 2^53, the block sum below 2^61
+    // predicted sum in front of the chunk in units of binade e (only used to
+    // place a crossing, which the walk confirms from the exact sum)
+    const double sp = ldexp(cs.pst[c], 52 - e);
+    const u64 S0 = sp < 0x1p53 ? (u64)sp : LIM;
+    const int lo = threadIdx.x * kIncrPer;
+    double p[kIncrPer];
+    u64 R[kIncrPer];
+    u64 mine = 0, nties = 0, nhuge = 0;
+    int tie = INT_MAX;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        acc += __shfl_xor(acc, off, 64);
-        bad |= __shfl_xor(bad, off, 64);
-    }
-    const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (lane == 0) {
-        lds_u[wid] = acc;
-        lds_i[wid] = bad;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        u64 tot = 0;
-        int b = 0;
-        for (int w = 0; w < kIncrThreads / 64; ++w) {
-            tot += lds_u[w];
-            b |= lds_i[w];
+    for (int k = 0; k < kIncrPer; ++k) {
+        const int j = lo + k;
+        p[k] = 0.0;
+        R[k] = 0;
+        if (j < len) {
+            p[k] = prob_at<DM>(a, c0 + j, dim);
+            const double q = ldexp(p[k], 52 - e);
+            if (!(q < 0x1p53)) {  // certainly crosses (or NaN)
+                R[k] = LIM;
+                ++nhuge;
+            } else {
+                const double fl = floor(q), fr = q - fl;
+                R[k] = (u64)fl + (fr > 0.5 ? 1 : 0);
+                if (fr == 0.5) {
+                    ++nties;
+                    if (tie == INT_MAX) tie = j;
+                }
+            }
         }
-        N[c] = tot;
-        slow[c] = (b || tot >= LIM) ? 1 : 0;
+        mine += R[k];
+    }
+    // clamped per thread: block sums stay below 2^61, no sum before the first
+    // crossing changes
+    if (mine > LIM) mine = LIM;
+    const XScan xa = xchg_scan_first<kIncrWaves, true>(mine, tie, X, r, (nties << 32) | nhuge);
+    const u64 N = xa.total;
+    const u64 ties = xa.total2 >> 32, huge = xa.total2 & 0xffffffffu;
+    int kind = kChunkExact;
+    if (S0 + N < LIM) {
+        if (huge == 0 && ties == 0) {
+            kind = kChunkPlain;
+        } else if (huge == 0 && ties == 1) {
+            kind = kChunkTie;
+            u64 run = xa.excl;  // exact: every partial sum is below 2^53
+#pragma unroll
+            for (int k = 0; k < kIncrPer; ++k) {
+                if (lo + k == xa.first) {
+                    cs.ej[c] = xa.first;
+                    cs.eA[c] = (run + R[k]) & 1;  // R[k] is the tie's floor
+                }
+                run += R[k];
+            }
+        }
+    } else {
+        // a crossing is predicted: the first element whose partial sum reaches 2^53 units
+        u64 run = S0 + xa.excl;
+        int mc = INT_MAX;
+        u64 A = 0, Rj = 0;
+        double pj = 0.0;
+#pragma unroll
+        for (int k = 0; k < kIncrPer; ++k) {
+            const int j = lo + k;
+            if (j < len && mc == INT_MAX) {
+                if (run + R[k] >= LIM) {
+                    mc = j;
+                    A = run - S0;
+                    Rj = R[k];
+                    pj = p[k];
+                }
+                run += R[k];
+            }
+        }
+        const int jx = xchg_scan_first<kIncrWaves>(0, mc, X, r).first;
+        // after it, increments in the next binade (units of 2u); a tie there or
+        // a second crossing leaves the chunk to the walk's element-wise sum
+        u64 B = 0, bad = 0;
+#pragma unroll
+        for (int k = 0; k < kIncrPer; ++k) {
+            const int j = lo + k;
+            if (j < len && j > jx) {
+                const double q2 = ldexp(p[k], 51 - e);
+                if (!(q2 < 0x1p53)) {
+                    ++bad;
+                } else {
+                    const double fl = floor(q2), fr = q2 - fl;
+                    B += (u64)fl + (fr > 0.5 ? 1 : 0);
+                    if (fr == 0.5) ++bad;
+                }
+            }
+        }
+        if (B > LIM) B = LIM;
+        const XScan xb = xchg_scan_first<kIncrWaves, true>(B, INT_MAX, X, r, bad);
+        // a tie at or before the crossing element also leaves it to the walk
+        if (xa.first > jx && xb.total2 == 0 && xb.total < LIM) kind = kChunkCross;
+        if (mc != INT_MAX && mc == jx) {
+            cs.ej[c] = jx;
+            cs.eA[c] = A;
+            cs.eR[c] = Rj;
+            cs.ep[c] = pj;
+        }
+        if (threadIdx.x == 0) cs.eB[c] = xb.total;
+    }
+    if (threadIdx.x == 0) {
+        cs.N[c] = N;
+        cs.kind[c] = kind;
     }
 }
 
 template <bool DM>
 __global__ __launch_bounds__(kCdfThreads) void k_cdf_walk(const double2* __restrict__ a, u64 dim, u64 M,
-                                                          const int* __restrict__ E, const u64* __restrict__ N,
-                                                          const int* __restrict__ slow, const double* __restrict__ T,
-                                                          double* __restrict__ start, int* __restrict__ exact,
-                                                          double* __restrict__ cdf) {
+                                                          ChunkSums cs, double* __restrict__ cdf) {
     __shared__ double P[kCdfChunk];
     __shared__ Xchg X[2];
     int r = 0;
@@ -512,21 +645,25 @@ __global__ __launch_bounds__(kCdfThreads) void k_cdf_walk(const double2* __restr
     const u64 LIM = 1ULL << 53;
     double s = 0.0;  // exact running sum, identical in every thread
     if (threadIdx.x == 0) cdf[0] = 0.0;
-    // chunk summaries of this thread's chunk in the window, loaded one window
-    // ahead (the next window's loads are in flight during an exact chunk)
-    double tT = 0.0;
-    int tslow = 0, tE = 0;
-    u64 tN = 0;
+    // summaries of this thread's chunk in the window, loaded one window ahead
+    // (the next window's loads are in flight during an exact chunk)
+    double tT = 0.0, tp = 0.0;
+    int tkind = 0, tE = 0;
+    u64 tN = 0, tA = 0, tR = 0, tB = 0;
     auto load = [&](u64 cc) {
         const u64 ci = cc + threadIdx.x;
         tT = 0.0;
-        tslow = tE = 0;
+        tkind = tE = 0;
         tN = 0;
         if (ci < M) {
-            tT = T[ci];
-            tslow = slow[ci];
-            tE = E[ci];
-            tN = N[ci];
+            tT = cs.T[ci];
+            tkind = cs.kind[ci];
+            tE = cs.E[ci];
+            tN = cs.N[ci];
+            tA = cs.eA[ci];  // meaningful for tie and crossing chunks only
+            tR = cs.eR[ci];
+            tB = cs.eB[ci];
+            tp = cs.ep[ci];
         }
     };
     u64 c = 0;
@@ -542,35 +679,68 @@ __global__ __launch_bounds__(kCdfThreads) void k_cdf_walk(const double2* __restr
             S = (u64)(s / u);
         }
         bool ok = false;
-        u64 v = 0;  // below 2^53 (a chunk whose sum reaches it is slow)
+        u64 v = 0;  // below 2^53 (a chunk whose sum reaches it is not plain)
         if (ci < M && isfinite(s)) {
             if (tT == 0.0) {
                 ok = true;
-            } else if (s != 0.0 && !tslow && tE == es) {
+            } else if (s != 0.0 && tkind == kChunkPlain && tE == es) {
                 ok = true;
                 v = tN;
             }
         }
-        const XScan xs = xchg_scan_first(v, ok ? INT_MAX : (int)threadIdx.x, X, r);
+        const XScan xs = xchg_scan_first<>(v, ok ? INT_MAX : (int)threadIdx.x, X, r);
         const int first_bad = xs.first;
         // Chunk ci is fast when every chunk up to it is ok and the sum stays
         // below 2^53 units through it; the sums never decrease, so the fast
         // chunks are the first `stop` of the window.
         const bool fast = (int)threadIdx.x < first_bad && S + xs.excl + v < LIM;
         if (fast) {
-            start[ci] = s == 0.0 ? 0.0 : (double)(S + xs.excl) * u;
-            exact[ci] = 0;
+            cs.start[ci] = s == 0.0 ? 0.0 : (double)(S + xs.excl) * u;
+            cs.exact[ci] = kStartPlain;
         }
-        // stop and the sum in front of chunk c + stop: from the first wave
-        // holding a chunk that is not fast
+        // A tie or crossing chunk settled from its precomputed values, valid
+        // for the first chunk that is not fast (whose start S + excl is exact).
+        bool rok = false;
+        int rcode = kStartExact;
+        double rend = 0.0, rsj = 0.0;
+        if (!fast && ci < M && s != 0.0 && isfinite(s) && tE == es && tT != 0.0) {
+            const u64 S0 = S + xs.excl;
+            if (tkind == kChunkTie) {
+                // fl(x + (F + 1/2) u) rounds to the even neighbour
+                const u64 endS = S0 + tN + ((S0 + tA) & 1);
+                if (endS < LIM) {
+                    rok = true;
+                    rcode = kStartTie;
+                    rend = (double)endS * u;
+                }
+            } else if (tkind == kChunkCross && S0 + tA < LIM && S0 + tA + tR >= LIM) {
+                const double sj = (double)(S0 + tA) * u + tp;  // the one IEEE addition
+                if (isfinite(sj) && ilogb(sj) == es + 1) {
+                    const double u2 = 2.0 * u;
+                    const u64 S1 = (u64)(sj / u2);  // exact: sj is a multiple of 2u below 2^53 * 2u
+                    if (S1 + tB < LIM) {
+                        rok = true;
+                        rcode = kStartCross;
+                        rend = (double)(S1 + tB) * u2;
+                        rsj = sj;
+                    }
+                }
+            }
+        }
+        // stop, the sum in front of chunk c + stop and that chunk's settlement:
+        // from the first wave holding a chunk that is not fast
         {
             const u64 nf = __ballot(!fast);
             const int fl = nf ? __ffsll((unsigned long long)nf) - 1 : 0;
             const u64 at = readlane_u64(S + xs.excl, fl);
+            const int rk = __builtin_amdgcn_readlane((int)rok, fl);
+            const double re = readlane_f64(rend, fl);
             Xchg& slot = X[r & 1];
             if (lane == 0) {
                 slot.mn[wid] = nf ? wid * 64 + fl : INT_MAX;
                 slot.val[wid] = at;
+                slot.ok[wid] = rk;
+                slot.dv[wid] = re;
             }
             __syncthreads();
             ++r;
@@ -578,64 +748,102 @@ __global__ __launch_bounds__(kCdfThreads) void k_cdf_walk(const double2* __restr
         const Xchg& got = X[(r - 1) & 1];
         const int tm = lane < kCdfWaves ? got.mn[lane] : INT_MAX;
         const u64 tv = lane < kCdfWaves ? got.val[lane] : 0;
+        const int tok = lane < kCdfWaves ? got.ok[lane] : 0;
+        const double tdv = lane < kCdfWaves ? got.dv[lane] : 0.0;
         const u64 has = __ballot(tm != INT_MAX);
         int stop = kCdfThreads;
         u64 Sstop = S + xs.total;
+        bool settled = false;
+        double s_after = 0.0;
         if (has) {
             const int w = __ffsll((unsigned long long)has) - 1;
             stop = __builtin_amdgcn_readlane(tm, w);
             Sstop = readlane_u64(tv, w);
+            settled = __builtin_amdgcn_readlane(tok, w) != 0;
+            s_after = readlane_f64(tdv, w);
         }
         if (stop > 0 && s != 0.0) s = (double)Sstop * u;
         c += (u64)stop;
-        const bool exact_next = stop < kCdfThreads && c < M;
-        load(c + (exact_next ? 1 : 0));
-        if (exact_next) {
-            if (threadIdx.x == 0) {
-                start[c] = s;
-                exact[c] = 1;
+        const bool handle = stop < kCdfThreads && c < M;
+        load(c + (handle ? 1 : 0));
+        if (handle) {
+            if (settled) {
+                if ((int)threadIdx.x == stop) {
+                    cs.start[c] = s;
+                    cs.exact[c] = rcode;
+                    if (rcode == kStartCross) cs.ep[c] = rsj;
+                }
+                s = s_after;
+            } else {
+                if (threadIdx.x == 0) {
+                    cs.start[c] = s;
+                    cs.exact[c] = kStartExact;
+                }
+                const int len = (int)min((u64)kCdfChunk, dim - c * kCdfChunk);
+                exact_chunk<DM>(a, dim, c * kCdfChunk, len, s, cdf, P, X, r);
             }
-            const int len = (int)min((u64)kCdfChunk, dim - c * kCdfChunk);
-            exact_chunk<DM>(a, dim, c * kCdfChunk, len, s, cdf, P, X, r);
             c += 1;
         }
     }
 }
 
 template <bool DM>
-__global__ __launch_bounds__(kCdfThreads) void k_cdf_fill(const double2* __restrict__ a, u64 dim,
-                                                          const int* __restrict__ E, const double* __restrict__ T,
-                                                          const double* __restrict__ start,
-                                                          const int* __restrict__ exact, double* __restrict__ cdf) {
+__global__ __launch_bounds__(kCdfThreads) void k_cdf_fill(const double2* __restrict__ a, u64 dim, ChunkSums cs,
+                                                          double* __restrict__ cdf) {
     __shared__ u64 scan_lds[17];
     const u64 c = blockIdx.x;
-    if (exact[c]) return;
+    const int code = cs.exact[c];
+    if (code == kStartExact) return;
     const u64 c0 = c * kCdfChunk;
     const int len = (int)min((u64)kCdfChunk, dim - c0);
     const int lo = threadIdx.x * kCdfPer;
-    const double s0 = start[c];
-    if (T[c] == 0.0) {
+    const double s0 = cs.start[c];
+    if (cs.T[c] == 0.0) {
         for (int k = 0; k < kCdfPer; ++k)
             if (lo + k < len) cdf[c0 + lo + k + 1] = s0;
         return;
     }
-    const int e = E[c];
+    const int e = cs.E[c];
     const double u = ldexp(1.0, e - 52);
     const u64 S = (u64)(s0 / u);
+    const int jx = code == kStartPlain ? -1 : cs.ej[c];
     u64 R[kCdfPer];
     u64 mine = 0;
 #pragma unroll
     for (int k = 0; k < kCdfPer; ++k) {
+        const int j = lo + k;
         R[k] = 0;
-        if (lo + k < len) {
-            const double q = ldexp(prob_at<DM>(a, c0 + lo + k, dim), 52 - e);
-            const double fl = floor(q), fr = q - fl;
-            R[k] = (u64)fl + (fr > 0.5 ? 1 : 0);
+        if (j < len) {
+            const double pr = prob_at<DM>(a, c0 + j, dim);
+            if (code == kStartCross && j > jx) {
+                const double q2 = ldexp(pr, 51 - e);  // next binade: units of 2u
+                const double fl = floor(q2), fr = q2 - fl;
+                R[k] = (u64)fl + (fr > 0.5 ? 1 : 0);
+            } else if (!(code == kStartCross && j == jx)) {
+                const double q = ldexp(pr, 52 - e);
+                const double fl = floor(q), fr = q - fl;
+                R[k] = (u64)fl + (fr > 0.5 ? 1 : 0);
+                if (code == kStartTie && j == jx) R[k] = (u64)fl + ((S + cs.eA[c]) & 1);
+            }
         }
         mine += R[k];
     }
     u64 total;
     u64 run = S + block_excl_scan(mine, scan_lds, &total);
+    if (code == kStartCross) {
+        const double sj = cs.ep[c];
+        const double u2 = 2.0 * u;
+        const u64 S1A = (u64)(sj / u2) - cs.eA[c] - S;  // units of 2u: S1 - A - S, added to run after the crossing
+#pragma unroll
+        for (int k = 0; k < kCdfPer; ++k) {
+            const int j = lo + k;
+            if (j < len) {
+                run += R[k];
+                cdf[c0 + j + 1] = j < jx ? (double)run * u : j == jx ? sj : (double)(run + S1A) * u2;
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < kCdfPer; ++k) {
         if (lo + k < len) {
@@ -649,7 +857,7 @@ __global__ __launch_bounds__(kCdfThreads) void k_cdf_fill(const double2* __restr
 // cdf_work_bytes(dim) bytes.
 size_t cdf_work_bytes(u64 dim) {
     const u64 M = (dim + kCdfChunk - 1) / kCdfChunk;
-    return (size_t)M * (sizeof(double) * 2 + sizeof(u64) + sizeof(int) * 3) + 1024;
+    return (size_t)M * (8 * sizeof(double) + 4 * sizeof(int)) + 1024;
 }
 
 template <bool DM>
@@ -659,18 +867,27 @@ void enqueue_cdf(const double2* a, u64 dim, double* cdf, void* work, hipStream_t
         return;
     }
     const u64 M = (dim + kCdfChunk - 1) / kCdfChunk;
-    char* w = (char*)work;
-    double* T = (double*)w;
-    double* start = T + M;
-    u64* N = (u64*)(start + M);
-    int* E = (int*)(N + M);
-    int* slow = E + M;
-    int* exact = slow + M;
-    k_chunk_total<DM><<<(unsigned)M, kIncrThreads, 0, st>>>(a, dim, T);
-    k_chunk_predict<<<1, kPredictThreads, 0, st>>>(T, M, E);
-    k_chunk_incr<DM><<<(unsigned)M, kIncrThreads, 0, st>>>(a, dim, E, T, N, slow);
-    k_cdf_walk<DM><<<1, kCdfThreads, 0, st>>>(a, dim, M, E, N, slow, T, start, exact, cdf);
-    k_cdf_fill<DM><<<(unsigned)M, kCdfThreads, 0, st>>>(a, dim, E, T, start, exact, cdf);
+    ChunkSums cs;
+    double* d = (double*)work;
+    cs.T = d;
+    cs.pst = d + M;
+    cs.ep = d + 2 * M;
+    cs.start = d + 3 * M;
+    u64* q = (u64*)(d + 4 * M);
+    cs.N = q;
+    cs.eA = q + M;
+    cs.eR = q + 2 * M;
+    cs.eB = q + 3 * M;
+    int* w = (int*)(q + 4 * M);
+    cs.E = w;
+    cs.kind = w + M;
+    cs.ej = w + 2 * M;
+    cs.exact = w + 3 * M;
+    k_chunk_total<DM><<<(unsigned)M, kIncrThreads, 0, st>>>(a, dim, cs.T);
+    k_chunk_predict<<<1, kPredictThreads, 0, st>>>(cs, M);
+    k_chunk_incr<DM><<<(unsigned)M, kIncrThreads, 0, st>>>(a, dim, cs);
+    k_cdf_walk<DM><<<1, kCdfThreads, 0, st>>>(a, dim, M, cs, cdf);
+    k_cdf_fill<DM><<<(unsigned)M, kCdfThreads, 0, st>>>(a, dim, cs, cdf);
 }
 
 // lower_bound(cdf[0..dim], u) - 1, as an unsigned 64-bit index (the reference's
