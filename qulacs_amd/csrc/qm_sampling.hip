@@ -513,6 +513,10 @@ __global__ __launch_bounds__(kPredictThreads) void k_chunk_predict(ChunkSums cs,
 template <bool DM>
 __global__ __launch_bounds__(kIncrThreads) void k_chunk_incr(const double2* __restrict__ a, u64 dim, ChunkSums cs) {
     __shared__ Xchg X[2];
+    // probabilities, loaded with consecutive lanes on consecutive elements and
+    // read back as 16 consecutive elements per thread (padded: element i at
+    // i + i / 16, which spreads those reads over the LDS banks)
+    __shared__ double Pl[kCdfChunk + kCdfChunk / kIncrPer];
     int r = 0;
     const u64 c = blockIdx.x;
     const double tc = cs.T[c];
@@ -526,6 +530,12 @@ __global__ __launch_bounds__(kIncrThreads) void k_chunk_incr(const double2* __re
     }
     const u64 c0 = c * kCdfChunk;
     const int len = (int)min((u64)kCdfChunk, dim - c0);
+#pragma unroll
+    for (int k = 0; k < kIncrPer; ++k) {
+        const int j = k * kIncrThreads + threadIdx.x;
+        Pl[j + j / kIncrPer] = j < len ? prob_at<DM>(a, c0 + j, dim) : 0.0;
+    }
+    __syncthreads();
     const u64 LIM = 1ULL << 53;
     // predicted sum in front of the chunk in units of binade e (only used to
     // place a crossing, which the walk confirms from the exact sum)
@@ -542,7 +552,7 @@ __global__ __launch_bounds__(kIncrThreads) void k_chunk_incr(const double2* __re
         p[k] = 0.0;
         R[k] = 0;
         if (j < len) {
-            p[k] = prob_at<DM>(a, c0 + j, dim);
+            p[k] = Pl[j + j / kIncrPer];
             const double q = ldexp(p[k], 52 - e);
             if (!(q < 0x1p53)) {  // certainly crosses (or NaN)
                 R[k] = LIM;
@@ -791,6 +801,9 @@ template <bool DM>
 __global__ __launch_bounds__(kCdfThreads) void k_cdf_fill(const double2* __restrict__ a, u64 dim, ChunkSums cs,
                                                           double* __restrict__ cdf) {
     __shared__ u64 scan_lds[17];
+    // probabilities in, CDF values out, both moved between HBM and LDS with
+    // consecutive lanes on consecutive elements (padded: element i at i + i / 4)
+    __shared__ double Pl[kCdfChunk + kCdfChunk / kCdfPer];
     const u64 c = blockIdx.x;
     const int code = cs.exact[c];
     if (code == kStartExact) return;
@@ -799,10 +812,18 @@ __global__ __launch_bounds__(kCdfThreads) void k_cdf_fill(const double2* __restr
     const int lo = threadIdx.x * kCdfPer;
     const double s0 = cs.start[c];
     if (cs.T[c] == 0.0) {
-        for (int k = 0; k < kCdfPer; ++k)
-            if (lo + k < len) cdf[c0 + lo + k + 1] = s0;
+        for (int k = 0; k < kCdfPer; ++k) {
+            const int j = k * kCdfThreads + threadIdx.x;
+            if (j < len) cdf[c0 + j + 1] = s0;
+        }
         return;
     }
+#pragma unroll
+    for (int k = 0; k < kCdfPer; ++k) {
+        const int j = k * kCdfThreads + threadIdx.x;
+        Pl[j + j / kCdfPer] = j < len ? prob_at<DM>(a, c0 + j, dim) : 0.0;
+    }
+    __syncthreads();
     const int e = cs.E[c];
     const double u = ldexp(1.0, e - 52);
     const u64 S = (u64)(s0 / u);
@@ -814,7 +835,7 @@ __global__ __launch_bounds__(kCdfThreads) void k_cdf_fill(const double2* __restr
         const int j = lo + k;
         R[k] = 0;
         if (j < len) {
-            const double pr = prob_at<DM>(a, c0 + j, dim);
+            const double pr = Pl[j + j / kCdfPer];
             if (code == kStartCross && j > jx) {
                 const double q2 = ldexp(pr, 51 - e);  // next binade: units of 2u
                 const double fl = floor(q2), fr = q2 - fl;
@@ -829,7 +850,8 @@ __global__ __launch_bounds__(kCdfThreads) void k_cdf_fill(const double2* __restr
         mine += R[k];
     }
     u64 total;
-    u64 run = S + block_excl_scan(mine, scan_lds, &total);
+    u64 run = S + block_excl_scan(mine, scan_lds, &total);  // its barriers also end the reads of Pl above
+    double v[kCdfPer];
     if (code == kStartCross) {
         const double sj = cs.ep[c];
         const double u2 = 2.0 * u;
@@ -837,19 +859,23 @@ __global__ __launch_bounds__(kCdfThreads) void k_cdf_fill(const double2* __restr
 #pragma unroll
         for (int k = 0; k < kCdfPer; ++k) {
             const int j = lo + k;
-            if (j < len) {
-                run += R[k];
-                cdf[c0 + j + 1] = j < jx ? (double)run * u : j == jx ? sj : (double)(run + S1A) * u2;
-            }
+            run += R[k];
+            v[k] = j < jx ? (double)run * u : j == jx ? sj : (double)(run + S1A) * u2;
         }
-        return;
+    } else {
+#pragma unroll
+        for (int k = 0; k < kCdfPer; ++k) {
+            run += R[k];
+            v[k] = (double)run * u;
+        }
     }
 #pragma unroll
+    for (int k = 0; k < kCdfPer; ++k) Pl[(lo + k) + (lo + k) / kCdfPer] = v[k];
+    __syncthreads();
+#pragma unroll
     for (int k = 0; k < kCdfPer; ++k) {
-        if (lo + k < len) {
-            run += R[k];
-            cdf[c0 + lo + k + 1] = (double)run * u;
-        }
+        const int j = k * kCdfThreads + threadIdx.x;
+        if (j < len) cdf[c0 + j + 1] = Pl[j + j / kCdfPer];
     }
 }
 
@@ -899,6 +925,15 @@ void enqueue_cdf(const double2* a, u64 dim, double* cdf, void* work, hipStream_t
 // or cdf[j] - u <= delta (1 <= j <= dim).  delta = margin * cdf[dim]; such a
 // sample is flagged (flag 1), any other one is certain (flag 0).  With margin
 // <= 0 nothing is flagged.
+__device__ __forceinline__ unsigned char boundary_flag(const double* __restrict__ cdf, u64 dim, double u, u64 j,
+                                                       double margin) {
+    if (!(margin > 0.0)) return 0;
+    const double delta = margin * cdf[dim];
+    if (j >= 2 && u - cdf[j - 1] <= delta) return 1;
+    if (j >= 1 && j <= dim && cdf[j] - u <= delta) return 1;
+    return 0;
+}
+
 __device__ __forceinline__ u64 search_checked(const double* __restrict__ cdf, u64 dim, double u, double margin,
                                               unsigned char* flag) {
     u64 lo = 0, hi = dim + 1;  // search [lo, hi) for the first cdf[i] >= u
@@ -907,24 +942,50 @@ __device__ __forceinline__ u64 search_checked(const double* __restrict__ cdf, u6
         if (cdf[mid] < u) lo = mid + 1;
         else hi = mid;
     }
-    if (flag) {
-        unsigned char f = 0;
-        if (margin > 0.0) {
-            const double delta = margin * cdf[dim];
-            if (lo >= 2 && u - cdf[lo - 1] <= delta) f = 1;
-            if (lo >= 1 && lo <= dim && cdf[lo] - u <= delta) f = 1;
-        }
-        *flag = f;
-    }
+    if (flag) *flag = boundary_flag(cdf, dim, u, lo, margin);
     return lo - 1;
 }
 
-__global__ void k_search(const double* __restrict__ cdf, u64 dim, const double* __restrict__ us, unsigned count,
-                         double margin, u64* __restrict__ out, unsigned char* __restrict__ flags) {
-    const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= count) return;
-    out[c] = search_checked(cdf, dim, us[c], margin, flags ? flags + c : nullptr);
+// The same search with one wave per uniform: each round the 64 lanes probe 64
+// evenly spaced entries of the candidate range at once, so a 2^20-entry CDF
+// takes four dependent rounds of loads instead of twenty-one.  Sums built by
+// the sequential loop never decrease, and once a NaN appears every later sum
+// is NaN, so "cdf[i] < u" holds on a prefix of the array and the first index
+// where it fails is the binary search's answer.
+constexpr int kSearchWaves = 4;
+__global__ __launch_bounds__(64 * kSearchWaves) void k_search(const double* __restrict__ cdf, u64 dim,
+                                                              const double* __restrict__ us, unsigned count,
+                                                              double margin, u64* __restrict__ out,
+                                                              unsigned char* __restrict__ flags) {
+    const unsigned lane = threadIdx.x & 63;
+    const unsigned c = blockIdx.x * kSearchWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (c >= count) return;  // wave-uniform
+    const double u = us[c];
+    // invariant: every index below lo has cdf < u; cdf[hi] >= u or hi = dim + 1
+    u64 lo = 0, hi = dim + 1;
+    while (hi - lo > 64) {
+        const u64 step = (hi - lo + 63) / 64;
+        const u64 idx = lo + lane * step;
+        const u64 b = __ballot(idx < hi && cdf[idx] < u);
+        const int k = __popcll(b);  // probes with cdf < u: lanes 0 .. k-1
+        if (k == 0) {
+            hi = lo;
+            break;
+        }
+        const u64 last = lo + (u64)(k - 1) * step;
+        lo = last + 1;
+        if (last + step < hi) hi = last + step;
+    }
+    if (hi > lo) {
+        const u64 idx = lo + lane;
+        lo += __popcll(__ballot(idx < hi && cdf[idx] < u));
+    }
+    if (lane == 0) {
+        out[c] = lo - 1;
+        if (flags) flags[c] = boundary_flag(cdf, dim, u, lo, margin);
+    }
 }
+inline unsigned search_blocks(unsigned count) { return (count + kSearchWaves - 1) / kSearchWaves; }
 
 size_t sampling_buffer_bytes(u64 dim, u32 count, bool flags) {
     const size_t cdf_bytes = ((size_t)(dim + 1) * sizeof(double) + 255) & ~size_t(255);
@@ -948,7 +1009,7 @@ bool enqueue_sampling_in(void* buffer, const void* state, u64 dim, const double*
     if (!hip_ok(hipMemcpyAsync(dus, h_uniforms, (size_t)count * 8, hipMemcpyHostToDevice, st), "enqueue_sampling"))
         return false;
     enqueue_cdf<false>((const double2*)state, dim, cdf, buf + cdf_bytes, st);
-    k_search<<<(count + 255) / 256, 256, 0, st>>>(cdf, dim, dus, count, margin, dout, dflags);
+    k_search<<<search_blocks(count), 64 * kSearchWaves, 0, st>>>(cdf, dim, dus, count, margin, dout, dflags);
     if (dflags &&
         !hip_ok(hipMemcpyAsync(h_flags, dflags, count, hipMemcpyDeviceToHost, st), "enqueue_sampling"))
         return false;
@@ -1101,7 +1162,7 @@ void sample(const double* uniforms, unsigned count, ITYPE* out, void* state, u64
     unsigned char* dflags = flags ? (unsigned char*)dout + io_bytes : nullptr;
     if (!hip_ok(hipMemcpyAsync(dus, uniforms, (size_t)count * 8, hipMemcpyHostToDevice, st), what)) return;
     enqueue_cdf<DM>((const double2*)state, dim, cdf, work, st);
-    k_search<<<(count + 255) / 256, 256, 0, st>>>(cdf, dim, dus, count, margin, dout, dflags);
+    k_search<<<search_blocks(count), 64 * kSearchWaves, 0, st>>>(cdf, dim, dus, count, margin, dout, dflags);
     if (!hip_ok(hipMemcpyAsync(out, dout, (size_t)count * 8, hipMemcpyDeviceToHost, st), what)) return;
     if (dflags && !hip_ok(hipMemcpyAsync(flags, dflags, count, hipMemcpyDeviceToHost, st), what)) return;
     if (!hip_ok(hipStreamSynchronize(st), what)) return;
