@@ -439,9 +439,9 @@ __global__ __launch_bounds__(kCdfThreads) void k_cdf_exact(const double2* __rest
 // A wrong prediction only sends a chunk to exact_chunk; every value written
 // equals the sequential loop's.
 constexpr int kPredictThreads = 1024;
-constexpr int kIncrThreads = 256;
+constexpr int kIncrThreads = 1024;
 constexpr int kIncrWaves = kIncrThreads / 64;
-constexpr int kIncrPer = kCdfChunk / kIncrThreads;  // elements per thread (16)
+constexpr int kIncrPer = kCdfChunk / kIncrThreads;  // elements per thread
 constexpr u64 kMultiBlockMinDim = 1ULL << 16;
 
 // chunk kinds (stage C)
@@ -514,8 +514,8 @@ template <bool DM>
 __global__ __launch_bounds__(kIncrThreads) void k_chunk_incr(const double2* __restrict__ a, u64 dim, ChunkSums cs) {
     __shared__ Xchg X[2];
     // probabilities, loaded with consecutive lanes on consecutive elements and
-    // read back as 16 consecutive elements per thread (padded: element i at
-    // i + i / 16, which spreads those reads over the LDS banks)
+    // read back as kIncrPer consecutive elements per thread (padded: element i
+    // at i + i / kIncrPer, which spreads those reads over the LDS banks)
     __shared__ double Pl[kCdfChunk + kCdfChunk / kIncrPer];
     int r = 0;
     const u64 c = blockIdx.x;
