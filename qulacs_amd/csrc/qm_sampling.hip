@@ -446,7 +446,8 @@ constexpr u64 kMultiBlockMinDim = 1ULL << 16;
 
 // chunk kinds (stage C)
 constexpr int kChunkPlain = 0;
-constexpr int kChunkTie = 1;
+constexpr int kChunkTie = 1;   // 1..kMaxTies ties, no crossing predicted
+constexpr int kMaxTies = 4;
 constexpr int kChunkCross = 2;
 constexpr int kChunkExact = 3;
 // how the walk left a chunk (stage D)
@@ -460,13 +461,14 @@ struct ChunkSums {
     double* pst;    // predicted running sum in front of each chunk
     double* ep;     // crossing chunk: probability of the crossing element; after the walk, the sum through it
     double* start;  // exact running sum in front of each chunk (walk)
-    u64* N;         // increments in the predicted binade (a tie counted rounded down)
-    u64* eA;        // tie: parity of the increments before it plus its floor; crossing: increments before it
+    u64* N;         // increments in the predicted binade (ties counted rounded down)
+    u64* eA;        // ties: bit k = parity of the increments before tie k (ties rounded down) plus its floor;
+                    // crossing: increments before it
     u64* eR;        // crossing: the crossing element's increment
-    u64* eB;        // crossing: increments after it, in the next binade
+    u64* eB;        // ties: element index of tie k in bits 12k..12k+11; crossing: increments after it, next binade
     int* E;         // predicted binade of the sum in front (INT_MIN: zero)
     int* kind;      // kChunk*
-    int* ej;        // tie / crossing: element index within the chunk
+    int* ej;        // ties: their number; crossing: element index within the chunk
     int* exact;     // kStart*
 };
 
@@ -544,6 +546,7 @@ __global__ __launch_bounds__(kIncrThreads) void k_chunk_incr(const double2* __re
     const int lo = threadIdx.x * kIncrPer;
     double p[kIncrPer];
     u64 R[kIncrPer];
+    bool tk[kIncrPer];
     u64 mine = 0, nties = 0, nhuge = 0;
     int tie = INT_MAX;
 #pragma unroll
@@ -551,6 +554,7 @@ __global__ __launch_bounds__(kIncrThreads) void k_chunk_incr(const double2* __re
         const int j = lo + k;
         p[k] = 0.0;
         R[k] = 0;
+        tk[k] = false;
         if (j < len) {
             p[k] = Pl[j + j / kIncrPer];
             const double q = ldexp(p[k], 52 - e);
@@ -561,6 +565,7 @@ __global__ __launch_bounds__(kIncrThreads) void k_chunk_incr(const double2* __re
                 const double fl = floor(q), fr = q - fl;
                 R[k] = (u64)fl + (fr > 0.5 ? 1 : 0);
                 if (fr == 0.5) {
+                    tk[k] = true;
                     ++nties;
                     if (tie == INT_MAX) tie = j;
                 }
@@ -578,16 +583,28 @@ __global__ __launch_bounds__(kIncrThreads) void k_chunk_incr(const double2* __re
     if (S0 + N < LIM) {
         if (huge == 0 && ties == 0) {
             kind = kChunkPlain;
-        } else if (huge == 0 && ties == 1) {
+        } else if (huge == 0 && ties <= kMaxTies) {
             kind = kChunkTie;
+            // rank of each tie, its position and the parity of the increments
+            // before it plus its floor (R[k] of a tie is its floor); the fields
+            // of different ties are disjoint, so their sums are their union
+            const XScan xt = xchg_scan_first<kIncrWaves>(nties, INT_MAX, X, r);
             u64 run = xa.excl;  // exact: every partial sum is below 2^53
+            u64 rank = xt.excl, bits = 0, posw = 0;
 #pragma unroll
             for (int k = 0; k < kIncrPer; ++k) {
-                if (lo + k == xa.first) {
-                    cs.ej[c] = xa.first;
-                    cs.eA[c] = (run + R[k]) & 1;  // R[k] is the tie's floor
+                if (tk[k]) {
+                    bits |= ((run + R[k]) & 1) << rank;
+                    posw |= (u64)(lo + k) << (12 * rank);
+                    ++rank;
                 }
                 run += R[k];
+            }
+            const XScan xo = xchg_scan_first<kIncrWaves, true>(bits, INT_MAX, X, r, posw);
+            if (threadIdx.x == 0) {
+                cs.ej[c] = (int)ties;
+                cs.eA[c] = xo.total;
+                cs.eB[c] = xo.total2;
             }
         }
     } else {
@@ -658,7 +675,7 @@ __global__ __launch_bounds__(kCdfThreads) void k_cdf_walk(const double2* __restr
     // summaries of this thread's chunk in the window, loaded one window ahead
     // (the next window's loads are in flight during an exact chunk)
     double tT = 0.0, tp = 0.0;
-    int tkind = 0, tE = 0;
+    int tkind = 0, tE = 0, tJ = 0;
     u64 tN = 0, tA = 0, tR = 0, tB = 0;
     auto load = [&](u64 cc) {
         const u64 ci = cc + threadIdx.x;
@@ -670,7 +687,8 @@ __global__ __launch_bounds__(kCdfThreads) void k_cdf_walk(const double2* __restr
             tkind = cs.kind[ci];
             tE = cs.E[ci];
             tN = cs.N[ci];
-            tA = cs.eA[ci];  // meaningful for tie and crossing chunks only
+            tJ = cs.ej[ci];  // meaningful for tie and crossing chunks only
+            tA = cs.eA[ci];
             tR = cs.eR[ci];
             tB = cs.eB[ci];
             tp = cs.ep[ci];
@@ -698,93 +716,120 @@ __global__ __launch_bounds__(kCdfThreads) void k_cdf_walk(const double2* __restr
                 v = tN;
             }
         }
-        const XScan xs = xchg_scan_first<>(v, ok ? INT_MAX : (int)threadIdx.x, X, r);
-        const int first_bad = xs.first;
-        // Chunk ci is fast when every chunk up to it is ok and the sum stays
-        // below 2^53 units through it; the sums never decrease, so the fast
-        // chunks are the first `stop` of the window.
-        const bool fast = (int)threadIdx.x < first_bad && S + xs.excl + v < LIM;
-        if (fast) {
-            cs.start[ci] = s == 0.0 ? 0.0 : (double)(S + xs.excl) * u;
-            cs.exact[ci] = kStartPlain;
-        }
-        // A tie or crossing chunk settled from its precomputed values, valid
-        // for the first chunk that is not fast (whose start S + excl is exact).
-        bool rok = false;
-        int rcode = kStartExact;
-        double rend = 0.0, rsj = 0.0;
-        if (!fast && ci < M && s != 0.0 && isfinite(s) && tE == es && tT != 0.0) {
-            const u64 S0 = S + xs.excl;
-            if (tkind == kChunkTie) {
-                // fl(x + (F + 1/2) u) rounds to the even neighbour
-                const u64 endS = S0 + tN + ((S0 + tA) & 1);
-                if (endS < LIM) {
-                    rok = true;
-                    rcode = kStartTie;
-                    rend = (double)endS * u;
-                }
-            } else if (tkind == kChunkCross && S0 + tA < LIM && S0 + tA + tR >= LIM) {
-                const double sj = (double)(S0 + tA) * u + tp;  // the one IEEE addition
-                if (isfinite(sj) && ilogb(sj) == es + 1) {
-                    const double u2 = 2.0 * u;
-                    const u64 S1 = (u64)(sj / u2);  // exact: sj is a multiple of 2u below 2^53 * 2u
-                    if (S1 + tB < LIM) {
-                        rok = true;
-                        rcode = kStartCross;
-                        rend = (double)(S1 + tB) * u2;
-                        rsj = sj;
+        const XScan xs = xchg_scan_first<>(v, INT_MAX, X, r);
+        // The window is walked in runs: a run ends at its first chunk that is
+        // not fast; a settled tie chunk there adds its increments (delta) to
+        // every later chunk of the window, and the next run starts after it.
+        u64 delta = 0;
+        int pos = 0;
+        int stop = kCdfThreads;
+        int settled = 0;  // 0, kStartTie or kStartCross for the chunk at stop
+        u64 Sstop = 0;
+        double s_after = 0.0;
+        double rsj = 0.0;  // this thread's sum through its crossing element (last run)
+        for (;;) {  // wave-uniform and block-uniform
+            const int t = (int)threadIdx.x;
+            const u64 S0 = S + xs.excl + delta;  // exact for the chunks of the current run and its stop
+            // Chunk ci is fast when every chunk of the run up to it is ok and
+            // the sum stays below 2^53 units through it; the sums never
+            // decrease, so the fast chunks are the first of the run.
+            const bool fast = t >= pos && ok && S0 + v < LIM;
+            if (fast) {
+                cs.start[ci] = s == 0.0 ? 0.0 : (double)S0 * u;
+                cs.exact[ci] = kStartPlain;
+            }
+            // the settlement of this chunk, should it be the run's stop
+            int rcode = 0;
+            double rend = 0.0;
+            u64 rdelta = 0;
+            if (t >= pos && !fast && ci < M && s != 0.0 && isfinite(s) && tE == es && tT != 0.0) {
+                if (tkind == kChunkTie) {
+                    // fl(x + (F + 1/2) u) rounds to the even neighbour: tie k
+                    // adds F + ((sum in front of it) mod 2)
+                    u64 add = 0;
+                    for (int k = 0; k < tJ; ++k) add += (S0 + ((tA >> k) & 1) + add) & 1;
+                    const u64 endS = S0 + tN + add;
+                    if (endS < LIM) {
+                        rcode = kStartTie;
+                        rdelta = tN + add;
+                        rend = (double)endS * u;
+                    }
+                } else if (tkind == kChunkCross && S0 + tA < LIM && S0 + tA + tR >= LIM) {
+                    const double sj = (double)(S0 + tA) * u + tp;  // the one IEEE addition
+                    if (isfinite(sj) && ilogb(sj) == es + 1) {
+                        const double u2 = 2.0 * u;
+                        const u64 S1 = (u64)(sj / u2);  // exact: sj is a multiple of 2u below 2^53 * 2u
+                        if (S1 + tB < LIM) {
+                            rcode = kStartCross;
+                            rend = (double)(S1 + tB) * u2;
+                            rsj = sj;
+                        }
                     }
                 }
             }
-        }
-        // stop, the sum in front of chunk c + stop and that chunk's settlement:
-        // from the first wave holding a chunk that is not fast
-        {
-            const u64 nf = __ballot(!fast);
-            const int fl = nf ? __ffsll((unsigned long long)nf) - 1 : 0;
-            const u64 at = readlane_u64(S + xs.excl, fl);
-            const int rk = __builtin_amdgcn_readlane((int)rok, fl);
-            const double re = readlane_f64(rend, fl);
-            Xchg& slot = X[r & 1];
-            if (lane == 0) {
-                slot.mn[wid] = nf ? wid * 64 + fl : INT_MAX;
-                slot.val[wid] = at;
-                slot.ok[wid] = rk;
-                slot.dv[wid] = re;
+            // the run's stop, the sum in front of it and its settlement: from
+            // the first wave holding a chunk of the run that is not fast
+            {
+                const u64 nf = __ballot(t >= pos && !fast);
+                const int fl = nf ? __ffsll((unsigned long long)nf) - 1 : 0;
+                Xchg& slot = X[r & 1];
+                const u64 at = readlane_u64(S0, fl);
+                const int rk = __builtin_amdgcn_readlane(rcode, fl);
+                const double re = readlane_f64(rend, fl);
+                const u64 rd = readlane_u64(rdelta, fl);
+                if (lane == 0) {
+                    slot.mn[wid] = nf ? wid * 64 + fl : INT_MAX;
+                    slot.val[wid] = at;
+                    slot.ok[wid] = rk;
+                    slot.dv[wid] = re;
+                    slot.sum2[wid] = rd;
+                }
+                __syncthreads();
+                ++r;
             }
-            __syncthreads();
-            ++r;
+            const Xchg& got = X[(r - 1) & 1];
+            const int tm = lane < kCdfWaves ? got.mn[lane] : INT_MAX;
+            const u64 has = __ballot(tm != INT_MAX);
+            stop = kCdfThreads;
+            Sstop = S + xs.total + delta;
+            settled = 0;
+            if (has) {
+                const int w = __ffsll((unsigned long long)has) - 1;
+                stop = __builtin_amdgcn_readlane(tm, w);
+                Sstop = readlane_u64(lane < kCdfWaves ? got.val[lane] : 0, w);
+                settled = __builtin_amdgcn_readlane(lane < kCdfWaves ? got.ok[lane] : 0, w);
+                s_after = readlane_f64(lane < kCdfWaves ? got.dv[lane] : 0.0, w);
+                const u64 dstop = readlane_u64(lane < kCdfWaves ? got.sum2[lane] : 0, w);
+                if (settled == kStartTie && c + (u64)stop < M) {
+                    if (t == stop) {
+                        cs.start[ci] = (double)Sstop * u;
+                        cs.exact[ci] = kStartTie;
+                    }
+                    delta += dstop;
+                    pos = stop + 1;
+                    if (pos < kCdfThreads) continue;
+                    // the window ends with the settled tie
+                    stop = kCdfThreads;
+                    Sstop = S + xs.total + delta;
+                    settled = 0;
+                }
+            }
+            break;
         }
-        const Xchg& got = X[(r - 1) & 1];
-        const int tm = lane < kCdfWaves ? got.mn[lane] : INT_MAX;
-        const u64 tv = lane < kCdfWaves ? got.val[lane] : 0;
-        const int tok = lane < kCdfWaves ? got.ok[lane] : 0;
-        const double tdv = lane < kCdfWaves ? got.dv[lane] : 0.0;
-        const u64 has = __ballot(tm != INT_MAX);
-        int stop = kCdfThreads;
-        u64 Sstop = S + xs.total;
-        bool settled = false;
-        double s_after = 0.0;
-        if (has) {
-            const int w = __ffsll((unsigned long long)has) - 1;
-            stop = __builtin_amdgcn_readlane(tm, w);
-            Sstop = readlane_u64(tv, w);
-            settled = __builtin_amdgcn_readlane(tok, w) != 0;
-            s_after = readlane_f64(tdv, w);
-        }
-        if (stop > 0 && s != 0.0) s = (double)Sstop * u;
+        if (s != 0.0 && isfinite(s)) s = (double)Sstop * u;  // the sum in front of chunk c + stop
         c += (u64)stop;
         const bool handle = stop < kCdfThreads && c < M;
+        if (handle && settled == kStartCross) {
+            if ((int)threadIdx.x == stop) {
+                cs.start[c] = s;
+                cs.exact[c] = kStartCross;
+                cs.ep[c] = rsj;  // the sum through the crossing element
+            }
+            s = s_after;
+        }
         load(c + (handle ? 1 : 0));
         if (handle) {
-            if (settled) {
-                if ((int)threadIdx.x == stop) {
-                    cs.start[c] = s;
-                    cs.exact[c] = rcode;
-                    if (rcode == kStartCross) cs.ep[c] = rsj;
-                }
-                s = s_after;
-            } else {
+            if (settled != kStartCross) {
                 if (threadIdx.x == 0) {
                     cs.start[c] = s;
                     cs.exact[c] = kStartExact;
@@ -827,7 +872,22 @@ __global__ __launch_bounds__(kCdfThreads) void k_cdf_fill(const double2* __restr
     const int e = cs.E[c];
     const double u = ldexp(1.0, e - 52);
     const u64 S = (u64)(s0 / u);
-    const int jx = code == kStartPlain ? -1 : cs.ej[c];
+    const int jx = code == kStartCross ? cs.ej[c] : -1;
+    // ties: the rounding bit of tie k follows from the parity of the sum in
+    // front of it (start, increments before it rounded down, earlier bits)
+    int nt = 0;
+    u64 tpos = 0, tbit = 0;
+    if (code == kStartTie) {
+        nt = cs.ej[c];
+        tpos = cs.eB[c];
+        const u64 A = cs.eA[c];
+        u64 add = 0;
+        for (int k = 0; k < nt; ++k) {
+            const u64 b = (S + ((A >> k) & 1) + add) & 1;
+            tbit |= b << k;
+            add += b;
+        }
+    }
     u64 R[kCdfPer];
     u64 mine = 0;
 #pragma unroll
@@ -844,7 +904,8 @@ __global__ __launch_bounds__(kCdfThreads) void k_cdf_fill(const double2* __restr
                 const double q = ldexp(pr, 52 - e);
                 const double fl = floor(q), fr = q - fl;
                 R[k] = (u64)fl + (fr > 0.5 ? 1 : 0);
-                if (code == kStartTie && j == jx) R[k] = (u64)fl + ((S + cs.eA[c]) & 1);
+                for (int k2 = 0; k2 < nt; ++k2)
+                    if (j == (int)((tpos >> (12 * k2)) & 0xfff)) R[k] = (u64)fl + ((tbit >> k2) & 1);
             }
         }
         mine += R[k];

@@ -307,6 +307,15 @@ def _sampling_states(rng, n):
         i -= i % 4
         sp[i: i + 4] = 2.0 ** (4 * k) * np.array([1.0, 2.0, 4.0, 8.0]) * (1 + 0.3j)
     yield "spikes", sp / np.linalg.norm(sp)
+    # a few exact ties per 4096 elements on a random state: probability 2^-54
+    # is half a unit of the sum's grid while the sum is in [1/2, 1), 2^-55 while
+    # it is in [1/4, 1/2) (not renormalised, which would round them)
+    st = random_state(rng, n)
+    idx = np.arange(dim)
+    pick = (idx % 1024) == 7
+    st[pick & (idx >= dim // 2)] = 2.0 ** -27
+    st[pick & (idx < dim // 2)] = 2.0 ** -28 * (1 + 1j)
+    yield "sparse_ties", st
 
 
 @pytest.mark.parametrize("n", [6, 12, 18])
