@@ -218,3 +218,37 @@ def test_33q_index_math_above_32_bits():
     obs.add_operator(0.5, "Z 32 Z 0")
     want = 0.5 * sum(abs(a) ** 2 * (1 - 2 * ((k >> 32) & 1)) * (1 - 2 * (k & 1)) for k, a in amp.items())
     assert abs(obs.get_expectation_value(s) - want) < 1e-12
+
+
+def test_16q_density_matrix_cdf_on_the_multi_block_path():
+    """A 16-qubit density matrix (4^16 entries, 64 GiB) has a 2^16-entry diagonal,
+    the size from which the CDF runs on the multi-block chunk walk.  Built on the
+    device from a real pure state, so every diagonal entry is real (|rho_ii| is
+    the entry itself); the diagonal is read back entry by entry and the device CDF
+    must equal its left-to-right sums bit for bit, with samples to match."""
+    from itertools import accumulate
+
+    n = 16
+    dim = 1 << n
+    rng = np.random.default_rng(16)
+    psi = rng.standard_normal(dim) * np.exp(-np.arange(dim) / (dim / 7))  # real, decaying: many binades
+    psi /= np.linalg.norm(psi)
+    c = _lib.call
+    with DevState.from_host(psi.astype(np.complex128)) as sv, DevState(dim * dim) as dm:
+        c("dm_initialize_with_pure_state_host", dm.ptr, sv.ptr, dim, None, 0)
+        diag = np.zeros(dim, dtype=np.complex128)
+        one = np.zeros(1, dtype=np.complex128)
+        for i in range(dim):
+            c("copy_quantum_state_from_device_to_host", one.ctypes.data, dm.ptr + 16 * i * (dim + 1), 1, None, 0)
+            diag[i] = one[0]
+        assert np.all(diag.imag == 0) and np.all(diag.real >= 0)
+        want = np.array([0.0] + list(accumulate(diag.real.tolist())))
+        got = np.zeros(dim + 1)
+        c("dm_sampling_cdf_host", got.ctypes.data_as(_lib.PD), dm.ptr, dim, None, 0)
+        bad = np.nonzero(got.view(np.uint64) != want.view(np.uint64))[0]
+        assert bad.size == 0, bad[:5]
+        us = np.concatenate([rng.random(2000) * want[-1], want[1:2000:7], np.nextafter(want[1:2000:7], 2.0)])
+        out = np.zeros(us.size, dtype=np.uint64)
+        c("dm_sampling_host", us.ctypes.data_as(_lib.PD), us.size, out.ctypes.data_as(_lib.PI), dm.ptr, dim, None, 0)
+        expect = (np.searchsorted(want, us, side="left") - 1).astype(np.int64).astype(np.uint64)
+        assert np.array_equal(out, expect)
