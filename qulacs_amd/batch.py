@@ -144,7 +144,8 @@ class FusedCircuit:
     (qm_circuit_*): one launch per stage of gates that fit a window of
     `local_qubits` qubits, register-blocked passes inside each stage."""
 
-    def __init__(self, n: int, gates, block_qubits: int = 4, local_qubits: int = 13):
+    def __init__(self, n: int, gates, block_qubits: int = 0, local_qubits: int = 0):
+        """block_qubits / local_qubits 0: chosen by qubit count (qm_circuit_create)."""
         gates = list(gates)
         arr, self._keep = _marshal(gates)
         self.n = n

@@ -1896,7 +1896,7 @@ void QuantumCircuit::build_plan() {
             lowered.push_back(bg);
         }
         if (members >= min_fused_run(qubit_count_)) {
-            qm_circuit* prog = qm_circuit_create(qubit_count_, lowered.data(), (UINT)lowered.size(), 4, 0);
+            qm_circuit* prog = qm_circuit_create(qubit_count_, lowered.data(), (UINT)lowered.size(), 0, 0);
             if (prog) {
                 plan_.push_back({i, j, prog});
                 i = j;

@@ -44,11 +44,10 @@ namespace {
 
 using clk = std::chrono::steady_clock;
 constexpr double kStarveSeconds = 0.25;
-// register-block widths of the batched and the fused executors, the fastest
-// measured for brickwork / layered circuits (batched: 3; fused, with the next
-// tile prefetched at every width: 4, 5-8 % ahead of 3 from 18 to 28 qubits)
+// register-block width of the batched executor, the fastest measured for
+// brickwork / layered circuits; large requests take the fused executor's
+// defaults, which qm_circuit_create picks by qubit count
 constexpr u32 kBatchBlockQubits = 3;
-constexpr u32 kLargeBlockQubits = 4;
 
 struct Request {
     long long id = 0;
@@ -444,7 +443,7 @@ long long qm_server_submit(qm_server* s, UINT n, const qm_batch_gate* gates, UIN
         qm_batch_destroy(probe);
         if (idx < 0) return -1;
     } else {
-        r->circuit = qm_circuit_create(n, r->gates.data(), gate_count, kLargeBlockQubits, 0);
+        r->circuit = qm_circuit_create(n, r->gates.data(), gate_count, 0, 0);
         // the program goes to the device here, in the submitting thread, so the
         // scheduler thread never waits on an upload
         if (!r->circuit || !use_device(s->device) || !circuit_prepare(r->circuit, s->device)) return -1;
