@@ -363,9 +363,9 @@ QM_EXPORT void qm_batch_stats(const qm_batch* batch, ITYPE* circuits, ITYPE* sta
  * qubits made of the batch gate kinds.  create() does host work only. */
 #define QM_CIRCUIT_MAX_QUBITS 40u
 typedef struct qm_circuit qm_circuit;
-/* block_qubits 2..4, local_qubits 8..13; 0 picks by qubit count (3 and 12 from
- * 14 to 19 qubits, else 4 and 13; the width grows to the largest dense gate,
- * at most 4 qubits); NULL on error. */
+/* block_qubits 2..4, local_qubits 8..13; 0 picks by qubit count (3 and 12 up to
+ * 19 qubits, else 4 and 13; the width grows to the largest dense gate, at most
+ * 4 qubits); NULL on error. */
 QM_EXPORT qm_circuit* qm_circuit_create(UINT qubit_count, const qm_batch_gate* gates, UINT gate_count, UINT block_qubits, UINT local_qubits);
 QM_EXPORT void qm_circuit_destroy(qm_circuit* circuit);
 /* Applies the circuit to `state` (dim = 2^qubit_count) in stream order. */

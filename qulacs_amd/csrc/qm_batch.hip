@@ -1459,11 +1459,12 @@ void* qm_batch_state_device_ptr(qm_batch* b, UINT index) {
 qm_circuit* qm_circuit_create(UINT n, const qm_batch_gate* gates, UINT gate_count, UINT block_qubits,
                               UINT local_qubits) {
     const char* what = "qm_circuit_create";
-    // Defaults (0): from 14 to 19 qubits a 13-qubit window leaves 2..64 tiles
+    // Defaults (0): up to 19 qubits a 13-qubit window leaves at most 64 tiles
     // per stage for 256 CUs; 12-qubit windows with 3-qubit blocks were 11-28 %
-    // faster there on 10-layer brickwork circuits (14 q: 0.130 -> 0.093 ms,
-    // 18 q: 0.179 -> 0.133 ms), equal at 20 and slower from 21 qubits up.
-    const bool small_state = n >= 14 && n <= 19;
+    // faster there on brickwork circuits (13 q: 0.198 -> 0.149 ms, 14 q: 0.130
+    // -> 0.093 ms, 18 q: 0.179 -> 0.133 ms; 10-12 q, one stage: 15-20 % from the
+    // block width alone), equal at 20 and slower from 21 qubits up.
+    const bool small_state = n <= 19;
     u32 K = block_qubits ? block_qubits : (small_state ? 3 : 4);
     const u32 lq = local_qubits ? local_qubits : (small_state ? 12 : kMaxLocal);
     if (n < kMinQubits || n > QM_CIRCUIT_MAX_QUBITS || K < 2 || K > 4 || lq < 8 || lq > kMaxLocal ||

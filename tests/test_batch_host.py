@@ -62,8 +62,8 @@ def test_fused_circuit_rejects_bad_arguments():
 
 
 def test_fused_defaults_by_qubit_count():
-    """Width and window 0 pick 3-qubit blocks and 12-qubit windows from 14 to 19
-    qubits, 4 and 13 otherwise; a 4-qubit dense gate widens the default block."""
+    """Width and window 0 pick 3-qubit blocks and 12-qubit windows up to 19
+    qubits, 4 and 13 above; a 4-qubit dense gate widens the default block."""
     from qulacs_amd.batch import FusedCircuit
 
     def stats(n, gates, **kw):
@@ -73,7 +73,7 @@ def test_fused_defaults_by_qubit_count():
         finally:
             fc.close()
 
-    for n, (k, lq) in ((16, (3, 12)), (19, (3, 12)), (20, (4, 13)), (22, (4, 13))):
+    for n, (k, lq) in ((8, (3, 12)), (13, (3, 12)), (16, (3, 12)), (19, (3, 12)), (20, (4, 13)), (22, (4, 13))):
         gates = [("dense", [q], np.eye(2)) for q in range(n)] + [("CZ", [q, q + 1], None) for q in range(n - 1)]
         assert stats(n, gates) == stats(n, gates, block_qubits=k, local_qubits=lq), n
     wide = [("dense", [q], np.eye(2)) for q in range(16)] + [("dense", [1, 5, 9, 14], np.eye(16))]
